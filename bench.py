@@ -1,0 +1,225 @@
+"""Benchmark of kettle's simulate() hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-vqe] [--no-cpu]
+
+Workload (one step): simulate() of the 30-qubit complex128 circuit "QFT on all qubits + 20 layers
+of random 1-qubit gates each followed by a CX brick ladder" on |0...0> (state resident in HBM;
+the reset to |0...0> is part of the step).  value = gates/s (kettle primitive gates after the QFT
+and swap expansion, divided by the step time).
+
+The JSON line also carries
+  roofline      the dominant kernel's (the fused tile pass) achieved HBM GB/s against 8 TB/s,
+                from HIP events on the stream libkq launches on, over the timed steps;
+  passes        achieved GB/s per gate class (low-index LDS tile, high-index tile, sweep);
+  cpu_baseline  the oracle (kettle's loops restated, 1 core) on a bounded sample of the same
+                circuit's gates at 30 qubits, as gates/s;
+  vqe           energy+gradient evaluations/s at 24 qubits, 960 parameters, 248 Pauli strings:
+                adjoint complex64 vs the plain per-parameter FP64 GPU path (parameter shift,
+                2P + 1 energies; timed on a bounded number of energies and scaled).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    sys.stderr.write(f"[bench] {msg}\n")
+    sys.stderr.flush()
+
+
+def cpu_baseline(ops, n: int, n_sample: int) -> dict:
+    from oracle import oracle as orc
+
+    idx = np.unique(np.linspace(0, len(ops) - 1, n_sample).astype(int))
+    sample = ops[idx]
+    log(f"cpu baseline: {len(sample)} gates at {n} qubits on 1 core")
+    secs = orc.time_gates(n, sample)
+    return {
+        "value": round(len(sample) / secs, 6),
+        "unit": "gates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{len(sample)} gates evenly spaced through the same {n}-qubit circuit "
+                  f"(gate indices {idx.tolist()}), oracle gate loop only, {secs:.1f} s",
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def vqe_section(n: int, layers: int, n_random: int, energies_sample: int) -> dict:
+    import kettle_amd as ka
+    from kettle_amd import workloads
+
+    w = workloads.vqe(n=n, layers=layers, n_random=n_random)
+    P = len(w.params)
+    # adjoint, complex64 state with FP64 reductions
+    st = ka.Statevector(n, ka.KQ_C64)
+    r = ka.energy_and_gradient(w.circuit, w.hamiltonian, w.params, state=st)   # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st.set_basis(0)
+        r = ka.energy_and_gradient(w.circuit, w.hamiltonian, w.params, state=st)
+    t_adj = (time.perf_counter() - t0) / reps
+    # the FP64 reference energy on the device (complex128 forward + expectation)
+    sv = ka.Statevector(n, ka.KQ_C128)
+    ka.simulate(w.circuit, sv)
+    e64 = ka.expectation_value(w.hamiltonian, sv).real
+    # plain per-parameter FP64 GPU path: one energy = simulate + expectation (complex128)
+    ops, _, _ = w.circuit.flatten(w.params)
+    t0 = time.perf_counter()
+    for i in range(energies_sample):
+        shifted = ops.copy()
+        shifted["angle"][i % len(ops)] += np.pi / 2
+        sv.set_basis(0)
+        ka.apply_gates(sv, shifted)
+        ka.expectation_value(w.hamiltonian, sv)
+    t_energy = (time.perf_counter() - t0) / energies_sample
+    t_plain = (2 * P + 1) * t_energy
+    return {
+        "config": f"{n}q n_local(RY,RZ; CX linear) x{layers}, {P} params, {w.hamiltonian.size()} Pauli strings",
+        "adjoint_c64_evals_per_s": round(1.0 / t_adj, 4),
+        "plain_fp64_param_shift_evals_per_s": round(1.0 / t_plain, 6),
+        "speedup": round(t_plain / t_adj, 2),
+        "energy_err_vs_fp64": abs(r.energy - e64),
+        "plain_timing": f"{energies_sample} complex128 energies timed ({t_energy * 1e3:.2f} ms each), x (2P+1)",
+        "checkpoint_restores": r.restores,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--layers", type=int, default=20)
+    ap.add_argument("--no-vqe", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-gates", type=int, default=3)
+    ap.add_argument("--vqe-qubits", type=int, default=24)
+    ap.add_argument("--vqe-energies", type=int, default=20)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the single-GPU shard is the measured unit); rerun with --gpus 1", file=sys.stderr)
+        return 2
+
+    import kettle_amd as ka
+    from kettle_amd import _lib, workloads
+
+    lib = _lib.load()
+    name = ctypes.create_string_buffer(256)
+    ncu = ctypes.c_int()
+    mem = ctypes.c_size_t()
+    _lib.check(lib.kq_device_info(name, 256, ctypes.byref(ncu), ctypes.byref(mem)))
+    log(f"device {name.value.decode()} CUs={ncu.value} mem={mem.value / 2**30:.0f} GiB")
+
+    n = args.qubits
+    circuit = workloads.qft_plus_layers(n=n, layers=args.layers)
+    ops, _, _ = circuit.flatten()
+    n_gates = len(ops)
+    log(f"workload: {n}q complex128 QFT + {args.layers} layers, {n_gates} gates")
+
+    sv = ka.Statevector(n, ka.KQ_C128)
+
+    def step():
+        sv.set_basis(0)
+        ka.apply_gates(sv, ops)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup} done")
+    _, n_passes, n_tiles = ka.plan_stats()
+    _lib.check(lib.kq_synchronize())
+    stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
+    _lib.check(lib.kq_profile_begin())
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        log(f"step {i + 1}/{args.steps} done")
+    _lib.check(lib.kq_synchronize())
+    elapsed = time.perf_counter() - t0
+    _lib.check(lib.kq_profile_end(_lib.ptr(stats)))
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    gates_per_s = n_gates / (elapsed / args.steps)
+    passes = {}
+    for i, cls in enumerate(_lib.PASS_CLASSES):
+        s = stats[i]
+        if s["launches"]:
+            passes[cls] = {
+                "launches_per_step": int(s["launches"]) // args.steps,
+                "gates_per_step": int(s["gates"]) // args.steps,
+                "avg_ms": round(float(s["ms"]) / int(s["launches"]), 4),
+                "GB_per_s": round(float(s["bytes"]) / (float(s["ms"]) * 1e-3) / 1e9, 1),
+            }
+    tile = stats[0].copy()
+    for f in ("ms", "bytes"):
+        tile[f] = stats[0][f] + stats[1][f]
+    tile["launches"] = stats[0]["launches"] + stats[1]["launches"]
+    kernel_ms = float(stats["ms"].sum()) / args.steps
+    achieved = float(tile["bytes"]) / (float(tile["ms"]) * 1e-3) / 1e9 if tile["ms"] else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_tile_pass<double>",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_launch": int(2 * 16 * (1 << n)),
+        "avg_launch_ms": round(float(tile["ms"]) / max(int(tile["launches"]), 1), 4),
+    }
+    out = {
+        "metric": "gates/s & HBM % at 30q c128; VQE energy+grad/s, adjoint vs plain GPU",
+        "value": round(gates_per_s, 3),
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n}q complex128 QFT + {args.layers} random 1q layers with CX brick ladders",
+            "qubits": n,
+            "gates": n_gates,
+            "passes_per_step": n_passes,
+            "tile_passes_per_step": n_tiles,
+            "kernel_ms_per_step": round(kernel_ms, 3),
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "passes": passes,
+    }
+    del sv
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_gates)
+    if not args.no_vqe:
+        log("vqe section")
+        out["vqe"] = vqe_section(args.vqe_qubits, 20, 200, args.vqe_energies)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,217 @@
+/*
+ * kq.h -- C-ABI of libkq, the MI355X (gfx950) engine under kettle's simulate() hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No torch or C++ types
+ * cross this boundary.  Every entry point returns KQ_OK (0) on success; on failure it returns a
+ * non-zero status and kq_last_error() gives a thread-local message.  Host-side validation
+ * failures that kettle reports with std::runtime_error return KQ_ERR_RUNTIME with kettle's
+ * message, so a C++ caller can rethrow the same exception type and text.
+ *
+ * Indexing is kettle's: little endian, qubit q <-> bit q of the amplitude index
+ * (reference: source/kettle_internal/state/qubit_state_conversion.cpp:37-83).
+ * Amplitudes cross the boundary as interleaved (re, im) doubles, which is the memory layout of
+ * kettle's std::vector<std::complex<double>> (include/kettle/state/statevector.hpp:94-96).
+ * Density matrices cross it column-major, (row r, col c) at r + c*N, which is the layout of the
+ * Eigen::MatrixXcd kettle stores (include/kettle/state/density_matrix.hpp).
+ */
+#ifndef KQ_ABI_H_INCLUDED
+#define KQ_ABI_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------- */
+#define KQ_OK            0
+#define KQ_ERR_INVALID   1   /* bad argument (null pointer, out-of-range qubit, bad size)      */
+#define KQ_ERR_HIP       2   /* a HIP runtime call failed                                      */
+#define KQ_ERR_ALLOC     3   /* device allocation failed or exceeds the configured cap         */
+#define KQ_ERR_RUNTIME   4   /* kettle-level std::runtime_error (message is kettle's text)     */
+#define KQ_ERR_OUT_OF_RANGE 5 /* kettle-level std::out_of_range                                */
+
+/* ---- precisions -------------------------------------------------------------------------- */
+#define KQ_C128 0            /* complex<double> amplitudes (kettle's only precision)           */
+#define KQ_C64  1            /* complex<float> amplitudes; norms, probabilities, CDFs,
+                                expectations and gradients are still accumulated in FP64     */
+
+/* ---- gate codes: the values of kettle's `enum class Gate : std::uint8_t`
+ *      (include/kettle/gates/primitive_gate.hpp:19-52), in the same order ------------------ */
+enum kq_gate {
+    KQ_H = 0, KQ_X, KQ_Y, KQ_Z, KQ_S, KQ_SDAG, KQ_T, KQ_TDAG, KQ_SX, KQ_SXDAG,
+    KQ_RX = 10, KQ_RY, KQ_RZ, KQ_P,
+    KQ_CH = 14, KQ_CX, KQ_CY, KQ_CZ, KQ_CS, KQ_CSDAG, KQ_CT, KQ_CTDAG, KQ_CSX, KQ_CSXDAG,
+    KQ_CRX = 24, KQ_CRY, KQ_CRZ, KQ_CP,
+    KQ_U = 28, KQ_CU = 29,
+    KQ_M = 30,
+    /* program-structure ops (kettle's non-gate CircuitElements, circuit_element.hpp:23-94) */
+    KQ_OP_IF = 64,           /* q0 = predicate offset, q1 = length of the body that follows    */
+    KQ_OP_IF_ELSE = 65,      /* q0 = predicate offset, q1 = if-body length, aux = else length  */
+    KQ_OP_LOG_STATE = 66,    /* StatevectorCircuitLogger: snapshot the state (callback)       */
+    KQ_OP_LOG_CREG = 67      /* ClassicalRegisterCircuitLogger: snapshot the register         */
+};
+
+/*
+ * One element of a flattened circuit.  Fields follow kettle's GateInfo
+ * (primitive_gate.hpp:71-79; packing in gates/primitive_gate/gate_create.cpp:34-244):
+ *   1-target gates (H..P, U):  q0 = target
+ *   1-control-1-target gates:  q0 = control, q1 = target
+ *   M:                         q0 = qubit,   q1 = classical bit
+ *   angle gates:               angle = the evaluated angle (literal or parameter value)
+ *   U / CU:                    u = {re00, im00, re01, im01, re10, im10, re11, im11}
+ *   aux:                       parameterised gates: index of the parameter (-1 = literal);
+ *                              KQ_OP_IF_ELSE: length of the else body
+ * The body of an IF / IF_ELSE is the next q1 (and then aux) ops of the same array, so nested
+ * sub-circuits flatten recursively.
+ */
+typedef struct kq_op {
+    int32_t gate;
+    int32_t q0;
+    int32_t q1;
+    int32_t aux;
+    double  angle;
+    double  u[8];
+} kq_op;
+
+/*
+ * Control-flow predicates (kettle's ControlFlowPredicate, control_flow_predicate.cpp:12-69)
+ * are packed in one int32 array.  A predicate at offset p is
+ *   preds[p] = kind (0 = IF, 1 = IF_NOT), preds[p+1] = n, then n pairs (bit_index, expected).
+ */
+
+/* classical register cells: -1 = not measured, else 0 / 1 (kettle's optional<int>) */
+
+/* Logger callback: kind is KQ_OP_LOG_STATE or KQ_OP_LOG_CREG.  For LOG_STATE, `amps` is a host
+ * copy of the state (interleaved complex doubles; 2*N doubles for a statevector, 2*N*N for a
+ * density matrix).  For LOG_CREG, `creg` is the register.  Both pointers are valid only for the
+ * duration of the call. */
+typedef void (*kq_logger_fn)(void* user, int32_t kind, const double* amps, const int8_t* creg,
+                             size_t n_bits);
+
+/* ---- library ------------------------------------------------------------------------------ */
+const char* kq_last_error(void);
+int kq_init(int device);                 /* selects the HIP device; idempotent                */
+int kq_device_info(char* name, size_t name_len, int* n_cu, size_t* total_mem);
+int kq_synchronize(void);
+
+/* ---- states (statevector or density matrix) ----------------------------------------------- */
+typedef struct kq_state kq_state;
+
+/* Statevector of n_qubits in |0...0> (kettle: Statevector(n), state/state.cpp:21-28). */
+int kq_sv_create(kq_state** out, uint32_t n_qubits, int precision);
+/* Density matrix |0..0><0..0| of n_qubits, stored as a 2n-qubit vector vec(rho). */
+int kq_dm_create(kq_state** out, uint32_t n_qubits, int precision);
+int kq_state_destroy(kq_state* s);
+int kq_state_info(const kq_state* s, uint32_t* n_qubits, int* precision, int* is_density);
+/* Raw device pointer and byte size of the amplitude buffer (for zero-copy interop). */
+int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes);
+/* Host <-> device copies of the whole state, interleaved complex doubles (2*N or 2*N*N). */
+int kq_state_upload(kq_state* s, const double* host);
+int kq_state_download(const kq_state* s, double* host);
+/* Computational basis state |index> (or |index><index| for a density matrix). */
+int kq_state_set_basis(kq_state* s, uint64_t index);
+int kq_state_copy(kq_state* dst, const kq_state* src);
+
+/* ---- simulate() ---------------------------------------------------------------------------
+ * ket::simulate(circuit, state, seed) for a flattened circuit (simulate.cpp:417-597 for a
+ * statevector, simulate_density_matrix.cpp:354-548 for a density matrix).  `creg` must hold
+ * n_bits cells; they are reset to -1 first.  has_seed = 0 draws mid-circuit measurements from
+ * std::random_device like kettle's get_prng_(nullopt) (common/prng.cpp:9-19).
+ * A density matrix rejects logger ops with kettle's error, as simulate_density_matrix.cpp:391. */
+int kq_simulate(kq_state* s, uint32_t circuit_n_qubits, const kq_op* ops, size_t n_ops,
+                const int32_t* preds, size_t n_preds, int has_seed, int32_t seed,
+                int8_t* creg, size_t n_bits, kq_logger_fn logger, void* logger_user);
+
+/* Straight-line gate list (no M, no control flow): the fused device path alone. */
+int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops);
+
+/* Gate fusion control: max_tile_qubits = 0 disables fusion (one sweep per gate). */
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass);
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass);
+/* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
+int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);
+
+/* ---- mid-circuit measurement (measure.hpp:33-57, measure.cpp:22-72;
+ *      density matrix: measure_density_matrix.cpp:21-95) ---------------------------------- */
+int kq_qubit_weights(kq_state* s, uint32_t qubit, double out[2]);
+int kq_collapse(kq_state* s, uint32_t qubit, int kept_bit, double scale);
+int kq_measure(kq_state* s, uint32_t qubit, int has_seed, int32_t seed, int* outcome);
+
+/* ---- probabilities and sampling (calculations/probabilities.cpp:49-124,
+ *      calculations/measurements.cpp:29-312) --------------------------------------------- */
+/* |amp_i|^2 (diag(rho) for a density matrix) with optional per-qubit bit-flip noise
+ * (noise = NULL or n_qubits values in [0,1]).  out: N doubles on the host. */
+int kq_probabilities(kq_state* s, const double* noise, double* out);
+/* perform_measurements_as_memory(state, shots, noise, seed): state indices per shot. */
+int kq_sample(kq_state* s, const double* noise, uint64_t n_shots, int has_seed, int32_t seed,
+              uint64_t* out_indices);
+/* The same sampler over caller-provided raw probabilities (perform_measurements_as_memory(
+ * probabilities_raw, ...)); probabilities are uploaded and scanned on the device. */
+int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_shots,
+                            int has_seed, int32_t seed, uint64_t* out_indices);
+
+/* ---- Pauli strings (operator/pauli/pauli_operator.cpp:76-105, simulation/simulate_pauli.cpp)
+ * A Pauli operator of S strings is given as:
+ *   term_offsets[S+1]: string s owns terms [term_offsets[s], term_offsets[s+1])
+ *   term_qubits[], term_kinds[] (1 = X, 2 = Y, 3 = Z; 0 = I is ignored), in kettle's terms() order
+ *   phases[S] (PauliPhase: 0 = +1, 1 = +i, 2 = -1, 3 = -i), coeffs[2*S] (re, im)
+ * Expectation = sum_s coeff_s * phase_s * <psi| P_s |psi>, accumulated in FP64. */
+int kq_pauli_expectation(kq_state* s, size_t n_strings, const uint32_t* term_offsets,
+                         const uint32_t* term_qubits, const uint8_t* term_kinds,
+                         const uint8_t* phases, const double* coeffs, double out[2]);
+/* ket::simulate(SparsePauliString, state): applies one string's terms in place. */
+int kq_apply_pauli_string(kq_state* s, size_t n_terms, const uint32_t* term_qubits,
+                          const uint8_t* term_kinds);
+
+/* ---- density-matrix read-out (state/density_matrix.cpp:150-278) ------------------------- */
+int kq_dm_trace(kq_state* s, double out[2]);
+/* partial_trace(rho, qubits): traces out `qubits`; out: (2^(n-k))^2 complex, column-major. */
+int kq_dm_partial_trace(kq_state* s, const uint32_t* qubits, size_t n, double* out);
+/* |psi><psi| of a statevector (statevector_to_density_matrix, density_matrix.cpp:170-185). */
+int kq_dm_from_statevector(kq_state* dm, const kq_state* sv);
+
+/* ---- adjoint energy gradient (no kettle counterpart: kettle's VQE is derivative-free) ------
+ * For a straight-line parameterised circuit (ops with aux >= 0 name parameter slots) and a
+ * Pauli operator H (same packing as kq_pauli_expectation, real weights), computes
+ * E = <psi|H|psi> and dE/dtheta_p for p < n_params by one forward and one reverse sweep.
+ * `state` is the work state (its precision is the compute precision); it ends holding psi(theta).
+ * Checkpoints every `checkpoint_every` gates (0 = sqrt(G)) bound inverse-gate drift: at each
+ * checkpoint the uncomputed state is compared with the stored one and restored when the
+ * max-abs deviation exceeds drift_tol.  n_restores (nullable) reports how often that happened. */
+int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops,
+                       size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
+                       const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs,
+                       size_t n_params, double* energy, double* grad,
+                       uint32_t checkpoint_every, double drift_tol, size_t* n_restores);
+
+/* ---- profiling ------------------------------------------------------------------------------
+ * While enabled, every gate pass launched by kq_simulate / kq_apply_gates is bracketed by HIP
+ * events on the state's own stream; durations and algorithmic bytes (read + write of every
+ * amplitude a pass touches) accumulate per class:
+ *   0 = tile pass whose tile is the low contiguous qubits (LDS-fused low-index gates)
+ *   1 = tile pass that includes higher qubits (strided high-index gates)
+ *   2 = single-gate sweep (lone diagonal gates) */
+typedef struct kq_pass_stats {
+    double   ms;          /* summed kernel time */
+    uint64_t launches;
+    double   bytes;       /* summed algorithmic bytes */
+    uint64_t gates;       /* gates applied by these launches */
+} kq_pass_stats;
+int kq_profile_begin(void);
+int kq_profile_end(kq_pass_stats out[3]);
+
+/* ---- debug / parity probes ------------------------------------------------------------------
+ * Device-computed pair maps in kettle's generator order (gate_pair_generator.hpp:24-160):
+ * flat pair index k -> (s0, s1).  For the 2-qubit map with four_way != 0, writes 4 indices per
+ * k in next_unset_and_set() order (c0t0, c0t1, c1t0, c1t1). */
+int kq_debug_pair_map_1q(uint32_t n_qubits, uint32_t target, uint64_t* s0, uint64_t* s1);
+int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, int four_way,
+                         uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KQ_ABI_H_INCLUDED */

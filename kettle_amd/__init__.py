@@ -1,0 +1,16 @@
+"""kettle_amd: kettle's simulate() hot path on MI355X (gfx950) through libkq's C-ABI.
+
+Importing the package does not touch the GPU; the first call that needs libkq loads
+``kettle_amd/lib/libkq.so`` and fails loudly if it has not been built.
+"""
+from ._lib import KQ_C128, KQ_C64, KqError
+from .circuit import (ControlFlowBooleanKind, ControlFlowPredicate, Gate, Matrix2X2, ParameterID, QuantumCircuit,
+                      extend_circuit)
+from .api import (DensityMatrix, EnergyGradient, PauliOperator, PauliPhase, PauliTerm, QuantumNoise,
+                  SimulationResult, SparsePauliString, Statevector, apply_gates, calculate_probabilities_raw,
+                  energy_and_gradient, expectation_value, partial_trace, perform_measurements_as_counts,
+                  perform_measurements_as_counts_marginal, perform_measurements_as_counts_raw,
+                  perform_measurements_as_memory, plan_stats, set_fusion, simulate, simulate_pauli_string,
+                  state_index_to_bitstring, statevector_to_density_matrix)
+
+__all__ = [name for name in dir() if not name.startswith("_")]

@@ -1,0 +1,426 @@
+"""State handles and kettle's free functions over libkq.
+
+Mirrors kettle's read-out API:
+  ``simulate``                        simulation/simulate.hpp:41, simulate_density_matrix.hpp:46
+  ``calculate_probabilities_raw``     calculations/probabilities.hpp:43-46
+  ``perform_measurements_as_memory``  calculations/measurements.hpp:42-60
+  ``perform_measurements_as_counts``  calculations/measurements.hpp:92-97 (and _raw / _marginal)
+  ``expectation_value``               operator/pauli/pauli_operator.hpp:82-84
+plus ``energy_and_gradient`` (adjoint sweep; kettle has no gradient).
+"""
+from __future__ import annotations
+
+import enum
+from collections import Counter
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import KQ_C128, KQ_C64, LOGGER_FN, check, load, ptr
+from .circuit import OP_LOG_CREG, OP_LOG_STATE, QuantumCircuit
+
+
+def _p(x):
+    return _lib.ctypes.c_void_p(x)
+
+
+class _State:
+    """Owning handle of a device state (statevector or vectorised density matrix)."""
+
+    _create = None
+
+    def __init__(self, n_qubits: int, precision: int = KQ_C128):
+        lib = load()
+        h = _lib.ctypes.c_void_p()
+        check(getattr(lib, self._create)(_lib.ctypes.byref(h), int(n_qubits), int(precision)))
+        self._h = h
+        self._n = int(n_qubits)
+        self._precision = int(precision)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def n_qubits(self) -> int:
+        return self._n
+
+    def n_states(self) -> int:
+        return 1 << self._n
+
+    @property
+    def precision(self) -> int:
+        return self._precision
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().kq_state_destroy(self._h)
+            self._h = _lib.ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _n_values(self) -> int:
+        raise NotImplementedError
+
+    def upload(self, amps: np.ndarray) -> None:
+        a = np.ascontiguousarray(amps, dtype=np.complex128).reshape(-1)
+        if a.size != self._n_values():
+            raise ValueError(f"expected {self._n_values()} complex values, got {a.size}")
+        check(load().kq_state_upload(self._h, ptr(a)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self._n_values(), dtype=np.complex128)
+        check(load().kq_state_download(self._h, ptr(out)))
+        return out
+
+    def set_basis(self, index: int) -> None:
+        check(load().kq_state_set_basis(self._h, int(index)))
+
+    def device_ptr(self) -> tuple[int, int]:
+        p = _lib.ctypes.c_void_p()
+        b = _lib.ctypes.c_size_t()
+        check(load().kq_state_device_ptr(self._h, _lib.ctypes.byref(p), _lib.ctypes.byref(b)))
+        return p.value, b.value
+
+
+class Statevector(_State):
+    """``ket::Statevector`` resident in HBM; starts in |0...0> like ``Statevector(n)``."""
+
+    _create = "kq_sv_create"
+
+    def __init__(self, n_qubits_or_amps, precision: int = KQ_C128):
+        if isinstance(n_qubits_or_amps, (int, np.integer)):
+            super().__init__(int(n_qubits_or_amps), precision)
+        else:
+            amps = np.asarray(n_qubits_or_amps, dtype=np.complex128).reshape(-1)
+            n = int(amps.size).bit_length() - 1
+            if amps.size < 2 or (1 << n) != amps.size:
+                raise RuntimeError("The provided coefficients must have a size equal to a power of 2.\n")
+            super().__init__(n, precision)
+            self.upload(amps)
+
+    def _n_values(self) -> int:
+        return 1 << self._n
+
+
+class DensityMatrix(_State):
+    """``ket::DensityMatrix`` (column-major, as Eigen stores it) resident in HBM as vec(rho)."""
+
+    _create = "kq_dm_create"
+
+    def _n_values(self) -> int:
+        return 1 << (2 * self._n)
+
+    def matrix(self) -> np.ndarray:
+        """rho as a 2-D array (rows = first index)."""
+        N = 1 << self._n
+        return self.download().reshape(N, N).T
+
+    def upload_matrix(self, rho: np.ndarray) -> None:
+        self.upload(np.asarray(rho, dtype=np.complex128).T.reshape(-1))
+
+    def trace(self) -> complex:
+        out = np.zeros(2)
+        check(load().kq_dm_trace(self._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    @staticmethod
+    def from_statevector(sv: Statevector) -> "DensityMatrix":
+        dm = DensityMatrix(sv.n_qubits(), sv.precision)
+        check(load().kq_dm_from_statevector(dm.handle, sv.handle))
+        return dm
+
+
+def statevector_to_density_matrix(sv: Statevector) -> DensityMatrix:
+    return DensityMatrix.from_statevector(sv)
+
+
+def partial_trace(dm: DensityMatrix, qubit_indices: Sequence[int]) -> np.ndarray:
+    """kettle's ``partial_trace`` (density_matrix.cpp:231-278): traces out ``qubit_indices``."""
+    qs = np.asarray(sorted(set(int(q) for q in qubit_indices)), dtype=np.uint32)
+    side = 1 << (dm.n_qubits() - len(qs))
+    out = np.empty(side * side, dtype=np.complex128)
+    check(load().kq_dm_partial_trace(dm.handle, ptr(qs), len(qs), ptr(out)))
+    return out.reshape(side, side).T
+
+
+@dataclass
+class SimulationResult:
+    classical_register: list[int | None]
+    loggers: list = field(default_factory=list)
+
+
+def simulate(circuit: QuantumCircuit, state: _State, prng_seed: int | None = None) -> SimulationResult:
+    """``ket::simulate(circuit, state, seed)``; returns the classical register and logger snapshots."""
+    ops, preds, _ = circuit.flatten()
+    n_bits = circuit.n_bits()
+    creg = np.full(max(n_bits, 1), -1, dtype=np.int8)
+    logs: list = []
+
+    def _logger(_user, kind, amps, cr, nb):
+        if kind == OP_LOG_STATE:
+            n = state.n_states()
+            logs.append(("statevector", np.ctypeslib.as_array(amps, shape=(2 * n,)).view(np.complex128).copy()))
+        elif kind == OP_LOG_CREG:
+            reg = np.ctypeslib.as_array(cr, shape=(nb,)).copy() if nb else np.zeros(0, np.int8)
+            logs.append(("classical_register", [None if v < 0 else int(v) for v in reg]))
+
+    cb = LOGGER_FN(_logger)
+    check(
+        load().kq_simulate(
+            state.handle, circuit.n_qubits(), ptr(ops), len(ops), ptr(preds), len(preds),
+            0 if prng_seed is None else 1, 0 if prng_seed is None else int(prng_seed), ptr(creg), n_bits, cb, None,
+        )
+    )
+    reg = [None if v < 0 else int(v) for v in creg[:n_bits]]
+    return SimulationResult(reg, logs)
+
+
+def apply_gates(state: _State, ops: np.ndarray) -> None:
+    """Run a straight-line op array (no measurement / control flow) through the fused path."""
+    ops = np.ascontiguousarray(ops)
+    check(load().kq_apply_gates(state.handle, ptr(ops), len(ops)))
+
+
+def plan_stats() -> tuple[int, int, int]:
+    c = _lib.ctypes
+    a, b, d = c.c_size_t(), c.c_size_t(), c.c_size_t()
+    check(load().kq_last_plan_stats(c.byref(a), c.byref(b), c.byref(d)))
+    return a.value, b.value, d.value
+
+
+def set_fusion(max_tile_qubits: int, max_gates_per_pass: int = 0) -> None:
+    check(load().kq_set_fusion(int(max_tile_qubits), int(max_gates_per_pass)))
+
+
+# ---- probabilities and sampling ----------------------------------------------------------------------
+
+class QuantumNoise:
+    """kettle's ``QuantumNoise`` (probabilities.cpp:20-47): per-qubit bit-flip probability."""
+
+    def __init__(self, n_qubits: int):
+        self._noise = np.zeros(n_qubits, dtype=np.float64)
+
+    def set(self, index: int, noise: float) -> None:
+        if index >= len(self._noise):
+            raise RuntimeError("ERROR: Out-of-bounds access for QuantumNoise probability.")
+        if not (0.0 <= noise <= 1.0):
+            raise RuntimeError("ERROR: Cannot set probability in QuantumNoise object outside of [0, 1].")
+        self._noise[index] = noise
+
+    def get(self, index: int) -> float:
+        if index >= len(self._noise):
+            raise RuntimeError("ERROR: Out-of-bounds access for QuantumNoise probability.")
+        return float(self._noise[index])
+
+    def array(self) -> np.ndarray:
+        return self._noise
+
+
+def calculate_probabilities_raw(state: _State, noise: QuantumNoise | None = None) -> np.ndarray:
+    out = np.empty(state.n_states(), dtype=np.float64)
+    nz = None if noise is None else np.ascontiguousarray(noise.array())
+    check(load().kq_probabilities(state.handle, ptr(nz) if nz is not None else _p(0), ptr(out)))
+    return out
+
+
+def _seed(seed):
+    return (0, 0) if seed is None else (1, int(seed))
+
+
+def perform_measurements_as_memory(state_or_probs, n_shots: int, noise: QuantumNoise | None = None,
+                                   seed: int | None = None) -> np.ndarray:
+    out = np.empty(int(n_shots), dtype=np.uint64)
+    hs, sv = _seed(seed)
+    if isinstance(state_or_probs, _State):
+        nz = None if noise is None else np.ascontiguousarray(noise.array())
+        check(load().kq_sample(state_or_probs.handle, ptr(nz) if nz is not None else _p(0), int(n_shots), hs, sv, ptr(out)))
+    else:
+        p = np.ascontiguousarray(state_or_probs, dtype=np.float64)
+        check(load().kq_sample_probabilities(ptr(p), p.size, int(n_shots), hs, sv, ptr(out)))
+    return out
+
+
+def state_index_to_bitstring(index: int, n_qubits: int, marginal: Sequence[int] = ()) -> str:
+    """Little-endian bitstring: character i is bit i; marginal qubits print as 'x'
+    (qubit_state_conversion.cpp:37-83, marginal.cpp:19-43)."""
+    marg = set(marginal)
+    return "".join("x" if i in marg else ("1" if (index >> i) & 1 else "0") for i in range(n_qubits))
+
+
+def perform_measurements_as_counts_raw(state_or_probs, n_shots: int, noise=None, seed=None) -> dict[int, int]:
+    mem = perform_measurements_as_memory(state_or_probs, n_shots, noise, seed)
+    return dict(sorted(Counter(int(i) for i in mem).items()))
+
+
+def perform_measurements_as_counts_marginal(state_or_probs, n_shots: int, marginal_qubits: Sequence[int] = (),
+                                            noise=None, seed=None) -> dict[str, int]:
+    if isinstance(state_or_probs, _State):
+        n = state_or_probs.n_qubits()
+    else:
+        size = len(state_or_probs)
+        if size == 0 or size & (size - 1):
+            raise RuntimeError("The number of probabilities must be a power of 2.\n")
+        n = size.bit_length() - 1
+    if any(q >= n for q in marginal_qubits):
+        raise RuntimeError("ERROR: marginal qubit index out of range.")
+    raw = perform_measurements_as_counts_raw(state_or_probs, n_shots, noise, seed)
+    counts: Counter = Counter()
+    for idx, c in raw.items():
+        counts[state_index_to_bitstring(idx, n, marginal_qubits)] += c
+    return dict(sorted(counts.items()))
+
+
+def perform_measurements_as_counts(state, n_shots: int, noise=None, seed=None) -> dict[str, int]:
+    return perform_measurements_as_counts_marginal(state, n_shots, (), noise, seed)
+
+
+# ---- Pauli strings ------------------------------------------------------------------------------------
+
+class PauliTerm(enum.IntEnum):
+    I = 0
+    X = 1
+    Y = 2
+    Z = 3
+
+
+class PauliPhase(enum.IntEnum):
+    PLUS_ONE = 0
+    PLUS_EYE = 1
+    MINUS_ONE = 2
+    MINUS_EYE = 3
+
+
+class SparsePauliString:
+    """kettle's ``SparsePauliString`` (sparse_pauli_string.hpp:42-144): (qubit, term) pairs."""
+
+    def __init__(self, terms, n_qubits: int | None = None, phase: PauliPhase = PauliPhase.PLUS_ONE):
+        if n_qubits is None:
+            # dense form: one PauliTerm per qubit
+            self._n = len(terms)
+            self._terms = [(i, PauliTerm(t)) for i, t in enumerate(terms)]
+        else:
+            self._n = int(n_qubits)
+            self._terms = [(int(q), PauliTerm(t)) for q, t in terms]
+        if self._n == 0:
+            raise RuntimeError("ERROR: the number of qubits in the SparsePauliString must be positive.\n")
+        self._phase = PauliPhase(phase)
+
+    def n_qubits(self) -> int:
+        return self._n
+
+    def terms(self) -> list[tuple[int, PauliTerm]]:
+        return list(self._terms)
+
+    def phase(self) -> PauliPhase:
+        return self._phase
+
+    def set_phase(self, phase: PauliPhase) -> None:
+        self._phase = PauliPhase(phase)
+
+
+class PauliOperator:
+    """kettle's ``PauliOperator`` (pauli_operator.hpp): sum of coefficient * string."""
+
+    def __init__(self, weighted=None, n_qubits: int | None = None):
+        self._strings: list[tuple[complex, SparsePauliString]] = []
+        if weighted:
+            for c, s in weighted:
+                self.add(c, s)
+            n_set = {s.n_qubits() for _, s in self._strings}
+            if len(n_set) != 1:
+                raise RuntimeError(
+                    "ERROR: construction of `PauliOperator` with WeightedPauliString instances requires\n"
+                    "all pauli strings to have the same number of qubits.\n"
+                )
+            self._n = n_set.pop()
+        else:
+            self._n = int(n_qubits or 0)
+
+    def n_qubits(self) -> int:
+        return self._n
+
+    def add(self, coefficient: complex, pauli_string: SparsePauliString) -> None:
+        self._strings.append((complex(coefficient), pauli_string))
+
+    def weighted_pauli_strings(self):
+        return list(self._strings)
+
+    def size(self) -> int:
+        return len(self._strings)
+
+    def packed(self):
+        """-> (term_offsets u32, term_qubits u32, term_kinds u8, phases u8, coeffs f64[2S])."""
+        offs = [0]
+        qs: list[int] = []
+        ks: list[int] = []
+        ph: list[int] = []
+        co: list[float] = []
+        for c, s in self._strings:
+            for q, t in s.terms():
+                qs.append(q)
+                ks.append(int(t))
+            offs.append(len(qs))
+            ph.append(int(s.phase()))
+            co.extend([c.real, c.imag])
+        return (np.asarray(offs, np.uint32), np.asarray(qs, np.uint32), np.asarray(ks, np.uint8),
+                np.asarray(ph, np.uint8), np.asarray(co, np.float64))
+
+
+def expectation_value(op, state: Statevector) -> complex:
+    """kettle's ``expectation_value`` for a ``PauliOperator`` or a single ``SparsePauliString``."""
+    if isinstance(op, SparsePauliString):
+        op = PauliOperator([(1.0, op)])
+    offs, qs, ks, ph, co = op.packed()
+    out = np.zeros(2)
+    check(load().kq_pauli_expectation(state.handle, op.size(), ptr(offs), ptr(qs), ptr(ks), ptr(ph), ptr(co), ptr(out)))
+    return complex(out[0], out[1])
+
+
+def simulate_pauli_string(pauli_string: SparsePauliString, state: Statevector) -> None:
+    """``ket::simulate(SparsePauliString, Statevector&)`` (simulate_pauli.cpp:124-128)."""
+    if pauli_string.n_qubits() != state.n_qubits():
+        raise RuntimeError("Invalid simulation; SparsePauliString and state have different number of qubits.")
+    terms = pauli_string.terms()
+    qs = np.asarray([q for q, _ in terms], np.uint32)
+    ks = np.asarray([int(t) for _, t in terms], np.uint8)
+    check(load().kq_apply_pauli_string(state.handle, len(terms), ptr(qs), ptr(ks)))
+
+
+# ---- adjoint gradient ------------------------------------------------------------------------------------
+
+@dataclass
+class EnergyGradient:
+    energy: float
+    gradient: np.ndarray
+    restores: int
+
+
+def energy_and_gradient(circuit: QuantumCircuit, hamiltonian: PauliOperator, params: Sequence[str] | None = None,
+                        precision: int = KQ_C64, state: Statevector | None = None, checkpoint_every: int = 0,
+                        drift_tol: float = 1.0e-5) -> EnergyGradient:
+    """E = <psi(theta)|H|psi(theta)> and dE/dtheta for the circuit's parameters (adjoint sweep).
+
+    ``params`` fixes the gradient order (default: the circuit's parameter creation order).
+    The sweep starts from ``state`` (default |0...0>) and leaves psi(theta) in it."""
+    ops, _, order = circuit.flatten(params)
+    if any(int(g) >= 30 for g in ops["gate"]):
+        raise RuntimeError("energy_and_gradient: the circuit must be straight-line (no M, control flow or loggers)")
+    if state is None:
+        state = Statevector(circuit.n_qubits(), precision)
+    offs, qs, ks, ph, co = hamiltonian.packed()
+    grad = np.zeros(len(order), dtype=np.float64)
+    e = _lib.ctypes.c_double()
+    nr = _lib.ctypes.c_size_t()
+    check(
+        load().kq_energy_gradient(
+            state.handle, ptr(ops), len(ops), hamiltonian.size(), ptr(offs), ptr(qs), ptr(ks), ptr(ph), ptr(co),
+            len(order), _lib.ctypes.byref(e), ptr(grad), int(checkpoint_every), float(drift_tol), _lib.ctypes.byref(nr),
+        )
+    )
+    return EnergyGradient(e.value, grad, nr.value)

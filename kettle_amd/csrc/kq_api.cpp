@@ -1,0 +1,1247 @@
+// The C-ABI entry points of libkq (include/kq.h) and their host-side logic: state handles, the
+// simulate() interpreter, measurement draws, sampling, Pauli canonicalisation and the adjoint
+// energy gradient.  Device work goes through the launchers in kq_kernels.hip.
+#include <algorithm>
+#include <bit>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <optional>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+namespace
+{
+
+thread_local std::string g_last_error;
+std::mutex g_init_mutex;
+bool g_initialised = false;
+int g_device = 0;
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return KQ_OK;
+    }
+    catch (const InvalidArg& e) {
+        g_last_error = e.what();
+        return KQ_ERR_INVALID;
+    }
+    catch (const HipError& e) {
+        g_last_error = e.what();
+        return KQ_ERR_HIP;
+    }
+    catch (const AllocError& e) {
+        g_last_error = e.what();
+        return KQ_ERR_ALLOC;
+    }
+    catch (const std::out_of_range& e) {
+        g_last_error = e.what();
+        return KQ_ERR_OUT_OF_RANGE;
+    }
+    catch (const std::exception& e) {
+        g_last_error = e.what();
+        return KQ_ERR_RUNTIME;
+    }
+    catch (...) {
+        g_last_error = "unknown exception";
+        return KQ_ERR_RUNTIME;
+    }
+}
+
+void ensure_init()
+{
+    std::lock_guard<std::mutex> lock {g_init_mutex};
+    if (!g_initialised) {
+        int count = 0;
+        KQ_HIP_CHECK(hipGetDeviceCount(&count));
+        if (count <= 0) {
+            throw HipError {"no HIP device available"};
+        }
+        KQ_HIP_CHECK(hipSetDevice(g_device));
+        hipDeviceProp_t prop;
+        KQ_HIP_CHECK(hipGetDeviceProperties(&prop, g_device));
+        device_lds_per_block() = prop.sharedMemPerBlock;
+        g_initialised = true;
+    }
+    else {
+        KQ_HIP_CHECK(hipSetDevice(g_device));
+    }
+}
+
+void check_state(const kq_state* s)
+{
+    if (s == nullptr || s->amps == nullptr) {
+        throw InvalidArg {"null or destroyed state handle"};
+    }
+}
+
+void* device_alloc(size_t bytes)
+{
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess || p == nullptr) {
+        (void) hipGetLastError();
+        throw AllocError {"hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    return p;
+}
+
+void* grow(void*& buf, size_t& have, size_t need)
+{
+    if (need > have) {
+        if (buf != nullptr) {
+            KQ_HIP_CHECK(hipFree(buf));
+            buf = nullptr;
+            have = 0;
+        }
+        const size_t bytes = std::max<size_t>(need, 1 << 16);
+        buf = device_alloc(bytes);
+        have = bytes;
+    }
+    return buf;
+}
+
+kq_state* create_state(uint32_t n, int precision, bool dm)
+{
+    if (precision != KQ_C128 && precision != KQ_C64) {
+        throw InvalidArg {"precision must be KQ_C128 or KQ_C64"};
+    }
+    if (n == 0) {
+        throw std::runtime_error {dm ? "ERROR: cannot perform a DensityMatrix simulation with 0 qubits.\n"
+                                     : "There must be at least 1 qubit in the Statevector.\n"};
+    }
+    const uint32_t nvec = dm ? 2 * n : n;
+    if (nvec > 36) {
+        throw InvalidArg {"state too large for one device"};
+    }
+    ensure_init();
+    auto* s = new kq_state {};
+    s->n = n;
+    s->nvec = nvec;
+    s->precision = precision;
+    s->is_dm = dm ? 1 : 0;
+    s->n_amps = uint64_t {1} << nvec;
+    try {
+        s->amps = device_alloc(s->n_amps * amp_bytes(precision));
+        KQ_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        launch_set_basis(s, 0);
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    }
+    catch (...) {
+        if (s->amps) {
+            (void) hipFree(s->amps);
+        }
+        if (s->stream) {
+            (void) hipStreamDestroy(s->stream);
+        }
+        delete s;
+        throw;
+    }
+    return s;
+}
+
+// ---- running straight-line gate batches -------------------------------------------------------
+
+uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
+{
+    const bool ctrl = g.control >= 0;
+    if (g.kind == GK_DIAG) {
+        const bool skip0 = g.m[0] == 1.0 && g.m[1] == 0.0;
+        return n_amps >> ((ctrl ? 1 : 0) + (skip0 ? 1 : 0));
+    }
+    return n_amps >> (ctrl ? 1 : 0);
+}
+
+void run_records(kq_state* s, const std::vector<GateRec>& recs)
+{
+    PlanStats& st = last_stats();
+    st.n_gates += recs.size();
+    if (recs.empty()) {
+        return;
+    }
+    const double ab = static_cast<double>(amp_bytes(s->precision));
+    const int max_k = fusion_max_k(s->precision, s->nvec);
+    if (max_k <= 0) {
+        for (const auto& g : recs) {
+            profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(g, s->n_amps)), 1, [&] { launch_gate_sweep(s, g); });
+            ++st.n_passes;
+        }
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        profiler_collect();
+        return;
+    }
+    const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates());
+    size_t total = 0;
+    for (const auto& p : passes) {
+        total += p.gates.size();
+    }
+    std::vector<GateRec> flat;
+    flat.reserve(total);
+    for (const auto& p : passes) {
+        flat.insert(flat.end(), p.gates.begin(), p.gates.end());
+    }
+    auto* d = static_cast<GateRec*>(grow(s->prog, s->prog_bytes, std::max<size_t>(total, 1) * sizeof(GateRec)));
+    if (total) {
+        KQ_HIP_CHECK(hipMemcpyAsync(d, flat.data(), total * sizeof(GateRec), hipMemcpyHostToDevice, s->stream));
+    }
+    size_t off = 0;
+    for (const auto& p : passes) {
+        if (p.sweep) {
+            profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s->n_amps)), 1,
+                     [&] { launch_gate_sweep(s, p.single); });
+        }
+        else {
+            const int cls = p.c >= p.k ? 0 : 1;
+            profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
+                     [&] { launch_tile_pass(s, p, d + off); });
+            off += p.gates.size();
+            ++st.n_tile_passes;
+        }
+        ++st.n_passes;
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    profiler_collect();
+}
+
+void run_ops(kq_state* s, const kq_op* ops, size_t n)
+{
+    std::vector<GateRec> recs;
+    recs.reserve(s->is_dm ? 2 * n : n);
+    for (size_t i = 0; i < n; ++i) {
+        if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+            throw InvalidArg {"kq_apply_gates: op " + std::to_string(i) + " is not a unitary gate"};
+        }
+        compile_gate(ops[i], s->n, s->is_dm != 0, recs);
+    }
+    run_records(s, recs);
+}
+
+// ---- measurement ---------------------------------------------------------------------------------
+
+// get_prng_ (common/prng.cpp:9-19)
+std::mt19937 get_prng(bool has_seed, int32_t seed)
+{
+    if (has_seed) {
+        return std::mt19937 {static_cast<std::mt19937::result_type>(seed)};
+    }
+    auto device = std::random_device {};
+    return std::mt19937 {device()};
+}
+
+// simulate_measurement_ (measure.hpp:33-57; measure_density_matrix.hpp:33-57): weights on the
+// device, the coin flip with kettle's std::discrete_distribution<int> on the host, collapse on
+// the device with scale sqrt(1 / w).
+int measure(kq_state* s, uint32_t q, bool has_seed, int32_t seed)
+{
+    if (q >= s->n) {
+        throw InvalidArg {"measured qubit out of range"};
+    }
+    double w[2];
+    launch_qubit_weights(s, q, w);
+    auto prng = get_prng(has_seed, seed);
+    auto flip = std::discrete_distribution<int> {{w[0], w[1]}};
+    const int outcome = flip(prng);
+    const double norm = outcome == 0 ? std::sqrt(1.0 / w[0]) : std::sqrt(1.0 / w[1]);
+    launch_collapse(s, q, outcome, norm);
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    return outcome;
+}
+
+// ControlFlowPredicate::operator() (control_flow_predicate.cpp:40-69)
+bool eval_predicate(const int32_t* preds, size_t n_preds, int32_t off, const int8_t* creg, size_t n_bits)
+{
+    if (off < 0 || static_cast<size_t>(off) + 2 > n_preds) {
+        throw InvalidArg {"predicate offset out of range"};
+    }
+    const int kind = preds[off];
+    const int n = preds[off + 1];
+    if (n <= 0 || static_cast<size_t>(off) + 2 + 2 * static_cast<size_t>(n) > n_preds) {
+        throw InvalidArg {"malformed predicate"};
+    }
+    bool out = true;
+    for (int i = 0; i < n; ++i) {
+        const int bit = preds[off + 2 + 2 * i];
+        const int expected = preds[off + 3 + 2 * i];
+        if (bit < 0 || static_cast<size_t>(bit) >= n_bits) {
+            throw InvalidArg {"predicate bit out of range"};
+        }
+        if (creg[bit] < 0) {
+            throw std::runtime_error {"There is no measured bit at classical register " + std::to_string(bit) + "\n"};
+        }
+        if (creg[bit] != expected) {
+            out = false;
+            break;
+        }
+    }
+    return kind == 0 ? out : !out;
+}
+
+std::vector<double> download_host(const kq_state* s)
+{
+    std::vector<double> h(2 * s->n_amps);
+    if (s->precision == KQ_C128) {
+        KQ_HIP_CHECK(hipMemcpy(h.data(), s->amps, s->n_amps * 16, hipMemcpyDeviceToHost));
+    }
+    else {
+        std::vector<float> f(2 * s->n_amps);
+        KQ_HIP_CHECK(hipMemcpy(f.data(), s->amps, s->n_amps * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < f.size(); ++i) {
+            h[i] = f[i];
+        }
+    }
+    return h;
+}
+
+// simulate_loop_body_iterative_ (simulate.cpp:417-521): an explicit stack of op ranges.
+// Consecutive gates of one range are compiled, fused and launched together; the batch is
+// flushed at every measurement, control-flow element and logger.
+void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds, size_t n_preds,
+              bool has_seed, int32_t seed, int8_t* creg, size_t n_bits, kq_logger_fn logger, void* user)
+{
+    if (circuit_n != s->n) {
+        throw std::runtime_error {"Invalid simulation; circuit and state have different number of qubits."};
+    }
+    if (n_bits > 0 && creg == nullptr) {
+        throw InvalidArg {"null classical register"};
+    }
+    for (size_t i = 0; i < n_bits; ++i) {
+        creg[i] = -1;
+    }
+    std::vector<GateRec> batch;
+    const auto flush = [&] {
+        run_records(s, batch);
+        batch.clear();
+    };
+    std::vector<std::pair<size_t, size_t>> stack;
+    stack.emplace_back(0, n_ops);
+    while (!stack.empty()) {
+        auto& top = stack.back();
+        if (top.first >= top.second) {
+            stack.pop_back();
+            continue;
+        }
+        const size_t here = top.first;
+        const kq_op& op = ops[here];
+        const auto body_end = [&](size_t len) {
+            if (here + 1 + len > top.second) {
+                throw InvalidArg {"control-flow body overruns its enclosing range"};
+            }
+        };
+        if (op.gate == KQ_OP_IF) {
+            const auto len = static_cast<size_t>(op.q1);
+            body_end(len);
+            top.first += 1 + len;
+            flush();
+            if (eval_predicate(preds, n_preds, op.q0, creg, n_bits)) {
+                stack.emplace_back(here + 1, here + 1 + len);
+            }
+        }
+        else if (op.gate == KQ_OP_IF_ELSE) {
+            const auto len_if = static_cast<size_t>(op.q1), len_else = static_cast<size_t>(op.aux);
+            body_end(len_if + len_else);
+            top.first += 1 + len_if + len_else;
+            flush();
+            if (eval_predicate(preds, n_preds, op.q0, creg, n_bits)) {
+                stack.emplace_back(here + 1, here + 1 + len_if);
+            }
+            else {
+                stack.emplace_back(here + 1 + len_if, here + 1 + len_if + len_else);
+            }
+        }
+        else if (op.gate == KQ_OP_LOG_STATE || op.gate == KQ_OP_LOG_CREG) {
+            ++top.first;
+            if (s->is_dm) {
+                throw std::runtime_error {"DEV ERROR: loggers haven't been implemented yet for the density matrix simulator.\n"};
+            }
+            flush();
+            if (logger != nullptr) {
+                if (op.gate == KQ_OP_LOG_STATE) {
+                    const auto h = download_host(s);
+                    logger(user, op.gate, h.data(), creg, n_bits);
+                }
+                else {
+                    logger(user, op.gate, nullptr, creg, n_bits);
+                }
+            }
+        }
+        else if (op.gate == KQ_M) {
+            ++top.first;
+            if (op.q1 < 0 || static_cast<size_t>(op.q1) >= n_bits) {
+                throw InvalidArg {"measurement bit out of range"};
+            }
+            flush();
+            creg[op.q1] = static_cast<int8_t>(measure(s, static_cast<uint32_t>(op.q0), has_seed, seed));
+        }
+        else {
+            ++top.first;
+            if (op.gate < 0 || op.gate > KQ_CU) {
+                throw InvalidArg {"unknown op code " + std::to_string(op.gate)};
+            }
+            compile_gate(op, s->n, s->is_dm != 0, batch);
+        }
+    }
+    flush();
+}
+
+// ---- sampling ---------------------------------------------------------------------------------------
+
+// ProbabilitySampler_ (measurements.cpp:288-312) with the CDF on the device.  The uniform draws
+// use kettle's std::uniform_real_distribution(0, last - offset) on std::mt19937 (libstdc++), so
+// the draw sequence is kettle's; lower_bound runs per shot on the device.
+void sample_from_device_probs(hipStream_t st, double* d_p, uint64_t n, uint64_t shots, bool has_seed, int32_t seed,
+                              uint64_t* out)
+{
+    const size_t sb = cdf_scratch_bytes(n);
+    void* sc = device_alloc(sb + shots * (sizeof(double) + sizeof(uint64_t)) + 256);
+    try {
+        double last = 0.0, second_last = 0.0;
+        launch_cdf(st, d_p, n, sc, sb, &last, &second_last);
+        const double offset = (last - second_last) * 1.0e-4;
+        auto prng = get_prng(has_seed, seed);
+        auto dist = std::uniform_real_distribution<double> {0.0, last - offset};
+        std::vector<double> u(shots);
+        for (auto& x : u) {
+            x = dist(prng);
+        }
+        auto* d_u = reinterpret_cast<double*>(static_cast<char*>(sc) + ((sb + 127) / 128) * 128);
+        auto* d_out = reinterpret_cast<uint64_t*>(d_u + shots);
+        if (shots) {
+            KQ_HIP_CHECK(hipMemcpyAsync(d_u, u.data(), shots * sizeof(double), hipMemcpyHostToDevice, st));
+            launch_lower_bound(st, d_p, n, d_u, shots, d_out);
+            KQ_HIP_CHECK(hipMemcpyAsync(out, d_out, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        }
+        KQ_HIP_CHECK(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < shots; ++i) {
+            if (out[i] >= n) {
+                throw std::runtime_error {"LOGIC BUG: Ended up with measurement of state past end of cumulative\nprobability distribution, which shouldn't happen?"};
+            }
+        }
+    }
+    catch (...) {
+        (void) hipFree(sc);
+        throw;
+    }
+    KQ_HIP_CHECK(hipFree(sc));
+}
+
+void check_noise(const double* noise, uint32_t n)
+{
+    if (noise == nullptr) {
+        return;
+    }
+    for (uint32_t q = 0; q < n; ++q) {
+        if (!(0.0 <= noise[q] && noise[q] <= 1.0)) {
+            throw std::runtime_error {"ERROR: Cannot set probability in QuantumNoise object outside of [0, 1]."};
+        }
+    }
+}
+
+// ---- Pauli strings ----------------------------------------------------------------------------------
+
+struct Canon
+{
+    std::complex<double> c {1.0, 0.0};
+    uint64_t x = 0;
+    uint64_t z = 0;
+};
+
+// Product of the string's terms in kettle's application order (simulate_pauli.cpp:66-85):
+// the operator is T_m ... T_1 with T_1 applied first.  Each term is (c, x, z) acting as
+// (P psi)_i = c (-1)^{|i & z|} psi_{i ^ x}; X = (1, m, 0), Z = (1, 0, m), Y = (-i, m, m).
+// Composing Q after P gives c_Q c_P (-1)^{|x_Q & z_P|}, x_Q ^ x_P, z_Q ^ z_P.
+Canon canonical(const uint32_t* q, const uint8_t* k, size_t nt, uint32_t n)
+{
+    Canon p;
+    for (size_t j = 0; j < nt; ++j) {
+        if (k[j] == 0) {
+            continue;
+        }
+        if (k[j] > 3) {
+            throw InvalidArg {"Pauli term kind must be 0..3"};
+        }
+        if (q[j] >= n) {
+            throw InvalidArg {"Pauli term qubit out of range"};
+        }
+        const uint64_t m = uint64_t {1} << q[j];
+        std::complex<double> cq {1.0, 0.0};
+        uint64_t xq = 0, zq = 0;
+        if (k[j] == 1) {
+            xq = m;
+        }
+        else if (k[j] == 2) {
+            xq = m;
+            zq = m;
+            cq = {0.0, -1.0};
+        }
+        else {
+            zq = m;
+        }
+        const double sign = (std::popcount(xq & p.z) & 1) ? -1.0 : 1.0;
+        p.c = cq * p.c * sign;
+        p.x ^= xq;
+        p.z ^= zq;
+    }
+    return p;
+}
+
+const std::complex<double> PHASES[4] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0}};
+
+void build_pauli(uint32_t n, size_t n_strings, const uint32_t* off, const uint32_t* q, const uint8_t* k,
+                 const uint8_t* phases, const double* coeffs, std::vector<PauliGroup>& groups,
+                 std::vector<PauliZ>& strings)
+{
+    if (n_strings > 0 && (off == nullptr || phases == nullptr || coeffs == nullptr)) {
+        throw InvalidArg {"null Pauli operator arrays"};
+    }
+    std::map<uint64_t, std::vector<PauliZ>> by_x;
+    for (size_t s = 0; s < n_strings; ++s) {
+        if (off[s + 1] < off[s]) {
+            throw InvalidArg {"term_offsets must be non-decreasing"};
+        }
+        if (phases[s] > 3) {
+            throw InvalidArg {"Pauli phase must be 0..3"};
+        }
+        const Canon cn = canonical(q + off[s], k + off[s], off[s + 1] - off[s], n);
+        const std::complex<double> w = std::complex<double> {coeffs[2 * s], coeffs[2 * s + 1]} * PHASES[phases[s]] * cn.c;
+        by_x[cn.x].push_back(PauliZ {cn.z, {w.real(), w.imag()}});
+    }
+    groups.clear();
+    strings.clear();
+    for (auto& [x, zs] : by_x) {
+        groups.push_back(PauliGroup {x, static_cast<uint32_t>(strings.size()), static_cast<uint32_t>(zs.size())});
+        strings.insert(strings.end(), zs.begin(), zs.end());
+    }
+}
+
+// ---- adjoint gradient -------------------------------------------------------------------------------
+
+GateRec inverse_of(const GateRec& g)
+{
+    GateRec r = g;
+    if (g.kind == GK_SWAP) {
+        return r;
+    }
+    if (g.kind == GK_DIAG) {
+        r.m[1] = -g.m[1];
+        r.m[7] = -g.m[7];
+        return r;
+    }
+    // conjugate transpose
+    r.m[0] = g.m[0]; r.m[1] = -g.m[1];
+    r.m[2] = g.m[4]; r.m[3] = -g.m[5];
+    r.m[4] = g.m[2]; r.m[5] = -g.m[3];
+    r.m[6] = g.m[6]; r.m[7] = -g.m[7];
+    return r;
+}
+
+// d/dtheta of kettle's angle-gate matrices (as written in gate_matrix of kq_plan.cpp)
+bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d)
+{
+    const double th = op.angle;
+    d = g;
+    d.kind = GK_DENSE;
+    std::fill(std::begin(d.m), std::end(d.m), 0.0);
+    switch (op.gate) {
+        case KQ_RX: case KQ_CRX: {
+            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
+            d.m[0] = -s / 2; d.m[3] = -c / 2; d.m[5] = -c / 2; d.m[6] = -s / 2;
+            return true;
+        }
+        case KQ_RY: case KQ_CRY: {
+            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
+            d.m[0] = -s / 2; d.m[2] = -c / 2; d.m[4] = c / 2; d.m[6] = -s / 2;
+            return true;
+        }
+        case KQ_RZ: case KQ_CRZ: {
+            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
+            d.m[0] = -s / 2; d.m[1] = -c / 2; d.m[6] = -s / 2; d.m[7] = c / 2;
+            return true;
+        }
+        case KQ_P: case KQ_CP: {
+            const double c = std::cos(th), s = std::sin(th);
+            d.m[6] = -s; d.m[7] = c;
+            return true;
+        }
+        default: break;
+    }
+    return false;
+}
+
+void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strings, const uint32_t* off,
+                     const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs, size_t n_params,
+                     double* energy, double* grad, uint32_t every, double drift_tol, size_t* n_restores)
+{
+    if (s->is_dm) {
+        throw InvalidArg {"energy gradient: statevector states only"};
+    }
+    std::vector<GateRec> recs;
+    recs.reserve(n_ops);
+    for (size_t i = 0; i < n_ops; ++i) {
+        if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+            throw InvalidArg {"energy gradient: op " + std::to_string(i) + " is not a unitary gate"};
+        }
+        if (ops[i].aux >= 0 && static_cast<size_t>(ops[i].aux) >= n_params) {
+            throw InvalidArg {"energy gradient: parameter index out of range"};
+        }
+        compile_gate(ops[i], s->n, false, recs);
+    }
+    std::vector<PauliGroup> groups;
+    std::vector<PauliZ> strings;
+    build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
+
+    const size_t G = recs.size();
+    const size_t bytes = s->n_amps * amp_bytes(s->precision);
+    if (every == 0) {
+        every = static_cast<uint32_t>(std::max<double>(1.0, std::ceil(std::sqrt(static_cast<double>(G)))));
+    }
+    // checkpoints at gate boundaries b = every, 2 every, ... (< G), capped at 32 GiB of memory
+    std::vector<size_t> bounds;
+    for (size_t b = every; b < G; b += every) {
+        bounds.push_back(b);
+    }
+    const size_t cap = std::max<size_t>(1, (size_t {32} << 30) / std::max<size_t>(bytes, 1));
+    if (bounds.size() > cap) {
+        bounds.clear();
+        const size_t step = (G + cap) / (cap + 1);
+        for (size_t b = step; b < G && bounds.size() < cap; b += step) {
+            bounds.push_back(b);
+        }
+    }
+    std::vector<void*> ckpt(bounds.size(), nullptr);
+    void* lam = nullptr;
+    void* accs = nullptr;
+    const auto cleanup = [&] {
+        for (auto* p : ckpt) {
+            if (p) {
+                (void) hipFree(p);
+            }
+        }
+        if (lam) {
+            (void) hipFree(lam);
+        }
+        if (accs) {
+            (void) hipFree(accs);
+        }
+    };
+    try {
+        lam = device_alloc(bytes);
+        for (auto& p : ckpt) {
+            p = device_alloc(bytes);
+        }
+        // forward sweep with checkpoints
+        size_t g0 = 0;
+        for (size_t j = 0; j <= bounds.size(); ++j) {
+            const size_t g1 = j < bounds.size() ? bounds[j] : G;
+            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(g0), recs.begin() + static_cast<long>(g1)));
+            if (j < bounds.size()) {
+                launch_copy(s, ckpt[j], s->amps, bytes);
+            }
+            g0 = g1;
+        }
+        // lambda = H psi, E = <psi | lambda>
+        launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
+        double e[2];
+        launch_dot(s, s->amps, lam, e);
+        *energy = e[0];
+        // reverse sweep
+        size_t n_param_gates = 0;
+        for (size_t i = 0; i < n_ops; ++i) {
+            n_param_gates += ops[i].aux >= 0 ? 1 : 0;
+        }
+        accs = device_alloc(std::max<size_t>(n_param_gates, 1) * 2 * sizeof(double));
+        std::vector<size_t> slot_gate;
+        size_t restores = 0;
+        void* psi_backup = nullptr;
+        (void) psi_backup;
+        // the current state buffer becomes psi; keep psi(theta) for the caller afterwards
+        void* psi = device_alloc(bytes);
+        launch_copy(s, psi, s->amps, bytes);
+        try {
+            long long jb = static_cast<long long>(bounds.size()) - 1;
+            for (size_t gi = G; gi-- > 0;) {
+                if (jb >= 0 && bounds[static_cast<size_t>(jb)] == gi + 1) {
+                    const double drift = launch_max_abs_diff(s, psi, ckpt[static_cast<size_t>(jb)]);
+                    if (drift > drift_tol) {
+                        launch_copy(s, psi, ckpt[static_cast<size_t>(jb)], bytes);
+                        ++restores;
+                    }
+                    --jb;
+                }
+                const GateRec inv = inverse_of(recs[gi]);
+                GateRec d;
+                const bool has_d = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], d);
+                double* acc = nullptr;
+                if (has_d) {
+                    acc = static_cast<double*>(accs) + 2 * slot_gate.size();
+                    slot_gate.push_back(gi);
+                }
+                launch_adjoint_gate(s, psi, lam, inv, has_d ? &d : nullptr, acc);
+            }
+            std::vector<double> h(2 * std::max<size_t>(slot_gate.size(), 1));
+            KQ_HIP_CHECK(hipMemcpyAsync(h.data(), accs, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+            std::fill(grad, grad + n_params, 0.0);
+            for (size_t j = 0; j < slot_gate.size(); ++j) {
+                grad[ops[slot_gate[j]].aux] += 2.0 * h[2 * j];
+            }
+        }
+        catch (...) {
+            (void) hipFree(psi);
+            throw;
+        }
+        KQ_HIP_CHECK(hipFree(psi));
+        if (n_restores) {
+            *n_restores = restores;
+        }
+    }
+    catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+PassProfiler& profiler()
+{
+    static PassProfiler p;
+    return p;
+}
+
+void profiler_collect()
+{
+    PassProfiler& p = profiler();
+    for (auto& e : p.pending) {
+        float ms = 0.0f;
+        KQ_HIP_CHECK(hipEventSynchronize(e.b));
+        KQ_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        p.cls[e.cls].ms += ms;
+        p.cls[e.cls].launches += 1;
+        p.cls[e.cls].bytes += e.bytes;
+        p.cls[e.cls].gates += e.gates;
+        (void) hipEventDestroy(e.a);
+        (void) hipEventDestroy(e.b);
+    }
+    p.pending.clear();
+}
+
+void* scratch(kq_state* s, size_t bytes) { return grow(s->scratch, s->scratch_bytes, bytes); }
+
+}  // namespace kq
+
+using namespace kq;
+
+extern "C" {
+
+const char* kq_last_error(void) { return g_last_error.c_str(); }
+
+int kq_init(int device)
+{
+    return guarded([&] {
+        {
+            std::lock_guard<std::mutex> lock {g_init_mutex};
+            g_device = device;
+            g_initialised = false;
+        }
+        ensure_init();
+    });
+}
+
+int kq_device_info(char* name, size_t name_len, int* n_cu, size_t* total_mem)
+{
+    return guarded([&] {
+        ensure_init();
+        hipDeviceProp_t prop;
+        KQ_HIP_CHECK(hipGetDeviceProperties(&prop, g_device));
+        if (name && name_len) {
+            std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        }
+        if (n_cu) {
+            *n_cu = prop.multiProcessorCount;
+        }
+        if (total_mem) {
+            *total_mem = prop.totalGlobalMem;
+        }
+    });
+}
+
+int kq_synchronize(void)
+{
+    return guarded([&] { KQ_HIP_CHECK(hipDeviceSynchronize()); });
+}
+
+int kq_sv_create(kq_state** out, uint32_t n_qubits, int precision)
+{
+    return guarded([&] {
+        if (!out) {
+            throw InvalidArg {"null output pointer"};
+        }
+        *out = create_state(n_qubits, precision, false);
+    });
+}
+
+int kq_dm_create(kq_state** out, uint32_t n_qubits, int precision)
+{
+    return guarded([&] {
+        if (!out) {
+            throw InvalidArg {"null output pointer"};
+        }
+        *out = create_state(n_qubits, precision, true);
+    });
+}
+
+int kq_state_destroy(kq_state* s)
+{
+    return guarded([&] {
+        if (!s) {
+            return;
+        }
+        (void) hipSetDevice(g_device);
+        if (s->stream) {
+            (void) hipStreamSynchronize(s->stream);
+            (void) hipStreamDestroy(s->stream);
+        }
+        if (s->amps) {
+            (void) hipFree(s->amps);
+        }
+        if (s->scratch) {
+            (void) hipFree(s->scratch);
+        }
+        if (s->prog) {
+            (void) hipFree(s->prog);
+        }
+        delete s;
+    });
+}
+
+int kq_state_info(const kq_state* s, uint32_t* n_qubits, int* precision, int* is_density)
+{
+    return guarded([&] {
+        check_state(s);
+        if (n_qubits) { *n_qubits = s->n; }
+        if (precision) { *precision = s->precision; }
+        if (is_density) { *is_density = s->is_dm; }
+    });
+}
+
+int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes)
+{
+    return guarded([&] {
+        check_state(s);
+        if (ptr) { *ptr = s->amps; }
+        if (bytes) { *bytes = s->n_amps * amp_bytes(s->precision); }
+    });
+}
+
+int kq_state_upload(kq_state* s, const double* host)
+{
+    return guarded([&] {
+        check_state(s);
+        if (!host) {
+            throw InvalidArg {"null host buffer"};
+        }
+        ensure_init();
+        if (s->precision == KQ_C128) {
+            KQ_HIP_CHECK(hipMemcpy(s->amps, host, s->n_amps * 16, hipMemcpyHostToDevice));
+        }
+        else {
+            std::vector<float> f(2 * s->n_amps);
+            for (size_t i = 0; i < f.size(); ++i) {
+                f[i] = static_cast<float>(host[i]);
+            }
+            KQ_HIP_CHECK(hipMemcpy(s->amps, f.data(), s->n_amps * 8, hipMemcpyHostToDevice));
+        }
+    });
+}
+
+int kq_state_download(const kq_state* s, double* host)
+{
+    return guarded([&] {
+        check_state(s);
+        if (!host) {
+            throw InvalidArg {"null host buffer"};
+        }
+        ensure_init();
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        const auto h = download_host(s);
+        std::memcpy(host, h.data(), h.size() * sizeof(double));
+    });
+}
+
+int kq_state_set_basis(kq_state* s, uint64_t index)
+{
+    return guarded([&] {
+        check_state(s);
+        const uint64_t N = uint64_t {1} << s->n;
+        if (index >= N) {
+            throw InvalidArg {"basis index out of range"};
+        }
+        ensure_init();
+        launch_set_basis(s, s->is_dm ? index + index * N : index);
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int kq_state_copy(kq_state* dst, const kq_state* src)
+{
+    return guarded([&] {
+        check_state(dst);
+        check_state(src);
+        if (dst->n_amps != src->n_amps || dst->precision != src->precision || dst->is_dm != src->is_dm) {
+            throw InvalidArg {"state copy: shapes differ"};
+        }
+        ensure_init();
+        KQ_HIP_CHECK(hipStreamSynchronize(src->stream));
+        launch_copy_kernel(dst, dst->amps, src->amps, dst->n_amps);
+        KQ_HIP_CHECK(hipStreamSynchronize(dst->stream));
+    });
+}
+
+int kq_simulate(kq_state* s, uint32_t circuit_n_qubits, const kq_op* ops, size_t n_ops, const int32_t* preds,
+                size_t n_preds, int has_seed, int32_t seed, int8_t* creg, size_t n_bits, kq_logger_fn logger,
+                void* logger_user)
+{
+    return guarded([&] {
+        check_state(s);
+        if (n_ops > 0 && ops == nullptr) {
+            throw InvalidArg {"null op array"};
+        }
+        ensure_init();
+        last_stats() = PlanStats {};
+        simulate(s, circuit_n_qubits, ops, n_ops, preds, n_preds, has_seed != 0, seed, creg, n_bits, logger, logger_user);
+    });
+}
+
+int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops)
+{
+    return guarded([&] {
+        check_state(s);
+        if (n_ops > 0 && ops == nullptr) {
+            throw InvalidArg {"null op array"};
+        }
+        ensure_init();
+        last_stats() = PlanStats {};
+        run_ops(s, ops, n_ops);
+    });
+}
+
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass)
+{
+    return guarded([&] {
+        if (max_tile_qubits > 13) {
+            throw InvalidArg {"tile size above 2^13 amplitudes does not fit LDS"};
+        }
+        set_fusion(max_tile_qubits, max_gates_per_pass);
+    });
+}
+
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass)
+{
+    return guarded([&] {
+        if (max_tile_qubits) { *max_tile_qubits = fusion_setting_k(); }
+        if (max_gates_per_pass) { *max_gates_per_pass = fusion_max_gates(); }
+    });
+}
+
+int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes)
+{
+    return guarded([&] {
+        const auto& st = last_stats();
+        if (n_gates) { *n_gates = st.n_gates; }
+        if (n_passes) { *n_passes = st.n_passes; }
+        if (n_tile_passes) { *n_tile_passes = st.n_tile_passes; }
+    });
+}
+
+int kq_qubit_weights(kq_state* s, uint32_t qubit, double out[2])
+{
+    return guarded([&] {
+        check_state(s);
+        if (qubit >= s->n) {
+            throw InvalidArg {"qubit out of range"};
+        }
+        ensure_init();
+        launch_qubit_weights(s, qubit, out);
+    });
+}
+
+int kq_collapse(kq_state* s, uint32_t qubit, int kept_bit, double scale)
+{
+    return guarded([&] {
+        check_state(s);
+        if (qubit >= s->n || (kept_bit != 0 && kept_bit != 1)) {
+            throw InvalidArg {"collapse: bad qubit or bit"};
+        }
+        ensure_init();
+        launch_collapse(s, qubit, kept_bit, scale);
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int kq_measure(kq_state* s, uint32_t qubit, int has_seed, int32_t seed, int* outcome)
+{
+    return guarded([&] {
+        check_state(s);
+        ensure_init();
+        const int o = measure(s, qubit, has_seed != 0, seed);
+        if (outcome) {
+            *outcome = o;
+        }
+    });
+}
+
+int kq_probabilities(kq_state* s, const double* noise, double* out)
+{
+    return guarded([&] {
+        check_state(s);
+        if (!out) {
+            throw InvalidArg {"null output"};
+        }
+        check_noise(noise, s->n);
+        ensure_init();
+        const uint64_t N = uint64_t {1} << s->n;
+        void* d = device_alloc(N * sizeof(double));
+        try {
+            launch_probabilities(s, static_cast<double*>(d), noise);
+            KQ_HIP_CHECK(hipMemcpyAsync(out, d, N * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        }
+        catch (...) {
+            (void) hipFree(d);
+            throw;
+        }
+        KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_sample(kq_state* s, const double* noise, uint64_t n_shots, int has_seed, int32_t seed, uint64_t* out_indices)
+{
+    return guarded([&] {
+        check_state(s);
+        if (n_shots > 0 && !out_indices) {
+            throw InvalidArg {"null output"};
+        }
+        check_noise(noise, s->n);
+        ensure_init();
+        const uint64_t N = uint64_t {1} << s->n;
+        void* d = device_alloc(N * sizeof(double));
+        try {
+            launch_probabilities(s, static_cast<double*>(d), noise);
+            sample_from_device_probs(s->stream, static_cast<double*>(d), N, n_shots, has_seed != 0, seed, out_indices);
+        }
+        catch (...) {
+            (void) hipFree(d);
+            throw;
+        }
+        KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_shots, int has_seed, int32_t seed,
+                            uint64_t* out_indices)
+{
+    return guarded([&] {
+        if (!probs || n_probs == 0) {
+            throw InvalidArg {"empty probability array"};
+        }
+        if (n_shots > 0 && !out_indices) {
+            throw InvalidArg {"null output"};
+        }
+        ensure_init();
+        hipStream_t st;
+        KQ_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        void* d = nullptr;
+        try {
+            d = device_alloc(n_probs * sizeof(double));
+            KQ_HIP_CHECK(hipMemcpyAsync(d, probs, n_probs * sizeof(double), hipMemcpyHostToDevice, st));
+            sample_from_device_probs(st, static_cast<double*>(d), n_probs, n_shots, has_seed != 0, seed, out_indices);
+        }
+        catch (...) {
+            if (d) {
+                (void) hipFree(d);
+            }
+            (void) hipStreamDestroy(st);
+            throw;
+        }
+        KQ_HIP_CHECK(hipFree(d));
+        KQ_HIP_CHECK(hipStreamDestroy(st));
+    });
+}
+
+int kq_pauli_expectation(kq_state* s, size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
+                         const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs, double out[2])
+{
+    return guarded([&] {
+        check_state(s);
+        if (s->is_dm) {
+            throw InvalidArg {"Pauli expectation: statevector states only"};
+        }
+        ensure_init();
+        std::vector<PauliGroup> groups;
+        std::vector<PauliZ> strings;
+        build_pauli(s->n, n_strings, term_offsets, term_qubits, term_kinds, phases, coeffs, groups, strings);
+        if (groups.empty()) {
+            out[0] = 0.0;
+            out[1] = 0.0;
+            return;
+        }
+        launch_pauli_expectation(s, groups, strings, out);
+    });
+}
+
+int kq_apply_pauli_string(kq_state* s, size_t n_terms, const uint32_t* term_qubits, const uint8_t* term_kinds)
+{
+    return guarded([&] {
+        check_state(s);
+        if (s->is_dm) {
+            throw InvalidArg {"Pauli string: statevector states only"};
+        }
+        ensure_init();
+        const Canon cn = canonical(term_qubits, term_kinds, n_terms, s->n);
+        const double w[2] = {cn.c.real(), cn.c.imag()};
+        launch_pauli_apply_one(s, cn.x, cn.z, w);
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int kq_dm_trace(kq_state* s, double out[2])
+{
+    return guarded([&] {
+        check_state(s);
+        if (!s->is_dm) {
+            throw InvalidArg {"trace: density matrices only"};
+        }
+        ensure_init();
+        launch_dm_trace(s, out);
+    });
+}
+
+int kq_dm_partial_trace(kq_state* s, const uint32_t* qubits, size_t n, double* out)
+{
+    return guarded([&] {
+        check_state(s);
+        if (!s->is_dm) {
+            throw InvalidArg {"partial trace: density matrices only"};
+        }
+        std::vector<uint32_t> qs(qubits, qubits + n);
+        for (auto x : qs) {
+            if (x >= s->n) {
+                throw std::runtime_error {"ERROR: Cannot take partial trace; found qubit index out of bounds\n"};
+            }
+        }
+        std::sort(qs.begin(), qs.end(), [](auto a, auto b) { return a > b; });
+        if (std::adjacent_find(qs.begin(), qs.end()) != qs.end()) {
+            throw InvalidArg {"partial trace: repeated qubit"};
+        }
+        ensure_init();
+        launch_dm_partial_trace(s, qs, out);
+    });
+}
+
+int kq_dm_from_statevector(kq_state* dm, const kq_state* sv)
+{
+    return guarded([&] {
+        check_state(dm);
+        check_state(sv);
+        if (!dm->is_dm || sv->is_dm || dm->n != sv->n || dm->precision != sv->precision) {
+            throw InvalidArg {"dm_from_statevector: shapes differ"};
+        }
+        ensure_init();
+        KQ_HIP_CHECK(hipStreamSynchronize(sv->stream));
+        launch_dm_from_sv(dm, sv);
+        KQ_HIP_CHECK(hipStreamSynchronize(dm->stream));
+    });
+}
+
+int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops, size_t n_strings, const uint32_t* term_offsets,
+                       const uint32_t* term_qubits, const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs,
+                       size_t n_params, double* energy, double* grad, uint32_t checkpoint_every, double drift_tol,
+                       size_t* n_restores)
+{
+    return guarded([&] {
+        check_state(state);
+        if (!energy || (n_params > 0 && !grad)) {
+            throw InvalidArg {"null output"};
+        }
+        ensure_init();
+        last_stats() = PlanStats {};
+        energy_gradient(state, ops, n_ops, n_strings, term_offsets, term_qubits, term_kinds, phases, coeffs, n_params,
+                        energy, grad, checkpoint_every, drift_tol, n_restores);
+    });
+}
+
+int kq_profile_begin(void)
+{
+    return guarded([&] {
+        PassProfiler& p = profiler();
+        profiler_collect();
+        p = PassProfiler {};
+        p.enabled = true;
+    });
+}
+
+int kq_profile_end(kq_pass_stats out[3])
+{
+    return guarded([&] {
+        PassProfiler& p = profiler();
+        profiler_collect();
+        if (out) {
+            for (int i = 0; i < 3; ++i) {
+                out[i] = p.cls[i];
+            }
+        }
+        p.enabled = false;
+    });
+}
+
+int kq_debug_pair_map_1q(uint32_t n_qubits, uint32_t target, uint64_t* s0, uint64_t* s1)
+{
+    return guarded([&] {
+        if (n_qubits == 0 || target >= n_qubits || n_qubits > 30) {
+            throw InvalidArg {"pair map: bad sizes"};
+        }
+        ensure_init();
+        const uint64_t np = uint64_t {1} << (n_qubits - 1);
+        void* d = device_alloc(2 * np * sizeof(uint64_t));
+        auto* d0 = static_cast<uint64_t*>(d);
+        launch_pair_map_1q(n_qubits, target, d0, d0 + np);
+        KQ_HIP_CHECK(hipDeviceSynchronize());
+        KQ_HIP_CHECK(hipMemcpy(s0, d0, np * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        KQ_HIP_CHECK(hipMemcpy(s1, d0 + np, np * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, int four_way, uint64_t* out)
+{
+    return guarded([&] {
+        if (n_qubits < 2 || control >= n_qubits || target >= n_qubits || control == target || n_qubits > 30) {
+            throw InvalidArg {"pair map: bad sizes"};
+        }
+        ensure_init();
+        const uint64_t np = uint64_t {1} << (n_qubits - 2);
+        const uint64_t words = np * (four_way ? 4 : 2);
+        void* d = device_alloc(words * sizeof(uint64_t));
+        launch_pair_map_2q(n_qubits, control, target, four_way, static_cast<uint64_t*>(d));
+        KQ_HIP_CHECK(hipDeviceSynchronize());
+        KQ_HIP_CHECK(hipMemcpy(out, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+}  // extern "C"

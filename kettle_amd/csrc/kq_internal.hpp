@@ -1,0 +1,199 @@
+// Internal declarations shared by the libkq translation units (host side + kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/kq.h"
+
+namespace kq
+{
+
+// ---- errors -------------------------------------------------------------------------------
+// Every C entry point runs its body through guarded(); these exception types map to status codes.
+struct InvalidArg : std::runtime_error { using std::runtime_error::runtime_error; };
+struct HipError : std::runtime_error { using std::runtime_error::runtime_error; };
+struct AllocError : std::runtime_error { using std::runtime_error::runtime_error; };
+// kettle-level errors keep kettle's exception type: std::runtime_error / std::out_of_range
+
+void set_error(const std::string& msg);
+
+#define KQ_HIP_CHECK(expr)                                                                    \
+    do {                                                                                      \
+        hipError_t kq_e_ = (expr);                                                            \
+        if (kq_e_ != hipSuccess) {                                                            \
+            throw ::kq::HipError {std::string {#expr} + ": " + hipGetErrorString(kq_e_)};     \
+        }                                                                                     \
+    } while (0)
+
+// ---- device gate records --------------------------------------------------------------------
+// A compiled gate acting on one target bit of the state vector (a density matrix is a vector of
+// 2n qubits: each kettle gate becomes two records, U on bit t and conj(U) on bit n + t).
+enum GateKind : uint32_t
+{
+    GK_DENSE = 0,   // (a, b) -> (m00 a + m01 b, m10 a + m11 b)
+    GK_DIAG = 1,    // a -> d0 a, b -> d1 b          (m[0..1] = d0, m[6..7] = d1)
+    GK_SWAP = 2,    // (a, b) -> (b, a)
+};
+
+struct GateRec
+{
+    uint32_t kind;
+    uint32_t target;
+    int32_t control;    // -1: no control
+    uint32_t pad;
+    double m[8];        // m00, m01, m10, m11 as (re, im)
+};
+
+// One fused pass: the tile qubit set and the gates applied inside the tile.  Tile gate records
+// carry local positions: target = lt | (gt << 8), pad = lc | (gc << 8), where lt / lc are the
+// local bits of target / control (0xFF when outside the tile) and gt / gc the global qubits.
+struct TilePass
+{
+    int k = 0;                   // tile qubits
+    int c = 0;                   // low contiguous qubits 0..c-1 in the tile
+    std::vector<int> qubits;     // qubits[j] = global qubit of local bit j (ascending)
+    std::vector<GateRec> gates;  // tile-encoded records
+    bool sweep = false;          // single gate: run the plain sweep kernel on `single`
+    GateRec single {};
+};
+
+struct PlanStats
+{
+    size_t n_gates = 0;
+    size_t n_passes = 0;
+    size_t n_tile_passes = 0;
+};
+
+// ---- states ---------------------------------------------------------------------------------
+}  // namespace kq
+
+struct kq_state
+{
+    uint32_t n = 0;          // qubits of the simulated system
+    uint32_t nvec = 0;       // qubits of the stored vector (n, or 2n for a density matrix)
+    int precision = KQ_C128;
+    int is_dm = 0;
+    uint64_t n_amps = 0;     // 2^nvec
+    void* amps = nullptr;    // device buffer of n_amps complex values
+    hipStream_t stream = nullptr;
+    // per-state device scratch (reductions, CDFs), grown on demand
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // per-state device buffer for uploaded gate programs
+    void* prog = nullptr;
+    size_t prog_bytes = 0;
+};
+
+namespace kq
+{
+
+inline size_t amp_bytes(int precision) { return precision == KQ_C64 ? 8 : 16; }
+
+void* scratch(kq_state* s, size_t bytes);
+
+// ---- pass profiling (HIP events on the state's stream) ------------------------------------------
+struct PassProfiler
+{
+    bool enabled = false;
+    kq_pass_stats cls[3] {};
+    // events recorded but not yet read: (class, start, stop, bytes, gates)
+    struct Pending { int cls; hipEvent_t a; hipEvent_t b; double bytes; uint64_t gates; };
+    std::vector<Pending> pending;
+};
+PassProfiler& profiler();
+// bracket one launch; `launch` is invoked between the two events
+template <typename F>
+void profiled(kq_state* s, int cls, double bytes, uint64_t gates, F&& launch)
+{
+    PassProfiler& p = profiler();
+    if (!p.enabled) {
+        launch();
+        return;
+    }
+    hipEvent_t a, b;
+    KQ_HIP_CHECK(hipEventCreate(&a));
+    KQ_HIP_CHECK(hipEventCreate(&b));
+    KQ_HIP_CHECK(hipEventRecord(a, s->stream));
+    launch();
+    KQ_HIP_CHECK(hipEventRecord(b, s->stream));
+    p.pending.push_back({cls, a, b, bytes, gates});
+}
+void profiler_collect();   // read and release pending events (after a stream sync)
+
+// ---- gate compilation and planning (kq_plan.cpp) --------------------------------------------
+// kettle gate (kq_op) -> one (statevector) or two (density matrix) device records
+void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>& out);
+bool is_diag(const GateRec& g);
+// Plan fused tile passes over a straight-line list of records.
+std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
+                                  int low_c, int max_gates);
+PlanStats& last_stats();
+int fusion_max_k(int precision, uint32_t nvec);
+// LDS bytes one workgroup may use on the current device (queried at initialisation)
+size_t& device_lds_per_block();
+constexpr size_t TILE_STATIC_LDS = 8192;   // the tile kernel's high-offset table
+int fusion_max_gates();
+void set_fusion(int max_k, int max_gates);
+int fusion_setting_k();
+
+// ---- kernel launchers (kq_kernels.hip) --------------------------------------------------------
+void launch_gate_sweep(kq_state* s, const GateRec& g);
+void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates);
+void launch_set_basis(kq_state* s, uint64_t index);
+// weights: SV -> (sum |a|^2 over bit q = 0, = 1); DM -> block Frobenius norms (kettle's quirk)
+void launch_qubit_weights(kq_state* s, uint32_t q, double out[2]);
+void launch_collapse(kq_state* s, uint32_t q, int kept, double scale);
+// probabilities into a device array of N doubles (|a|^2 or Re diag rho); noise per qubit
+void launch_probabilities(kq_state* s, double* d_probs, const double* noise_host);
+void launch_noise(hipStream_t st, double* d_probs, uint32_t n, uint32_t q, double noise);
+// inclusive scan of d_probs in place; returns last value and the largest value < last
+void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, size_t scratch_bytes,
+                double* last, double* second_last);
+void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots,
+                        uint64_t* d_out);
+size_t cdf_scratch_bytes(uint64_t n);
+
+struct PauliGroup
+{
+    uint64_t x;          // flip mask
+    uint32_t first;      // first string of the group in the string table
+    uint32_t count;
+};
+struct PauliZ
+{
+    uint64_t z;          // sign mask
+    double w[2];         // total complex weight (coefficient * phase * canonical factor)
+};
+// sum_i conj(a_i) * sum_g (sum_{s in g} w_s (-1)^{|i & z_s|}) a_{i ^ x_g}
+void launch_pauli_expectation(kq_state* s, const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings,
+                              double out[2]);
+// b_i = sum_g (sum_{s in g} w_s (-1)^{|i & z_s|}) a_{i ^ x_g}  (b = H a), same precision as a
+void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const std::vector<PauliGroup>& groups,
+                            const std::vector<PauliZ>& strings);
+// in-place (c, x, z) operator: a_i <- w (-1)^{|i & z|} a_{i ^ x}
+void launch_pauli_apply_one(kq_state* s, uint64_t x, uint64_t z, const double w[2]);
+
+void launch_dm_trace(kq_state* s, double out[2]);
+void launch_dm_partial_trace(kq_state* s, const std::vector<uint32_t>& traced, double* out_host);
+void launch_dm_from_sv(kq_state* dm, const kq_state* sv);
+
+void launch_pair_map_1q(uint32_t n, uint32_t t, uint64_t* d_s0, uint64_t* d_s1);
+void launch_pair_map_2q(uint32_t n, uint32_t c, uint32_t t, int four_way, uint64_t* d_out);
+
+// adjoint sweep helpers (kq_adjoint.cpp uses these)
+// One gate on two vectors: psi <- G^dag psi, then <lam| dG psi> accumulated (if dG != null),
+// then lam <- G^dag lam.  Result written to d_partial[slot] as (re, im) after reduction.
+void launch_adjoint_gate(kq_state* s, void* psi, void* lam, const GateRec& inv, const GateRec* deriv,
+                         double* d_acc);
+void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
+void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
+double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
+void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
+
+}  // namespace kq

@@ -1,0 +1,1334 @@
+// Device kernels of libkq for gfx950 (CDNA4, wave64) and their host-side launchers.
+//
+// Storage: a state is 2^nvec complex amplitudes, interleaved (re, im), FP64 (KQ_C128) or FP32
+// (KQ_C64).  Qubit q is bit q of the amplitude index (kettle's little-endian convention,
+// state/qubit_state_conversion.cpp:37-83).  A density matrix is stored column-major as a
+// 2n-qubit vector: bit q is row qubit q, bit n+q is column qubit q.
+//
+// All reductions (norms, probabilities, expectations, gradients) accumulate in FP64 whatever the
+// amplitude precision, and are deterministic: fixed per-block partials, then one fixed-order
+// final reduction (no floating-point atomics).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+namespace
+{
+
+constexpr int BLOCK = 256;
+constexpr int MAX_TILE_Q = 16;
+constexpr int MAX_FREE_Q = 48;
+constexpr int RED_BLOCKS = 2048;   // upper bound on partial-sum blocks of a reduction
+
+template <typename R> struct AmpT;
+template <> struct AmpT<double> { using T = double2; };
+template <> struct AmpT<float> { using T = float2; };
+template <typename R> using Amp = typename AmpT<R>::T;
+
+struct DevGate
+{
+    uint32_t kind;
+    uint32_t target;
+    int32_t control;
+    uint32_t pad;
+    double m[8];
+};
+
+// ---- small device helpers -----------------------------------------------------------------
+
+__device__ __forceinline__ uint64_t insert_zero(uint64_t p, uint32_t j)
+{
+    const uint64_t low = p & ((uint64_t {1} << j) - 1);
+    return ((p >> j) << (j + 1)) | low;
+}
+
+// insert zeros at two distinct positions lo < hi
+__device__ __forceinline__ uint64_t insert_zero2(uint64_t p, uint32_t lo, uint32_t hi)
+{
+    return insert_zero(insert_zero(p, lo), hi);
+}
+
+template <typename R>
+struct Cx
+{
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
+{
+    return {a.re * br - a.im * bi, a.re * bi + a.im * br};
+}
+
+template <typename R>
+__device__ __forceinline__ void apply_dense(Cx<R>& a, Cx<R>& b, const R* m)
+{
+    const Cx<R> na {a.re * m[0] - a.im * m[1] + b.re * m[2] - b.im * m[3],
+                    a.re * m[1] + a.im * m[0] + b.re * m[3] + b.im * m[2]};
+    const Cx<R> nb {a.re * m[4] - a.im * m[5] + b.re * m[6] - b.im * m[7],
+                    a.re * m[5] + a.im * m[4] + b.re * m[7] + b.im * m[6]};
+    a = na;
+    b = nb;
+}
+
+template <typename R>
+__device__ __forceinline__ Cx<R> ld(const Amp<R>* p, uint64_t i)
+{
+    const Amp<R> v = p[i];
+    return {v.x, v.y};
+}
+
+template <typename R>
+__device__ __forceinline__ void st(Amp<R>* p, uint64_t i, Cx<R> v)
+{
+    Amp<R> w;
+    w.x = v.re;
+    w.y = v.im;
+    p[i] = w;
+}
+
+// block-wide sum of a double2 (wave64 shuffles, then LDS across waves); result on thread 0
+__device__ double2 block_sum2(double a, double b)
+{
+    __shared__ double2 red[BLOCK / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[wave] = make_double2(a, b);
+    }
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            r.x += red[w].x;
+            r.y += red[w].y;
+        }
+    }
+    __syncthreads();
+    return r;
+}
+
+// final fixed-order reduction of block partials: one block, each thread a strided slice
+__global__ void k_reduce_partials(const double2* part, int n, double2* out)
+{
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        a += part[i].x;
+        b += part[i].y;
+    }
+    const double2 r = block_sum2(a, b);
+    if (threadIdx.x == 0) {
+        *out = r;
+    }
+}
+
+// ---- gate sweep: one gate over the whole vector ---------------------------------------------
+// Enumeration is in coalesced order (low bits fastest): item p -> the amplitude index with the
+// fixed bits (target 0, control 1) inserted.  The set of pairs is exactly the set kettle's
+// generators yield (gate_pair_generator.hpp), so per-pair results are identical.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_gate_sweep(Amp<R>* a, uint64_t n_items, DevGate g)
+{
+    R m[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        m[i] = static_cast<R>(g.m[i]);
+    }
+    const uint32_t t = g.target;
+    const bool ctrl = g.control >= 0;
+    const uint32_t c = ctrl ? static_cast<uint32_t>(g.control) : 0;
+    const uint32_t lo = ctrl ? min(c, t) : t;
+    const uint32_t hi = ctrl ? max(c, t) : t;
+    const uint64_t tbit = uint64_t {1} << t;
+    const uint64_t cbit = ctrl ? (uint64_t {1} << c) : 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    if (g.kind == GK_DIAG) {
+        // only amplitudes with control = 1 change; with d0 == 1 only those with target = 1.
+        // The host decides (bit 0 of pad) from the FP64 coefficient and sizes n_items to match.
+        const bool skip0 = (g.pad & 1u) != 0;
+        for (uint64_t p = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; p < n_items; p += stride) {
+            uint64_t i;
+            if (ctrl && skip0) {
+                i = insert_zero2(p, lo, hi) | cbit | tbit;
+            }
+            else if (ctrl) {
+                i = insert_zero(p, c) | cbit;
+            }
+            else if (skip0) {
+                i = insert_zero(p, t) | tbit;
+            }
+            else {
+                i = p;
+            }
+            Cx<R> v = ld<R>(a, i);
+            if (i & tbit) {
+                v = cmul(v, m[6], m[7]);
+            }
+            else {
+                v = cmul(v, m[0], m[1]);
+            }
+            st<R>(a, i, v);
+        }
+        return;
+    }
+    for (uint64_t p = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; p < n_items; p += stride) {
+        const uint64_t i0 = (ctrl ? insert_zero2(p, lo, hi) : insert_zero(p, t)) | cbit;
+        const uint64_t i1 = i0 | tbit;
+        Cx<R> x = ld<R>(a, i0);
+        Cx<R> y = ld<R>(a, i1);
+        if (g.kind == GK_SWAP) {
+            st<R>(a, i0, y);
+            st<R>(a, i1, x);
+        }
+        else {
+            apply_dense(x, y, m);
+            st<R>(a, i0, x);
+            st<R>(a, i1, y);
+        }
+    }
+}
+
+// ---- fused tile pass ----------------------------------------------------------------------------
+// A tile is the 2^k amplitudes that share the values of all qubits outside the tile set Q.
+// Q always holds the low qubits 0..c-1, so a tile is 2^(k-c) contiguous runs of 2^c amplitudes
+// (coalesced HBM traffic); the other k-c tile qubits may be any higher qubits, which is how
+// high-index gates are fused.  The tile lives in LDS while the pass's gates are applied; each
+// gate costs LDS traffic only, the pass costs one HBM read and one HBM write of the state.
+struct TileDesc
+{
+    int k;
+    int c;
+    int nfree;
+    int pad;
+    uint8_t qpos[MAX_TILE_Q];     // global qubit of local bit j
+    uint8_t freepos[MAX_FREE_Q];  // global positions of the non-tile qubits, ascending
+};
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_tile_pass(Amp<R>* a, TileDesc d, const DevGate* gates, int n_gates)
+{
+    extern __shared__ unsigned char smem_raw[];
+    Amp<R>* tile = reinterpret_cast<Amp<R>*>(smem_raw);
+    __shared__ uint64_t hoff[1 << 10];
+
+    const int k = d.k;
+    const int c = d.c;
+    const uint32_t tsize = 1u << k;
+    const uint32_t lowmask = (1u << c) - 1;
+    const int nh = k - c;
+
+    // base index: block id deposited into the free (non-tile) bit positions
+    uint64_t base = 0;
+    {
+        uint64_t b = blockIdx.x;
+        for (int j = 0; j < d.nfree; ++j) {
+            base |= ((b >> j) & 1ull) << d.freepos[j];
+        }
+    }
+    for (uint32_t h = threadIdx.x; h < (1u << nh); h += BLOCK) {
+        uint64_t off = 0;
+        for (int j = 0; j < nh; ++j) {
+            off |= static_cast<uint64_t>((h >> j) & 1u) << d.qpos[c + j];
+        }
+        hoff[h] = off;
+    }
+    __syncthreads();
+
+    for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
+        tile[l] = a[base | (l & lowmask) | hoff[l >> c]];
+    }
+
+    bool need_sync = true;
+    for (int gi = 0; gi < n_gates; ++gi) {
+        const DevGate& g = gates[gi];
+        // local positions are encoded in the record: target = local bit or 0xFF.. for outside
+        const int lt = static_cast<int>(g.target & 0xFFu);
+        const int gt = static_cast<int>((g.target >> 8) & 0xFFu);
+        const int lc = static_cast<int>(g.pad & 0xFFu);
+        const int gc = static_cast<int>((g.pad >> 8) & 0xFFu);
+        const bool has_c = g.control >= 0;
+        R m[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            m[i] = static_cast<R>(g.m[i]);
+        }
+        // a control outside the tile is uniform over the tile
+        if (has_c && lc == 0xFF && !((base >> gc) & 1ull)) {
+            continue;
+        }
+        if (g.kind == GK_DIAG) {
+            if (need_sync) {
+                __syncthreads();
+                need_sync = false;
+            }
+            for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
+                if (has_c && lc != 0xFF && !((l >> lc) & 1u)) {
+                    continue;
+                }
+                const uint32_t bit = lt != 0xFF ? ((l >> lt) & 1u) : static_cast<uint32_t>((base >> gt) & 1ull);
+                const Amp<R> v = tile[l];
+                Cx<R> x {v.x, v.y};
+                x = bit ? cmul(x, m[6], m[7]) : cmul(x, m[0], m[1]);
+                Amp<R> w;
+                w.x = x.re;
+                w.y = x.im;
+                tile[l] = w;
+            }
+            continue;
+        }
+        __syncthreads();
+        need_sync = true;
+        const uint32_t tb = 1u << lt;
+        if (has_c && lc != 0xFF) {
+            const uint32_t lo = min(lt, lc), hi = max(lt, lc);
+            const uint32_t cb = 1u << lc;
+            for (uint32_t p = threadIdx.x; p < (tsize >> 2); p += BLOCK) {
+                const uint32_t l0 = static_cast<uint32_t>(insert_zero2(p, lo, hi)) | cb;
+                const uint32_t l1 = l0 | tb;
+                const Amp<R> u0 = tile[l0], u1 = tile[l1];
+                if (g.kind == GK_SWAP) {
+                    tile[l0] = u1;
+                    tile[l1] = u0;
+                }
+                else {
+                    Cx<R> x {u0.x, u0.y}, y {u1.x, u1.y};
+                    apply_dense(x, y, m);
+                    Amp<R> w0, w1;
+                    w0.x = x.re; w0.y = x.im; w1.x = y.re; w1.y = y.im;
+                    tile[l0] = w0;
+                    tile[l1] = w1;
+                }
+            }
+        }
+        else {
+            for (uint32_t p = threadIdx.x; p < (tsize >> 1); p += BLOCK) {
+                const uint32_t l0 = static_cast<uint32_t>(insert_zero(p, lt));
+                const uint32_t l1 = l0 | tb;
+                const Amp<R> u0 = tile[l0], u1 = tile[l1];
+                if (g.kind == GK_SWAP) {
+                    tile[l0] = u1;
+                    tile[l1] = u0;
+                }
+                else {
+                    Cx<R> x {u0.x, u0.y}, y {u1.x, u1.y};
+                    apply_dense(x, y, m);
+                    Amp<R> w0, w1;
+                    w0.x = x.re; w0.y = x.im; w1.x = y.re; w1.y = y.im;
+                    tile[l0] = w0;
+                    tile[l1] = w1;
+                }
+            }
+        }
+    }
+    if (need_sync) {
+        __syncthreads();
+    }
+    for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
+        a[base | (l & lowmask) | hoff[l >> c]] = tile[l];
+    }
+}
+
+// ---- misc state kernels -----------------------------------------------------------------------
+template <typename R>
+__global__ void k_set_basis(Amp<R>* a, uint64_t n, uint64_t index)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        Amp<R> v;
+        v.x = i == index ? R(1) : R(0);
+        v.y = R(0);
+        a[i] = v;
+    }
+}
+
+// SV (dm_n == 0): w_b = sum |a_i|^2 over bit q = b.  DM (dm_n = n): w_b = sum |x|^2 over vec
+// indices with row bit q = col bit q = b (measure_density_matrix.cpp:21-51).
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_qubit_weights(const Amp<R>* a, uint64_t n_amps, uint32_t q, uint32_t dm_n,
+                                                         double2* part)
+{
+    double w0 = 0.0, w1 = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n_amps; i += stride) {
+        const Amp<R> v = a[i];
+        const double p = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y;
+        const uint32_t br = (i >> q) & 1u;
+        if (dm_n) {
+            const uint32_t bc = (i >> (q + dm_n)) & 1u;
+            if (br != bc) {
+                continue;
+            }
+        }
+        if (br) {
+            w1 += p;
+        }
+        else {
+            w0 += p;
+        }
+    }
+    const double2 r = block_sum2(w0, w1);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+    }
+}
+
+template <typename R>
+__global__ void k_collapse(Amp<R>* a, uint64_t n_amps, uint32_t q, uint32_t dm_n, uint32_t kept, double scale)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n_amps; i += stride) {
+        bool keep = ((i >> q) & 1u) == kept;
+        if (dm_n) {
+            keep = keep && (((i >> (q + dm_n)) & 1u) == kept);
+        }
+        Amp<R> v = a[i];
+        if (keep) {
+            v.x = static_cast<R>(v.x * scale);
+            v.y = static_cast<R>(v.y * scale);
+        }
+        else {
+            v.x = R(0);
+            v.y = R(0);
+        }
+        a[i] = v;
+    }
+}
+
+// probabilities: SV |a_i|^2 (std::norm); DM Re rho(i, i)
+template <typename R>
+__global__ void k_probabilities(const Amp<R>* a, uint64_t n_out, uint32_t dm_n, double* p)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n_out; i += stride) {
+        if (dm_n) {
+            p[i] = static_cast<double>(a[i + i * n_out].x);
+        }
+        else {
+            const Amp<R> v = a[i];
+            const double re = v.x, im = v.y;
+            p[i] = re * re + im * im;
+        }
+    }
+}
+
+// apply_noise_ (probabilities.cpp:110-124) for one qubit: pairs are independent
+__global__ void k_noise(double* p, uint64_t n_pairs, uint32_t q, double noise)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t k = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; k < n_pairs; k += stride) {
+        const uint64_t i0 = insert_zero(k, q);
+        const uint64_t i1 = i0 | (uint64_t {1} << q);
+        const double p0 = p[i0], p1 = p[i1];
+        p[i0] = ((1.0 - noise) * p0) + (noise * p1);
+        p[i1] = ((1.0 - noise) * p1) + (noise * p0);
+    }
+}
+
+// ---- inclusive scan (CDF) -----------------------------------------------------------------------
+constexpr int SCAN_PER_THREAD = 8;
+constexpr uint64_t SCAN_CHUNK = uint64_t {BLOCK} * SCAN_PER_THREAD;
+
+__device__ double block_exclusive_scan(double v, double* total)
+{
+    __shared__ double wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    double incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const double o = __shfl_up(incl, off, 64);
+        if (lane >= off) {
+            incl += o;
+        }
+    }
+    if (lane == 63) {
+        wsum[wave] = incl;
+    }
+    __syncthreads();
+    double wave_off = 0.0, tot = 0.0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        if (w < wave) {
+            wave_off += wsum[w];
+        }
+        tot += wsum[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_off + incl - v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_block_sums(const double* p, uint64_t n, double* sums)
+{
+    const uint64_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * uint64_t {SCAN_PER_THREAD};
+    double s = 0.0;
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        if (base + j < n) {
+            s += p[base + j];
+        }
+    }
+    double total;
+    block_exclusive_scan(s, &total);
+    if (threadIdx.x == 0) {
+        sums[blockIdx.x] = total;
+    }
+}
+
+// exclusive scan of the block sums in one block (each thread a contiguous slice)
+__global__ __launch_bounds__(BLOCK) void k_scan_sums(double* sums, uint64_t nb)
+{
+    const uint64_t per = (nb + BLOCK - 1) / BLOCK;
+    const uint64_t b0 = threadIdx.x * per;
+    double s = 0.0;
+    for (uint64_t j = 0; j < per; ++j) {
+        if (b0 + j < nb) {
+            s += sums[b0 + j];
+        }
+    }
+    double total;
+    double run = block_exclusive_scan(s, &total);
+    for (uint64_t j = 0; j < per; ++j) {
+        if (b0 + j < nb) {
+            const double v = sums[b0 + j];
+            sums[b0 + j] = run;
+            run += v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(double* p, uint64_t n, const double* offs)
+{
+    const uint64_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * uint64_t {SCAN_PER_THREAD};
+    double v[SCAN_PER_THREAD];
+    double s = 0.0;
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        v[j] = base + j < n ? p[base + j] : 0.0;
+        s += v[j];
+    }
+    double total;
+    double run = block_exclusive_scan(s, &total) + offs[blockIdx.x];
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        run += v[j];
+        if (base + j < n) {
+            p[base + j] = run;
+        }
+    }
+}
+
+// largest index j with cdf[j] < last (cumulative_end_offset_, measurements.cpp:235-256)
+__global__ __launch_bounds__(BLOCK) void k_last_below(const double* cdf, uint64_t n, double last, long long* out)
+{
+    __shared__ long long best[BLOCK];
+    long long b = -1;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i + 1 < n; i += stride) {
+        if (cdf[i] < last) {
+            b = static_cast<long long>(i);
+        }
+    }
+    best[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            best[threadIdx.x] = max(best[threadIdx.x], best[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicMax(out, best[0]);
+    }
+}
+
+// std::ranges::lower_bound per shot (measurements.cpp:297-312)
+__global__ void k_lower_bound(const double* cdf, uint64_t n, const double* u, uint64_t shots, uint64_t* out)
+{
+    const uint64_t s = blockIdx.x * uint64_t {BLOCK} + threadIdx.x;
+    if (s >= shots) {
+        return;
+    }
+    const double x = u[s];
+    uint64_t lo = 0, len = n;
+    while (len > 0) {
+        const uint64_t half = len >> 1;
+        if (cdf[lo + half] < x) {
+            lo = lo + half + 1;
+            len = len - half - 1;
+        }
+        else {
+            len = half;
+        }
+    }
+    out[s] = lo;
+}
+
+// ---- Pauli strings ------------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void pauli_coef(uint64_t i, const PauliGroup& g, const PauliZ* zs, double& cr, double& ci)
+{
+    cr = 0.0;
+    ci = 0.0;
+    for (uint32_t s = 0; s < g.count; ++s) {
+        const PauliZ z = zs[g.first + s];
+        const double sg = (__popcll(i & z.z) & 1) ? -1.0 : 1.0;
+        cr += sg * z.w[0];
+        ci += sg * z.w[1];
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_pauli_expect(const Amp<R>* a, uint64_t n, const PauliGroup* groups, int ng,
+                                                        const PauliZ* zs, double2* part)
+{
+    double er = 0.0, ei = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        const Amp<R> v = a[i];
+        const double ar = v.x, ai = v.y;
+        double hr = 0.0, hi = 0.0;
+        for (int gi = 0; gi < ng; ++gi) {
+            const PauliGroup g = groups[gi];
+            double cr, ci;
+            pauli_coef<R>(i, g, zs, cr, ci);
+            const Amp<R> u = a[i ^ g.x];
+            const double br = u.x, bi = u.y;
+            hr += cr * br - ci * bi;
+            hi += cr * bi + ci * br;
+        }
+        // conj(a) * (H a)_i
+        er += ar * hr + ai * hi;
+        ei += ar * hi - ai * hr;
+    }
+    const double2 r = block_sum2(er, ei);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_pauli_apply_sum(const Amp<R>* a, Amp<R>* b, uint64_t n, const PauliGroup* groups,
+                                                           int ng, const PauliZ* zs)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        double hr = 0.0, hi = 0.0;
+        for (int gi = 0; gi < ng; ++gi) {
+            const PauliGroup g = groups[gi];
+            double cr, ci;
+            pauli_coef<R>(i, g, zs, cr, ci);
+            const Amp<R> u = a[i ^ g.x];
+            const double br = u.x, bi = u.y;
+            hr += cr * br - ci * bi;
+            hi += cr * bi + ci * br;
+        }
+        Amp<R> w;
+        w.x = static_cast<R>(hr);
+        w.y = static_cast<R>(hi);
+        b[i] = w;
+    }
+}
+
+// in place a_i <- w (-1)^{|i & z|} a_{i ^ x}; for x != 0 each pair (i, i ^ x) is handled by the
+// index whose highest bit of x is 0
+template <typename R>
+__global__ void k_pauli_apply_one(Amp<R>* a, uint64_t n, uint64_t x, uint64_t z, double wr, double wi)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    if (x == 0) {
+        for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+            const double sg = (__popcll(i & z) & 1) ? -1.0 : 1.0;
+            const Amp<R> v = a[i];
+            const double vr = v.x, vi = v.y;
+            Amp<R> w;
+            w.x = static_cast<R>(sg * (wr * vr - wi * vi));
+            w.y = static_cast<R>(sg * (wr * vi + wi * vr));
+            a[i] = w;
+        }
+        return;
+    }
+    const uint32_t h = 63 - __clzll(x);
+    for (uint64_t p = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; p < n / 2; p += stride) {
+        const uint64_t i = insert_zero(p, h);
+        const uint64_t j = i ^ x;
+        const Amp<R> vi_ = a[i], vj_ = a[j];
+        const double si = (__popcll(i & z) & 1) ? -1.0 : 1.0;
+        const double sj = (__popcll(j & z) & 1) ? -1.0 : 1.0;
+        Amp<R> ni, nj;
+        ni.x = static_cast<R>(si * (wr * vj_.x - wi * vj_.y));
+        ni.y = static_cast<R>(si * (wr * vj_.y + wi * vj_.x));
+        nj.x = static_cast<R>(sj * (wr * vi_.x - wi * vi_.y));
+        nj.y = static_cast<R>(sj * (wr * vi_.y + wi * vi_.x));
+        a[i] = ni;
+        a[j] = nj;
+    }
+}
+
+// ---- density-matrix read-outs ----------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_dm_trace(const Amp<R>* a, uint64_t N, double2* part)
+{
+    double tr = 0.0, ti = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < N; i += stride) {
+        const Amp<R> v = a[i + i * N];
+        tr += v.x;
+        ti += v.y;
+    }
+    const double2 r = block_sum2(tr, ti);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+    }
+}
+
+// partial_trace (density_matrix.cpp:231-278): out(ir, ic) = sum_e rho(ir (+) e, ic (+) e) over
+// assignments e of the traced qubits.  The sum runs as a binary counter over e with bit 0 of e on
+// the highest traced qubit, which is the association kettle's one-qubit-at-a-time loop produces.
+template <typename R>
+__global__ void k_dm_partial_trace(const Amp<R>* a, uint32_t n, uint64_t kept_mask, uint64_t traced_desc_packed_lo,
+                                   uint64_t traced_desc_packed_hi, uint32_t n_traced, uint64_t n_out_side, double2* out)
+{
+    const uint64_t idx = blockIdx.x * uint64_t {BLOCK} + threadIdx.x;
+    if (idx >= n_out_side * n_out_side) {
+        return;
+    }
+    const uint64_t ir = idx % n_out_side;
+    const uint64_t ic = idx / n_out_side;
+    const uint64_t N = uint64_t {1} << n;
+    // deposit ir / ic into the kept positions (ascending)
+    uint64_t r0 = 0, c0 = 0;
+    {
+        uint32_t j = 0;
+        for (uint32_t q = 0; q < n; ++q) {
+            if ((kept_mask >> q) & 1ull) {
+                r0 |= ((ir >> j) & 1ull) << q;
+                c0 |= ((ic >> j) & 1ull) << q;
+                ++j;
+            }
+        }
+    }
+    uint8_t tq[32];
+    for (uint32_t j = 0; j < n_traced; ++j) {
+        const uint64_t word = j < 8 ? traced_desc_packed_lo : traced_desc_packed_hi;
+        tq[j] = static_cast<uint8_t>((word >> (8 * (j & 7))) & 0xFF);
+    }
+    double sr[33], si[33];
+    int top = 0;
+    const uint64_t ne = uint64_t {1} << n_traced;
+    for (uint64_t e = 0; e < ne; ++e) {
+        uint64_t off = 0;
+        for (uint32_t j = 0; j < n_traced; ++j) {
+            off |= ((e >> j) & 1ull) << tq[j];
+        }
+        const Amp<R> v = a[(r0 | off) + (c0 | off) * N];
+        double xr = v.x, xi = v.y;
+        uint64_t cnt = e;
+        while (cnt & 1ull) {
+            --top;
+            xr = sr[top] + xr;
+            xi = si[top] + xi;
+            cnt >>= 1;
+        }
+        sr[top] = xr;
+        si[top] = xi;
+        ++top;
+    }
+    out[idx] = make_double2(sr[0], si[0]);
+}
+
+template <typename R>
+__global__ void k_dm_from_sv(Amp<R>* rho, const Amp<R>* psi, uint64_t N)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < N * N; i += stride) {
+        const uint64_t r = i % N, c = i / N;
+        const Amp<R> x = psi[r], y = psi[c];
+        Amp<R> w;
+        w.x = x.x * y.x + x.y * y.y;
+        w.y = x.y * y.x - x.x * y.y;
+        rho[i] = w;
+    }
+}
+
+// ---- kettle generator order, for the index-map parity checks ---------------------------------------
+__global__ void k_pair_map_1q(uint32_t n, uint32_t t, uint64_t* s0, uint64_t* s1)
+{
+    const uint64_t np = uint64_t {1} << (n - 1);
+    const uint64_t i0_max = uint64_t {1} << t;
+    const uint64_t i1_max = uint64_t {1} << (n - t - 1);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t k = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; k < np; k += stride) {
+        const uint64_t i0 = k / i1_max;
+        const uint64_t i1 = k - i0 * i1_max;
+        const uint64_t a = i0 + 2 * i1 * i0_max;
+        s0[k] = a;
+        s1[k] = a + i0_max;
+    }
+}
+
+__global__ void k_pair_map_2q(uint32_t n, uint32_t c, uint32_t t, int four, uint64_t* out)
+{
+    const uint32_t lo = min(c, t), hi = max(c, t);
+    const uint64_t np = n < 2 ? 0 : (uint64_t {1} << (n - 2));
+    const uint64_t i1_max = uint64_t {1} << (hi - lo - 1);
+    const uint64_t i2_max = uint64_t {1} << (n - hi - 1);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t k = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; k < np; k += stride) {
+        const uint64_t i0 = k / (i1_max * i2_max);
+        const uint64_t r = k - i0 * i1_max * i2_max;
+        const uint64_t i1 = r / i2_max;
+        const uint64_t i2 = r - i1 * i2_max;
+        const uint64_t c0t0 = i0 + i1 * (uint64_t {1} << (lo + 1)) + i2 * (uint64_t {1} << (hi + 1));
+        const uint64_t tb = uint64_t {1} << t, cb = uint64_t {1} << c;
+        if (four) {
+            out[4 * k] = c0t0;
+            out[4 * k + 1] = c0t0 + tb;
+            out[4 * k + 2] = c0t0 + cb;
+            out[4 * k + 3] = c0t0 + tb + cb;
+        }
+        else {
+            out[2 * k] = c0t0 + cb;
+            out[2 * k + 1] = c0t0 + cb + tb;
+        }
+    }
+}
+
+// ---- adjoint: one gate on (psi, lambda) plus the derivative overlap ------------------------------
+// psi <- inv psi; acc += <lam | d psi>; lam <- inv lam   (inv = G^dagger, d = dG/dtheta)
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_adjoint_gate(Amp<R>* psi, Amp<R>* lam, uint64_t n_items, DevGate inv,
+                                                        DevGate d, int has_d, double2* part)
+{
+    R m[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        m[i] = static_cast<R>(inv.m[i]);
+    }
+    const uint32_t t = inv.target;
+    const bool ctrl = inv.control >= 0;
+    const uint32_t c = ctrl ? static_cast<uint32_t>(inv.control) : 0;
+    const uint32_t lo = ctrl ? min(c, t) : t;
+    const uint32_t hi = ctrl ? max(c, t) : t;
+    const uint64_t tbit = uint64_t {1} << t;
+    const uint64_t cbit = ctrl ? (uint64_t {1} << c) : 0;
+    double accr = 0.0, acci = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t p = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; p < n_items; p += stride) {
+        const uint64_t i0 = (ctrl ? insert_zero2(p, lo, hi) : insert_zero(p, t)) | cbit;
+        const uint64_t i1 = i0 | tbit;
+        Cx<R> x = ld<R>(psi, i0), y = ld<R>(psi, i1);
+        if (inv.kind == GK_SWAP) {
+            const Cx<R> tmp = x;
+            x = y;
+            y = tmp;
+        }
+        else if (inv.kind == GK_DIAG) {
+            x = cmul(x, m[0], m[1]);
+            y = cmul(y, m[6], m[7]);
+        }
+        else {
+            apply_dense(x, y, m);
+        }
+        st<R>(psi, i0, x);
+        st<R>(psi, i1, y);
+        Cx<R> u = ld<R>(lam, i0), v = ld<R>(lam, i1);
+        if (has_d) {
+            // mu = d (x, y), in FP64
+            const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
+            const double m0r = xr * d.m[0] - xi * d.m[1] + yr * d.m[2] - yi * d.m[3];
+            const double m0i = xr * d.m[1] + xi * d.m[0] + yr * d.m[3] + yi * d.m[2];
+            const double m1r = xr * d.m[4] - xi * d.m[5] + yr * d.m[6] - yi * d.m[7];
+            const double m1i = xr * d.m[5] + xi * d.m[4] + yr * d.m[7] + yi * d.m[6];
+            const double ur = u.re, ui = u.im, vr = v.re, vi = v.im;
+            accr += ur * m0r + ui * m0i + vr * m1r + vi * m1i;
+            acci += ur * m0i - ui * m0r + vr * m1i - vi * m1r;
+        }
+        if (inv.kind == GK_SWAP) {
+            const Cx<R> tmp = u;
+            u = v;
+            v = tmp;
+        }
+        else if (inv.kind == GK_DIAG) {
+            u = cmul(u, m[0], m[1]);
+            v = cmul(v, m[6], m[7]);
+        }
+        else {
+            apply_dense(u, v, m);
+        }
+        st<R>(lam, i0, u);
+        st<R>(lam, i1, v);
+    }
+    if (has_d) {
+        const double2 r = block_sum2(accr, acci);
+        if (threadIdx.x == 0) {
+            part[blockIdx.x] = r;
+        }
+    }
+}
+
+// plain copy of n amplitudes (also the calibration kernel for the HBM traffic counters)
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_copy(Amp<R>* dst, const Amp<R>* src, uint64_t n)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        dst[i] = src[i];
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_dot(const Amp<R>* a, const Amp<R>* b, uint64_t n, double2* part)
+{
+    double sr = 0.0, si = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        const Amp<R> x = a[i], y = b[i];
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+        sr += xr * yr + xi * yi;
+        si += xr * yi - xi * yr;
+    }
+    const double2 r = block_sum2(sr, si);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_max_abs_diff(const Amp<R>* a, const Amp<R>* b, uint64_t n, double2* part)
+{
+    double mx = 0.0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        const Amp<R> x = a[i], y = b[i];
+        const double dr = static_cast<double>(x.x) - y.x, di = static_cast<double>(x.y) - y.y;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+    __shared__ double red[BLOCK];
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = make_double2(red[0], 0.0);
+    }
+}
+
+// ---- launch helpers ----------------------------------------------------------------------------------
+
+unsigned grid_for(uint64_t items, unsigned per_thread = 1, unsigned cap = 1u << 20)
+{
+    const uint64_t threads = (items + per_thread - 1) / per_thread;
+    uint64_t blocks = (threads + BLOCK - 1) / BLOCK;
+    blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, cap));
+    return static_cast<unsigned>(blocks);
+}
+
+DevGate to_dev(const GateRec& g)
+{
+    DevGate d {};
+    d.kind = g.kind;
+    d.target = g.target;
+    d.control = g.control;
+    d.pad = g.pad;
+    std::memcpy(d.m, g.m, sizeof d.m);
+    return d;
+}
+
+// two-stage deterministic reduction into a host double2
+template <typename Launch>
+void reduce2(kq_state* s, unsigned blocks, Launch&& launch, double out[2])
+{
+    auto* part = static_cast<double2*>(scratch(s, (RED_BLOCKS + 1) * sizeof(double2)));
+    launch(part);
+    KQ_HIP_CHECK(hipGetLastError());
+    k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(part, static_cast<int>(blocks), part + RED_BLOCKS);
+    KQ_HIP_CHECK(hipGetLastError());
+    double2 h;
+    KQ_HIP_CHECK(hipMemcpyAsync(&h, part + RED_BLOCKS, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    out[0] = h.x;
+    out[1] = h.y;
+}
+
+}  // namespace
+
+#define KQ_DISPATCH(s, KERNEL_CALL_DOUBLE, KERNEL_CALL_FLOAT) \
+    do {                                                      \
+        if ((s)->precision == KQ_C64) {                       \
+            KERNEL_CALL_FLOAT;                                \
+        }                                                     \
+        else {                                                \
+            KERNEL_CALL_DOUBLE;                               \
+        }                                                     \
+        KQ_HIP_CHECK(hipGetLastError());                      \
+    } while (0)
+
+void launch_gate_sweep(kq_state* s, const GateRec& g)
+{
+    DevGate d = to_dev(g);
+    const bool ctrl = g.control >= 0;
+    uint64_t items;
+    if (g.kind == GK_DIAG) {
+        const bool skip0 = g.m[0] == 1.0 && g.m[1] == 0.0;
+        d.pad = skip0 ? 1u : 0u;
+        items = s->n_amps >> ((ctrl ? 1 : 0) + (skip0 ? 1 : 0));
+    }
+    else {
+        items = s->n_amps >> (ctrl ? 2 : 1);
+    }
+    if (items == 0) {
+        return;
+    }
+    const unsigned blocks = grid_for(items, 4);
+    KQ_DISPATCH(s,
+                (k_gate_sweep<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), items, d)),
+                (k_gate_sweep<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), items, d)));
+}
+
+void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates)
+{
+    TileDesc d {};
+    d.k = p.k;
+    d.c = p.c;
+    uint64_t qmask = 0;
+    for (int j = 0; j < p.k; ++j) {
+        d.qpos[j] = static_cast<uint8_t>(p.qubits[j]);
+        qmask |= uint64_t {1} << p.qubits[j];
+    }
+    int nf = 0;
+    for (uint32_t q = 0; q < s->nvec; ++q) {
+        if (!((qmask >> q) & 1ull)) {
+            d.freepos[nf++] = static_cast<uint8_t>(q);
+        }
+    }
+    d.nfree = nf;
+    if (p.k - p.c > 10) {
+        throw InvalidArg {"tile pass: too many high tile qubits"};
+    }
+    const uint64_t blocks = uint64_t {1} << nf;
+    if (blocks > 0x7fffffffull) {
+        throw InvalidArg {"tile pass: grid too large"};
+    }
+    const size_t lds = (size_t {1} << p.k) * amp_bytes(s->precision);
+    if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
+        throw InvalidArg {"tile pass: tile does not fit the workgroup's LDS"};
+    }
+    static bool attr_set = false;
+    if (!attr_set) {
+        // allow dynamic LDS up to the device limit; a runtime that does not need this may refuse
+        const int maxdyn = static_cast<int>(device_lds_per_block() - TILE_STATIC_LDS);
+        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<double>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
+        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<float>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
+        (void) hipGetLastError();
+        attr_set = true;
+    }
+    const int ng = static_cast<int>(p.gates.size());
+    KQ_DISPATCH(s,
+                (k_tile_pass<double><<<static_cast<unsigned>(blocks), BLOCK, lds, s->stream>>>(
+                    static_cast<double2*>(s->amps), d, static_cast<const DevGate*>(d_gates), ng)),
+                (k_tile_pass<float><<<static_cast<unsigned>(blocks), BLOCK, lds, s->stream>>>(
+                    static_cast<float2*>(s->amps), d, static_cast<const DevGate*>(d_gates), ng)));
+}
+
+void launch_set_basis(kq_state* s, uint64_t index)
+{
+    const unsigned blocks = grid_for(s->n_amps, 4);
+    KQ_DISPATCH(s, (k_set_basis<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), s->n_amps, index)),
+                (k_set_basis<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), s->n_amps, index)));
+}
+
+void launch_qubit_weights(kq_state* s, uint32_t q, double out[2])
+{
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    const uint32_t dm_n = s->is_dm ? s->n : 0;
+    reduce2(s, blocks, [&](double2* part) {
+        KQ_DISPATCH(s,
+                    (k_qubit_weights<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), s->n_amps, q, dm_n, part)),
+                    (k_qubit_weights<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), s->n_amps, q, dm_n, part)));
+    }, out);
+}
+
+void launch_collapse(kq_state* s, uint32_t q, int kept, double scale)
+{
+    const unsigned blocks = grid_for(s->n_amps, 4);
+    const uint32_t dm_n = s->is_dm ? s->n : 0;
+    KQ_DISPATCH(s,
+                (k_collapse<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), s->n_amps, q, dm_n, static_cast<uint32_t>(kept), scale)),
+                (k_collapse<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), s->n_amps, q, dm_n, static_cast<uint32_t>(kept), scale)));
+}
+
+void launch_probabilities(kq_state* s, double* d_probs, const double* noise_host)
+{
+    const uint64_t n_out = uint64_t {1} << s->n;
+    const unsigned blocks = grid_for(n_out, 4);
+    const uint32_t dm_n = s->is_dm ? s->n : 0;
+    KQ_DISPATCH(s,
+                (k_probabilities<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), n_out, dm_n, d_probs)),
+                (k_probabilities<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), n_out, dm_n, d_probs)));
+    if (noise_host != nullptr) {
+        for (uint32_t q = 0; q < s->n; ++q) {
+            launch_noise(s->stream, d_probs, s->n, q, noise_host[q]);
+        }
+    }
+}
+
+void launch_noise(hipStream_t st, double* d_probs, uint32_t n, uint32_t q, double noise)
+{
+    const uint64_t np = uint64_t {1} << (n - 1);
+    k_noise<<<grid_for(np, 4), BLOCK, 0, st>>>(d_probs, np, q, noise);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+size_t cdf_scratch_bytes(uint64_t n)
+{
+    const uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    return nb * sizeof(double) + 64;
+}
+
+void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, size_t scratch_bytes, double* last,
+                double* second_last)
+{
+    const uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (scratch_bytes < cdf_scratch_bytes(n)) {
+        throw InvalidArg {"cdf: scratch too small"};
+    }
+    auto* sums = static_cast<double*>(scratch_ptr);
+    auto* best = reinterpret_cast<long long*>(static_cast<char*>(scratch_ptr) + nb * sizeof(double));
+    k_scan_block_sums<<<static_cast<unsigned>(nb), BLOCK, 0, st>>>(d_cdf, n, sums);
+    KQ_HIP_CHECK(hipGetLastError());
+    k_scan_sums<<<1, BLOCK, 0, st>>>(sums, nb);
+    KQ_HIP_CHECK(hipGetLastError());
+    k_scan_apply<<<static_cast<unsigned>(nb), BLOCK, 0, st>>>(d_cdf, n, sums);
+    KQ_HIP_CHECK(hipGetLastError());
+    KQ_HIP_CHECK(hipMemcpyAsync(last, d_cdf + (n - 1), sizeof(double), hipMemcpyDeviceToHost, st));
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+    const long long init = -1;
+    KQ_HIP_CHECK(hipMemcpyAsync(best, &init, sizeof init, hipMemcpyHostToDevice, st));
+    k_last_below<<<grid_for(n, 8, 4096), BLOCK, 0, st>>>(d_cdf, n, *last, best);
+    KQ_HIP_CHECK(hipGetLastError());
+    long long j = -1;
+    KQ_HIP_CHECK(hipMemcpyAsync(&j, best, sizeof j, hipMemcpyDeviceToHost, st));
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+    *second_last = 0.0;
+    if (j >= 0) {
+        KQ_HIP_CHECK(hipMemcpyAsync(second_last, d_cdf + j, sizeof(double), hipMemcpyDeviceToHost, st));
+        KQ_HIP_CHECK(hipStreamSynchronize(st));
+    }
+}
+
+void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots, uint64_t* d_out)
+{
+    if (shots == 0) {
+        return;
+    }
+    k_lower_bound<<<static_cast<unsigned>((shots + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(d_cdf, n, d_u, shots, d_out);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+namespace
+{
+// upload Pauli tables into the state's scratch after `reserve` bytes; returns device pointers
+void upload_pauli(kq_state* s, size_t reserve, const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings,
+                  PauliGroup** dg, PauliZ** dz)
+{
+    const size_t gb = groups.size() * sizeof(PauliGroup);
+    const size_t zb = strings.size() * sizeof(PauliZ);
+    auto* base = static_cast<char*>(scratch(s, reserve + gb + zb + 256));
+    *dg = reinterpret_cast<PauliGroup*>(base + reserve);
+    *dz = reinterpret_cast<PauliZ*>(base + reserve + ((gb + 127) / 128) * 128);
+    if (reserve + ((gb + 127) / 128) * 128 + zb > reserve + gb + zb + 256) {
+        throw InvalidArg {"pauli: scratch layout"};
+    }
+    if (gb) {
+        KQ_HIP_CHECK(hipMemcpyAsync(*dg, groups.data(), gb, hipMemcpyHostToDevice, s->stream));
+    }
+    if (zb) {
+        KQ_HIP_CHECK(hipMemcpyAsync(*dz, strings.data(), zb, hipMemcpyHostToDevice, s->stream));
+    }
+}
+}  // namespace
+
+void launch_pauli_expectation(kq_state* s, const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings,
+                              double out[2])
+{
+    const size_t reserve = (RED_BLOCKS + 1) * sizeof(double2);
+    PauliGroup* dg;
+    PauliZ* dz;
+    upload_pauli(s, reserve, groups, strings, &dg, &dz);
+    auto* part = static_cast<double2*>(s->scratch);
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    const int ng = static_cast<int>(groups.size());
+    KQ_DISPATCH(s,
+                (k_pauli_expect<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), s->n_amps, dg, ng, dz, part)),
+                (k_pauli_expect<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), s->n_amps, dg, ng, dz, part)));
+    k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(part, static_cast<int>(blocks), part + RED_BLOCKS);
+    KQ_HIP_CHECK(hipGetLastError());
+    double2 h;
+    KQ_HIP_CHECK(hipMemcpyAsync(&h, part + RED_BLOCKS, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    out[0] = h.x;
+    out[1] = h.y;
+}
+
+void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const std::vector<PauliGroup>& groups,
+                            const std::vector<PauliZ>& strings)
+{
+    PauliGroup* dg;
+    PauliZ* dz;
+    upload_pauli(s, 0, groups, strings, &dg, &dz);
+    const unsigned blocks = grid_for(s->n_amps, 4);
+    const int ng = static_cast<int>(groups.size());
+    KQ_DISPATCH(s,
+                (k_pauli_apply_sum<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), s->n_amps, dg, ng, dz)),
+                (k_pauli_apply_sum<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(d_in), static_cast<float2*>(d_out), s->n_amps, dg, ng, dz)));
+}
+
+void launch_pauli_apply_one(kq_state* s, uint64_t x, uint64_t z, const double w[2])
+{
+    const unsigned blocks = grid_for(s->n_amps, 4);
+    KQ_DISPATCH(s,
+                (k_pauli_apply_one<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), s->n_amps, x, z, w[0], w[1])),
+                (k_pauli_apply_one<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), s->n_amps, x, z, w[0], w[1])));
+}
+
+void launch_dm_trace(kq_state* s, double out[2])
+{
+    const uint64_t N = uint64_t {1} << s->n;
+    const unsigned blocks = grid_for(N, 4, RED_BLOCKS);
+    reduce2(s, blocks, [&](double2* part) {
+        KQ_DISPATCH(s,
+                    (k_dm_trace<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), N, part)),
+                    (k_dm_trace<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), N, part)));
+    }, out);
+}
+
+void launch_dm_partial_trace(kq_state* s, const std::vector<uint32_t>& traced_desc, double* out_host)
+{
+    // traced_desc: traced qubits in descending order (bit 0 of the counter = highest qubit)
+    const uint32_t n = s->n;
+    const uint32_t nt = static_cast<uint32_t>(traced_desc.size());
+    if (nt > 16) {
+        throw InvalidArg {"partial trace: at most 16 traced qubits"};
+    }
+    uint64_t lo = 0, hi = 0, traced_mask = 0;
+    for (uint32_t j = 0; j < nt; ++j) {
+        traced_mask |= uint64_t {1} << traced_desc[j];
+        if (j < 8) {
+            lo |= static_cast<uint64_t>(traced_desc[j]) << (8 * j);
+        }
+        else {
+            hi |= static_cast<uint64_t>(traced_desc[j]) << (8 * (j - 8));
+        }
+    }
+    const uint64_t kept_mask = ((uint64_t {1} << n) - 1) & ~traced_mask;
+    const uint64_t side = uint64_t {1} << (n - nt);
+    const uint64_t n_out = side * side;
+    auto* d_out = static_cast<double2*>(scratch(s, n_out * sizeof(double2)));
+    const unsigned blocks = static_cast<unsigned>((n_out + BLOCK - 1) / BLOCK);
+    KQ_DISPATCH(s,
+                (k_dm_partial_trace<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), n, kept_mask, lo, hi, nt, side, d_out)),
+                (k_dm_partial_trace<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), n, kept_mask, lo, hi, nt, side, d_out)));
+    KQ_HIP_CHECK(hipMemcpyAsync(out_host, d_out, n_out * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+}
+
+void launch_dm_from_sv(kq_state* dm, const kq_state* sv)
+{
+    const uint64_t N = uint64_t {1} << sv->n;
+    const unsigned blocks = grid_for(N * N, 4);
+    KQ_DISPATCH(dm,
+                (k_dm_from_sv<double><<<blocks, BLOCK, 0, dm->stream>>>(static_cast<double2*>(dm->amps), static_cast<const double2*>(sv->amps), N)),
+                (k_dm_from_sv<float><<<blocks, BLOCK, 0, dm->stream>>>(static_cast<float2*>(dm->amps), static_cast<const float2*>(sv->amps), N)));
+}
+
+void launch_pair_map_1q(uint32_t n, uint32_t t, uint64_t* d_s0, uint64_t* d_s1)
+{
+    const uint64_t np = uint64_t {1} << (n - 1);
+    k_pair_map_1q<<<grid_for(np, 1), BLOCK>>>(n, t, d_s0, d_s1);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pair_map_2q(uint32_t n, uint32_t c, uint32_t t, int four_way, uint64_t* d_out)
+{
+    const uint64_t np = n < 2 ? 0 : (uint64_t {1} << (n - 2));
+    if (np == 0) {
+        return;
+    }
+    k_pair_map_2q<<<grid_for(np, 1), BLOCK>>>(n, c, t, four_way, d_out);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_adjoint_gate(kq_state* s, void* psi, void* lam, const GateRec& inv, const GateRec* deriv, double* d_acc)
+{
+    const bool ctrl = inv.control >= 0;
+    const uint64_t items = s->n_amps >> (ctrl ? 2 : 1);
+    if (items == 0) {
+        return;
+    }
+    const unsigned blocks = grid_for(items, 4, RED_BLOCKS);
+    const DevGate di = to_dev(inv);
+    const DevGate dd = deriv ? to_dev(*deriv) : DevGate {};
+    auto* part = static_cast<double2*>(scratch(s, (RED_BLOCKS + 1) * sizeof(double2)));
+    const int has_d = deriv ? 1 : 0;
+    KQ_DISPATCH(s,
+                (k_adjoint_gate<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(psi), static_cast<double2*>(lam), items, di, dd, has_d, part)),
+                (k_adjoint_gate<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(psi), static_cast<float2*>(lam), items, di, dd, has_d, part)));
+    if (deriv) {
+        k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(part, static_cast<int>(blocks), reinterpret_cast<double2*>(d_acc));
+        KQ_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes)
+{
+    KQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream));
+}
+
+void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps)
+{
+    const unsigned blocks = grid_for(n_amps, 4);
+    KQ_DISPATCH(s,
+                (k_copy<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(dst), static_cast<const double2*>(src), n_amps)),
+                (k_copy<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(dst), static_cast<const float2*>(src), n_amps)));
+}
+
+double launch_max_abs_diff(kq_state* s, const void* a, const void* b)
+{
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    auto* part = static_cast<double2*>(scratch(s, (RED_BLOCKS + 1) * sizeof(double2)));
+    KQ_DISPATCH(s,
+                (k_max_abs_diff<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(a), static_cast<const double2*>(b), s->n_amps, part)),
+                (k_max_abs_diff<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(a), static_cast<const float2*>(b), s->n_amps, part)));
+    std::vector<double2> h(blocks);
+    KQ_HIP_CHECK(hipMemcpyAsync(h.data(), part, blocks * sizeof(double2), hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    double mx = 0.0;
+    for (const auto& v : h) {
+        mx = std::max(mx, v.x);
+    }
+    return mx;
+}
+
+void launch_dot(kq_state* s, const void* a, const void* b, double out[2])
+{
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    reduce2(s, blocks, [&](double2* part) {
+        KQ_DISPATCH(s,
+                    (k_dot<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(a), static_cast<const double2*>(b), s->n_amps, part)),
+                    (k_dot<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(a), static_cast<const float2*>(b), s->n_amps, part)));
+    }, out);
+}
+
+}  // namespace kq

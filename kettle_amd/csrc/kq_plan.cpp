@@ -1,0 +1,255 @@
+// Gate compilation (kettle gate -> device record) and the fusion planner.
+//
+// Coefficients are evaluated here, on the host, with kettle's own expressions: cos/sin of
+// theta / 2.0 for RX/RY/RZ and of theta for P (operations.cpp:111-173), M_SQRT1_2 and 0.5
+// literals for H, T, SX (operations.cpp:13-109).  The device never evaluates a transcendental.
+#include <algorithm>
+#include <bit>
+#include <cmath>
+#include <cstring>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+namespace
+{
+
+struct M
+{
+    double m[8];
+};
+
+M dense(double a, double b, double c, double d, double e, double f, double g, double h)
+{
+    return M {{a, b, c, d, e, f, g, h}};
+}
+
+// kettle's per-pair formula of each gate, written as the 2x2 matrix it applies
+GateKind gate_matrix(int gate, double theta, const double* u, M& out)
+{
+    const double r = M_SQRT1_2;
+    switch (gate) {
+        case KQ_H: case KQ_CH: out = dense(r, 0, r, 0, r, 0, -r, 0); return GK_DENSE;
+        case KQ_X: case KQ_CX: out = dense(0, 0, 1, 0, 1, 0, 0, 0); return GK_SWAP;
+        case KQ_Y: case KQ_CY: out = dense(0, 0, 0, -1, 0, 1, 0, 0); return GK_DENSE;
+        case KQ_Z: case KQ_CZ: out = dense(1, 0, 0, 0, 0, 0, -1, 0); return GK_DIAG;
+        case KQ_S: case KQ_CS: out = dense(1, 0, 0, 0, 0, 0, 0, 1); return GK_DIAG;
+        case KQ_SDAG: case KQ_CSDAG: out = dense(1, 0, 0, 0, 0, 0, 0, -1); return GK_DIAG;
+        case KQ_T: case KQ_CT: out = dense(1, 0, 0, 0, 0, 0, r, r); return GK_DIAG;
+        case KQ_TDAG: case KQ_CTDAG: out = dense(1, 0, 0, 0, 0, 0, r, -r); return GK_DIAG;
+        case KQ_SX: case KQ_CSX: out = dense(0.5, 0.5, 0.5, -0.5, 0.5, -0.5, 0.5, 0.5); return GK_DENSE;
+        case KQ_SXDAG: case KQ_CSXDAG: out = dense(0.5, -0.5, 0.5, 0.5, 0.5, 0.5, 0.5, -0.5); return GK_DENSE;
+        case KQ_RX: case KQ_CRX: {
+            const double c = std::cos(theta / 2.0), s = std::sin(theta / 2.0);
+            out = dense(c, 0, 0, -s, 0, -s, c, 0);
+            return GK_DENSE;
+        }
+        case KQ_RY: case KQ_CRY: {
+            const double c = std::cos(theta / 2.0), s = std::sin(theta / 2.0);
+            out = dense(c, 0, -s, 0, s, 0, c, 0);
+            return GK_DENSE;
+        }
+        case KQ_RZ: case KQ_CRZ: {
+            const double c = std::cos(theta / 2.0), s = std::sin(theta / 2.0);
+            out = dense(c, -s, 0, 0, 0, 0, c, s);
+            return GK_DIAG;
+        }
+        case KQ_P: case KQ_CP: {
+            const double c = std::cos(theta), s = std::sin(theta);
+            out = dense(1, 0, 0, 0, 0, 0, c, s);
+            return GK_DIAG;
+        }
+        case KQ_U: case KQ_CU: {
+            std::memcpy(out.m, u, sizeof out.m);
+            const bool diag = u[2] == 0.0 && u[3] == 0.0 && u[4] == 0.0 && u[5] == 0.0;
+            return diag ? GK_DIAG : GK_DENSE;
+        }
+        default: break;
+    }
+    throw InvalidArg {"unknown gate code " + std::to_string(gate)};
+}
+
+bool is_controlled(int gate) { return (gate >= KQ_CH && gate <= KQ_CP) || gate == KQ_CU; }
+
+GateRec make_rec(GateKind kind, uint32_t target, int32_t control, const M& m)
+{
+    GateRec g {};
+    g.kind = kind;
+    g.target = target;
+    g.control = control;
+    std::memcpy(g.m, m.m, sizeof g.m);
+    return g;
+}
+
+PlanStats g_stats;
+int g_max_k = -1;          // -1: automatic
+int g_max_gates = 48;
+
+}  // namespace
+
+PlanStats& last_stats() { return g_stats; }
+
+size_t& device_lds_per_block()
+{
+    static size_t bytes = 65536;
+    return bytes;
+}
+
+void set_fusion(int max_k, int max_gates)
+{
+    g_max_k = max_k;
+    if (max_gates > 0) {
+        g_max_gates = max_gates;
+    }
+}
+
+int fusion_setting_k() { return g_max_k; }
+int fusion_max_gates() { return g_max_gates; }
+
+// Tile size: 2^k amplitudes in LDS.  64 KiB of tile (k = 12 for complex128, 13 for complex64)
+// leaves room for two workgroups per CU within gfx950's 160 KiB of LDS.
+int fusion_max_k(int precision, uint32_t nvec)
+{
+    int k = g_max_k;
+    if (k < 0) {
+        k = precision == KQ_C64 ? 13 : 12;
+    }
+    const size_t ab = precision == KQ_C64 ? 8 : 16;
+    while (k > 1 && (size_t {1} << k) * ab + TILE_STATIC_LDS > device_lds_per_block()) {
+        --k;
+    }
+    return std::min<int>(k, static_cast<int>(nvec));
+}
+
+bool is_diag(const GateRec& g) { return g.kind == GK_DIAG; }
+
+void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>& out)
+{
+    M m;
+    const GateKind kind = gate_matrix(op.gate, op.angle, op.u, m);
+    const bool ctrl = is_controlled(op.gate);
+    const uint32_t target = static_cast<uint32_t>(ctrl ? op.q1 : op.q0);
+    const int32_t control = ctrl ? op.q0 : -1;
+    if (target >= n || (ctrl && static_cast<uint32_t>(control) >= n)) {
+        throw InvalidArg {"gate qubit index out of range"};
+    }
+    if (ctrl && static_cast<uint32_t>(control) == target) {
+        throw InvalidArg {"control and target qubits coincide"};
+    }
+    out.push_back(make_rec(kind, target, control, m));
+    if (is_dm) {
+        // rho <- U rho U^dagger: U on the row bit t, conj(U) on the column bit n + t
+        M cm = m;
+        for (int i = 1; i < 8; i += 2) {
+            cm.m[i] = -cm.m[i];
+        }
+        out.push_back(make_rec(kind, target + n, ctrl ? control + static_cast<int32_t>(n) : -1, cm));
+    }
+}
+
+// Greedy pass builder.  A pass owns a tile qubit set Q (|Q| <= k, always containing the low
+// qubits 0..low_c-1).  Gates are taken in circuit order; a gate joins the current pass when
+//   - its non-diagonal target is in Q or Q still has room for it, and
+//   - it commutes with every gate deferred so far (it shares no qubit with a deferred
+//     non-diagonal target, and its own non-diagonal target touches no deferred gate).
+// Diagonal gates and controls never need to be in Q: their effect depends only on index bits.
+std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k, int low_c, int max_gates)
+{
+    std::vector<TilePass> passes;
+    const int k = std::min<int>(max_k, static_cast<int>(nvec));
+    // a tile covering every qubit needs no free slots; otherwise keep at least one slot free
+    const int c = k == static_cast<int>(nvec) ? k : std::max(0, std::min(low_c, k - 1));
+    std::vector<size_t> remaining(gates.size());
+    for (size_t i = 0; i < gates.size(); ++i) {
+        remaining[i] = i;
+    }
+    while (!remaining.empty()) {
+        uint64_t q_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
+        int q_count = c;
+        std::vector<size_t> taken;
+        std::vector<size_t> deferred;
+        uint64_t d_any = 0, d_nd = 0;
+        for (const size_t r : remaining) {
+            const GateRec& g = gates[r];
+            const uint64_t tbit = uint64_t {1} << g.target;
+            const uint64_t nd = is_diag(g) ? 0 : tbit;
+            const uint64_t all = tbit | (g.control >= 0 ? (uint64_t {1} << g.control) : 0);
+            bool ok = static_cast<int>(taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd);
+            if (ok) {
+                const uint64_t need = nd & ~q_mask;
+                const int add = std::popcount(need);
+                if (q_count + add > k) {
+                    ok = false;
+                }
+                else {
+                    q_mask |= need;
+                    q_count += add;
+                }
+            }
+            if (ok) {
+                taken.push_back(r);
+            }
+            else {
+                deferred.push_back(r);
+                d_any |= all;
+                d_nd |= nd;
+            }
+        }
+        if (taken.empty()) {
+            throw InvalidArg {"fusion plan: no gate fits a tile"};
+        }
+        TilePass p;
+        if (taken.size() == 1 && is_diag(gates[taken[0]])) {
+            // a lone diagonal gate touches only a quarter / half of the state: plain sweep
+            p.sweep = true;
+            p.single = gates[taken[0]];
+        }
+        else {
+            // fill Q with the lowest free qubits (longer contiguous runs coalesce better)
+            for (uint32_t q = 0; q < nvec && q_count < k; ++q) {
+                if (!((q_mask >> q) & 1ull)) {
+                    q_mask |= uint64_t {1} << q;
+                    ++q_count;
+                }
+            }
+            p.k = k;
+            for (uint32_t q = 0; q < nvec; ++q) {
+                if ((q_mask >> q) & 1ull) {
+                    p.qubits.push_back(static_cast<int>(q));
+                }
+            }
+            p.c = std::countr_one(q_mask);
+            if (p.c > k) {
+                p.c = k;
+            }
+            // keep the high-offset table within 2^10 entries
+            if (p.k - p.c > 10) {
+                throw InvalidArg {"fusion plan: tile has more than 10 non-contiguous qubits"};
+            }
+            const auto local = [&](int q) { return std::popcount(q_mask & ((uint64_t {1} << q) - 1)); };
+            for (const size_t r : taken) {
+                GateRec g = gates[r];
+                const bool t_in = (q_mask >> g.target) & 1ull;
+                const uint32_t lt = t_in ? static_cast<uint32_t>(local(static_cast<int>(g.target))) : 0xFFu;
+                uint32_t lc = 0xFFu, gc = 0xFFu;
+                if (g.control >= 0) {
+                    if ((q_mask >> g.control) & 1ull) {
+                        lc = static_cast<uint32_t>(local(g.control));
+                    }
+                    else {
+                        gc = static_cast<uint32_t>(g.control);
+                    }
+                }
+                g.target = lt | (static_cast<uint32_t>(gates[r].target) << 8);
+                g.pad = lc | (gc << 8);
+                p.gates.push_back(g);
+            }
+        }
+        passes.push_back(std::move(p));
+        remaining = std::move(deferred);
+    }
+    return passes;
+}
+
+}  // namespace kq
