@@ -127,9 +127,11 @@ int kq_simulate(kq_state* s, uint32_t circuit_n_qubits, const kq_op* ops, size_t
 /* Straight-line gate list (no M, no control flow): the fused device path alone. */
 int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops);
 
-/* Gate fusion control: max_tile_qubits = 0 disables fusion (one sweep per gate). */
-int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass);
-int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass);
+/* Gate fusion control: max_tile_qubits = 0 disables fusion (one sweep per gate), -1 picks the
+ * largest tile that fits LDS; max_gates_per_pass and max_stages_per_pass (register stages, each
+ * costing one LDS remap of the tile) bound a pass; values <= 0 leave a setting unchanged. */
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass);
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass);
 /* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
 int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);
 

@@ -182,21 +182,30 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         profiler_collect();
         return;
     }
-    const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates());
-    size_t total = 0;
+    const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
+                                    fusion_max_stages());
+    size_t n_recs = 0, n_stages = 0;
     for (const auto& p : passes) {
-        total += p.gates.size();
+        n_recs += p.gates.size();
+        n_stages += p.stages.size();
     }
-    std::vector<GateRec> flat;
-    flat.reserve(total);
+    // one upload: [gate records][stage records]
+    const size_t gate_bytes = std::max<size_t>(n_recs, 1) * sizeof(GateRec);
+    const size_t stage_bytes = std::max<size_t>(n_stages, 1) * sizeof(TileStage);
+    std::vector<unsigned char> blob(gate_bytes + stage_bytes);
+    size_t go = 0, so = 0;
     for (const auto& p : passes) {
-        flat.insert(flat.end(), p.gates.begin(), p.gates.end());
+        std::memcpy(blob.data() + go * sizeof(GateRec), p.gates.data(), p.gates.size() * sizeof(GateRec));
+        go += p.gates.size();
+        std::memcpy(blob.data() + gate_bytes + so * sizeof(TileStage), p.stages.data(), p.stages.size() * sizeof(TileStage));
+        so += p.stages.size();
     }
-    auto* d = static_cast<GateRec*>(grow(s->prog, s->prog_bytes, std::max<size_t>(total, 1) * sizeof(GateRec)));
-    if (total) {
-        KQ_HIP_CHECK(hipMemcpyAsync(d, flat.data(), total * sizeof(GateRec), hipMemcpyHostToDevice, s->stream));
-    }
-    size_t off = 0;
+    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
+    KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
+    const auto* d_gates = reinterpret_cast<const GateRec*>(d);
+    const auto* d_stages = reinterpret_cast<const TileStage*>(d + gate_bytes);
+    go = 0;
+    so = 0;
     for (const auto& p : passes) {
         if (p.sweep) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s->n_amps)), 1,
@@ -205,8 +214,9 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         else {
             const int cls = p.c >= p.k ? 0 : 1;
             profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                     [&] { launch_tile_pass(s, p, d + off); });
-            off += p.gates.size();
+                     [&] { launch_tile_pass(s, p, d_gates + go, d_stages + so); });
+            go += p.gates.size();
+            so += p.stages.size();
             ++st.n_tile_passes;
         }
         ++st.n_passes;
@@ -940,21 +950,22 @@ int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops)
     });
 }
 
-int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass)
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass)
 {
     return guarded([&] {
         if (max_tile_qubits > 13) {
             throw InvalidArg {"tile size above 2^13 amplitudes does not fit LDS"};
         }
-        set_fusion(max_tile_qubits, max_gates_per_pass);
+        set_fusion(max_tile_qubits, max_gates_per_pass, max_stages_per_pass);
     });
 }
 
-int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass)
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass)
 {
     return guarded([&] {
         if (max_tile_qubits) { *max_tile_qubits = fusion_setting_k(); }
         if (max_gates_per_pass) { *max_gates_per_pass = fusion_max_gates(); }
+        if (max_stages_per_pass) { *max_stages_per_pass = fusion_max_stages(); }
     });
 }
 

@@ -50,15 +50,35 @@ struct GateRec
     double m[8];        // m00, m01, m10, m11 as (re, im)
 };
 
-// One fused pass: the tile qubit set and the gates applied inside the tile.  Tile gate records
-// carry local positions: target = lt | (gt << 8), pad = lc | (gc << 8), where lt / lc are the
-// local bits of target / control (0xFF when outside the tile) and gt / gc the global qubits.
+// One fused pass.  The tile is the 2^k amplitudes sharing the values of every qubit outside the
+// tile set; local bit j of a tile index is global qubit `qubits[j]`, and local bits 0..c-1 are
+// the contiguous global qubits 0..c-1.  Inside the tile each thread holds 2^rq amplitudes in
+// registers (rq "register bits" of the local index); the pass runs as stages, each with its own
+// register bits, and the tile moves through LDS only between stages whose register bits differ.
+struct TileStage
+{
+    uint8_t reg[8];          // local bit held by register position p (ascending), p < rq
+    uint32_t thr_mask;       // local bits that form the thread index (ascending)
+    uint32_t g_begin;        // gate range of the stage in the pass's gate array
+    uint32_t g_count;
+    uint32_t pad;
+};
+
+// Gate records of a tile pass encode where each index bit comes from:
+//   enc = (idx << 2) | src,  src 0: register position idx, 1: thread-index bit idx,
+//                             2: global qubit idx (outside the tile, uniform per tile)
+// target: the encoded target bit (non-diagonal gates always have src 0); control: -1 or encoded.
 struct TilePass
 {
     int k = 0;                   // tile qubits
     int c = 0;                   // low contiguous qubits 0..c-1 in the tile
+    int rq = 0;                  // register bits per thread
     std::vector<int> qubits;     // qubits[j] = global qubit of local bit j (ascending)
     std::vector<GateRec> gates;  // tile-encoded records
+    std::vector<TileStage> stages;
+    bool load_direct = false;    // load HBM straight into stage 0's layout
+    bool store_direct = false;   // store HBM straight from the last stage's layout
+    TileStage io_layout {};      // layout used for HBM I/O when not direct
     bool sweep = false;          // single gate: run the plain sweep kernel on `single`
     GateRec single {};
 };
@@ -132,19 +152,21 @@ void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>&
 bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
-                                  int low_c, int max_gates);
+                                  int low_c, int max_gates, int rq, int max_stages);
+int fusion_max_stages();
+int register_qubits(int precision);
 PlanStats& last_stats();
 int fusion_max_k(int precision, uint32_t nvec);
 // LDS bytes one workgroup may use on the current device (queried at initialisation)
 size_t& device_lds_per_block();
 constexpr size_t TILE_STATIC_LDS = 8192;   // the tile kernel's high-offset table
 int fusion_max_gates();
-void set_fusion(int max_k, int max_gates);
+void set_fusion(int max_k, int max_gates, int max_stages);
 int fusion_setting_k();
 
 // ---- kernel launchers (kq_kernels.hip) --------------------------------------------------------
 void launch_gate_sweep(kq_state* s, const GateRec& g);
-void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates);
+void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates, const void* d_stages);
 void launch_set_basis(kq_state* s, uint64_t index);
 // weights: SV -> (sum |a|^2 over bit q = 0, = 1); DM -> block Frobenius norms (kettle's quirk)
 void launch_qubit_weights(kq_state* s, uint32_t q, double out[2]);

@@ -22,8 +22,6 @@ namespace
 {
 
 constexpr int BLOCK = 256;
-constexpr int MAX_TILE_Q = 16;
-constexpr int MAX_FREE_Q = 48;
 constexpr int RED_BLOCKS = 2048;   // upper bound on partial-sum blocks of a reduction
 
 template <typename R> struct AmpT;
@@ -195,146 +193,6 @@ __global__ __launch_bounds__(BLOCK) void k_gate_sweep(Amp<R>* a, uint64_t n_item
             st<R>(a, i0, x);
             st<R>(a, i1, y);
         }
-    }
-}
-
-// ---- fused tile pass ----------------------------------------------------------------------------
-// A tile is the 2^k amplitudes that share the values of all qubits outside the tile set Q.
-// Q always holds the low qubits 0..c-1, so a tile is 2^(k-c) contiguous runs of 2^c amplitudes
-// (coalesced HBM traffic); the other k-c tile qubits may be any higher qubits, which is how
-// high-index gates are fused.  The tile lives in LDS while the pass's gates are applied; each
-// gate costs LDS traffic only, the pass costs one HBM read and one HBM write of the state.
-struct TileDesc
-{
-    int k;
-    int c;
-    int nfree;
-    int pad;
-    uint8_t qpos[MAX_TILE_Q];     // global qubit of local bit j
-    uint8_t freepos[MAX_FREE_Q];  // global positions of the non-tile qubits, ascending
-};
-
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_tile_pass(Amp<R>* a, TileDesc d, const DevGate* gates, int n_gates)
-{
-    extern __shared__ unsigned char smem_raw[];
-    Amp<R>* tile = reinterpret_cast<Amp<R>*>(smem_raw);
-    __shared__ uint64_t hoff[1 << 10];
-
-    const int k = d.k;
-    const int c = d.c;
-    const uint32_t tsize = 1u << k;
-    const uint32_t lowmask = (1u << c) - 1;
-    const int nh = k - c;
-
-    // base index: block id deposited into the free (non-tile) bit positions
-    uint64_t base = 0;
-    {
-        uint64_t b = blockIdx.x;
-        for (int j = 0; j < d.nfree; ++j) {
-            base |= ((b >> j) & 1ull) << d.freepos[j];
-        }
-    }
-    for (uint32_t h = threadIdx.x; h < (1u << nh); h += BLOCK) {
-        uint64_t off = 0;
-        for (int j = 0; j < nh; ++j) {
-            off |= static_cast<uint64_t>((h >> j) & 1u) << d.qpos[c + j];
-        }
-        hoff[h] = off;
-    }
-    __syncthreads();
-
-    for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
-        tile[l] = a[base | (l & lowmask) | hoff[l >> c]];
-    }
-
-    bool need_sync = true;
-    for (int gi = 0; gi < n_gates; ++gi) {
-        const DevGate& g = gates[gi];
-        // local positions are encoded in the record: target = local bit or 0xFF.. for outside
-        const int lt = static_cast<int>(g.target & 0xFFu);
-        const int gt = static_cast<int>((g.target >> 8) & 0xFFu);
-        const int lc = static_cast<int>(g.pad & 0xFFu);
-        const int gc = static_cast<int>((g.pad >> 8) & 0xFFu);
-        const bool has_c = g.control >= 0;
-        R m[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            m[i] = static_cast<R>(g.m[i]);
-        }
-        // a control outside the tile is uniform over the tile
-        if (has_c && lc == 0xFF && !((base >> gc) & 1ull)) {
-            continue;
-        }
-        if (g.kind == GK_DIAG) {
-            if (need_sync) {
-                __syncthreads();
-                need_sync = false;
-            }
-            for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
-                if (has_c && lc != 0xFF && !((l >> lc) & 1u)) {
-                    continue;
-                }
-                const uint32_t bit = lt != 0xFF ? ((l >> lt) & 1u) : static_cast<uint32_t>((base >> gt) & 1ull);
-                const Amp<R> v = tile[l];
-                Cx<R> x {v.x, v.y};
-                x = bit ? cmul(x, m[6], m[7]) : cmul(x, m[0], m[1]);
-                Amp<R> w;
-                w.x = x.re;
-                w.y = x.im;
-                tile[l] = w;
-            }
-            continue;
-        }
-        __syncthreads();
-        need_sync = true;
-        const uint32_t tb = 1u << lt;
-        if (has_c && lc != 0xFF) {
-            const uint32_t lo = min(lt, lc), hi = max(lt, lc);
-            const uint32_t cb = 1u << lc;
-            for (uint32_t p = threadIdx.x; p < (tsize >> 2); p += BLOCK) {
-                const uint32_t l0 = static_cast<uint32_t>(insert_zero2(p, lo, hi)) | cb;
-                const uint32_t l1 = l0 | tb;
-                const Amp<R> u0 = tile[l0], u1 = tile[l1];
-                if (g.kind == GK_SWAP) {
-                    tile[l0] = u1;
-                    tile[l1] = u0;
-                }
-                else {
-                    Cx<R> x {u0.x, u0.y}, y {u1.x, u1.y};
-                    apply_dense(x, y, m);
-                    Amp<R> w0, w1;
-                    w0.x = x.re; w0.y = x.im; w1.x = y.re; w1.y = y.im;
-                    tile[l0] = w0;
-                    tile[l1] = w1;
-                }
-            }
-        }
-        else {
-            for (uint32_t p = threadIdx.x; p < (tsize >> 1); p += BLOCK) {
-                const uint32_t l0 = static_cast<uint32_t>(insert_zero(p, lt));
-                const uint32_t l1 = l0 | tb;
-                const Amp<R> u0 = tile[l0], u1 = tile[l1];
-                if (g.kind == GK_SWAP) {
-                    tile[l0] = u1;
-                    tile[l1] = u0;
-                }
-                else {
-                    Cx<R> x {u0.x, u0.y}, y {u1.x, u1.y};
-                    apply_dense(x, y, m);
-                    Amp<R> w0, w1;
-                    w0.x = x.re; w0.y = x.im; w1.x = y.re; w1.y = y.im;
-                    tile[l0] = w0;
-                    tile[l1] = w1;
-                }
-            }
-        }
-    }
-    if (need_sync) {
-        __syncthreads();
-    }
-    for (uint32_t l = threadIdx.x; l < tsize; l += BLOCK) {
-        a[base | (l & lowmask) | hoff[l >> c]] = tile[l];
     }
 }
 
@@ -993,53 +851,6 @@ void launch_gate_sweep(kq_state* s, const GateRec& g)
     KQ_DISPATCH(s,
                 (k_gate_sweep<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), items, d)),
                 (k_gate_sweep<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), items, d)));
-}
-
-void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates)
-{
-    TileDesc d {};
-    d.k = p.k;
-    d.c = p.c;
-    uint64_t qmask = 0;
-    for (int j = 0; j < p.k; ++j) {
-        d.qpos[j] = static_cast<uint8_t>(p.qubits[j]);
-        qmask |= uint64_t {1} << p.qubits[j];
-    }
-    int nf = 0;
-    for (uint32_t q = 0; q < s->nvec; ++q) {
-        if (!((qmask >> q) & 1ull)) {
-            d.freepos[nf++] = static_cast<uint8_t>(q);
-        }
-    }
-    d.nfree = nf;
-    if (p.k - p.c > 10) {
-        throw InvalidArg {"tile pass: too many high tile qubits"};
-    }
-    const uint64_t blocks = uint64_t {1} << nf;
-    if (blocks > 0x7fffffffull) {
-        throw InvalidArg {"tile pass: grid too large"};
-    }
-    const size_t lds = (size_t {1} << p.k) * amp_bytes(s->precision);
-    if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
-        throw InvalidArg {"tile pass: tile does not fit the workgroup's LDS"};
-    }
-    static bool attr_set = false;
-    if (!attr_set) {
-        // allow dynamic LDS up to the device limit; a runtime that does not need this may refuse
-        const int maxdyn = static_cast<int>(device_lds_per_block() - TILE_STATIC_LDS);
-        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<double>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
-        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_pass<float>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
-        (void) hipGetLastError();
-        attr_set = true;
-    }
-    const int ng = static_cast<int>(p.gates.size());
-    KQ_DISPATCH(s,
-                (k_tile_pass<double><<<static_cast<unsigned>(blocks), BLOCK, lds, s->stream>>>(
-                    static_cast<double2*>(s->amps), d, static_cast<const DevGate*>(d_gates), ng)),
-                (k_tile_pass<float><<<static_cast<unsigned>(blocks), BLOCK, lds, s->stream>>>(
-                    static_cast<float2*>(s->amps), d, static_cast<const DevGate*>(d_gates), ng)));
 }
 
 void launch_set_basis(kq_state* s, uint64_t index)

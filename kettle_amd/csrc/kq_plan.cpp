@@ -84,7 +84,8 @@ GateRec make_rec(GateKind kind, uint32_t target, int32_t control, const M& m)
 
 PlanStats g_stats;
 int g_max_k = -1;          // -1: automatic
-int g_max_gates = 48;
+int g_max_gates = 4096;
+int g_max_stages = 6;
 
 }  // namespace
 
@@ -96,15 +97,20 @@ size_t& device_lds_per_block()
     return bytes;
 }
 
-void set_fusion(int max_k, int max_gates)
+void set_fusion(int max_k, int max_gates, int max_stages)
 {
     g_max_k = max_k;
     if (max_gates > 0) {
         g_max_gates = max_gates;
     }
+    if (max_stages > 0) {
+        g_max_stages = max_stages;
+    }
 }
 
 int fusion_setting_k() { return g_max_k; }
+int fusion_max_stages() { return g_max_stages; }
+int register_qubits(int precision) { return precision == KQ_C64 ? 5 : 4; }
 int fusion_max_gates() { return g_max_gates; }
 
 // Tile size: 2^k amplitudes in LDS.  64 KiB of tile (k = 12 for complex128, 13 for complex64)
@@ -148,16 +154,62 @@ void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>&
     }
 }
 
+namespace
+{
+
+uint32_t enc(uint32_t src, uint32_t idx) { return (idx << 2) | src; }
+
+// choose the register bits of each stage: the stage's target bits, completed from the previous
+// stage's register bits (equal layouts need no LDS remap), then from local bits >= 3 (keeps the
+// lowest local bits on lanes for coalesced HBM access), then any
+void complete_stage(TileStage& st, uint32_t need, uint32_t prev, int k, int rq)
+{
+    uint32_t regs = need;
+    const auto fill = [&](uint32_t from) {
+        for (int b = 0; b < k && std::popcount(regs) < rq; ++b) {
+            if (((from >> b) & 1u) && !((regs >> b) & 1u)) {
+                regs |= 1u << b;
+            }
+        }
+    };
+    fill(prev);
+    fill(((1u << k) - 1) & ~7u);
+    fill((1u << k) - 1);
+    const uint32_t all = (k >= 32) ? ~0u : ((1u << k) - 1);
+    int p = 0;
+    for (int b = 0; b < k; ++b) {
+        if ((regs >> b) & 1u) {
+            st.reg[p++] = static_cast<uint8_t>(b);
+        }
+    }
+    st.thr_mask = all & ~regs;
+}
+
+uint32_t stage_regs(const TileStage& st, int rq)
+{
+    uint32_t m = 0;
+    for (int p = 0; p < rq; ++p) {
+        m |= 1u << st.reg[p];
+    }
+    return m;
+}
+
+}  // namespace
+
 // Greedy pass builder.  A pass owns a tile qubit set Q (|Q| <= k, always containing the low
 // qubits 0..low_c-1).  Gates are taken in circuit order; a gate joins the current pass when
-//   - its non-diagonal target is in Q or Q still has room for it, and
+//   - its non-diagonal target is in Q or Q still has room for it,
 //   - it commutes with every gate deferred so far (it shares no qubit with a deferred
-//     non-diagonal target, and its own non-diagonal target touches no deferred gate).
+//     non-diagonal target, and its own non-diagonal target touches no deferred gate), and
+//   - its target fits the current register stage (at most rq distinct non-diagonal targets per
+//     stage) or a new stage may still be opened (at most max_stages per pass).
 // Diagonal gates and controls never need to be in Q: their effect depends only on index bits.
-std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k, int low_c, int max_gates)
+std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k, int low_c, int max_gates,
+                                  int rq_max, int max_stages)
 {
     std::vector<TilePass> passes;
     const int k = std::min<int>(max_k, static_cast<int>(nvec));
+    const int rq = std::min(rq_max, k);
     // a tile covering every qubit needs no free slots; otherwise keep at least one slot free
     const int c = k == static_cast<int>(nvec) ? k : std::max(0, std::min(low_c, k - 1));
     std::vector<size_t> remaining(gates.size());
@@ -168,6 +220,8 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
         uint64_t q_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
         int q_count = c;
         std::vector<size_t> taken;
+        std::vector<int> taken_stage;
+        std::vector<uint64_t> stage_targets {0};
         std::vector<size_t> deferred;
         uint64_t d_any = 0, d_nd = 0;
         for (const size_t r : remaining) {
@@ -176,19 +230,31 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
             const uint64_t nd = is_diag(g) ? 0 : tbit;
             const uint64_t all = tbit | (g.control >= 0 ? (uint64_t {1} << g.control) : 0);
             bool ok = static_cast<int>(taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd);
-            if (ok) {
-                const uint64_t need = nd & ~q_mask;
-                const int add = std::popcount(need);
-                if (q_count + add > k) {
+            uint64_t need = 0;
+            bool new_stage = false;
+            if (ok && nd) {
+                need = nd & ~q_mask;
+                if (q_count + std::popcount(need) > k) {
                     ok = false;
                 }
-                else {
-                    q_mask |= need;
-                    q_count += add;
+                else if (!(stage_targets.back() & nd) && std::popcount(stage_targets.back()) >= rq) {
+                    if (static_cast<int>(stage_targets.size()) >= max_stages) {
+                        ok = false;
+                    }
+                    else {
+                        new_stage = true;
+                    }
                 }
             }
             if (ok) {
+                q_mask |= need;
+                q_count += std::popcount(need);
+                if (new_stage) {
+                    stage_targets.push_back(0);
+                }
+                stage_targets.back() |= nd;
                 taken.push_back(r);
+                taken_stage.push_back(static_cast<int>(stage_targets.size()) - 1);
             }
             else {
                 deferred.push_back(r);
@@ -204,47 +270,89 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
             // a lone diagonal gate touches only a quarter / half of the state: plain sweep
             p.sweep = true;
             p.single = gates[taken[0]];
+            passes.push_back(std::move(p));
+            remaining = std::move(deferred);
+            continue;
         }
-        else {
-            // fill Q with the lowest free qubits (longer contiguous runs coalesce better)
-            for (uint32_t q = 0; q < nvec && q_count < k; ++q) {
-                if (!((q_mask >> q) & 1ull)) {
-                    q_mask |= uint64_t {1} << q;
-                    ++q_count;
-                }
+        // fill Q with the lowest free qubits (longer contiguous runs coalesce better)
+        for (uint32_t q = 0; q < nvec && q_count < k; ++q) {
+            if (!((q_mask >> q) & 1ull)) {
+                q_mask |= uint64_t {1} << q;
+                ++q_count;
             }
-            p.k = k;
+        }
+        p.k = k;
+        p.rq = rq;
+        for (uint32_t q = 0; q < nvec; ++q) {
+            if ((q_mask >> q) & 1ull) {
+                p.qubits.push_back(static_cast<int>(q));
+            }
+        }
+        p.c = std::min(std::countr_one(q_mask), k);
+        // keep the high-offset table within 2^10 entries
+        if (p.k - p.c > 10) {
+            throw InvalidArg {"fusion plan: tile has more than 10 non-contiguous qubits"};
+        }
+        const auto local = [&](uint64_t q) { return static_cast<uint32_t>(std::popcount(q_mask & ((uint64_t {1} << q) - 1))); };
+        // stages: register bits and layouts
+        const int n_st = static_cast<int>(stage_targets.size());
+        p.stages.resize(static_cast<size_t>(n_st));
+        uint32_t prev = 0;
+        for (int si = 0; si < n_st; ++si) {
+            uint32_t need_local = 0;
             for (uint32_t q = 0; q < nvec; ++q) {
-                if ((q_mask >> q) & 1ull) {
-                    p.qubits.push_back(static_cast<int>(q));
+                if ((stage_targets[si] >> q) & 1ull) {
+                    need_local |= 1u << local(q);
                 }
             }
-            p.c = std::countr_one(q_mask);
-            if (p.c > k) {
-                p.c = k;
+            complete_stage(p.stages[si], need_local, prev, k, rq);
+            prev = stage_regs(p.stages[si], rq);
+        }
+        // gates, grouped by stage (taken order is stage order), with encoded bit sources
+        const auto encode = [&](const TileStage& st, uint64_t q) -> uint32_t {
+            if (!((q_mask >> q) & 1ull)) {
+                return enc(2, static_cast<uint32_t>(q));
             }
-            // keep the high-offset table within 2^10 entries
-            if (p.k - p.c > 10) {
-                throw InvalidArg {"fusion plan: tile has more than 10 non-contiguous qubits"};
+            const uint32_t lb = local(q);
+            for (int pp = 0; pp < rq; ++pp) {
+                if (st.reg[pp] == lb) {
+                    return enc(0, static_cast<uint32_t>(pp));
+                }
             }
-            const auto local = [&](int q) { return std::popcount(q_mask & ((uint64_t {1} << q) - 1)); };
-            for (const size_t r : taken) {
-                GateRec g = gates[r];
-                const bool t_in = (q_mask >> g.target) & 1ull;
-                const uint32_t lt = t_in ? static_cast<uint32_t>(local(static_cast<int>(g.target))) : 0xFFu;
-                uint32_t lc = 0xFFu, gc = 0xFFu;
+            return enc(1, static_cast<uint32_t>(std::popcount(st.thr_mask & ((1u << lb) - 1))));
+        };
+        size_t gi = 0;
+        for (int si = 0; si < n_st; ++si) {
+            TileStage& st = p.stages[si];
+            st.g_begin = static_cast<uint32_t>(p.gates.size());
+            while (gi < taken.size() && taken_stage[gi] == si) {
+                GateRec g = gates[taken[gi]];
+                const uint32_t t_enc = encode(st, g.target);
+                if (!is_diag(g) && (t_enc & 3u) != 0) {
+                    throw InvalidArg {"fusion plan: non-diagonal target outside the stage registers"};
+                }
                 if (g.control >= 0) {
-                    if ((q_mask >> g.control) & 1ull) {
-                        lc = static_cast<uint32_t>(local(g.control));
-                    }
-                    else {
-                        gc = static_cast<uint32_t>(g.control);
-                    }
+                    g.control = static_cast<int32_t>(encode(st, static_cast<uint64_t>(g.control)));
                 }
-                g.target = lt | (static_cast<uint32_t>(gates[r].target) << 8);
-                g.pad = lc | (gc << 8);
+                g.target = t_enc;
+                g.pad = 0;
                 p.gates.push_back(g);
+                ++gi;
             }
+            st.g_count = static_cast<uint32_t>(p.gates.size()) - st.g_begin;
+        }
+        // HBM I/O layout: the stage's own when its register bits leave local bits 0..2 on lanes
+        const uint32_t low3 = k > rq + 2 ? 7u : 0u;
+        p.load_direct = (stage_regs(p.stages.front(), rq) & low3) == 0;
+        p.store_direct = (stage_regs(p.stages.back(), rq) & low3) == 0;
+        complete_stage(p.io_layout, 0, 0, k, rq);
+        {
+            // default I/O layout: the top rq local bits in registers
+            uint32_t top = 0;
+            for (int b = k - 1; b >= 0 && std::popcount(top) < rq; --b) {
+                top |= 1u << b;
+            }
+            complete_stage(p.io_layout, top, 0, k, rq);
         }
         passes.push_back(std::move(p));
         remaining = std::move(deferred);
