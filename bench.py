@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--cpu-gates", type=int, default=3)
     ap.add_argument("--vqe-qubits", type=int, default=24)
     ap.add_argument("--vqe-energies", type=int, default=20)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -131,6 +133,16 @@ def main() -> int:
     log(f"device {name.value.decode()} CUs={ncu.value} mem={mem.value / 2**30:.0f} GiB")
 
     n = args.qubits
+    if args.calib_copy:
+        # a plain copy kernel of known size: reads and writes 16 * 2^n bytes per launch
+        src = ka.Statevector(n, ka.KQ_C128)
+        dst = ka.Statevector(n, ka.KQ_C128)
+        for _ in range(args.calib_copy):
+            _lib.check(lib.kq_state_copy(dst.handle, src.handle))
+        _lib.check(lib.kq_synchronize())
+        print(json.dumps({"calib_copy": args.calib_copy, "bytes_read_per_launch": 16 << n,
+                          "bytes_written_per_launch": 16 << n}))
+        return 0
     circuit = workloads.qft_plus_layers(n=n, layers=args.layers)
     ops, _, _ = circuit.flatten()
     n_gates = len(ops)

@@ -130,8 +130,13 @@ int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops);
 /* Gate fusion control: max_tile_qubits = 0 disables fusion (one sweep per gate), -1 picks the
  * largest tile that fits LDS; max_gates_per_pass and max_stages_per_pass (register stages, each
  * costing one LDS remap of the tile) bound a pass; values <= 0 leave a setting unchanged. */
-int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass);
-int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass);
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass,
+                  double max_cost_per_pass);
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass,
+                  double* max_cost_per_pass);
+/* max_cost_per_pass bounds a pass's arithmetic in units of one general complex 2x2 gate (real or
+ * axis-rotation matrices count 1/2, one-sided diagonals 1/4, swaps 1/10): above roughly the HBM
+ * time of one sweep a pass stops being memory-bound. */
 /* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
 int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);
 

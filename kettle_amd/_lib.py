@@ -65,8 +65,8 @@ _SIGNATURES = {
         [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_size_t, c_int, c_int32, c_void_p, c_size_t, LOGGER_FN, c_void_p],
     ),
     "kq_apply_gates": (c_int, [c_void_p, c_void_p, c_size_t]),
-    "kq_set_fusion": (c_int, [c_int, c_int, c_int]),
-    "kq_get_fusion": (c_int, [POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+    "kq_set_fusion": (c_int, [c_int, c_int, c_int, c_double]),
+    "kq_get_fusion": (c_int, [POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)]),
     "kq_last_plan_stats": (c_int, [POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_qubit_weights": (c_int, [c_void_p, c_uint32, c_void_p]),
     "kq_collapse": (c_int, [c_void_p, c_uint32, c_int, c_double]),

@@ -194,9 +194,11 @@ def plan_stats() -> tuple[int, int, int]:
     return a.value, b.value, d.value
 
 
-def set_fusion(max_tile_qubits: int, max_gates_per_pass: int = 0, max_stages_per_pass: int = 0) -> None:
+def set_fusion(max_tile_qubits: int, max_gates_per_pass: int = 0, max_stages_per_pass: int = 0,
+               max_cost_per_pass: float = 0.0) -> None:
     """Tile size 2^max_tile_qubits (0: no fusion, -1: largest that fits LDS); <= 0 keeps a bound."""
-    check(load().kq_set_fusion(int(max_tile_qubits), int(max_gates_per_pass), int(max_stages_per_pass)))
+    check(load().kq_set_fusion(int(max_tile_qubits), int(max_gates_per_pass), int(max_stages_per_pass),
+                               float(max_cost_per_pass)))
 
 
 # ---- probabilities and sampling ----------------------------------------------------------------------

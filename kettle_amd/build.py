@@ -46,6 +46,12 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write("\n".join(noise) + "\n")
 
 
+# per-source extra flags.  The tile kernel dispatches gates to many specialised, structurally
+# identical code paths; LLVM's common-code sinking would merge them into one block that addresses
+# the per-thread amplitude slots through a phi of pointers, which forces the slots into scratch.
+EXTRA_FLAGS = {"kq_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
 def build_kq(jobs: int = 4) -> Path:
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     LIB_DIR.mkdir(parents=True, exist_ok=True)
@@ -54,7 +60,7 @@ def build_kq(jobs: int = 4) -> Path:
     for src in KQ_SOURCES:
         obj = BUILD_DIR / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmds.append([HIPCC, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(str(obj))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for fut in [ex.submit(_run, c) for c in cmds]:

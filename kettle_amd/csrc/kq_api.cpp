@@ -157,7 +157,7 @@ kq_state* create_state(uint32_t n, int precision, bool dm)
 uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 {
     const bool ctrl = g.control >= 0;
-    if (g.kind == GK_DIAG) {
+    if (g.kind == GK_DIAG || g.kind == GK_DIAG1) {
         const bool skip0 = g.m[0] == 1.0 && g.m[1] == 0.0;
         return n_amps >> ((ctrl ? 1 : 0) + (skip0 ? 1 : 0));
     }
@@ -173,7 +173,8 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
     }
     const double ab = static_cast<double>(amp_bytes(s->precision));
     const int max_k = fusion_max_k(s->precision, s->nvec);
-    if (max_k <= 0) {
+    // tiles smaller than one thread's register block (states of a few qubits) run gate by gate
+    if (max_k < register_qubits(s->precision)) {
         for (const auto& g : recs) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(g, s->n_amps)), 1, [&] { launch_gate_sweep(s, g); });
             ++st.n_passes;
@@ -543,7 +544,7 @@ GateRec inverse_of(const GateRec& g)
     if (g.kind == GK_SWAP) {
         return r;
     }
-    if (g.kind == GK_DIAG) {
+    if (g.kind == GK_DIAG || g.kind == GK_DIAG1) {
         r.m[1] = -g.m[1];
         r.m[7] = -g.m[7];
         return r;
@@ -950,22 +951,23 @@ int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops)
     });
 }
 
-int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass)
+int kq_set_fusion(int max_tile_qubits, int max_gates_per_pass, int max_stages_per_pass, double max_cost_per_pass)
 {
     return guarded([&] {
         if (max_tile_qubits > 13) {
             throw InvalidArg {"tile size above 2^13 amplitudes does not fit LDS"};
         }
-        set_fusion(max_tile_qubits, max_gates_per_pass, max_stages_per_pass);
+        set_fusion(max_tile_qubits, max_gates_per_pass, max_stages_per_pass, max_cost_per_pass);
     });
 }
 
-int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass)
+int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages_per_pass, double* max_cost_per_pass)
 {
     return guarded([&] {
         if (max_tile_qubits) { *max_tile_qubits = fusion_setting_k(); }
         if (max_gates_per_pass) { *max_gates_per_pass = fusion_max_gates(); }
         if (max_stages_per_pass) { *max_stages_per_pass = fusion_max_stages(); }
+        if (max_cost_per_pass) { *max_cost_per_pass = fusion_max_cost(); }
     });
 }
 

@@ -36,9 +36,12 @@ void set_error(const std::string& msg);
 // 2n qubits: each kettle gate becomes two records, U on bit t and conj(U) on bit n + t).
 enum GateKind : uint32_t
 {
-    GK_DENSE = 0,   // (a, b) -> (m00 a + m01 b, m10 a + m11 b)
+    GK_DENSE = 0,   // (a, b) -> (m00 a + m01 b, m10 a + m11 b), general complex matrix
     GK_DIAG = 1,    // a -> d0 a, b -> d1 b          (m[0..1] = d0, m[6..7] = d1)
     GK_SWAP = 2,    // (a, b) -> (b, a)
+    GK_REAL = 3,    // real matrix (H, RY): half the multiplies of GK_DENSE
+    GK_AXIS = 4,    // real diagonal, imaginary off-diagonal (RX, Y): half the multiplies
+    GK_DIAG1 = 5,   // diagonal with d0 == 1 exactly (Z, S, T, P and their controlled forms)
 };
 
 struct GateRec
@@ -161,7 +164,10 @@ int fusion_max_k(int precision, uint32_t nvec);
 size_t& device_lds_per_block();
 constexpr size_t TILE_STATIC_LDS = 8192;   // the tile kernel's high-offset table
 int fusion_max_gates();
-void set_fusion(int max_k, int max_gates, int max_stages);
+void set_fusion(int max_k, int max_gates, int max_stages, double max_cost);
+double fusion_max_cost();
+// arithmetic cost of a gate in a tile pass, in units of one general complex 2x2 gate
+double gate_cost(const GateRec& g);
 int fusion_setting_k();
 
 // ---- kernel launchers (kq_kernels.hip) --------------------------------------------------------

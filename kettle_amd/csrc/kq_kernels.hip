@@ -150,7 +150,7 @@ __global__ __launch_bounds__(BLOCK) void k_gate_sweep(Amp<R>* a, uint64_t n_item
     const uint64_t tbit = uint64_t {1} << t;
     const uint64_t cbit = ctrl ? (uint64_t {1} << c) : 0;
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
-    if (g.kind == GK_DIAG) {
+    if (g.kind == GK_DIAG || g.kind == GK_DIAG1) {
         // only amplitudes with control = 1 change; with d0 == 1 only those with target = 1.
         // The host decides (bit 0 of pad) from the FP64 coefficient and sizes n_items to match.
         const bool skip0 = (g.pad & 1u) != 0;
@@ -686,7 +686,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_gate(Amp<R>* psi, Amp<R>* lam
             x = y;
             y = tmp;
         }
-        else if (inv.kind == GK_DIAG) {
+        else if (inv.kind == GK_DIAG || inv.kind == GK_DIAG1) {
             x = cmul(x, m[0], m[1]);
             y = cmul(y, m[6], m[7]);
         }
@@ -712,7 +712,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_gate(Amp<R>* psi, Amp<R>* lam
             u = v;
             v = tmp;
         }
-        else if (inv.kind == GK_DIAG) {
+        else if (inv.kind == GK_DIAG || inv.kind == GK_DIAG1) {
             u = cmul(u, m[0], m[1]);
             v = cmul(v, m[6], m[7]);
         }
@@ -836,7 +836,7 @@ void launch_gate_sweep(kq_state* s, const GateRec& g)
     DevGate d = to_dev(g);
     const bool ctrl = g.control >= 0;
     uint64_t items;
-    if (g.kind == GK_DIAG) {
+    if (g.kind == GK_DIAG || g.kind == GK_DIAG1) {
         const bool skip0 = g.m[0] == 1.0 && g.m[1] == 0.0;
         d.pad = skip0 ? 1u : 0u;
         items = s->n_amps >> ((ctrl ? 1 : 0) + (skip0 ? 1 : 0));

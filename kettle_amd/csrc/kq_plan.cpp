@@ -72,6 +72,26 @@ GateKind gate_matrix(int gate, double theta, const double* u, M& out)
 
 bool is_controlled(int gate) { return (gate >= KQ_CH && gate <= KQ_CP) || gate == KQ_CU; }
 
+// the cheapest exact arithmetic form of a matrix (kernels skip the terms that are zero)
+GateKind classify(const M& m)
+{
+    const double* e = m.m;
+    if (e[2] == 0.0 && e[3] == 0.0 && e[4] == 0.0 && e[5] == 0.0) {
+        return (e[0] == 1.0 && e[1] == 0.0) ? GK_DIAG1 : GK_DIAG;
+    }
+    if (e[0] == 0.0 && e[1] == 0.0 && e[6] == 0.0 && e[7] == 0.0 && e[2] == 1.0 && e[3] == 0.0 && e[4] == 1.0
+        && e[5] == 0.0) {
+        return GK_SWAP;
+    }
+    if (e[1] == 0.0 && e[3] == 0.0 && e[5] == 0.0 && e[7] == 0.0) {
+        return GK_REAL;
+    }
+    if (e[1] == 0.0 && e[7] == 0.0 && e[2] == 0.0 && e[4] == 0.0) {
+        return GK_AXIS;
+    }
+    return GK_DENSE;
+}
+
 GateRec make_rec(GateKind kind, uint32_t target, int32_t control, const M& m)
 {
     GateRec g {};
@@ -86,6 +106,7 @@ PlanStats g_stats;
 int g_max_k = -1;          // -1: automatic
 int g_max_gates = 4096;
 int g_max_stages = 6;
+double g_max_cost = 1.0e9;
 
 }  // namespace
 
@@ -97,8 +118,13 @@ size_t& device_lds_per_block()
     return bytes;
 }
 
-void set_fusion(int max_k, int max_gates, int max_stages)
+double fusion_max_cost() { return g_max_cost; }
+
+void set_fusion(int max_k, int max_gates, int max_stages, double max_cost)
 {
+    if (max_cost > 0.0) {
+        g_max_cost = max_cost;
+    }
     g_max_k = max_k;
     if (max_gates > 0) {
         g_max_gates = max_gates;
@@ -110,16 +136,16 @@ void set_fusion(int max_k, int max_gates, int max_stages)
 
 int fusion_setting_k() { return g_max_k; }
 int fusion_max_stages() { return g_max_stages; }
-int register_qubits(int precision) { return precision == KQ_C64 ? 5 : 4; }
+int register_qubits(int) { return 4; }
 int fusion_max_gates() { return g_max_gates; }
 
-// Tile size: 2^k amplitudes in LDS.  64 KiB of tile (k = 12 for complex128, 13 for complex64)
-// leaves room for two workgroups per CU within gfx950's 160 KiB of LDS.
+// Tile size: 2^k amplitudes.  k = 12 gives 256 threads of 16 register amplitudes each and a
+// 64 KiB (complex128) / 32 KiB (complex64) LDS tile for the stage remaps.
 int fusion_max_k(int precision, uint32_t nvec)
 {
     int k = g_max_k;
     if (k < 0) {
-        k = precision == KQ_C64 ? 13 : 12;
+        k = 12;
     }
     const size_t ab = precision == KQ_C64 ? 8 : 16;
     while (k > 1 && (size_t {1} << k) * ab + TILE_STATIC_LDS > device_lds_per_block()) {
@@ -128,12 +154,26 @@ int fusion_max_k(int precision, uint32_t nvec)
     return std::min<int>(k, static_cast<int>(nvec));
 }
 
-bool is_diag(const GateRec& g) { return g.kind == GK_DIAG; }
+bool is_diag(const GateRec& g) { return g.kind == GK_DIAG || g.kind == GK_DIAG1; }
+
+double gate_cost(const GateRec& g)
+{
+    switch (g.kind) {
+        case GK_DENSE: return 1.0;
+        case GK_REAL: case GK_AXIS: return 0.5;
+        case GK_DIAG: return 0.5;
+        case GK_DIAG1: return 0.25;
+        case GK_SWAP: return 0.1;
+        default: break;
+    }
+    return 1.0;
+}
 
 void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>& out)
 {
     M m;
-    const GateKind kind = gate_matrix(op.gate, op.angle, op.u, m);
+    (void) gate_matrix(op.gate, op.angle, op.u, m);
+    const GateKind kind = classify(m);
     const bool ctrl = is_controlled(op.gate);
     const uint32_t target = static_cast<uint32_t>(ctrl ? op.q1 : op.q0);
     const int32_t control = ctrl ? op.q0 : -1;
@@ -150,7 +190,7 @@ void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>&
         for (int i = 1; i < 8; i += 2) {
             cm.m[i] = -cm.m[i];
         }
-        out.push_back(make_rec(kind, target + n, ctrl ? control + static_cast<int32_t>(n) : -1, cm));
+        out.push_back(make_rec(classify(cm), target + n, ctrl ? control + static_cast<int32_t>(n) : -1, cm));
     }
 }
 
@@ -224,12 +264,15 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
         std::vector<uint64_t> stage_targets {0};
         std::vector<size_t> deferred;
         uint64_t d_any = 0, d_nd = 0;
+        double cost = 0.0;
+        const double max_cost = fusion_max_cost();
         for (const size_t r : remaining) {
             const GateRec& g = gates[r];
             const uint64_t tbit = uint64_t {1} << g.target;
             const uint64_t nd = is_diag(g) ? 0 : tbit;
             const uint64_t all = tbit | (g.control >= 0 ? (uint64_t {1} << g.control) : 0);
-            bool ok = static_cast<int>(taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd);
+            bool ok = static_cast<int>(taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd)
+                   && (taken.empty() || cost + gate_cost(g) <= max_cost);
             uint64_t need = 0;
             bool new_stage = false;
             if (ok && nd) {
@@ -247,6 +290,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
                 }
             }
             if (ok) {
+                cost += gate_cost(g);
                 q_mask |= need;
                 q_count += std::popcount(need);
                 if (new_stage) {

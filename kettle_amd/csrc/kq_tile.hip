@@ -4,8 +4,7 @@
 // the pass's tile set.  The tile is read from HBM once and written once; all gates of the pass
 // are applied in between without touching HBM.
 //
-// Register blocking.  Each thread holds 2^RQ amplitudes in VGPRs (RQ = 4 for complex128, 5 for
-// complex64: 64 VGPRs of data either way).  Which RQ bits of the local tile index live in a
+// Register blocking.  Each thread holds 2^RQ = 16 amplitudes in VGPRs.  Which RQ bits of the local tile index live in a
 // thread's registers is the "layout"; the remaining bits form the thread index.  A pass is a
 // sequence of stages; within a stage every non-diagonal gate acts on a register bit, so it is
 // plain register arithmetic on pairs (v[r], v[r | 1 << p]).  Diagonal gates and controls may sit
@@ -94,7 +93,6 @@ struct TileRegs
     Cx<R> v[1 << RQ];
     uint32_t reg_bit[RQ];   // local bit of register position p
     uint32_t tpart;         // local-index bits contributed by the thread index
-    int rq;                 // register bits in use (min(RQ, k))
 
     __device__ __forceinline__ void set_layout(const DevStage& st, uint32_t tid)
     {
@@ -117,7 +115,7 @@ struct TileRegs
         bool same = true;
 #pragma unroll
         for (int p = 0; p < RQ; ++p) {
-            same = same && (p >= rq || reg_bit[p] == st.reg[p]);
+            same = same && (reg_bit[p] == st.reg[p]);
         }
         return same;
     }
@@ -156,35 +154,79 @@ struct Unroll<N, N>
     }
 };
 
-template <typename R, int RQ, int P>
-__device__ __forceinline__ void pair_gate(TileRegs<R, RQ>& t, const R* m, uint32_t kind, int csrc, uint32_t cidx)
+// One gate on register position P: pairs (v[r], v[r | 1 << P]).  KIND selects the arithmetic:
+// the zero entries of real, axis-rotation and diagonal matrices are skipped at compile time.
+// A control on a register bit (csrc == 0) is tested per slot with a uniform (scalar) branch.
+template <typename R>
+struct Coef
 {
-    Unroll<0, (1 << RQ)>::run([&]<int r>() {
+    R m[8];
+};
+
+template <typename R, int RQ, int P, int KIND>
+__device__ __forceinline__ void reg_gate(TileRegs<R, RQ>& t, const Coef<R> c, int csrc, uint32_t cidx)
+{
+    const R* m = c.m;
+    Unroll<0, (1 << RQ)>::run([&]<int r>() __attribute__((always_inline)) {
         if constexpr (((r >> P) & 1) == 0) {
             constexpr int r1 = r | (1 << P);
-            if (P < t.rq) {
-                const bool on = csrc != 0 || ((r >> cidx) & 1u);
-                if (on) {
-                    const Cx<R> a = t.v[r];
-                    const Cx<R> b = t.v[r1];
-                    if (kind == GK_SWAP) {
-                        t.v[r] = b;
-                        t.v[r1] = a;
-                    }
-                    else {
-                        t.v[r] = {a.re * m[0] - a.im * m[1] + b.re * m[2] - b.im * m[3],
-                                  a.re * m[1] + a.im * m[0] + b.re * m[3] + b.im * m[2]};
-                        t.v[r1] = {a.re * m[4] - a.im * m[5] + b.re * m[6] - b.im * m[7],
-                                   a.re * m[5] + a.im * m[4] + b.re * m[7] + b.im * m[6]};
-                    }
+            if (csrc != 0 || ((r >> cidx) & 1u)) {
+                const Cx<R> a = t.v[r];
+                const Cx<R> b = t.v[r1];
+                if constexpr (KIND == GK_SWAP) {
+                    t.v[r] = b;
+                    t.v[r1] = a;
+                }
+                else if constexpr (KIND == GK_REAL) {
+                    t.v[r] = {a.re * m[0] + b.re * m[2], a.im * m[0] + b.im * m[2]};
+                    t.v[r1] = {a.re * m[4] + b.re * m[6], a.im * m[4] + b.im * m[6]};
+                }
+                else if constexpr (KIND == GK_AXIS) {
+                    // m00 = m[0], m01 = i m[3], m10 = i m[5], m11 = m[6]
+                    t.v[r] = {a.re * m[0] - b.im * m[3], a.im * m[0] + b.re * m[3]};
+                    t.v[r1] = {b.re * m[6] - a.im * m[5], b.im * m[6] + a.re * m[5]};
+                }
+                else if constexpr (KIND == GK_DIAG1) {
+                    t.v[r1] = cmul(b, m[6], m[7]);
+                }
+                else if constexpr (KIND == GK_DIAG) {
+                    t.v[r] = cmul(a, m[0], m[1]);
+                    t.v[r1] = cmul(b, m[6], m[7]);
+                }
+                else {
+                    t.v[r] = {a.re * m[0] - a.im * m[1] + b.re * m[2] - b.im * m[3],
+                              a.re * m[1] + a.im * m[0] + b.re * m[3] + b.im * m[2]};
+                    t.v[r1] = {a.re * m[4] - a.im * m[5] + b.re * m[6] - b.im * m[7],
+                               a.re * m[5] + a.im * m[4] + b.re * m[7] + b.im * m[6]};
                 }
             }
         }
     });
 }
 
+template <typename R, int RQ, int KIND>
+__device__ __forceinline__ void reg_gate_at(TileRegs<R, RQ>& t, const DevGate& g, uint32_t pos, int csrc, uint32_t cidx)
+{
+    Coef<R> m;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        m.m[i] = static_cast<R>(g.m[i]);
+    }
+    switch (pos) {
+        case 0: reg_gate<R, RQ, 0, KIND>(t, m, csrc, cidx); break;
+        case 1: reg_gate<R, RQ, 1, KIND>(t, m, csrc, cidx); break;
+        case 2: reg_gate<R, RQ, 2, KIND>(t, m, csrc, cidx); break;
+        case 3: reg_gate<R, RQ, 3, KIND>(t, m, csrc, cidx); break;
+        default:
+            if constexpr (RQ > 4) {
+                reg_gate<R, RQ, (RQ > 4 ? 4 : 0), KIND>(t, m, csrc, cidx);
+            }
+            break;
+    }
+}
+
 template <typename R, int RQ>
-__global__ __launch_bounds__(TILE_BLOCK_MAX) void k_tile_reg(Amp<R>* __restrict__ a, TileArgs d,
+__global__ __launch_bounds__(TILE_BLOCK_MAX) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_tile_reg(Amp<R>* __restrict__ a, TileArgs d,
                                                              const DevStage* __restrict__ stages,
                                                              const DevGate* __restrict__ gates)
 {
@@ -216,36 +258,28 @@ __global__ __launch_bounds__(TILE_BLOCK_MAX) void k_tile_reg(Amp<R>* __restrict_
     __syncthreads();
 
     TileRegs<R, RQ> t;
-    t.rq = k < RQ ? k : RQ;
-    const int ra = 1 << t.rq;
     const auto gidx = [&](uint32_t l) { return base | (l & lowmask) | hoff[l >> c]; };
 
     const DevStage s0 = stages[0];
     t.set_layout(d.load_direct ? s0 : d.io, tid);
-    Unroll<0, RA>::run([&]<int r>() {
-        if (r < ra) {
-            const Amp<R> x = a[gidx(t.template local<r>())];
-            t.v[r] = {x.x, x.y};
-        }
+    Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+        const Amp<R> x = a[gidx(t.template local<r>())];
+        t.v[r] = {x.x, x.y};
     });
 
     const auto remap = [&](const DevStage& to) {
         __syncthreads();
-        Unroll<0, RA>::run([&]<int r>() {
-            if (r < ra) {
-                Amp<R> x;
-                x.x = t.v[r].re;
-                x.y = t.v[r].im;
-                tile[swz(t.template local<r>())] = x;
-            }
+        Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+            Amp<R> x;
+            x.x = t.v[r].re;
+            x.y = t.v[r].im;
+            tile[swz(t.template local<r>())] = x;
         });
         __syncthreads();
         t.set_layout(to, tid);
-        Unroll<0, RA>::run([&]<int r>() {
-            if (r < ra) {
-                const Amp<R> x = tile[swz(t.template local<r>())];
-                t.v[r] = {x.x, x.y};
-            }
+        Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+            const Amp<R> x = tile[swz(t.template local<r>())];
+            t.v[r] = {x.x, x.y};
         });
     };
 
@@ -269,35 +303,29 @@ __global__ __launch_bounds__(TILE_BLOCK_MAX) void k_tile_reg(Amp<R>* __restrict_
             if (csrc == 2 && !((base >> cidx) & 1ull)) {
                 continue;
             }
-            R m[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                m[i] = static_cast<R>(g.m[i]);
-            }
-            if (kind == GK_DIAG) {
-                const uint32_t tbit_uniform = tsrc == 1 ? ((tid >> tidx) & 1u) : static_cast<uint32_t>((base >> tidx) & 1ull);
-                Unroll<0, RA>::run([&]<int r>() {
-                    if (r < ra) {
-                        const bool on = csrc != 0 || ((r >> cidx) & 1u);
-                        if (on) {
-                            const uint32_t bit = tsrc == 0 ? ((r >> tidx) & 1u) : tbit_uniform;
-                            t.v[r] = bit ? cmul(t.v[r], m[6], m[7]) : cmul(t.v[r], m[0], m[1]);
-                        }
+            if ((kind == GK_DIAG || kind == GK_DIAG1) && tsrc != 0) {
+                // the target bit is uniform over the thread: one factor for every slot
+                const uint32_t bit = tsrc == 1 ? ((tid >> tidx) & 1u) : static_cast<uint32_t>((base >> tidx) & 1ull);
+                if (!bit && kind == GK_DIAG1) {
+                    continue;
+                }
+                const R fr = static_cast<R>(bit ? g.m[6] : g.m[0]);
+                const R fi = static_cast<R>(bit ? g.m[7] : g.m[1]);
+                Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+                    if (csrc != 0 || ((r >> cidx) & 1u)) {
+                        t.v[r] = cmul(t.v[r], fr, fi);
                     }
                 });
                 continue;
             }
-            // non-diagonal: the target is register position tidx
-            switch (tidx) {
-                case 0: pair_gate<R, RQ, 0>(t, m, kind, csrc, cidx); break;
-                case 1: pair_gate<R, RQ, 1>(t, m, kind, csrc, cidx); break;
-                case 2: pair_gate<R, RQ, 2>(t, m, kind, csrc, cidx); break;
-                case 3: pair_gate<R, RQ, 3>(t, m, kind, csrc, cidx); break;
-                default:
-                    if constexpr (RQ > 4) {
-                        pair_gate<R, RQ, (RQ > 4 ? 4 : 0)>(t, m, kind, csrc, cidx);
-                    }
-                    break;
+            // the target is register position tidx
+            switch (kind) {
+                case GK_SWAP: reg_gate_at<R, RQ, GK_SWAP>(t, g, tidx, csrc, cidx); break;
+                case GK_REAL: reg_gate_at<R, RQ, GK_REAL>(t, g, tidx, csrc, cidx); break;
+                case GK_AXIS: reg_gate_at<R, RQ, GK_AXIS>(t, g, tidx, csrc, cidx); break;
+                case GK_DIAG1: reg_gate_at<R, RQ, GK_DIAG1>(t, g, tidx, csrc, cidx); break;
+                case GK_DIAG: reg_gate_at<R, RQ, GK_DIAG>(t, g, tidx, csrc, cidx); break;
+                default: reg_gate_at<R, RQ, GK_DENSE>(t, g, tidx, csrc, cidx); break;
             }
         }
     }
@@ -305,13 +333,11 @@ __global__ __launch_bounds__(TILE_BLOCK_MAX) void k_tile_reg(Amp<R>* __restrict_
     if (!d.store_direct && !t.same_layout(d.io)) {
         remap(d.io);
     }
-    Unroll<0, RA>::run([&]<int r>() {
-        if (r < ra) {
-            Amp<R> x;
-            x.x = t.v[r].re;
-            x.y = t.v[r].im;
-            a[gidx(t.template local<r>())] = x;
-        }
+    Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+        Amp<R> x;
+        x.x = t.v[r].re;
+        x.y = t.v[r].im;
+        a[gidx(t.template local<r>())] = x;
     });
 }
 
@@ -333,7 +359,7 @@ void set_lds_attr()
 void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates, const void* d_stages)
 {
     const int rq_kernel = register_qubits(s->precision);
-    if (p.rq != std::min(rq_kernel, p.k)) {
+    if (p.rq != rq_kernel || p.k < rq_kernel) {
         throw InvalidArg {"tile pass: register bits do not match the kernel"};
     }
     if (p.k > MAX_TILE_Q || p.k - p.c > 10 || p.stages.empty()) {
@@ -376,8 +402,8 @@ void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates, const
     const auto* g = static_cast<const DevGate*>(d_gates);
     const auto* st = static_cast<const DevStage*>(d_stages);
     if (s->precision == KQ_C64) {
-        set_lds_attr<float, 5>();
-        k_tile_reg<float, 5><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<float2*>(s->amps), d, st, g);
+        set_lds_attr<float, 4>();
+        k_tile_reg<float, 4><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<float2*>(s->amps), d, st, g);
     }
     else {
         set_lds_attr<double, 4>();

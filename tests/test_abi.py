@@ -142,7 +142,7 @@ def test_state_creation_without_gpu_reports_hip_error():
 
 def test_invalid_arguments_are_rejected_before_device_work():
     lib = _lib.load()
-    assert lib.kq_set_fusion(20, 0, 0) == _lib.KQ_ERR_INVALID
+    assert lib.kq_set_fusion(20, 0, 0, 0.0) == _lib.KQ_ERR_INVALID
     assert b"LDS" in lib.kq_last_error()
     assert lib.kq_state_upload(None, None) == _lib.KQ_ERR_INVALID
     assert lib.kq_simulate(None, 1, None, 0, None, 0, 0, 0, None, 0, _lib.LOGGER_FN(), None) == _lib.KQ_ERR_INVALID
