@@ -1,6 +1,7 @@
 #pragma once
 
 #include <cstddef>
+#include <ranges>
 #include <initializer_list>
 #include <memory>
 #include <string_view>
@@ -110,16 +111,16 @@ public:
     void add_tdag_gate(std::size_t target);
     void add_sx_gate(std::size_t target);
     void add_sxdag_gate(std::size_t target);
-    template <typename C> void add_h_gate(const C& ts) { for (auto t : ts) { add_h_gate(t); } }
-    template <typename C> void add_x_gate(const C& ts) { for (auto t : ts) { add_x_gate(t); } }
-    template <typename C> void add_y_gate(const C& ts) { for (auto t : ts) { add_y_gate(t); } }
-    template <typename C> void add_z_gate(const C& ts) { for (auto t : ts) { add_z_gate(t); } }
-    template <typename C> void add_s_gate(const C& ts) { for (auto t : ts) { add_s_gate(t); } }
-    template <typename C> void add_sdag_gate(const C& ts) { for (auto t : ts) { add_sdag_gate(t); } }
-    template <typename C> void add_t_gate(const C& ts) { for (auto t : ts) { add_t_gate(t); } }
-    template <typename C> void add_tdag_gate(const C& ts) { for (auto t : ts) { add_tdag_gate(t); } }
-    template <typename C> void add_sx_gate(const C& ts) { for (auto t : ts) { add_sx_gate(t); } }
-    template <typename C> void add_sxdag_gate(const C& ts) { for (auto t : ts) { add_sxdag_gate(t); } }
+    template <std::ranges::range C> void add_h_gate(const C& ts) { for (auto t : ts) { add_h_gate(t); } }
+    template <std::ranges::range C> void add_x_gate(const C& ts) { for (auto t : ts) { add_x_gate(t); } }
+    template <std::ranges::range C> void add_y_gate(const C& ts) { for (auto t : ts) { add_y_gate(t); } }
+    template <std::ranges::range C> void add_z_gate(const C& ts) { for (auto t : ts) { add_z_gate(t); } }
+    template <std::ranges::range C> void add_s_gate(const C& ts) { for (auto t : ts) { add_s_gate(t); } }
+    template <std::ranges::range C> void add_sdag_gate(const C& ts) { for (auto t : ts) { add_sdag_gate(t); } }
+    template <std::ranges::range C> void add_t_gate(const C& ts) { for (auto t : ts) { add_t_gate(t); } }
+    template <std::ranges::range C> void add_tdag_gate(const C& ts) { for (auto t : ts) { add_tdag_gate(t); } }
+    template <std::ranges::range C> void add_sx_gate(const C& ts) { for (auto t : ts) { add_sx_gate(t); } }
+    template <std::ranges::range C> void add_sxdag_gate(const C& ts) { for (auto t : ts) { add_sxdag_gate(t); } }
     void add_h_gate(std::initializer_list<std::size_t> ts) { for (auto t : ts) { add_h_gate(t); } }
     void add_x_gate(std::initializer_list<std::size_t> ts) { for (auto t : ts) { add_x_gate(t); } }
 

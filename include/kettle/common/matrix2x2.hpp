@@ -2,6 +2,7 @@
 
 #include <complex>
 
+#include "kettle/common/mathtools.hpp"
 #include "kettle/common/tolerance.hpp"
 
 // kettle's 2x2 complex matrix (include/kettle/common/matrix2x2.hpp in the reference).
@@ -22,8 +23,6 @@ auto conjugate_transpose(const Matrix2X2& mat) -> Matrix2X2;
 auto determinant(const Matrix2X2& mat) -> std::complex<double>;
 auto trace(const Matrix2X2& mat) -> std::complex<double>;
 auto almost_eq(const Matrix2X2& left, const Matrix2X2& right,
-               double tolerance_sq = COMPLEX_ALMOST_EQ_TOLERANCE_SQ) noexcept -> bool;
-auto almost_eq(const std::complex<double>& left, const std::complex<double>& right,
                double tolerance_sq = COMPLEX_ALMOST_EQ_TOLERANCE_SQ) noexcept -> bool;
 
 }  // namespace ket

@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "kettle/common/mathtools.hpp"
 #include "kettle/common/tolerance.hpp"
 #include "kettle/state/endian.hpp"
 #include "kettle/state/qubit_state_conversion.hpp"

@@ -72,6 +72,40 @@ def build_kq(jobs: int = 4) -> Path:
     return out
 
 
+KET_SOURCES = ["ket_common.cpp", "ket_state.cpp", "ket_circuit.cpp", "ket_simulate.cpp", "ket_measure.cpp",
+               "ket_pauli.cpp", "ket_optimize.cpp"]
+HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-I", str(ROOT / "include")]
+
+
+def build_ket(jobs: int = 4) -> Path:
+    """The C++20 kettle-compatible host library (ket:: API) over libkq's C-ABI."""
+    BUILD_DIR.mkdir(parents=True, exist_ok=True)
+    host = ROOT / "kettle_amd" / "host"
+    objs = []
+    cmds = []
+    for src in KET_SOURCES:
+        obj = BUILD_DIR / (src + ".o")
+        cmds.append(["g++", *HOST_FLAGS, "-c", str(host / src), "-o", str(obj)])
+        objs.append(str(obj))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for fut in [ex.submit(_run, c) for c in cmds]:
+            fut.result()
+    out = LIB_DIR / "libket.so"
+    tmp = LIB_DIR / "libket.so.tmp"
+    _run(["g++", "-shared", "-o", str(tmp), *objs, "-L", str(LIB_DIR), "-lkq", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cpp_tests() -> Path:
+    """The C++ API test executable (tests/cpp), linked against libket and libkq."""
+    src = ROOT / "tests" / "cpp" / "ket_api_test.cpp"
+    out = ROOT / "tests" / "cpp" / "ket_api_test"
+    _run(["g++", *HOST_FLAGS, str(src), "-o", str(out), "-L", str(LIB_DIR), "-lket", "-lkq",
+          f"-Wl,-rpath,{LIB_DIR}"])
+    return out
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "liboracle.so"
