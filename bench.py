@@ -189,7 +189,7 @@ def main() -> int:
     achieved = float(tile["bytes"]) / (float(tile["ms"]) * 1e-3) / 1e9 if tile["ms"] else 0.0
     roofline = {
         "bound": "hbm",
-        "kernel": "k_tile_pass<double>",
+        "kernel": "k_tile<double>",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",

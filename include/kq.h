@@ -217,6 +217,18 @@ int kq_debug_pair_map_1q(uint32_t n_qubits, uint32_t target, uint64_t* s0, uint6
 int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, int four_way,
                          uint64_t* out);
 
+/* Host-only view of the fusion plan of a straight-line gate list (no device work, no GPU
+ * needed).  Each op compiles to one device gate record (two for a density matrix: U on the row
+ * bits, then conj(U) on the column bits, records 2i and 2i+1).  For the j-th record applied,
+ * order[j] is its record index and pass_of[j] the pass that applies it; for each pass,
+ * pass_kind is its profiling class (0, 1, 2 as above) and pass_cost its arithmetic cost in the
+ * units of kq_set_fusion's max_cost_per_pass.  order and pass_of hold one entry per record,
+ * pass_kind and pass_cost at least as many; lds_bytes is the per-workgroup LDS the tile is
+ * fitted to (0 = the value of the initialised device, 64 KiB before kq_init). */
+int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op* ops, size_t n_ops,
+                  size_t lds_bytes, uint32_t* order, uint32_t* pass_of, int32_t* pass_kind,
+                  double* pass_cost, size_t* n_passes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -88,6 +88,8 @@ _SIGNATURES = {
     "kq_profile_end": (c_int, [c_void_p]),
     "kq_debug_pair_map_1q": (c_int, [c_uint32, c_uint32, c_void_p, c_void_p]),
     "kq_debug_pair_map_2q": (c_int, [c_uint32, c_uint32, c_uint32, c_int, c_void_p]),
+    "kq_debug_plan": (c_int, [c_uint32, c_int, c_int, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
+                              c_void_p, c_void_p]),
 }
 
 _lib = None

@@ -194,6 +194,29 @@ def plan_stats() -> tuple[int, int, int]:
     return a.value, b.value, d.value
 
 
+@dataclass
+class FusionPlan:
+    """Host-only fusion plan of a straight-line gate list (``kq_debug_plan``)."""
+    order: np.ndarray       # record index of each applied gate, in execution order
+    pass_of: np.ndarray     # pass that applies it
+    pass_kind: np.ndarray   # per pass: 0 low-qubit tile, 1 tile with high qubits, 2 single-gate sweep
+    pass_cost: np.ndarray   # per pass: arithmetic cost (general 2x2 gate = 1)
+
+
+def debug_plan(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, is_density: bool = False,
+               lds_bytes: int = 160 * 1024) -> FusionPlan:
+    n_rec = len(ops) * (2 if is_density else 1)
+    order = np.zeros(max(n_rec, 1), dtype=np.uint32)
+    pass_of = np.zeros_like(order)
+    kind = np.zeros(max(n_rec, 1), dtype=np.int32)
+    cost = np.zeros(max(n_rec, 1), dtype=np.float64)
+    n_passes = _lib.ctypes.c_size_t()
+    check(load().kq_debug_plan(int(n_qubits), int(precision), int(bool(is_density)), ptr(ops), len(ops), int(lds_bytes),
+                               ptr(order), ptr(pass_of), ptr(kind), ptr(cost), _lib.ctypes.byref(n_passes)))
+    npass = n_passes.value
+    return FusionPlan(order[:n_rec], pass_of[:n_rec], kind[:npass], cost[:npass])
+
+
 def set_fusion(max_tile_qubits: int, max_gates_per_pass: int = 0, max_stages_per_pass: int = 0,
                max_cost_per_pass: float = 0.0) -> None:
     """Tile size 2^max_tile_qubits (0: no fusion, -1: largest that fits LDS); <= 0 keeps a bound."""

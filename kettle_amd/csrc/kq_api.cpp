@@ -185,39 +185,50 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
     }
     const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
                                     fusion_max_stages());
-    size_t n_recs = 0, n_stages = 0;
-    for (const auto& p : passes) {
-        n_recs += p.gates.size();
-        n_stages += p.stages.size();
+    // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts]
+    std::vector<TileProgram> progs(passes.size());
+    size_t total = 0;
+    const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
+    std::vector<size_t> offs(passes.size());
+    for (size_t i = 0; i < passes.size(); ++i) {
+        if (passes[i].sweep) {
+            continue;
+        }
+        lower_tile_pass(passes[i], s->precision, s->nvec, progs[i]);
+        offs[i] = total;
+        const TileProgram& tp = progs[i];
+        total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
+               + align(tp.layouts.size() * sizeof(TileLayout));
     }
-    // one upload: [gate records][stage records]
-    const size_t gate_bytes = std::max<size_t>(n_recs, 1) * sizeof(GateRec);
-    const size_t stage_bytes = std::max<size_t>(n_stages, 1) * sizeof(TileStage);
-    std::vector<unsigned char> blob(gate_bytes + stage_bytes);
-    size_t go = 0, so = 0;
-    for (const auto& p : passes) {
-        std::memcpy(blob.data() + go * sizeof(GateRec), p.gates.data(), p.gates.size() * sizeof(GateRec));
-        go += p.gates.size();
-        std::memcpy(blob.data() + gate_bytes + so * sizeof(TileStage), p.stages.data(), p.stages.size() * sizeof(TileStage));
-        so += p.stages.size();
+    std::vector<unsigned char> blob(std::max<size_t>(total, 256), 0);
+    for (size_t i = 0; i < passes.size(); ++i) {
+        if (passes[i].sweep) {
+            continue;
+        }
+        const TileProgram& tp = progs[i];
+        size_t o = offs[i];
+        std::memcpy(blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+        o += align(tp.gates.size() * sizeof(TileGate));
+        std::memcpy(blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+        o += align(tp.steps.size() * sizeof(TileStep));
+        std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
     }
     auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
     KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
-    const auto* d_gates = reinterpret_cast<const GateRec*>(d);
-    const auto* d_stages = reinterpret_cast<const TileStage*>(d + gate_bytes);
-    go = 0;
-    so = 0;
-    for (const auto& p : passes) {
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const TilePass& p = passes[i];
         if (p.sweep) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s->n_amps)), 1,
                      [&] { launch_gate_sweep(s, p.single); });
         }
         else {
+            const TileProgram& tp = progs[i];
+            const unsigned char* g = d + offs[i];
+            const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
+            const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
             const int cls = p.c >= p.k ? 0 : 1;
             profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                     [&] { launch_tile_pass(s, p, d_gates + go, d_stages + so); });
-            go += p.gates.size();
-            so += p.stages.size();
+                     [&] { launch_tile_pass(s, tp, g, sp, ly); });
             ++st.n_tile_passes;
         }
         ++st.n_passes;
@@ -1254,6 +1265,58 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
         KQ_HIP_CHECK(hipDeviceSynchronize());
         KQ_HIP_CHECK(hipMemcpy(out, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
         KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                  uint32_t* order, uint32_t* pass_of, int32_t* pass_kind, double* pass_cost, size_t* n_passes)
+{
+    return guarded([&] {
+        if (n_qubits == 0 || n_qubits > 40 || (precision != KQ_C128 && precision != KQ_C64) || (!ops && n_ops)
+            || !order || !pass_of || !pass_kind || !pass_cost || !n_passes) {
+            throw InvalidArg {"debug plan: bad arguments"};
+        }
+        std::vector<GateRec> recs;
+        for (size_t i = 0; i < n_ops; ++i) {
+            if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+                throw InvalidArg {"debug plan: op " + std::to_string(i) + " is not a unitary gate"};
+            }
+            compile_gate(ops[i], n_qubits, is_density != 0, recs);
+        }
+        const uint32_t nvec = is_density ? 2 * n_qubits : n_qubits;
+        const int max_k = fusion_max_k_for(precision, nvec, lds_bytes ? lds_bytes : device_lds_per_block());
+        size_t np = 0, j = 0;
+        if (max_k < register_qubits(precision)) {
+            for (size_t r = 0; r < recs.size(); ++r, ++np) {
+                order[j] = static_cast<uint32_t>(r);
+                pass_of[j++] = static_cast<uint32_t>(np);
+                pass_kind[np] = 2;
+                pass_cost[np] = gate_cost(recs[r]);
+            }
+        }
+        else {
+            const auto passes = plan_passes(recs, nvec, max_k, 5, fusion_max_gates(), register_qubits(precision),
+                                            fusion_max_stages());
+            for (const auto& p : passes) {
+                if (!p.sweep) {
+                    TileProgram tp;
+                    lower_tile_pass(p, precision, nvec, tp);   // validates the layouts and variants
+                }
+                double cost = 0.0;
+                for (const uint32_t r : p.src) {
+                    order[j] = r;
+                    pass_of[j++] = static_cast<uint32_t>(np);
+                    cost += gate_cost(recs[r]);
+                }
+                pass_kind[np] = p.sweep ? 2 : (p.c >= p.k ? 0 : 1);
+                pass_cost[np] = cost;
+                ++np;
+            }
+        }
+        if (j != recs.size()) {
+            throw InvalidArg {"debug plan: the plan does not apply every gate exactly once"};
+        }
+        *n_passes = np;
     });
 }
 

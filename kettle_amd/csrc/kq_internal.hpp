@@ -84,7 +84,62 @@ struct TilePass
     TileStage io_layout {};      // layout used for HBM I/O when not direct
     bool sweep = false;          // single gate: run the plain sweep kernel on `single`
     GateRec single {};
+    std::vector<uint32_t> src;   // input record index of each applied gate, in execution order
 };
+
+// ---- lowered tile program (what the tile kernel reads) ----------------------------------------
+// Gate variants of the tile kernel.  A gate whose target is register position P (0..3) runs the
+// straight-line code of variant (kind, P, C), C = 0 for no register control or 1 + the register
+// position of its control; a diagonal gate whose target bit is a thread or global bit scales
+// every slot (or the slots whose register control is set) by one factor per lane: variant
+// TV_UNIFORM_BASE + C.
+enum TileKind : uint32_t { TK_DENSE = 0, TK_REAL, TK_AXIS, TK_DIAG, TK_DIAG1, TK_SWAP, TK_COUNT };
+constexpr uint32_t tile_code(uint32_t kind, uint32_t pos, uint32_t ctrl) { return (kind * 4 + pos) * 5 + ctrl; }
+constexpr uint32_t TV_UNIFORM_BASE = TK_COUNT * 4 * 5;
+
+struct TileGate
+{
+    uint32_t code;
+    uint32_t lane_ctrl;      // thread-index bits that must all be 1 (controls on thread bits)
+    uint32_t lane_sel;       // uniform-diagonal: the thread-index bit selecting d1 (0: see glb_sel)
+    uint32_t pad;
+    uint64_t glb_ctrl;       // amplitude-index bits outside the tile that must all be 1
+    uint64_t glb_sel;        // uniform-diagonal: the index bit outside the tile selecting d1
+    double m[8];             // coefficients: 8 doubles (complex128) or 8 floats first (complex64)
+};
+
+// Where a register layout puts the tile: slot r of thread t holds local index S_r | T_t.  Both
+// the LDS swizzle and the global index are XOR-linear in the local index, so the address of a
+// slot is (thread part) ^ (slot part) with the slot parts uniform.
+struct TileLayout
+{
+    uint32_t lds_slot[16];   // swizzled LDS byte offset of S_r
+    uint32_t lds_thr[8];     // swizzled LDS byte offset of thread bit j
+    uint64_t hbm_slot[16];   // amplitude offset of S_r in the state vector
+    uint64_t hbm_thr[8];     // amplitude offset of thread bit j
+};
+
+struct TileStep
+{
+    uint32_t g_begin;
+    uint32_t g_count;
+    uint32_t layout;         // index into the pass's layouts (0 = load layout)
+    uint32_t pad;
+};
+
+struct TileProgram
+{
+    int k = 0;
+    int n_thr_bits = 0;
+    uint32_t out_layout = 0;
+    std::vector<uint8_t> freepos;     // global qubits outside the tile, ascending
+    std::vector<TileGate> gates;
+    std::vector<TileStep> steps;
+    std::vector<TileLayout> layouts;
+};
+
+// lower a planned pass into the tile kernel's program (coefficients rounded to the precision)
+void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out);
 
 struct PlanStats
 {
@@ -160,9 +215,10 @@ int fusion_max_stages();
 int register_qubits(int precision);
 PlanStats& last_stats();
 int fusion_max_k(int precision, uint32_t nvec);
+int fusion_max_k_for(int precision, uint32_t nvec, size_t lds_bytes);
 // LDS bytes one workgroup may use on the current device (queried at initialisation)
 size_t& device_lds_per_block();
-constexpr size_t TILE_STATIC_LDS = 8192;   // the tile kernel's high-offset table
+constexpr size_t TILE_STATIC_LDS = 0;      // static LDS of the tile kernel besides the tile
 int fusion_max_gates();
 void set_fusion(int max_k, int max_gates, int max_stages, double max_cost);
 double fusion_max_cost();
@@ -172,7 +228,9 @@ int fusion_setting_k();
 
 // ---- kernel launchers (kq_kernels.hip) --------------------------------------------------------
 void launch_gate_sweep(kq_state* s, const GateRec& g);
-void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates, const void* d_stages);
+// d_prog: the uploaded [gates][steps][layouts] of `prog`
+void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
+                      const void* d_layouts);
 void launch_set_basis(kq_state* s, uint64_t index);
 // weights: SV -> (sum |a|^2 over bit q = 0, = 1); DM -> block Frobenius norms (kettle's quirk)
 void launch_qubit_weights(kq_state* s, uint32_t q, double out[2]);

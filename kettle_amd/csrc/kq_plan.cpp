@@ -141,14 +141,16 @@ int fusion_max_gates() { return g_max_gates; }
 
 // Tile size: 2^k amplitudes.  k = 12 gives 256 threads of 16 register amplitudes each and a
 // 64 KiB (complex128) / 32 KiB (complex64) LDS tile for the stage remaps.
-int fusion_max_k(int precision, uint32_t nvec)
+int fusion_max_k(int precision, uint32_t nvec) { return fusion_max_k_for(precision, nvec, device_lds_per_block()); }
+
+int fusion_max_k_for(int precision, uint32_t nvec, size_t lds_bytes)
 {
     int k = g_max_k;
     if (k < 0) {
         k = 12;
     }
     const size_t ab = precision == KQ_C64 ? 8 : 16;
-    while (k > 1 && (size_t {1} << k) * ab + TILE_STATIC_LDS > device_lds_per_block()) {
+    while (k > 1 && (size_t {1} << k) * ab + TILE_STATIC_LDS > lds_bytes) {
         --k;
     }
     return std::min<int>(k, static_cast<int>(nvec));
@@ -314,6 +316,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
             // a lone diagonal gate touches only a quarter / half of the state: plain sweep
             p.sweep = true;
             p.single = gates[taken[0]];
+            p.src.push_back(static_cast<uint32_t>(taken[0]));
             passes.push_back(std::move(p));
             remaining = std::move(deferred);
             continue;
@@ -381,6 +384,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
                 g.target = t_enc;
                 g.pad = 0;
                 p.gates.push_back(g);
+                p.src.push_back(static_cast<uint32_t>(taken[gi]));
                 ++gi;
             }
             st.g_count = static_cast<uint32_t>(p.gates.size()) - st.g_begin;
@@ -402,6 +406,209 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
         remaining = std::move(deferred);
     }
     return passes;
+}
+
+namespace
+{
+
+uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
+
+TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab)
+{
+    TileLayout L {};
+    const auto hbm = [&](uint32_t local) {
+        uint64_t off = 0;
+        for (int j = 0; j < p.k; ++j) {
+            if ((local >> j) & 1u) {
+                off |= uint64_t {1} << p.qubits[static_cast<size_t>(j)];
+            }
+        }
+        return off;
+    };
+    for (uint32_t r = 0; r < 16; ++r) {
+        uint32_t s_r = 0;
+        for (int q = 0; q < p.rq; ++q) {
+            if ((r >> q) & 1u) {
+                s_r |= 1u << st.reg[q];
+            }
+        }
+        L.lds_slot[r] = static_cast<uint32_t>(lds_swizzle(s_r) * ab);
+        L.hbm_slot[r] = hbm(s_r);
+    }
+    int j = 0;
+    for (int b = 0; b < p.k; ++b) {
+        if ((st.thr_mask >> b) & 1u) {
+            L.lds_thr[j] = static_cast<uint32_t>(lds_swizzle(1u << b) * ab);
+            L.hbm_thr[j] = hbm(1u << b);
+            ++j;
+        }
+    }
+    return L;
+}
+
+bool same_regs(const TileStage& a, const TileStage& b, int rq)
+{
+    for (int q = 0; q < rq; ++q) {
+        if (a.reg[q] != b.reg[q]) {
+            return false;
+        }
+    }
+    return true;
+}
+
+uint32_t tile_kind(uint32_t kind)
+{
+    switch (kind) {
+        case GK_DENSE: return TK_DENSE;
+        case GK_REAL: return TK_REAL;
+        case GK_AXIS: return TK_AXIS;
+        case GK_DIAG: return TK_DIAG;
+        case GK_DIAG1: return TK_DIAG1;
+        case GK_SWAP: return TK_SWAP;
+        default: break;
+    }
+    throw InvalidArg {"tile lowering: unknown gate kind"};
+}
+
+}  // namespace
+
+// Turn the planner's bit encoding (enc = idx << 2 | src; src 0 register position, 1 thread bit,
+// 2 global qubit) into kernel variants, lane / workgroup conditions and per-layout offset tables.
+void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out)
+{
+    if (p.rq != 4 || p.k < 4 || p.k > 12 || p.stages.empty()) {
+        throw InvalidArg {"tile lowering: unsupported tile shape"};
+    }
+    const size_t ab = amp_bytes(precision);
+    out = TileProgram {};
+    out.k = p.k;
+    out.n_thr_bits = p.k - p.rq;
+    uint64_t qmask = 0;
+    for (const int q : p.qubits) {
+        qmask |= uint64_t {1} << q;
+    }
+    for (uint32_t q = 0; q < nvec; ++q) {
+        if (!((qmask >> q) & 1ull)) {
+            out.freepos.push_back(static_cast<uint8_t>(q));
+        }
+    }
+    // layouts: [load][one per distinct consecutive stage layout][store]
+    const TileStage& first = p.load_direct ? p.stages.front() : p.io_layout;
+    out.layouts.push_back(make_layout(first, p, ab));
+    const TileStage* cur = &first;
+    uint32_t cur_idx = 0;
+    for (const TileStage& st : p.stages) {
+        if (!same_regs(st, *cur, p.rq)) {
+            out.layouts.push_back(make_layout(st, p, ab));
+            cur = &st;
+            cur_idx = static_cast<uint32_t>(out.layouts.size() - 1);
+        }
+        TileStep step {};
+        step.g_begin = st.g_begin;
+        step.g_count = st.g_count;
+        step.layout = cur_idx;
+        out.steps.push_back(step);
+    }
+    if (p.store_direct || same_regs(p.io_layout, *cur, p.rq)) {
+        out.out_layout = cur_idx;
+    }
+    else {
+        out.layouts.push_back(make_layout(p.io_layout, p, ab));
+        out.out_layout = static_cast<uint32_t>(out.layouts.size() - 1);
+    }
+    // every layout must address each amplitude of the state exactly once: the slot bits, the
+    // thread bits and the free qubits are disjoint and together cover all nvec index bits
+    const uint64_t all = nvec >= 64 ? ~0ull : ((uint64_t {1} << nvec) - 1);
+    const uint64_t free_bits = all & ~qmask;
+    const uint32_t tile_bytes = static_cast<uint32_t>((size_t {1} << p.k) * ab);
+    for (const TileLayout& L : out.layouts) {
+        uint64_t slot_bits = 0, thr_bits = 0;
+        for (int q = 0; q < p.rq; ++q) {
+            slot_bits |= L.hbm_slot[1u << q];
+        }
+        for (int j = 0; j < out.n_thr_bits; ++j) {
+            if (std::popcount(L.hbm_thr[j]) != 1 || (thr_bits & L.hbm_thr[j])) {
+                throw InvalidArg {"tile lowering: thread bits overlap"};
+            }
+            thr_bits |= L.hbm_thr[j];
+        }
+        if ((slot_bits & thr_bits) || ((slot_bits | thr_bits) & free_bits) || ((slot_bits | thr_bits | free_bits) != all)
+            || std::popcount(slot_bits) != p.rq) {
+            throw InvalidArg {"tile lowering: layout does not cover the state exactly once"};
+        }
+        for (int r = 0; r < 16; ++r) {
+            if (L.lds_slot[r] >= tile_bytes) {
+                throw InvalidArg {"tile lowering: LDS offset outside the tile"};
+            }
+        }
+        for (int j = 0; j < out.n_thr_bits; ++j) {
+            if (L.lds_thr[j] >= tile_bytes) {
+                throw InvalidArg {"tile lowering: LDS offset outside the tile"};
+            }
+        }
+    }
+    // gates
+    out.gates.reserve(p.gates.size());
+    for (const GateRec& g : p.gates) {
+        TileGate t {};
+        uint32_t creg = 0;
+        if (g.control >= 0) {
+            const uint32_t src = static_cast<uint32_t>(g.control) & 3u, idx = static_cast<uint32_t>(g.control) >> 2;
+            if (src == 0) {
+                creg = idx + 1;
+            }
+            else if (src == 1) {
+                t.lane_ctrl |= 1u << idx;
+            }
+            else {
+                t.glb_ctrl |= uint64_t {1} << idx;
+            }
+        }
+        const uint32_t tsrc = g.target & 3u, tidx = g.target >> 2;
+        double m[8];
+        std::memcpy(m, g.m, sizeof m);
+        if (tsrc == 0) {
+            if (creg == tidx + 1) {
+                throw InvalidArg {"tile lowering: control equals target"};
+            }
+            t.code = tile_code(tile_kind(g.kind), tidx, creg);
+        }
+        else {
+            if (!is_diag(g)) {
+                throw InvalidArg {"tile lowering: non-diagonal gate outside the registers"};
+            }
+            t.code = TV_UNIFORM_BASE + creg;
+            if (g.kind == GK_DIAG1) {
+                // d0 = 1: the gate acts only where the target bit is set, with the factor d1
+                if (tsrc == 1) {
+                    t.lane_ctrl |= 1u << tidx;
+                }
+                else {
+                    t.glb_ctrl |= uint64_t {1} << tidx;
+                }
+                m[0] = m[6];
+                m[1] = m[7];
+            }
+            else if (tsrc == 1) {
+                t.lane_sel = 1u << tidx;
+            }
+            else {
+                t.glb_sel = uint64_t {1} << tidx;
+            }
+        }
+        if (precision == KQ_C64) {
+            float f[8];
+            for (int i = 0; i < 8; ++i) {
+                f[i] = static_cast<float>(m[i]);
+            }
+            std::memset(t.m, 0, sizeof t.m);
+            std::memcpy(t.m, f, sizeof f);
+        }
+        else {
+            std::memcpy(t.m, m, sizeof m);
+        }
+        out.gates.push_back(t);
+    }
 }
 
 }  // namespace kq

@@ -1,21 +1,28 @@
 // The fused tile pass: the hot kernel of simulate() for gfx950.
 //
-// One workgroup owns one tile: the 2^k amplitudes that share the values of every qubit outside
-// the pass's tile set.  The tile is read from HBM once and written once; all gates of the pass
-// are applied in between without touching HBM.
+// One workgroup owns one tile: the 2^k amplitudes that share the values of every qubit outside the
+// pass's tile set.  The tile is read from HBM once and written once; all gates of the pass are
+// applied in between without touching HBM.
 //
-// Register blocking.  Each thread holds 2^RQ = 16 amplitudes in VGPRs.  Which RQ bits of the local tile index live in a
-// thread's registers is the "layout"; the remaining bits form the thread index.  A pass is a
-// sequence of stages; within a stage every non-diagonal gate acts on a register bit, so it is
-// plain register arithmetic on pairs (v[r], v[r | 1 << p]).  Diagonal gates and controls may sit
-// on any bit: register bits are known per register slot, thread bits per thread, and bits outside
-// the tile are uniform over the workgroup.  Between stages whose layouts differ the tile goes
-// through LDS once (write, barrier, read); LDS addresses are XOR-swizzled on the low four bits so
-// lanes whose local indices differ by 16 or 256 land on different banks.
+// Register blocking.  Each thread holds 16 amplitudes in VGPRs: slot r holds local index
+// S_r | T_t, where the four "register bits" of the local index are spread over r and the other
+// k - 4 bits come from the thread index.  A pass is a sequence of steps; within a step every
+// non-diagonal gate acts on a register bit, so it is straight-line register arithmetic on the
+// pairs (v[r], v[r | 1 << P]).  Between steps whose register bits differ the tile goes through LDS
+// once (write, barrier, read).
 //
-// HBM access: local bits 0..c-1 are the contiguous global qubits 0..c-1, and the I/O layout keeps
-// local bits 0..2 on lanes, so every wavefront load or store covers runs of >= 8 consecutive
-// amplitudes (>= 128 B for complex128).
+// Gate variants.  The host lowers every gate to a variant code (kind, register position P of the
+// target, register position of the control or none) plus lane / workgroup conditions for controls
+// on thread bits or outside the tile; each variant is one straight-line block whose slot pattern is
+// fixed at compile time, so no per-slot branch exists and the amplitudes stay in the same VGPRs.
+// Diagonal gates whose target is a thread or outside bit scale the slots by one factor per lane.
+//
+// Addressing.  The LDS swizzle (l ^ (((l >> 4) ^ (l >> 8)) & 15), which spreads lanes whose local
+// indices differ by 16 or 256 over different banks) and the local -> global index map are both
+// XOR-linear in the local index, so every address is (per-thread part) ^ (per-slot part), the
+// per-slot parts being uniform values the host precomputes per layout.  HBM I/O uses a layout whose
+// thread bits are the lowest local bits, so each wavefront access covers 64 consecutive
+// amplitudes.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -27,8 +34,7 @@ namespace kq
 namespace
 {
 
-constexpr int TILE_BLOCK_MAX = 256;
-constexpr int MAX_TILE_Q = 16;
+constexpr int TILE_BLOCK = 256;
 constexpr int MAX_FREE_Q = 48;
 
 template <typename R> struct AmpT;
@@ -36,44 +42,14 @@ template <> struct AmpT<double> { using T = double2; };
 template <> struct AmpT<float> { using T = float2; };
 template <typename R> using Amp = typename AmpT<R>::T;
 
-struct DevGate
-{
-    uint32_t kind;
-    uint32_t target;
-    int32_t control;
-    uint32_t pad;
-    double m[8];
-};
-
-struct DevStage
-{
-    uint8_t reg[8];
-    uint32_t thr_mask;
-    uint32_t g_begin;
-    uint32_t g_count;
-    uint32_t pad;
-};
-
 struct TileArgs
 {
-    int k;
-    int c;
     int nfree;
-    int n_stages;
-    int load_direct;
-    int store_direct;
-    uint8_t qpos[MAX_TILE_Q];
+    int n_thr_bits;
+    int n_steps;
+    uint32_t out_layout;
     uint8_t freepos[MAX_FREE_Q];
-    DevStage io;
 };
-
-static_assert(sizeof(DevGate) == sizeof(GateRec), "device gate record layout");
-static_assert(sizeof(DevStage) == sizeof(TileStage), "device stage record layout");
-
-__device__ __forceinline__ uint32_t swz(uint32_t l)
-{
-    return l ^ (((l >> 4) ^ (l >> 8)) & 15u);
-}
 
 template <typename R>
 struct Cx
@@ -86,53 +62,6 @@ __device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
 {
     return {a.re * br - a.im * bi, a.re * bi + a.im * br};
 }
-
-template <typename R, int RQ>
-struct TileRegs
-{
-    Cx<R> v[1 << RQ];
-    uint32_t reg_bit[RQ];   // local bit of register position p
-    uint32_t tpart;         // local-index bits contributed by the thread index
-
-    __device__ __forceinline__ void set_layout(const DevStage& st, uint32_t tid)
-    {
-#pragma unroll
-        for (int p = 0; p < RQ; ++p) {
-            reg_bit[p] = st.reg[p];
-        }
-        uint32_t t = 0, src = tid, mask = st.thr_mask;
-        while (mask) {
-            const uint32_t b = __ffs(mask) - 1;
-            t |= (src & 1u) << b;
-            src >>= 1;
-            mask &= mask - 1;
-        }
-        tpart = t;
-    }
-
-    __device__ __forceinline__ bool same_layout(const DevStage& st) const
-    {
-        bool same = true;
-#pragma unroll
-        for (int p = 0; p < RQ; ++p) {
-            same = same && (reg_bit[p] == st.reg[p]);
-        }
-        return same;
-    }
-
-    template <int r>
-    __device__ __forceinline__ uint32_t local() const
-    {
-        uint32_t l = tpart;
-#pragma unroll
-        for (int p = 0; p < RQ; ++p) {
-            if ((r >> p) & 1) {
-                l |= 1u << reg_bit[p];
-            }
-        }
-        return l;
-    }
-};
 
 // compile-time loop over register slots
 template <int I, int N>
@@ -154,92 +83,122 @@ struct Unroll<N, N>
     }
 };
 
-// One gate on register position P: pairs (v[r], v[r | 1 << P]).  KIND selects the arithmetic:
-// the zero entries of real, axis-rotation and diagonal matrices are skipped at compile time.
-// A control on a register bit (csrc == 0) is tested per slot with a uniform (scalar) branch.
-template <typename R>
-struct Coef
+// Variant (KIND, P, C): pairs (v[r], v[r | 1 << P]) for every r with bit P clear and, if C > 0,
+// bit C - 1 set.  The zero entries of real, axis-rotation and diagonal matrices are skipped.
+template <typename R, int KIND, int P, int C>
+__device__ __forceinline__ void apply(Cx<R> (&v)[16], const R* __restrict__ m)
 {
-    R m[8];
-};
-
-template <typename R, int RQ, int P, int KIND>
-__device__ __forceinline__ void reg_gate(TileRegs<R, RQ>& t, const Coef<R> c, int csrc, uint32_t cidx)
-{
-    const R* m = c.m;
-    Unroll<0, (1 << RQ)>::run([&]<int r>() __attribute__((always_inline)) {
-        if constexpr (((r >> P) & 1) == 0) {
-            constexpr int r1 = r | (1 << P);
-            if (csrc != 0 || ((r >> cidx) & 1u)) {
-                const Cx<R> a = t.v[r];
-                const Cx<R> b = t.v[r1];
-                if constexpr (KIND == GK_SWAP) {
-                    t.v[r] = b;
-                    t.v[r1] = a;
+    if constexpr (C == P + 1) {
+        return;   // never emitted by the host
+    }
+    else {
+        const R m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7];
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1)) {
+                constexpr int r1 = r | (1 << P);
+                const Cx<R> a = v[r];
+                const Cx<R> b = v[r1];
+                if constexpr (KIND == TK_SWAP) {
+                    v[r] = b;
+                    v[r1] = a;
                 }
-                else if constexpr (KIND == GK_REAL) {
-                    t.v[r] = {a.re * m[0] + b.re * m[2], a.im * m[0] + b.im * m[2]};
-                    t.v[r1] = {a.re * m[4] + b.re * m[6], a.im * m[4] + b.im * m[6]};
+                else if constexpr (KIND == TK_REAL) {
+                    v[r] = {a.re * m0 + b.re * m2, a.im * m0 + b.im * m2};
+                    v[r1] = {a.re * m4 + b.re * m6, a.im * m4 + b.im * m6};
                 }
-                else if constexpr (KIND == GK_AXIS) {
-                    // m00 = m[0], m01 = i m[3], m10 = i m[5], m11 = m[6]
-                    t.v[r] = {a.re * m[0] - b.im * m[3], a.im * m[0] + b.re * m[3]};
-                    t.v[r1] = {b.re * m[6] - a.im * m[5], b.im * m[6] + a.re * m[5]};
+                else if constexpr (KIND == TK_AXIS) {
+                    // m00 = m0, m01 = i m3, m10 = i m5, m11 = m6
+                    v[r] = {a.re * m0 - b.im * m3, a.im * m0 + b.re * m3};
+                    v[r1] = {b.re * m6 - a.im * m5, b.im * m6 + a.re * m5};
                 }
-                else if constexpr (KIND == GK_DIAG1) {
-                    t.v[r1] = cmul(b, m[6], m[7]);
+                else if constexpr (KIND == TK_DIAG1) {
+                    v[r1] = cmul(b, m6, m7);
                 }
-                else if constexpr (KIND == GK_DIAG) {
-                    t.v[r] = cmul(a, m[0], m[1]);
-                    t.v[r1] = cmul(b, m[6], m[7]);
+                else if constexpr (KIND == TK_DIAG) {
+                    v[r] = cmul(a, m0, m1);
+                    v[r1] = cmul(b, m6, m7);
                 }
                 else {
-                    t.v[r] = {a.re * m[0] - a.im * m[1] + b.re * m[2] - b.im * m[3],
-                              a.re * m[1] + a.im * m[0] + b.re * m[3] + b.im * m[2]};
-                    t.v[r1] = {a.re * m[4] - a.im * m[5] + b.re * m[6] - b.im * m[7],
-                               a.re * m[5] + a.im * m[4] + b.re * m[7] + b.im * m[6]};
+                    v[r] = {a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3, a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2};
+                    v[r1] = {a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7, a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6};
                 }
             }
+        });
+    }
+}
+
+// Uniform diagonal: every slot (C = 0) or the slots with register bit C - 1 set, times (fr, fi).
+template <typename R, int C>
+__device__ __forceinline__ void scale(Cx<R> (&v)[16], R fr, R fi)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
+            v[r] = cmul(v[r], fr, fi);
         }
     });
 }
 
-template <typename R, int RQ, int KIND>
-__device__ __forceinline__ void reg_gate_at(TileRegs<R, RQ>& t, const DevGate& g, uint32_t pos, int csrc, uint32_t cidx)
+#define KQ_TILE_CASE(K, P, C) \
+    case tile_code(K, P, C): apply<R, K, P, C>(v, m); break;
+#define KQ_TILE_CASES_C(K, P) \
+    KQ_TILE_CASE(K, P, 0) KQ_TILE_CASE(K, P, 1) KQ_TILE_CASE(K, P, 2) KQ_TILE_CASE(K, P, 3) KQ_TILE_CASE(K, P, 4)
+#define KQ_TILE_CASES_P(K) \
+    KQ_TILE_CASES_C(K, 0) KQ_TILE_CASES_C(K, 1) KQ_TILE_CASES_C(K, 2) KQ_TILE_CASES_C(K, 3)
+
+template <typename R>
+__device__ __forceinline__ void apply_gate(Cx<R> (&v)[16], uint32_t code, const R* __restrict__ m)
 {
-    Coef<R> m;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        m.m[i] = static_cast<R>(g.m[i]);
-    }
-    switch (pos) {
-        case 0: reg_gate<R, RQ, 0, KIND>(t, m, csrc, cidx); break;
-        case 1: reg_gate<R, RQ, 1, KIND>(t, m, csrc, cidx); break;
-        case 2: reg_gate<R, RQ, 2, KIND>(t, m, csrc, cidx); break;
-        case 3: reg_gate<R, RQ, 3, KIND>(t, m, csrc, cidx); break;
-        default:
-            if constexpr (RQ > 4) {
-                reg_gate<R, RQ, (RQ > 4 ? 4 : 0), KIND>(t, m, csrc, cidx);
-            }
-            break;
+    switch (code) {
+        KQ_TILE_CASES_P(TK_DENSE)
+        KQ_TILE_CASES_P(TK_REAL)
+        KQ_TILE_CASES_P(TK_AXIS)
+        KQ_TILE_CASES_P(TK_DIAG)
+        KQ_TILE_CASES_P(TK_DIAG1)
+        KQ_TILE_CASES_P(TK_SWAP)
+        default: break;
     }
 }
 
-template <typename R, int RQ>
-__global__ __launch_bounds__(TILE_BLOCK_MAX) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_tile_reg(Amp<R>* __restrict__ a, TileArgs d,
-                                                             const DevStage* __restrict__ stages,
-                                                             const DevGate* __restrict__ gates)
+template <typename R>
+__device__ __forceinline__ void apply_uniform(Cx<R> (&v)[16], uint32_t c, R fr, R fi)
 {
-    constexpr int RA = 1 << RQ;
-    extern __shared__ unsigned char smem_raw[];
-    Amp<R>* tile = reinterpret_cast<Amp<R>*>(smem_raw);
-    __shared__ uint64_t hoff[1 << 10];
+    switch (c) {
+        case 0: scale<R, 0>(v, fr, fi); break;
+        case 1: scale<R, 1>(v, fr, fi); break;
+        case 2: scale<R, 2>(v, fr, fi); break;
+        case 3: scale<R, 3>(v, fr, fi); break;
+        case 4: scale<R, 4>(v, fr, fi); break;
+        default: break;
+    }
+}
 
+__device__ __forceinline__ uint32_t lds_thread_part(const TileLayout& L, uint32_t tid, int nbits)
+{
+    uint32_t x = 0;
+    for (int j = 0; j < nbits; ++j) {
+        x ^= ((tid >> j) & 1u) ? L.lds_thr[j] : 0u;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t hbm_thread_part(const TileLayout& L, uint32_t tid, int nbits)
+{
+    uint64_t x = 0;
+    for (int j = 0; j < nbits; ++j) {
+        x |= ((tid >> j) & 1u) ? L.hbm_thr[j] : 0ull;
+    }
+    return x;
+}
+
+template <typename R>
+__global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, const TileArgs d,
+                                                     const TileStep* __restrict__ steps,
+                                                     const TileLayout* __restrict__ layouts,
+                                                     const TileGate* __restrict__ gates)
+{
+    extern __shared__ unsigned char smem_raw[];
     const uint32_t tid = threadIdx.x;
-    const int k = d.k;
-    const int c = d.c;
-    const int nh = k - c;
-    const uint32_t lowmask = (1u << c) - 1;
+    const int tb = d.n_thr_bits;
 
     uint64_t base = 0;
     {
@@ -248,107 +207,89 @@ __global__ __launch_bounds__(TILE_BLOCK_MAX) __attribute__((amdgpu_waves_per_eu(
             base |= ((b >> j) & 1ull) << d.freepos[j];
         }
     }
-    for (uint32_t h = tid; h < (1u << nh); h += blockDim.x) {
-        uint64_t off = 0;
-        for (int j = 0; j < nh; ++j) {
-            off |= static_cast<uint64_t>((h >> j) & 1u) << d.qpos[c + j];
-        }
-        hoff[h] = off;
+
+    Cx<R> v[16];
+    {
+        const TileLayout& L = layouts[0];
+        const Amp<R>* p = a + (base | hbm_thread_part(L, tid, tb));
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            const Amp<R> x = p[L.hbm_slot[r]];
+            v[r] = {x.x, x.y};
+        });
     }
-    __syncthreads();
 
-    TileRegs<R, RQ> t;
-    const auto gidx = [&](uint32_t l) { return base | (l & lowmask) | hoff[l >> c]; };
-
-    const DevStage s0 = stages[0];
-    t.set_layout(d.load_direct ? s0 : d.io, tid);
-    Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
-        const Amp<R> x = a[gidx(t.template local<r>())];
-        t.v[r] = {x.x, x.y};
-    });
-
-    const auto remap = [&](const DevStage& to) {
+    uint32_t cur = 0;
+    const auto remap = [&](uint32_t to) {
+        const TileLayout& A = layouts[cur];
+        const TileLayout& B = layouts[to];
+        const uint32_t xa = lds_thread_part(A, tid, tb);
         __syncthreads();
-        Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
             Amp<R> x;
-            x.x = t.v[r].re;
-            x.y = t.v[r].im;
-            tile[swz(t.template local<r>())] = x;
+            x.x = v[r].re;
+            x.y = v[r].im;
+            *reinterpret_cast<Amp<R>*>(smem_raw + (xa ^ A.lds_slot[r])) = x;
         });
+        const uint32_t xb = lds_thread_part(B, tid, tb);
         __syncthreads();
-        t.set_layout(to, tid);
-        Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
-            const Amp<R> x = tile[swz(t.template local<r>())];
-            t.v[r] = {x.x, x.y};
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            const Amp<R> x = *reinterpret_cast<const Amp<R>*>(smem_raw + (xb ^ B.lds_slot[r]));
+            v[r] = {x.x, x.y};
         });
+        cur = to;
     };
 
-    for (int si = 0; si < d.n_stages; ++si) {
-        const DevStage st = stages[si];
-        if (!t.same_layout(st)) {
-            remap(st);
+    for (int si = 0; si < d.n_steps; ++si) {
+        const TileStep st = steps[si];
+        if (st.layout != cur) {
+            remap(st.layout);
         }
-        for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
-            const DevGate& g = gates[gi];
-            const uint32_t kind = g.kind;
-            const int tsrc = static_cast<int>(g.target & 3u);
-            const uint32_t tidx = g.target >> 2;
-            const bool has_c = g.control >= 0;
-            const int csrc = has_c ? static_cast<int>(static_cast<uint32_t>(g.control) & 3u) : 3;
-            const uint32_t cidx = has_c ? (static_cast<uint32_t>(g.control) >> 2) : 0u;
-            // controls on thread bits or outside the tile are uniform per thread / per tile
-            if (csrc == 1 && !((tid >> cidx) & 1u)) {
-                continue;
+        const uint32_t g_end = st.g_begin + st.g_count;
+        for (uint32_t gi = st.g_begin; gi < g_end; ++gi) {
+            const TileGate& g = gates[gi];
+            if ((base & g.glb_ctrl) != g.glb_ctrl) {
+                continue;   // a control (or a d0 = 1 diagonal's target) outside the tile is 0
             }
-            if (csrc == 2 && !((base >> cidx) & 1ull)) {
-                continue;
+            if ((tid & g.lane_ctrl) != g.lane_ctrl) {
+                continue;   // a control on a thread bit is 0 for this lane
             }
-            if ((kind == GK_DIAG || kind == GK_DIAG1) && tsrc != 0) {
-                // the target bit is uniform over the thread: one factor for every slot
-                const uint32_t bit = tsrc == 1 ? ((tid >> tidx) & 1u) : static_cast<uint32_t>((base >> tidx) & 1ull);
-                if (!bit && kind == GK_DIAG1) {
-                    continue;
-                }
-                const R fr = static_cast<R>(bit ? g.m[6] : g.m[0]);
-                const R fi = static_cast<R>(bit ? g.m[7] : g.m[1]);
-                Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
-                    if (csrc != 0 || ((r >> cidx) & 1u)) {
-                        t.v[r] = cmul(t.v[r], fr, fi);
-                    }
-                });
-                continue;
+            const R* m = reinterpret_cast<const R*>(g.m);
+            const uint32_t code = g.code;
+            if (code >= TV_UNIFORM_BASE) {
+                const bool sel = g.lane_sel ? ((tid & g.lane_sel) != 0) : ((base & g.glb_sel) != 0);
+                const R fr = sel ? m[6] : m[0];
+                const R fi = sel ? m[7] : m[1];
+                apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
             }
-            // the target is register position tidx
-            switch (kind) {
-                case GK_SWAP: reg_gate_at<R, RQ, GK_SWAP>(t, g, tidx, csrc, cidx); break;
-                case GK_REAL: reg_gate_at<R, RQ, GK_REAL>(t, g, tidx, csrc, cidx); break;
-                case GK_AXIS: reg_gate_at<R, RQ, GK_AXIS>(t, g, tidx, csrc, cidx); break;
-                case GK_DIAG1: reg_gate_at<R, RQ, GK_DIAG1>(t, g, tidx, csrc, cidx); break;
-                case GK_DIAG: reg_gate_at<R, RQ, GK_DIAG>(t, g, tidx, csrc, cidx); break;
-                default: reg_gate_at<R, RQ, GK_DENSE>(t, g, tidx, csrc, cidx); break;
+            else {
+                apply_gate<R>(v, code, m);
             }
         }
     }
 
-    if (!d.store_direct && !t.same_layout(d.io)) {
-        remap(d.io);
+    if (d.out_layout != cur) {
+        remap(d.out_layout);
     }
-    Unroll<0, RA>::run([&]<int r>() __attribute__((always_inline)) {
-        Amp<R> x;
-        x.x = t.v[r].re;
-        x.y = t.v[r].im;
-        a[gidx(t.template local<r>())] = x;
-    });
+    {
+        const TileLayout& L = layouts[cur];
+        Amp<R>* p = a + (base | hbm_thread_part(L, tid, tb));
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            Amp<R> x;
+            x.x = v[r].re;
+            x.y = v[r].im;
+            p[L.hbm_slot[r]] = x;
+        });
+    }
 }
 
-template <typename R, int RQ>
+template <typename R>
 void set_lds_attr()
 {
     static bool done = false;
     if (!done) {
         const int maxdyn = static_cast<int>(device_lds_per_block() - TILE_STATIC_LDS);
-        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_reg<R, RQ>),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
+        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   maxdyn);
         (void) hipGetLastError();
         done = true;
     }
@@ -356,58 +297,48 @@ void set_lds_attr()
 
 }  // namespace
 
-void launch_tile_pass(kq_state* s, const TilePass& p, const void* d_gates, const void* d_stages)
+void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
+                      const void* d_layouts)
 {
-    const int rq_kernel = register_qubits(s->precision);
-    if (p.rq != rq_kernel || p.k < rq_kernel) {
-        throw InvalidArg {"tile pass: register bits do not match the kernel"};
+    if (prog.k < 4 || prog.k > 12 || prog.n_thr_bits != prog.k - 4 || prog.steps.empty() || prog.layouts.empty()
+        || prog.out_layout >= prog.layouts.size()) {
+        throw InvalidArg {"tile pass: bad program shape"};
     }
-    if (p.k > MAX_TILE_Q || p.k - p.c > 10 || p.stages.empty()) {
-        throw InvalidArg {"tile pass: bad tile shape"};
+    if (prog.freepos.size() > static_cast<size_t>(MAX_FREE_Q) || prog.freepos.size() + prog.k != s->nvec) {
+        throw InvalidArg {"tile pass: free qubits do not complete the state"};
     }
-    TileArgs d {};
-    d.k = p.k;
-    d.c = p.c;
-    d.n_stages = static_cast<int>(p.stages.size());
-    d.load_direct = p.load_direct ? 1 : 0;
-    d.store_direct = p.store_direct ? 1 : 0;
-    std::memcpy(&d.io, &p.io_layout, sizeof d.io);
-    uint64_t qmask = 0;
-    for (int j = 0; j < p.k; ++j) {
-        d.qpos[j] = static_cast<uint8_t>(p.qubits[j]);
-        qmask |= uint64_t {1} << p.qubits[j];
-    }
-    int nf = 0;
-    for (uint32_t q = 0; q < s->nvec; ++q) {
-        if (!((qmask >> q) & 1ull)) {
-            if (nf >= MAX_FREE_Q) {
-                throw InvalidArg {"tile pass: too many free qubits"};
-            }
-            d.freepos[nf++] = static_cast<uint8_t>(q);
+    for (const TileStep& st : prog.steps) {
+        if (st.layout >= prog.layouts.size() || st.g_begin + st.g_count > prog.gates.size()) {
+            throw InvalidArg {"tile pass: step out of range"};
         }
     }
-    d.nfree = nf;
-    const uint64_t blocks = uint64_t {1} << nf;
+    TileArgs d {};
+    d.nfree = static_cast<int>(prog.freepos.size());
+    d.n_thr_bits = prog.n_thr_bits;
+    d.n_steps = static_cast<int>(prog.steps.size());
+    d.out_layout = prog.out_layout;
+    for (size_t j = 0; j < prog.freepos.size(); ++j) {
+        d.freepos[j] = prog.freepos[j];
+    }
+    const uint64_t blocks = uint64_t {1} << d.nfree;
     if (blocks > 0x7fffffffull) {
         throw InvalidArg {"tile pass: grid too large"};
     }
-    const size_t lds = (size_t {1} << p.k) * amp_bytes(s->precision);
+    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
     if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
         throw InvalidArg {"tile pass: tile does not fit the workgroup's LDS"};
     }
-    const unsigned threads = 1u << (p.k - p.rq);
-    if (threads > TILE_BLOCK_MAX) {
-        throw InvalidArg {"tile pass: too many threads per tile"};
-    }
-    const auto* g = static_cast<const DevGate*>(d_gates);
-    const auto* st = static_cast<const DevStage*>(d_stages);
+    const unsigned threads = 1u << prog.n_thr_bits;
+    const auto* g = static_cast<const TileGate*>(d_gates);
+    const auto* st = static_cast<const TileStep*>(d_steps);
+    const auto* ly = static_cast<const TileLayout*>(d_layouts);
     if (s->precision == KQ_C64) {
-        set_lds_attr<float, 4>();
-        k_tile_reg<float, 4><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<float2*>(s->amps), d, st, g);
+        set_lds_attr<float>();
+        k_tile<float><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<float2*>(s->amps), d, st, ly, g);
     }
     else {
-        set_lds_attr<double, 4>();
-        k_tile_reg<double, 4><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<double2*>(s->amps), d, st, g);
+        set_lds_attr<double>();
+        k_tile<double><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<double2*>(s->amps), d, st, ly, g);
     }
     KQ_HIP_CHECK(hipGetLastError());
 }

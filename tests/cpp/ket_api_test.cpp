@@ -312,7 +312,13 @@ int main()
         REQUIRE(std::abs(dm.trace() - 1.0) < 1e-14);
         const auto reduced = ket::partial_trace(dm, {2});
         REQUIRE(reduced.n_qubits() == 2);
-        REQUIRE(std::abs(reduced.matrix()(0, 3) - 0.5) < 1e-14);
+        // tracing out qubit 2 (bit 2 of the index): reduced(r, k) = sum_b psi[r + 4b] conj(psi[k + 4b])
+        for (std::size_t r = 0; r < 4; ++r) {
+            for (std::size_t k = 0; k < 4; ++k) {
+                const auto expected = s[r] * std::conj(s[k]) + s[r + 4] * std::conj(s[k + 4]);
+                REQUIRE(std::abs(reduced.matrix()(r, k) - expected) < 1e-14);
+            }
+        }
         auto from_sv = ket::statevector_to_density_matrix(s);
         REQUIRE(ket::almost_eq(from_sv, dm));
     });

@@ -1,0 +1,141 @@
+"""The fusion planner on the CPU (kq_debug_plan: host-only, no device).
+
+The planner reorders gates (a gate may run before earlier gates it commutes with) and groups them
+into passes.  These tests replay the planned order on the oracle and require the same state as the
+circuit order, for statevectors and density matrices, over tile sizes, stage limits and cost caps;
+kq_debug_plan also lowers every pass and checks that each layout addresses every amplitude exactly
+once.  Tolerance 1e-12 (reordering commuting gates changes rounding only)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import kettle_amd as ka
+from kettle_amd.circuit import Gate
+from kat_cases import gate_matrix, random_gate_circuit
+from oracle import oracle as orc
+
+CONTROLLED = {g for g in Gate if Gate.CH <= g <= Gate.CP} | {Gate.CU}
+
+
+@pytest.fixture(autouse=True)
+def _fusion_default():
+    yield
+    ka.set_fusion(-1, 4096, 6, 1.0e9)
+
+
+def op_matrix(op) -> np.ndarray:
+    g = Gate(int(op["gate"]))
+    if g in (Gate.U, Gate.CU):
+        u = op["u"]
+        return np.array([[u[0] + 1j * u[1], u[2] + 1j * u[3]], [u[4] + 1j * u[5], u[6] + 1j * u[7]]])
+    base = Gate[g.name[1:]] if g in CONTROLLED else g
+    return gate_matrix(base, float(op["angle"]))
+
+
+def as_u_op(m: np.ndarray, target: int, control: int | None) -> tuple:
+    u = np.array([m[0, 0].real, m[0, 0].imag, m[0, 1].real, m[0, 1].imag,
+                  m[1, 0].real, m[1, 0].imag, m[1, 1].real, m[1, 1].imag])
+    if control is None:
+        return (int(Gate.U), target, 0, -1, 0.0, u)
+    return (int(Gate.CU), control, target, -1, 0.0, u)
+
+
+def records(ops, n: int, density: bool) -> list[tuple]:
+    """Each op as the U / CU records the device runs (two per op for a density matrix)."""
+    out = []
+    for op in ops:
+        g = Gate(int(op["gate"]))
+        ctrl = g in CONTROLLED
+        t = int(op["q1"] if ctrl else op["q0"])
+        c = int(op["q0"]) if ctrl else None
+        m = op_matrix(op)
+        out.append(as_u_op(m, t, c))
+        if density:
+            out.append(as_u_op(np.conj(m), t + n, None if c is None else c + n))
+    return out
+
+
+def to_ops(recs) -> np.ndarray:
+    arr = np.zeros(len(recs), dtype=ka._lib.OP_DTYPE)
+    for i, r in enumerate(recs):
+        arr[i] = r
+    return arr
+
+
+def random_state(nq: int, rng) -> np.ndarray:
+    a = rng.normal(size=1 << nq) + 1j * rng.normal(size=1 << nq)
+    return a / np.linalg.norm(a)
+
+
+def check_plan(circuit: ka.QuantumCircuit, density: bool, seed: int) -> ka.FusionPlan:
+    n = circuit.n_qubits()
+    ops, _, _ = circuit.flatten()
+    plan = ka.debug_plan(ops, n, ka.KQ_C128, density)
+    n_rec = len(ops) * (2 if density else 1)
+    assert sorted(plan.order.tolist()) == list(range(n_rec))
+    assert np.all(np.diff(plan.pass_of.astype(np.int64)) >= 0)
+    recs = records(ops, n, density)
+    nv = 2 * n if density else n
+    start = random_state(nv, np.random.default_rng(seed))
+    in_order, _ = orc.sv_simulate(start, nv, to_ops(recs))
+    planned, _ = orc.sv_simulate(start, nv, to_ops([recs[i] for i in plan.order]))
+    assert np.max(np.abs(in_order - planned)) <= 1e-12
+    if density:
+        # the record replay is the density-matrix update itself
+        rho, _ = orc.dm_simulate(start, n, ops)
+        assert np.max(np.abs(rho - in_order)) <= 1e-12
+    return plan
+
+
+@pytest.mark.parametrize("n", [4, 5, 7, 9, 12, 14])
+@pytest.mark.parametrize("k,stages,cost", [(-1, 6, 1e9), (5, 2, 1e9), (8, 3, 4.0), (12, 8, 12.0), (4, 1, 1e9)])
+def test_planned_order_statevector(n, k, stages, cost):
+    ka.set_fusion(k, 4096, stages, cost)
+    c = random_gate_circuit(n, 12 * n, np.random.default_rng(100 + n))
+    plan = check_plan(c, False, n)
+    assert np.all(plan.pass_cost[plan.pass_kind != 2] <= max(cost, 1.0) + 1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 6])
+@pytest.mark.parametrize("k", [-1, 6])
+def test_planned_order_density_matrix(n, k):
+    ka.set_fusion(k, 4096, 4, 1e9)
+    c = random_gate_circuit(n, 10 * n, np.random.default_rng(200 + n))
+    check_plan(c, True, n)
+
+
+def test_diagonal_runs_do_not_claim_tile_slots():
+    """A long run of controlled phases on high qubits fuses into the tile of the low qubits."""
+    n = 16
+    c = ka.QuantumCircuit(n)
+    for q in range(6):
+        c.add_h_gate(q)
+    for a in range(n):
+        for b in range(a + 1, n):
+            c.add_cp_gate(a, b, 0.1 * (a + b))
+    for q in range(6):
+        c.add_h_gate(q)
+    ka.set_fusion(-1, 4096, 6, 1e9)
+    plan = check_plan(c, False, 1)
+    assert len(plan.pass_kind) == 1
+
+
+def test_30q_workload_plan_shape():
+    """The 30-qubit QFT + layers circuit: every pass a tile pass within the cost cap."""
+    from kettle_amd import workloads
+
+    ka.set_fusion(-1, 4096, 6, 16.0)
+    ops, _, _ = workloads.qft_plus_layers(30, 20).flatten()
+    plan = ka.debug_plan(ops, 30)
+    assert sorted(plan.order.tolist()) == list(range(len(ops)))
+    assert np.all(plan.pass_cost <= 16.0 + 1e-12)
+    assert 20 <= len(plan.pass_kind) <= 80
+
+
+def test_debug_plan_rejects_non_gates():
+    c = ka.QuantumCircuit(2)
+    c.add_m_gate(0)
+    ops, _, _ = c.flatten()
+    with pytest.raises(ka.KqError):
+        ka.debug_plan(ops, 2)
