@@ -12,6 +12,7 @@
 #include <kettle/gates/primitive_gate.hpp>
 #include <kettle/gates/random_u_gates.hpp>
 #include <kettle/gradient/energy_gradient.hpp>
+#include <kettle/io/read_pauli_operator.hpp>
 #include <kettle/operator/pauli/pauli_operator.hpp>
 #include <kettle/operator/pauli/sparse_pauli_string.hpp>
 #include <kettle/optimize/n_local.hpp>

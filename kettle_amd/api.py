@@ -349,6 +349,14 @@ class SparsePauliString:
     def set_phase(self, phase: PauliPhase) -> None:
         self._phase = PauliPhase(phase)
 
+    def add(self, qubit: int, term: PauliTerm) -> None:
+        """kettle's ``SparsePauliString::add`` (sparse_pauli_string.cpp): a new qubit only."""
+        if qubit >= self._n:
+            raise RuntimeError("ERROR: cannot perform operation on SparsePauliString with index beyond qubit range.\n")
+        if any(q == qubit for q, _ in self._terms):
+            raise RuntimeError("ERROR: Pauli term is already present in the string\n")
+        self._terms.append((int(qubit), PauliTerm(term)))
+
 
 class PauliOperator:
     """kettle's ``PauliOperator`` (pauli_operator.hpp): sum of coefficient * string."""

@@ -73,7 +73,7 @@ def build_kq(jobs: int = 4) -> Path:
 
 
 KET_SOURCES = ["ket_common.cpp", "ket_state.cpp", "ket_circuit.cpp", "ket_simulate.cpp", "ket_measure.cpp",
-               "ket_pauli.cpp", "ket_optimize.cpp"]
+               "ket_pauli.cpp", "ket_optimize.cpp", "ket_io.cpp"]
 HOST_FLAGS = ["-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-I", str(ROOT / "include")]
 
 
@@ -97,13 +97,20 @@ def build_ket(jobs: int = 4) -> Path:
     return out
 
 
-def build_cpp_tests() -> Path:
-    """The C++ API test executable (tests/cpp), linked against libket and libkq."""
-    src = ROOT / "tests" / "cpp" / "ket_api_test.cpp"
-    out = ROOT / "tests" / "cpp" / "ket_api_test"
-    _run(["g++", *HOST_FLAGS, str(src), "-o", str(out), "-L", str(LIB_DIR), "-lket", "-lkq",
-          f"-Wl,-rpath,{LIB_DIR}"])
-    return out
+CPP_TESTS = ["ket_api_test", "ket_host_test"]
+
+
+def build_cpp_tests() -> list[Path]:
+    """The C++ API test executables (tests/cpp), linked against libket and libkq: ket_api_test
+    runs on the GPU, ket_host_test covers the host-only parts (builders, readers, validation)."""
+    outs = []
+    for name in CPP_TESTS:
+        src = ROOT / "tests" / "cpp" / f"{name}.cpp"
+        out = ROOT / "tests" / "cpp" / name
+        _run(["g++", *HOST_FLAGS, str(src), "-o", str(out), "-L", str(LIB_DIR), "-lket", "-lkq",
+              f"-Wl,-rpath,{LIB_DIR}"])
+        outs.append(out)
+    return outs
 
 
 def build_oracle() -> Path:

@@ -16,11 +16,12 @@ import pytest
 ROOT = Path(__file__).resolve().parent.parent
 LIBKET = ROOT / "kettle_amd" / "lib" / "libket.so"
 EXE = ROOT / "tests" / "cpp" / "ket_api_test"
+HOST_EXE = ROOT / "tests" / "cpp" / "ket_host_test"
 
 
 def _require_built() -> None:
-    if not LIBKET.exists() or not EXE.exists():
-        pytest.fail("libket.so / tests/cpp/ket_api_test missing: run __graft_entry__.build() first")
+    if not LIBKET.exists() or not EXE.exists() or not HOST_EXE.exists():
+        pytest.fail("libket.so / tests/cpp executables missing: run __graft_entry__.build() first")
 
 
 def test_libket_links_against_libkq():
@@ -44,6 +45,15 @@ def test_libket_exports_simulate():
     for name in ["simulate", "expectation_value", "perform_measurements_as_counts_marginal", "partial_trace",
                  "energy_and_gradient", "n_local"]:
         assert re.search(rf" T ket::{name}(\[abi:\w+\])?\(", out), name
+
+
+def test_cpp_host_cases():
+    """Host-only C++ cases (readers, builders, Pauli bookkeeping) run without a device."""
+    _require_built()
+    proc = subprocess.run([str(HOST_EXE)], capture_output=True, text=True, timeout=120)
+    print(proc.stdout[-4000:])
+    failed = [line for line in proc.stdout.splitlines() if line.startswith("FAIL")]
+    assert proc.returncode == 0 and not failed, "\n".join(failed) or proc.stderr
 
 
 @pytest.mark.gpu
