@@ -601,6 +601,89 @@ bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d)
     return false;
 }
 
+// One segment of the reverse sweep through fused adjoint tile passes.  inv[i] is the inverse of the
+// i-th gate in reverse order and deriv[i] its derivative (or null); values[i] receives
+// Re <lambda| dG |psi> for each i with a derivative.
+void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<GateRec>& inv,
+                         const std::vector<const GateRec*>& deriv, std::vector<double>& values)
+{
+    const int max_k = fusion_max_k(s->precision, s->nvec);
+    const auto passes = plan_passes(inv, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
+                                    fusion_max_stages(), false);
+    const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
+    std::vector<TileProgram> progs(passes.size());
+    std::vector<size_t> offs(passes.size());
+    size_t total = 0;
+    uint64_t n_rows_total = 0;
+    std::vector<uint64_t> row_off;
+    std::vector<uint32_t> row_cnt;
+    std::vector<size_t> part_off(passes.size());
+    uint64_t part_total = 0;
+    for (size_t i = 0; i < passes.size(); ++i) {
+        lower_tile_pass(passes[i], s->precision, s->nvec, progs[i], &deriv);
+        const TileProgram& tp = progs[i];
+        offs[i] = total;
+        total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
+               + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+        const uint64_t n_part = uint64_t {tile_pass_blocks(tp)} * tile_pass_waves(tp);
+        part_off[i] = part_total;
+        for (size_t k = 0; k < tp.derivs.size(); ++k) {
+            row_off.push_back(part_total + k * n_part);
+            row_cnt.push_back(static_cast<uint32_t>(n_part));
+        }
+        part_total += tp.derivs.size() * n_part;
+        n_rows_total += tp.derivs.size();
+    }
+    const size_t tables = align(row_off.size() * sizeof(uint64_t)) + align(row_cnt.size() * sizeof(uint32_t));
+    std::vector<unsigned char> blob(std::max<size_t>(total + tables, 256), 0);
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const TileProgram& tp = progs[i];
+        size_t o = offs[i];
+        std::memcpy(blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+        o += align(tp.gates.size() * sizeof(TileGate));
+        std::memcpy(blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+        o += align(tp.steps.size() * sizeof(TileStep));
+        std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+        o += align(tp.layouts.size() * sizeof(TileLayout));
+        std::memcpy(blob.data() + o, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
+    }
+    std::memcpy(blob.data() + total, row_off.data(), row_off.size() * sizeof(uint64_t));
+    std::memcpy(blob.data() + total + align(row_off.size() * sizeof(uint64_t)), row_cnt.data(),
+                row_cnt.size() * sizeof(uint32_t));
+    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
+    KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
+    // partials, then one sum per derivative slot
+    const size_t part_bytes = align(std::max<uint64_t>(part_total, 1) * sizeof(double));
+    auto* aux = static_cast<unsigned char*>(grow(s->aux, s->aux_bytes, part_bytes + std::max<uint64_t>(n_rows_total, 1) * sizeof(double)));
+    auto* d_part = reinterpret_cast<double*>(aux);
+    auto* d_sums = reinterpret_cast<double*>(aux + part_bytes);
+    const double ab = static_cast<double>(amp_bytes(s->precision));
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const TileProgram& tp = progs[i];
+        const unsigned char* g = d + offs[i];
+        const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
+        const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
+        const unsigned char* dv = ly + align(tp.layouts.size() * sizeof(TileLayout));
+        profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
+                 [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, d_part + part_off[i]); });
+    }
+    std::vector<double> sums(n_rows_total);
+    if (n_rows_total) {
+        launch_sum_rows(s->stream, d_part, reinterpret_cast<const uint64_t*>(d + total),
+                        reinterpret_cast<const uint32_t*>(d + total + align(row_off.size() * sizeof(uint64_t))),
+                        static_cast<uint32_t>(n_rows_total), d_sums);
+        KQ_HIP_CHECK(hipMemcpyAsync(sums.data(), d_sums, n_rows_total * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    profiler_collect();
+    size_t row = 0;
+    for (const TileProgram& tp : progs) {
+        for (const uint32_t src : tp.deriv_src) {
+            values[src] = sums[row++];
+        }
+    }
+}
+
 void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strings, const uint32_t* off,
                      const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs, size_t n_params,
                      double* energy, double* grad, uint32_t every, double drift_tol, size_t* n_restores)
@@ -628,7 +711,9 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     if (every == 0) {
         every = static_cast<uint32_t>(std::max<double>(1.0, std::ceil(std::sqrt(static_cast<double>(G)))));
     }
-    // checkpoints at gate boundaries b = every, 2 every, ... (< G), capped at 32 GiB of memory
+    // checkpoints at gate boundaries b = every, 2 every, ... (< G), capped at 32 GiB of memory; the
+    // forward and the reverse sweep are planned segment by segment between them, so the state at
+    // each boundary is exactly the checkpointed one
     std::vector<size_t> bounds;
     for (size_t b = every; b < G; b += every) {
         bounds.push_back(b);
@@ -641,20 +726,23 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
             bounds.push_back(b);
         }
     }
+    std::vector<size_t> seg {0};
+    seg.insert(seg.end(), bounds.begin(), bounds.end());
+    seg.push_back(G);
     std::vector<void*> ckpt(bounds.size(), nullptr);
     void* lam = nullptr;
     void* accs = nullptr;
+    void* psi = nullptr;
     const auto cleanup = [&] {
         for (auto* p : ckpt) {
             if (p) {
                 (void) hipFree(p);
             }
         }
-        if (lam) {
-            (void) hipFree(lam);
-        }
-        if (accs) {
-            (void) hipFree(accs);
+        for (void* p : {lam, accs, psi}) {
+            if (p) {
+                (void) hipFree(p);
+            }
         }
     };
     try {
@@ -663,67 +751,76 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
             p = device_alloc(bytes);
         }
         // forward sweep with checkpoints
-        size_t g0 = 0;
-        for (size_t j = 0; j <= bounds.size(); ++j) {
-            const size_t g1 = j < bounds.size() ? bounds[j] : G;
-            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(g0), recs.begin() + static_cast<long>(g1)));
+        for (size_t j = 0; j + 1 < seg.size(); ++j) {
+            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(seg[j]), recs.begin() + static_cast<long>(seg[j + 1])));
             if (j < bounds.size()) {
                 launch_copy(s, ckpt[j], s->amps, bytes);
             }
-            g0 = g1;
         }
         // lambda = H psi, E = <psi | lambda>
         launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
         double e[2];
         launch_dot(s, s->amps, lam, e);
         *energy = e[0];
-        // reverse sweep
-        size_t n_param_gates = 0;
-        for (size_t i = 0; i < n_ops; ++i) {
-            n_param_gates += ops[i].aux >= 0 ? 1 : 0;
-        }
-        accs = device_alloc(std::max<size_t>(n_param_gates, 1) * 2 * sizeof(double));
-        std::vector<size_t> slot_gate;
-        size_t restores = 0;
-        void* psi_backup = nullptr;
-        (void) psi_backup;
-        // the current state buffer becomes psi; keep psi(theta) for the caller afterwards
-        void* psi = device_alloc(bytes);
+        // reverse sweep on a copy of psi (the state keeps psi(theta) for the caller)
+        psi = device_alloc(bytes);
         launch_copy(s, psi, s->amps, bytes);
-        try {
-            long long jb = static_cast<long long>(bounds.size()) - 1;
-            for (size_t gi = G; gi-- > 0;) {
-                if (jb >= 0 && bounds[static_cast<size_t>(jb)] == gi + 1) {
-                    const double drift = launch_max_abs_diff(s, psi, ckpt[static_cast<size_t>(jb)]);
-                    if (drift > drift_tol) {
-                        launch_copy(s, psi, ckpt[static_cast<size_t>(jb)], bytes);
-                        ++restores;
+        std::vector<GateRec> derivs(G);
+        std::vector<char> has_d(G, 0);
+        for (size_t gi = 0; gi < G; ++gi) {
+            has_d[gi] = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]) ? 1 : 0;
+        }
+        const bool fused = fusion_max_k(s->precision, s->nvec) >= register_qubits(s->precision);
+        std::fill(grad, grad + n_params, 0.0);
+        std::vector<double> h;
+        if (!fused) {
+            accs = device_alloc(std::max<size_t>(G, 1) * 2 * sizeof(double));
+        }
+        size_t restores = 0;
+        for (size_t j = seg.size() - 1; j-- > 0;) {
+            const size_t b0 = seg[j], b1 = seg[j + 1];
+            if (fused) {
+                std::vector<GateRec> inv;
+                std::vector<const GateRec*> dptr;
+                inv.reserve(b1 - b0);
+                for (size_t gi = b1; gi-- > b0;) {
+                    inv.push_back(inverse_of(recs[gi]));
+                    dptr.push_back(has_d[gi] ? &derivs[gi] : nullptr);
+                }
+                std::vector<double> values(inv.size(), 0.0);
+                run_adjoint_segment(s, psi, lam, inv, dptr, values);
+                for (size_t i = 0; i < inv.size(); ++i) {
+                    const size_t gi = b1 - 1 - i;
+                    if (has_d[gi]) {
+                        grad[ops[gi].aux] += 2.0 * values[i];
                     }
-                    --jb;
                 }
-                const GateRec inv = inverse_of(recs[gi]);
-                GateRec d;
-                const bool has_d = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], d);
-                double* acc = nullptr;
-                if (has_d) {
-                    acc = static_cast<double*>(accs) + 2 * slot_gate.size();
-                    slot_gate.push_back(gi);
-                }
-                launch_adjoint_gate(s, psi, lam, inv, has_d ? &d : nullptr, acc);
             }
-            std::vector<double> h(2 * std::max<size_t>(slot_gate.size(), 1));
-            KQ_HIP_CHECK(hipMemcpyAsync(h.data(), accs, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-            KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
-            std::fill(grad, grad + n_params, 0.0);
-            for (size_t j = 0; j < slot_gate.size(); ++j) {
-                grad[ops[slot_gate[j]].aux] += 2.0 * h[2 * j];
+            else {
+                for (size_t gi = b1; gi-- > b0;) {
+                    launch_adjoint_gate(s, psi, lam, inverse_of(recs[gi]), has_d[gi] ? &derivs[gi] : nullptr,
+                                        has_d[gi] ? static_cast<double*>(accs) + 2 * gi : nullptr);
+                }
+                h.resize(2 * (b1 - b0));
+                KQ_HIP_CHECK(hipMemcpyAsync(h.data(), static_cast<double*>(accs) + 2 * b0, h.size() * sizeof(double),
+                                            hipMemcpyDeviceToHost, s->stream));
+                KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+                for (size_t gi = b0; gi < b1; ++gi) {
+                    if (has_d[gi]) {
+                        grad[ops[gi].aux] += 2.0 * h[2 * (gi - b0)];
+                    }
+                }
+            }
+            // psi should now be the forward state after gates [0, b0): compare with the checkpoint
+            if (j >= 1) {
+                const double drift = launch_max_abs_diff(s, psi, ckpt[j - 1]);
+                if (drift > drift_tol) {
+                    launch_copy(s, psi, ckpt[j - 1], bytes);
+                    ++restores;
+                }
             }
         }
-        catch (...) {
-            (void) hipFree(psi);
-            throw;
-        }
-        KQ_HIP_CHECK(hipFree(psi));
+        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
         if (n_restores) {
             *n_restores = restores;
         }
@@ -846,6 +943,9 @@ int kq_state_destroy(kq_state* s)
         }
         if (s->prog) {
             (void) hipFree(s->prog);
+        }
+        if (s->aux) {
+            (void) hipFree(s->aux);
         }
         delete s;
     });

@@ -136,10 +136,17 @@ struct TileProgram
     std::vector<TileGate> gates;
     std::vector<TileStep> steps;
     std::vector<TileLayout> layouts;
+    std::vector<TileGate> derivs;     // adjoint passes: derivative records (pad = partial slot)
+    std::vector<uint32_t> deriv_src;  // input record index of each derivative slot
 };
 
-// lower a planned pass into the tile kernel's program (coefficients rounded to the precision)
-void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out);
+// Lower a planned pass into the tile kernel's program (coefficients rounded to the precision).
+// deriv_of_src (adjoint passes): for input record i, its derivative dG (or null); a gate with one
+// gets TileGate::pad = derivative index + 1 and its derivative record the next partial slot.
+void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out,
+                     const std::vector<const GateRec*>* deriv_of_src = nullptr);
+uint32_t tile_pass_blocks(const TileProgram& prog);
+uint32_t tile_pass_waves(const TileProgram& prog);
 
 struct PlanStats
 {
@@ -166,6 +173,9 @@ struct kq_state
     // per-state device buffer for uploaded gate programs
     void* prog = nullptr;
     size_t prog_bytes = 0;
+    // per-state device buffer for adjoint partial sums
+    void* aux = nullptr;
+    size_t aux_bytes = 0;
 };
 
 namespace kq
@@ -209,8 +219,9 @@ void profiler_collect();   // read and release pending events (after a stream sy
 void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>& out);
 bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
+// allow_sweep: a pass holding a single diagonal gate may run as a plain sweep (off for adjoint plans)
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
-                                  int low_c, int max_gates, int rq, int max_stages);
+                                  int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true);
 int fusion_max_stages();
 int register_qubits(int precision);
 PlanStats& last_stats();
@@ -231,6 +242,12 @@ void launch_gate_sweep(kq_state* s, const GateRec& g);
 // d_prog: the uploaded [gates][steps][layouts] of `prog`
 void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
                       const void* d_layouts);
+// adjoint reverse-sweep pass over (psi, lam); partials: [slot][blocks * waves] doubles
+void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgram& prog, const void* d_gates,
+                              const void* d_steps, const void* d_layouts, const void* d_derivs, double* d_partials);
+// out[i] = sum of rows[offs[i] .. offs[i] + counts[i]) in a fixed order (one row per slot)
+void launch_sum_rows(hipStream_t st, const double* rows, const uint64_t* d_offs, const uint32_t* d_counts, uint32_t n,
+                     double* d_out);
 void launch_set_basis(kq_state* s, uint64_t index);
 // weights: SV -> (sum |a|^2 over bit q = 0, = 1); DM -> block Frobenius norms (kettle's quirk)
 void launch_qubit_weights(kq_state* s, uint32_t q, double out[2]);

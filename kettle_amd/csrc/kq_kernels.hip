@@ -757,6 +757,31 @@ __global__ __launch_bounds__(BLOCK) void k_dot(const Amp<R>* a, const Amp<R>* b,
     }
 }
 
+// out[i] = sum of rows[offs[i] .. offs[i] + counts[i]) (one block per row, fixed summation order)
+__global__ __launch_bounds__(BLOCK) void k_sum_rows(const double* __restrict__ rows, const uint64_t* __restrict__ offs,
+                                                    const uint32_t* __restrict__ counts, double* __restrict__ out)
+{
+    __shared__ double red[BLOCK];
+    const uint32_t row = blockIdx.x;
+    const double* p = rows + offs[row];
+    const uint32_t n = counts[row];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        acc += p[i];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            red[threadIdx.x] += red[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[row] = red[0];
+    }
+}
+
 template <typename R>
 __global__ __launch_bounds__(BLOCK) void k_max_abs_diff(const Amp<R>* a, const Amp<R>* b, uint64_t n, double2* part)
 {
@@ -1100,6 +1125,16 @@ void launch_adjoint_gate(kq_state* s, void* psi, void* lam, const GateRec& inv, 
         k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(part, static_cast<int>(blocks), reinterpret_cast<double2*>(d_acc));
         KQ_HIP_CHECK(hipGetLastError());
     }
+}
+
+void launch_sum_rows(hipStream_t st, const double* rows, const uint64_t* d_offs, const uint32_t* d_counts, uint32_t n,
+                     double* d_out)
+{
+    if (n == 0) {
+        return;
+    }
+    k_sum_rows<<<n, BLOCK, 0, st>>>(rows, d_offs, d_counts, d_out);
+    KQ_HIP_CHECK(hipGetLastError());
 }
 
 void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes)

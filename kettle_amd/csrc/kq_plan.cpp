@@ -247,7 +247,7 @@ uint32_t stage_regs(const TileStage& st, int rq)
 //     stage) or a new stage may still be opened (at most max_stages per pass).
 // Diagonal gates and controls never need to be in Q: their effect depends only on index bits.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k, int low_c, int max_gates,
-                                  int rq_max, int max_stages)
+                                  int rq_max, int max_stages, bool allow_sweep)
 {
     std::vector<TilePass> passes;
     const int k = std::min<int>(max_k, static_cast<int>(nvec));
@@ -312,7 +312,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
             throw InvalidArg {"fusion plan: no gate fits a tile"};
         }
         TilePass p;
-        if (taken.size() == 1 && is_diag(gates[taken[0]])) {
+        if (allow_sweep && taken.size() == 1 && is_diag(gates[taken[0]])) {
             // a lone diagonal gate touches only a quarter / half of the state: plain sweep
             p.sweep = true;
             p.single = gates[taken[0]];
@@ -470,11 +470,77 @@ uint32_t tile_kind(uint32_t kind)
     throw InvalidArg {"tile lowering: unknown gate kind"};
 }
 
+// One encoded gate record -> kernel variant.  derivative: the record is a derivative dG (never
+// given the d0 = 1 treatment, its diagonal may have d0 = 0).
+TileGate lower_gate(const GateRec& g, int precision, bool derivative)
+{
+    TileGate t {};
+    uint32_t creg = 0;
+    if (g.control >= 0) {
+        const uint32_t src = static_cast<uint32_t>(g.control) & 3u, idx = static_cast<uint32_t>(g.control) >> 2;
+        if (src == 0) {
+            creg = idx + 1;
+        }
+        else if (src == 1) {
+            t.lane_ctrl |= 1u << idx;
+        }
+        else {
+            t.glb_ctrl |= uint64_t {1} << idx;
+        }
+    }
+    const uint32_t tsrc = g.target & 3u, tidx = g.target >> 2;
+    double m[8];
+    std::memcpy(m, g.m, sizeof m);
+    const bool diag = g.kind == GK_DIAG || g.kind == GK_DIAG1;
+    if (tsrc == 0) {
+        if (creg == tidx + 1) {
+            throw InvalidArg {"tile lowering: control equals target"};
+        }
+        t.code = tile_code(tile_kind(g.kind), tidx, creg);
+    }
+    else {
+        if (!diag) {
+            throw InvalidArg {"tile lowering: non-diagonal gate outside the registers"};
+        }
+        t.code = TV_UNIFORM_BASE + creg;
+        if (g.kind == GK_DIAG1 && !derivative) {
+            // d0 = 1: the gate acts only where the target bit is set, with the factor d1
+            if (tsrc == 1) {
+                t.lane_ctrl |= 1u << tidx;
+            }
+            else {
+                t.glb_ctrl |= uint64_t {1} << tidx;
+            }
+            m[0] = m[6];
+            m[1] = m[7];
+        }
+        else if (tsrc == 1) {
+            t.lane_sel = 1u << tidx;
+        }
+        else {
+            t.glb_sel = uint64_t {1} << tidx;
+        }
+    }
+    if (precision == KQ_C64) {
+        float f[8];
+        for (int i = 0; i < 8; ++i) {
+            f[i] = static_cast<float>(m[i]);
+        }
+        std::memset(t.m, 0, sizeof t.m);
+        std::memcpy(t.m, f, sizeof f);
+    }
+    else {
+        std::memcpy(t.m, m, sizeof m);
+    }
+    return t;
+}
+
 }  // namespace
 
 // Turn the planner's bit encoding (enc = idx << 2 | src; src 0 register position, 1 thread bit,
 // 2 global qubit) into kernel variants, lane / workgroup conditions and per-layout offset tables.
-void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out)
+void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out,
+                     const std::vector<const GateRec*>* deriv_of_src)
 {
     if (p.rq != 4 || p.k < 4 || p.k > 12 || p.stages.empty()) {
         throw InvalidArg {"tile lowering: unsupported tile shape"};
@@ -547,65 +613,23 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
             }
         }
     }
-    // gates
+    // gates (and derivative records)
     out.gates.reserve(p.gates.size());
-    for (const GateRec& g : p.gates) {
-        TileGate t {};
-        uint32_t creg = 0;
-        if (g.control >= 0) {
-            const uint32_t src = static_cast<uint32_t>(g.control) & 3u, idx = static_cast<uint32_t>(g.control) >> 2;
-            if (src == 0) {
-                creg = idx + 1;
-            }
-            else if (src == 1) {
-                t.lane_ctrl |= 1u << idx;
-            }
-            else {
-                t.glb_ctrl |= uint64_t {1} << idx;
-            }
-        }
-        const uint32_t tsrc = g.target & 3u, tidx = g.target >> 2;
-        double m[8];
-        std::memcpy(m, g.m, sizeof m);
-        if (tsrc == 0) {
-            if (creg == tidx + 1) {
-                throw InvalidArg {"tile lowering: control equals target"};
-            }
-            t.code = tile_code(tile_kind(g.kind), tidx, creg);
-        }
-        else {
-            if (!is_diag(g)) {
-                throw InvalidArg {"tile lowering: non-diagonal gate outside the registers"};
-            }
-            t.code = TV_UNIFORM_BASE + creg;
-            if (g.kind == GK_DIAG1) {
-                // d0 = 1: the gate acts only where the target bit is set, with the factor d1
-                if (tsrc == 1) {
-                    t.lane_ctrl |= 1u << tidx;
-                }
-                else {
-                    t.glb_ctrl |= uint64_t {1} << tidx;
-                }
-                m[0] = m[6];
-                m[1] = m[7];
-            }
-            else if (tsrc == 1) {
-                t.lane_sel = 1u << tidx;
-            }
-            else {
-                t.glb_sel = uint64_t {1} << tidx;
-            }
-        }
-        if (precision == KQ_C64) {
-            float f[8];
-            for (int i = 0; i < 8; ++i) {
-                f[i] = static_cast<float>(m[i]);
-            }
-            std::memset(t.m, 0, sizeof t.m);
-            std::memcpy(t.m, f, sizeof f);
-        }
-        else {
-            std::memcpy(t.m, m, sizeof m);
+    for (size_t i = 0; i < p.gates.size(); ++i) {
+        TileGate t = lower_gate(p.gates[i], precision, false);
+        const GateRec* dr = deriv_of_src ? (*deriv_of_src)[p.src[i]] : nullptr;
+        if (dr) {
+            // the derivative acts on the same target / control bits as the gate
+            GateRec dg = *dr;
+            dg.target = p.gates[i].target;
+            dg.control = p.gates[i].control;
+            const bool diag = dg.m[2] == 0.0 && dg.m[3] == 0.0 && dg.m[4] == 0.0 && dg.m[5] == 0.0;
+            dg.kind = diag ? GK_DIAG : GK_DENSE;
+            TileGate td = lower_gate(dg, precision, true);
+            td.pad = static_cast<uint32_t>(out.derivs.size());
+            out.derivs.push_back(td);
+            out.deriv_src.push_back(p.src[i]);
+            t.pad = static_cast<uint32_t>(out.derivs.size());
         }
         out.gates.push_back(t);
     }

@@ -172,6 +172,72 @@ __device__ __forceinline__ void apply_uniform(Cx<R> (&v)[16], uint32_t c, R fr, 
     }
 }
 
+// <w| dG |v> restricted to the slots a derivative record touches, real part, FP64 accumulation.
+// Register target P (any matrix, dense formula) with register control C.
+template <typename R, int P, int C>
+__device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16], const R* __restrict__ m)
+{
+    const R m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7];
+    double acc = 0.0;
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1) && C != P + 1) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r];
+            const Cx<R> b = v[r1];
+            const R d0r = a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3;
+            const R d0i = a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2;
+            const R d1r = a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7;
+            const R d1i = a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6;
+            acc += static_cast<double>(w[r].re * d0r + w[r].im * d0i) + static_cast<double>(w[r1].re * d1r + w[r1].im * d1i);
+        }
+    });
+    return acc;
+}
+
+// uniform diagonal derivative: sum over the slots (with register control C) of Re(conj(w) f v)
+template <typename R, int C>
+__device__ __forceinline__ double ddot_uniform(const Cx<R> (&v)[16], const Cx<R> (&w)[16], R fr, R fi)
+{
+    double acc = 0.0;
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
+            const Cx<R> d = cmul(v[r], fr, fi);
+            acc += static_cast<double>(w[r].re * d.re + w[r].im * d.im);
+        }
+    });
+    return acc;
+}
+
+#define KQ_DDOT_CASE(P, C) \
+    case (P) * 5 + (C): return ddot<R, P, C>(v, w, m);
+
+template <typename R>
+__device__ __forceinline__ double ddot_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t pc, const R* __restrict__ m)
+{
+    switch (pc) {
+        KQ_DDOT_CASE(0, 0) KQ_DDOT_CASE(0, 2) KQ_DDOT_CASE(0, 3) KQ_DDOT_CASE(0, 4)
+        KQ_DDOT_CASE(1, 0) KQ_DDOT_CASE(1, 1) KQ_DDOT_CASE(1, 3) KQ_DDOT_CASE(1, 4)
+        KQ_DDOT_CASE(2, 0) KQ_DDOT_CASE(2, 1) KQ_DDOT_CASE(2, 2) KQ_DDOT_CASE(2, 4)
+        KQ_DDOT_CASE(3, 0) KQ_DDOT_CASE(3, 1) KQ_DDOT_CASE(3, 2) KQ_DDOT_CASE(3, 3)
+        default: break;
+    }
+    return 0.0;
+}
+
+template <typename R>
+__device__ __forceinline__ double ddot_uniform_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t c, R fr, R fi)
+{
+    switch (c) {
+        case 0: return ddot_uniform<R, 0>(v, w, fr, fi);
+        case 1: return ddot_uniform<R, 1>(v, w, fr, fi);
+        case 2: return ddot_uniform<R, 2>(v, w, fr, fi);
+        case 3: return ddot_uniform<R, 3>(v, w, fr, fi);
+        case 4: return ddot_uniform<R, 4>(v, w, fr, fi);
+        default: break;
+    }
+    return 0.0;
+}
+
 __device__ __forceinline__ uint32_t lds_thread_part(const TileLayout& L, uint32_t tid, int nbits)
 {
     uint32_t x = 0;
@@ -190,11 +256,29 @@ __device__ __forceinline__ uint64_t hbm_thread_part(const TileLayout& L, uint32_
     return x;
 }
 
-template <typename R>
-__global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, const TileArgs d,
+// Sum of `x` over the first `width` lanes of the wavefront (width a power of two <= 64); every
+// lane of the wave must be active.
+__device__ __forceinline__ double wave_sum(double x, int width)
+{
+    for (int off = 1; off < width; off <<= 1) {
+        x += __shfl_xor(x, off, 64);
+    }
+    return x;
+}
+
+// ADJ = false: one state, gates applied in order.
+// ADJ = true: the reverse sweep of the adjoint gradient over two states, psi (a) and lambda (b);
+// the gate records are the inverse gates G^dag in reverse circuit order.  For each gate: psi <- G^dag
+// psi, then, if the record has a derivative record (pad = index + 1), Re <lambda| dG |psi> over the
+// tile is reduced per wavefront into partials[slot * n_part + block * waves + wave]; then
+// lambda <- G^dag lambda.
+template <typename R, bool ADJ>
+__global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs d,
                                                      const TileStep* __restrict__ steps,
                                                      const TileLayout* __restrict__ layouts,
-                                                     const TileGate* __restrict__ gates)
+                                                     const TileGate* __restrict__ gates,
+                                                     const TileGate* __restrict__ derivs, double* __restrict__ partials,
+                                                     uint32_t n_part)
 {
     extern __shared__ unsigned char smem_raw[];
     const uint32_t tid = threadIdx.x;
@@ -202,42 +286,57 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, con
 
     uint64_t base = 0;
     {
-        const uint64_t b = blockIdx.x;
+        const uint64_t blk = blockIdx.x;
         for (int j = 0; j < d.nfree; ++j) {
-            base |= ((b >> j) & 1ull) << d.freepos[j];
+            base |= ((blk >> j) & 1ull) << d.freepos[j];
         }
     }
 
     Cx<R> v[16];
+    Cx<R> w[ADJ ? 16 : 1];
     {
         const TileLayout& L = layouts[0];
-        const Amp<R>* p = a + (base | hbm_thread_part(L, tid, tb));
+        const uint64_t off = base | hbm_thread_part(L, tid, tb);
         Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            const Amp<R> x = p[L.hbm_slot[r]];
+            const Amp<R> x = a[off + L.hbm_slot[r]];
             v[r] = {x.x, x.y};
+            if constexpr (ADJ) {
+                const Amp<R> y = b[off + L.hbm_slot[r]];
+                w[r] = {y.x, y.y};
+            }
         });
     }
 
     uint32_t cur = 0;
+    const auto remap_one = [&](Cx<R> (&u)[16], const TileLayout& A, const TileLayout& B, uint32_t xa, uint32_t xb) {
+        __syncthreads();
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            Amp<R> x;
+            x.x = u[r].re;
+            x.y = u[r].im;
+            *reinterpret_cast<Amp<R>*>(smem_raw + (xa ^ A.lds_slot[r])) = x;
+        });
+        __syncthreads();
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            const Amp<R> x = *reinterpret_cast<const Amp<R>*>(smem_raw + (xb ^ B.lds_slot[r]));
+            u[r] = {x.x, x.y};
+        });
+    };
     const auto remap = [&](uint32_t to) {
         const TileLayout& A = layouts[cur];
         const TileLayout& B = layouts[to];
         const uint32_t xa = lds_thread_part(A, tid, tb);
-        __syncthreads();
-        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            Amp<R> x;
-            x.x = v[r].re;
-            x.y = v[r].im;
-            *reinterpret_cast<Amp<R>*>(smem_raw + (xa ^ A.lds_slot[r])) = x;
-        });
         const uint32_t xb = lds_thread_part(B, tid, tb);
-        __syncthreads();
-        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            const Amp<R> x = *reinterpret_cast<const Amp<R>*>(smem_raw + (xb ^ B.lds_slot[r]));
-            v[r] = {x.x, x.y};
-        });
+        remap_one(v, A, B, xa, xb);
+        if constexpr (ADJ) {
+            remap_one(w, A, B, xa, xb);
+        }
         cur = to;
     };
+
+    const int width = (1 << tb) < 64 ? (1 << tb) : 64;
+    const uint32_t waves = (blockDim.x + 63) / 64;
+    const uint32_t part_col = blockIdx.x * waves + tid / 64;
 
     for (int si = 0; si < d.n_steps; ++si) {
         const TileStep st = steps[si];
@@ -247,22 +346,65 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, con
         const uint32_t g_end = st.g_begin + st.g_count;
         for (uint32_t gi = st.g_begin; gi < g_end; ++gi) {
             const TileGate& g = gates[gi];
-            if ((base & g.glb_ctrl) != g.glb_ctrl) {
-                continue;   // a control (or a d0 = 1 diagonal's target) outside the tile is 0
-            }
-            if ((tid & g.lane_ctrl) != g.lane_ctrl) {
-                continue;   // a control on a thread bit is 0 for this lane
-            }
+            const bool ok = (base & g.glb_ctrl) == g.glb_ctrl && (tid & g.lane_ctrl) == g.lane_ctrl;
             const R* m = reinterpret_cast<const R*>(g.m);
             const uint32_t code = g.code;
+            R fr = 0, fi = 0;
             if (code >= TV_UNIFORM_BASE) {
                 const bool sel = g.lane_sel ? ((tid & g.lane_sel) != 0) : ((base & g.glb_sel) != 0);
-                const R fr = sel ? m[6] : m[0];
-                const R fi = sel ? m[7] : m[1];
-                apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
+                fr = sel ? m[6] : m[0];
+                fi = sel ? m[7] : m[1];
+            }
+            if constexpr (!ADJ) {
+                if ((base & g.glb_ctrl) != g.glb_ctrl) {
+                    continue;   // a control (or a d0 = 1 diagonal's target) outside the tile is 0
+                }
+                if ((tid & g.lane_ctrl) != g.lane_ctrl) {
+                    continue;   // a control on a thread bit is 0 for this lane
+                }
+                if (code >= TV_UNIFORM_BASE) {
+                    apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
+                }
+                else {
+                    apply_gate<R>(v, code, m);
+                }
             }
             else {
-                apply_gate<R>(v, code, m);
+                if (ok) {
+                    if (code >= TV_UNIFORM_BASE) {
+                        apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
+                    }
+                    else {
+                        apply_gate<R>(v, code, m);
+                    }
+                }
+                if (g.pad != 0) {
+                    const TileGate& dg = derivs[g.pad - 1];
+                    const R* dm = reinterpret_cast<const R*>(dg.m);
+                    double acc = 0.0;
+                    if ((base & dg.glb_ctrl) == dg.glb_ctrl && (tid & dg.lane_ctrl) == dg.lane_ctrl) {
+                        if (dg.code >= TV_UNIFORM_BASE) {
+                            const bool sel = dg.lane_sel ? ((tid & dg.lane_sel) != 0) : ((base & dg.glb_sel) != 0);
+                            acc = ddot_uniform_gate<R>(v, w, dg.code - TV_UNIFORM_BASE, sel ? dm[6] : dm[0],
+                                                       sel ? dm[7] : dm[1]);
+                        }
+                        else {
+                            acc = ddot_gate<R>(v, w, dg.code % 20, dm);
+                        }
+                    }
+                    acc = wave_sum(acc, width);
+                    if ((tid & 63u) == 0) {
+                        partials[static_cast<size_t>(dg.pad) * n_part + part_col] = acc;
+                    }
+                }
+                if (ok) {
+                    if (code >= TV_UNIFORM_BASE) {
+                        apply_uniform<R>(w, code - TV_UNIFORM_BASE, fr, fi);
+                    }
+                    else {
+                        apply_gate<R>(w, code, m);
+                    }
+                }
             }
         }
     }
@@ -272,33 +414,36 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, con
     }
     {
         const TileLayout& L = layouts[cur];
-        Amp<R>* p = a + (base | hbm_thread_part(L, tid, tb));
+        const uint64_t off = base | hbm_thread_part(L, tid, tb);
         Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
             Amp<R> x;
             x.x = v[r].re;
             x.y = v[r].im;
-            p[L.hbm_slot[r]] = x;
+            a[off + L.hbm_slot[r]] = x;
+            if constexpr (ADJ) {
+                Amp<R> y;
+                y.x = w[r].re;
+                y.y = w[r].im;
+                b[off + L.hbm_slot[r]] = y;
+            }
         });
     }
 }
 
-template <typename R>
+template <typename R, bool ADJ>
 void set_lds_attr()
 {
     static bool done = false;
     if (!done) {
         const int maxdyn = static_cast<int>(device_lds_per_block() - TILE_STATIC_LDS);
-        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<R>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   maxdyn);
+        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<R, ADJ>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
         (void) hipGetLastError();
         done = true;
     }
 }
 
-}  // namespace
-
-void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
-                      const void* d_layouts)
+TileArgs check_program(const kq_state* s, const TileProgram& prog)
 {
     if (prog.k < 4 || prog.k > 12 || prog.n_thr_bits != prog.k - 4 || prog.steps.empty() || prog.layouts.empty()
         || prog.out_layout >= prog.layouts.size()) {
@@ -312,6 +457,11 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
             throw InvalidArg {"tile pass: step out of range"};
         }
     }
+    for (const TileGate& g : prog.gates) {
+        if (g.pad > prog.derivs.size()) {
+            throw InvalidArg {"tile pass: derivative index out of range"};
+        }
+    }
     TileArgs d {};
     d.nfree = static_cast<int>(prog.freepos.size());
     d.n_thr_bits = prog.n_thr_bits;
@@ -320,27 +470,70 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
     for (size_t j = 0; j < prog.freepos.size(); ++j) {
         d.freepos[j] = prog.freepos[j];
     }
-    const uint64_t blocks = uint64_t {1} << d.nfree;
-    if (blocks > 0x7fffffffull) {
+    if (d.nfree > 31) {
         throw InvalidArg {"tile pass: grid too large"};
     }
-    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
-    if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
+    if ((size_t {1} << prog.k) * amp_bytes(s->precision) + TILE_STATIC_LDS > device_lds_per_block()) {
         throw InvalidArg {"tile pass: tile does not fit the workgroup's LDS"};
     }
+    return d;
+}
+
+}  // namespace
+
+uint32_t tile_pass_blocks(const TileProgram& prog) { return 1u << prog.freepos.size(); }
+uint32_t tile_pass_waves(const TileProgram& prog) { return ((1u << prog.n_thr_bits) + 63) / 64; }
+
+void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
+                      const void* d_layouts)
+{
+    const TileArgs d = check_program(s, prog);
+    if (!prog.derivs.empty()) {
+        throw InvalidArg {"tile pass: derivative records need the adjoint launch"};
+    }
+    const unsigned blocks = tile_pass_blocks(prog);
     const unsigned threads = 1u << prog.n_thr_bits;
+    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
     const auto* g = static_cast<const TileGate*>(d_gates);
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
     if (s->precision == KQ_C64) {
-        set_lds_attr<float>();
-        k_tile<float><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<float2*>(s->amps), d, st, ly, g);
+        set_lds_attr<float, false>();
+        k_tile<float, false><<<blocks, threads, lds, s->stream>>>(static_cast<float2*>(s->amps), nullptr, d, st, ly, g,
+                                                                   nullptr, nullptr, 0);
     }
     else {
-        set_lds_attr<double>();
-        k_tile<double><<<static_cast<unsigned>(blocks), threads, lds, s->stream>>>(static_cast<double2*>(s->amps), d, st, ly, g);
+        set_lds_attr<double, false>();
+        k_tile<double, false><<<blocks, threads, lds, s->stream>>>(static_cast<double2*>(s->amps), nullptr, d, st, ly, g,
+                                                                    nullptr, nullptr, 0);
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgram& prog, const void* d_gates,
+                              const void* d_steps, const void* d_layouts, const void* d_derivs, double* d_partials)
+{
+    const TileArgs d = check_program(s, prog);
+    const unsigned blocks = tile_pass_blocks(prog);
+    const unsigned threads = 1u << prog.n_thr_bits;
+    const uint32_t n_part = blocks * tile_pass_waves(prog);
+    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
+    const auto* g = static_cast<const TileGate*>(d_gates);
+    const auto* st = static_cast<const TileStep*>(d_steps);
+    const auto* ly = static_cast<const TileLayout*>(d_layouts);
+    const auto* dv = static_cast<const TileGate*>(d_derivs);
+    if (s->precision == KQ_C64) {
+        set_lds_attr<float, true>();
+        k_tile<float, true><<<blocks, threads, lds, s->stream>>>(static_cast<float2*>(psi), static_cast<float2*>(lam), d, st,
+                                                                  ly, g, dv, d_partials, n_part);
+    }
+    else {
+        set_lds_attr<double, true>();
+        k_tile<double, true><<<blocks, threads, lds, s->stream>>>(static_cast<double2*>(psi), static_cast<double2*>(lam), d,
+                                                                   st, ly, g, dv, d_partials, n_part);
     }
     KQ_HIP_CHECK(hipGetLastError());
 }
 
 }  // namespace kq
+
