@@ -44,11 +44,11 @@ template <typename R> using Amp = typename AmpT<R>::T;
 
 struct TileArgs
 {
-    int nfree;
+    uint64_t free_mask;      // global qubits outside the tile; block index bit j -> j-th set bit
     int n_thr_bits;
     int n_steps;
     uint32_t out_layout;
-    uint8_t freepos[MAX_FREE_Q];
+    uint32_t pad;
 };
 
 template <typename R>
@@ -61,6 +61,35 @@ template <typename R>
 __device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
 {
     return {a.re * br - a.im * bi, a.re * bi + a.im * br};
+}
+
+// the eight coefficients of a gate record, already in registers
+template <typename R>
+struct Coef
+{
+    R m[8];
+};
+
+// a gate record's coefficients: doubles (complex128) or the floats packed in its first 32 bytes
+template <typename R>
+__device__ __forceinline__ Coef<R> coef_of(const TileGate& g)
+{
+    Coef<R> c;
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            c.m[i] = g.m[i];
+        }
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float2 f = __builtin_bit_cast(float2, g.m[i]);
+            c.m[2 * i] = f.x;
+            c.m[2 * i + 1] = f.y;
+        }
+    }
+    return c;
 }
 
 // compile-time loop over register slots
@@ -86,13 +115,13 @@ struct Unroll<N, N>
 // Variant (KIND, P, C): pairs (v[r], v[r | 1 << P]) for every r with bit P clear and, if C > 0,
 // bit C - 1 set.  The zero entries of real, axis-rotation and diagonal matrices are skipped.
 template <typename R, int KIND, int P, int C>
-__device__ __forceinline__ void apply(Cx<R> (&v)[16], const R* __restrict__ m)
+__device__ __forceinline__ void apply(Cx<R> (&v)[16], const Coef<R>& m)
 {
     if constexpr (C == P + 1) {
         return;   // never emitted by the host
     }
     else {
-        const R m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7];
+        const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
         Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
             if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1)) {
                 constexpr int r1 = r | (1 << P);
@@ -146,7 +175,7 @@ __device__ __forceinline__ void scale(Cx<R> (&v)[16], R fr, R fi)
     KQ_TILE_CASES_C(K, 0) KQ_TILE_CASES_C(K, 1) KQ_TILE_CASES_C(K, 2) KQ_TILE_CASES_C(K, 3)
 
 template <typename R>
-__device__ __forceinline__ void apply_gate(Cx<R> (&v)[16], uint32_t code, const R* __restrict__ m)
+__device__ __forceinline__ void apply_gate(Cx<R> (&v)[16], uint32_t code, const Coef<R>& m)
 {
     switch (code) {
         KQ_TILE_CASES_P(TK_DENSE)
@@ -175,9 +204,9 @@ __device__ __forceinline__ void apply_uniform(Cx<R> (&v)[16], uint32_t c, R fr, 
 // <w| dG |v> restricted to the slots a derivative record touches, real part, FP64 accumulation.
 // Register target P (any matrix, dense formula) with register control C.
 template <typename R, int P, int C>
-__device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16], const R* __restrict__ m)
+__device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16], const Coef<R>& m)
 {
-    const R m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7];
+    const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
     double acc = 0.0;
     Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
         if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1) && C != P + 1) {
@@ -212,7 +241,7 @@ __device__ __forceinline__ double ddot_uniform(const Cx<R> (&v)[16], const Cx<R>
     case (P) * 5 + (C): return ddot<R, P, C>(v, w, m);
 
 template <typename R>
-__device__ __forceinline__ double ddot_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t pc, const R* __restrict__ m)
+__device__ __forceinline__ double ddot_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t pc, const Coef<R>& m)
 {
     switch (pc) {
         KQ_DDOT_CASE(0, 0) KQ_DDOT_CASE(0, 2) KQ_DDOT_CASE(0, 3) KQ_DDOT_CASE(0, 4)
@@ -238,11 +267,15 @@ __device__ __forceinline__ double ddot_uniform_gate(const Cx<R> (&v)[16], const 
     return 0.0;
 }
 
+// Thread parts of a layout's addresses.  The per-bit tables are read unconditionally (one batch of
+// scalar loads) and selected per lane.
 __device__ __forceinline__ uint32_t lds_thread_part(const TileLayout& L, uint32_t tid, int nbits)
 {
     uint32_t x = 0;
-    for (int j = 0; j < nbits; ++j) {
-        x ^= ((tid >> j) & 1u) ? L.lds_thr[j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t wj = L.lds_thr[j];
+        x ^= (j < nbits && ((tid >> j) & 1u)) ? wj : 0u;
     }
     return x;
 }
@@ -250,8 +283,10 @@ __device__ __forceinline__ uint32_t lds_thread_part(const TileLayout& L, uint32_
 __device__ __forceinline__ uint64_t hbm_thread_part(const TileLayout& L, uint32_t tid, int nbits)
 {
     uint64_t x = 0;
-    for (int j = 0; j < nbits; ++j) {
-        x |= ((tid >> j) & 1u) ? L.hbm_thr[j] : 0ull;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint64_t wj = L.hbm_thr[j];
+        x |= (j < nbits && ((tid >> j) & 1u)) ? wj : 0ull;
     }
     return x;
 }
@@ -284,11 +319,16 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
     const uint32_t tid = threadIdx.x;
     const int tb = d.n_thr_bits;
 
+    // deposit the block index into the free qubits (uniform: scalar instructions only)
     uint64_t base = 0;
     {
-        const uint64_t blk = blockIdx.x;
-        for (int j = 0; j < d.nfree; ++j) {
-            base |= ((blk >> j) & 1ull) << d.freepos[j];
+        uint64_t blk = blockIdx.x;
+        uint64_t mask = d.free_mask;
+        while (mask != 0) {
+            const int q = __builtin_ctzll(mask);
+            base |= (blk & 1ull) << q;
+            blk >>= 1;
+            mask &= mask - 1;
         }
     }
 
@@ -343,23 +383,30 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
         if (st.layout != cur) {
             remap(st.layout);
         }
-        const uint32_t g_end = st.g_begin + st.g_count;
-        for (uint32_t gi = st.g_begin; gi < g_end; ++gi) {
-            const TileGate& g = gates[gi];
-            const bool ok = (base & g.glb_ctrl) == g.glb_ctrl && (tid & g.lane_ctrl) == g.lane_ctrl;
-            const R* m = reinterpret_cast<const R*>(g.m);
+        const TileGate* __restrict__ gp = gates + st.g_begin;
+        const uint32_t g_count = st.g_count;
+        TileGate next = gp[0];
+        for (uint32_t gi = 0; gi < g_count; ++gi) {
+            // the record is copied into scalar registers once; the next one is fetched meanwhile
+            const TileGate g = next;
+            if (gi + 1 < g_count) {
+                next = gp[gi + 1];
+            }
+            const Coef<R> m = coef_of<R>(g);
             const uint32_t code = g.code;
+            const bool glb_ok = (base & g.glb_ctrl) == g.glb_ctrl;
+            const bool lane_ok = (tid & g.lane_ctrl) == g.lane_ctrl;
             R fr = 0, fi = 0;
             if (code >= TV_UNIFORM_BASE) {
                 const bool sel = g.lane_sel ? ((tid & g.lane_sel) != 0) : ((base & g.glb_sel) != 0);
-                fr = sel ? m[6] : m[0];
-                fi = sel ? m[7] : m[1];
+                fr = sel ? m.m[6] : m.m[0];
+                fi = sel ? m.m[7] : m.m[1];
             }
             if constexpr (!ADJ) {
-                if ((base & g.glb_ctrl) != g.glb_ctrl) {
+                if (!glb_ok) {
                     continue;   // a control (or a d0 = 1 diagonal's target) outside the tile is 0
                 }
-                if ((tid & g.lane_ctrl) != g.lane_ctrl) {
+                if (!lane_ok) {
                     continue;   // a control on a thread bit is 0 for this lane
                 }
                 if (code >= TV_UNIFORM_BASE) {
@@ -370,6 +417,7 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
                 }
             }
             else {
+                const bool ok = glb_ok && lane_ok;
                 if (ok) {
                     if (code >= TV_UNIFORM_BASE) {
                         apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
@@ -379,14 +427,14 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
                     }
                 }
                 if (g.pad != 0) {
-                    const TileGate& dg = derivs[g.pad - 1];
-                    const R* dm = reinterpret_cast<const R*>(dg.m);
+                    const TileGate dg = derivs[g.pad - 1];
+                    const Coef<R> dm = coef_of<R>(dg);
                     double acc = 0.0;
                     if ((base & dg.glb_ctrl) == dg.glb_ctrl && (tid & dg.lane_ctrl) == dg.lane_ctrl) {
                         if (dg.code >= TV_UNIFORM_BASE) {
                             const bool sel = dg.lane_sel ? ((tid & dg.lane_sel) != 0) : ((base & dg.glb_sel) != 0);
-                            acc = ddot_uniform_gate<R>(v, w, dg.code - TV_UNIFORM_BASE, sel ? dm[6] : dm[0],
-                                                       sel ? dm[7] : dm[1]);
+                            acc = ddot_uniform_gate<R>(v, w, dg.code - TV_UNIFORM_BASE, sel ? dm.m[6] : dm.m[0],
+                                                       sel ? dm.m[7] : dm.m[1]);
                         }
                         else {
                             acc = ddot_gate<R>(v, w, dg.code % 20, dm);
@@ -463,14 +511,16 @@ TileArgs check_program(const kq_state* s, const TileProgram& prog)
         }
     }
     TileArgs d {};
-    d.nfree = static_cast<int>(prog.freepos.size());
     d.n_thr_bits = prog.n_thr_bits;
     d.n_steps = static_cast<int>(prog.steps.size());
     d.out_layout = prog.out_layout;
-    for (size_t j = 0; j < prog.freepos.size(); ++j) {
-        d.freepos[j] = prog.freepos[j];
+    for (const uint8_t q : prog.freepos) {
+        if (q >= 64 || ((d.free_mask >> q) & 1ull)) {
+            throw InvalidArg {"tile pass: bad free qubit list"};
+        }
+        d.free_mask |= uint64_t {1} << q;
     }
-    if (d.nfree > 31) {
+    if (prog.freepos.size() > 31) {
         throw InvalidArg {"tile pass: grid too large"};
     }
     if ((size_t {1} << prog.k) * amp_bytes(s->precision) + TILE_STATIC_LDS > device_lds_per_block()) {
