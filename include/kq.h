@@ -193,6 +193,27 @@ int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops,
                        size_t n_params, double* energy, double* grad,
                        uint32_t checkpoint_every, double drift_tol, size_t* n_restores);
 
+/* ---- batched energy + adjoint gradient (no kettle counterpart) --------------------------------
+ * Many independent straight-line parameterised circuits, each with its own qubit count, Pauli
+ * operator (same packing as kq_pauli_expectation, real weights) and parameters, evaluated together:
+ * kettle's VQE cost function (vqe_basic.cpp:61-82) for thousands of instances per call.  Instances
+ * are planned once (at the first run after the last kq_batch_add); kq_batch_set_parameters only
+ * changes angles.  kq_batch_run computes, for every instance i in add order, energies[i] and its
+ * gradient at grads + (sum of n_params of the instances before i); drift[i] (nullable) is the
+ * max-abs distance of the uncomputed state from |0...0> after the reverse sweep, a measure of
+ * inverse-gate drift.  use_graph != 0 replays the launch sequence as a captured hipGraph from the
+ * second run on.  A batch owns its stream and device memory; calls on one batch are serialised. */
+typedef struct kq_batch kq_batch;
+int kq_batch_create(kq_batch** out, int precision);
+int kq_batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params,
+                 size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
+                 const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs, size_t* index);
+int kq_batch_set_parameters(kq_batch* b, size_t index, const double* values);
+int kq_batch_run(kq_batch* b, int use_graph, double* energies, double* grads, double* drift);
+int kq_batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params_total, size_t* device_bytes,
+                  size_t* n_launches);
+int kq_batch_destroy(kq_batch* b);
+
 /* ---- profiling ------------------------------------------------------------------------------
  * While enabled, every gate pass launched by kq_simulate / kq_apply_gates is bracketed by HIP
  * events on the state's own stream; durations and algorithmic bytes (read + write of every

@@ -406,6 +406,35 @@ class PauliOperator:
                 np.asarray(ph, np.uint8), np.asarray(co, np.float64))
 
 
+class PackedPauliOperator:
+    """A Pauli operator held directly in kq_pauli_expectation's packing (term offsets, term qubits,
+    term kinds, phases, coefficients as re/im pairs); interchangeable with ``PauliOperator`` wherever
+    only ``packed()``, ``size()`` and ``n_qubits()`` are used."""
+
+    def __init__(self, n_qubits: int, offsets, qubits, kinds, phases, coeffs):
+        self._n = int(n_qubits)
+        self._p = (np.ascontiguousarray(offsets, np.uint32), np.ascontiguousarray(qubits, np.uint32),
+                   np.ascontiguousarray(kinds, np.uint8), np.ascontiguousarray(phases, np.uint8),
+                   np.ascontiguousarray(coeffs, np.float64))
+
+    def n_qubits(self) -> int:
+        return self._n
+
+    def size(self) -> int:
+        return len(self._p[3])
+
+    def packed(self):
+        return self._p
+
+    def to_operator(self) -> "PauliOperator":
+        offs, qs, ks, ph, co = self._p
+        op = PauliOperator(n_qubits=self._n)
+        for s in range(self.size()):
+            terms = [(int(qs[j]), int(ks[j])) for j in range(offs[s], offs[s + 1])]
+            op.add(complex(co[2 * s], co[2 * s + 1]), SparsePauliString(terms, self._n, PauliPhase(int(ph[s]))))
+        return op
+
+
 def expectation_value(op, state: Statevector) -> complex:
     """kettle's ``expectation_value`` for a ``PauliOperator`` or a single ``SparsePauliString``."""
     if isinstance(op, SparsePauliString):

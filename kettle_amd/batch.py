@@ -1,0 +1,97 @@
+"""Batched energy + adjoint gradient of many independent parameterised circuits (kq_batch_*).
+
+    b = EnergyGradientBatch(KQ_C64)
+    for circuit, hamiltonian in instances:
+        b.add(circuit, hamiltonian)              # parameter order: the circuit's creation order
+    r = b.run()                                  # r.energies[i], r.gradients[i], r.drift[i]
+    b.set_parameters(i, new_values); r = b.run() # later runs replay a captured hipGraph
+
+Each instance is kettle's VQE cost function (vqe_basic.cpp:61-82): a straight-line parameterised
+circuit on |0...0> and the expectation of a Pauli operator, here with its full gradient.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import KQ_C64, check, load, ptr
+from .api import PauliOperator
+from .circuit import QuantumCircuit
+
+
+@dataclass
+class BatchResult:
+    energies: np.ndarray           # (n_instances,)
+    gradients: list[np.ndarray]    # per instance, in its parameter order
+    drift: np.ndarray              # max |psi - |0...0>| after the reverse sweep, per instance
+
+
+class EnergyGradientBatch:
+    def __init__(self, precision: int = KQ_C64):
+        self.precision = precision
+        self._h = ctypes.c_void_p()
+        check(load().kq_batch_create(ctypes.byref(self._h), int(precision)))
+        self._n_params: list[int] = []
+
+    def __del__(self):
+        self.close()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().kq_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __len__(self) -> int:
+        return len(self._n_params)
+
+    def add(self, circuit: QuantumCircuit, hamiltonian: PauliOperator, params: Sequence[str] | None = None) -> int:
+        ops, _, order = circuit.flatten(params)
+        if any(int(g) >= 30 for g in ops["gate"]):
+            raise RuntimeError("batch: circuits must be straight-line (no M, control flow or loggers)")
+        offs, qs, ks, ph, co = hamiltonian.packed()
+        idx = ctypes.c_size_t()
+        check(load().kq_batch_add(self._h, circuit.n_qubits(), ptr(ops), len(ops), len(order), hamiltonian.size(),
+                                  ptr(offs), ptr(qs), ptr(ks), ptr(ph), ptr(co), ctypes.byref(idx)))
+        self._n_params.append(len(order))
+        return idx.value
+
+    def add_ops(self, n_qubits: int, ops: np.ndarray, n_params: int, hamiltonian: PauliOperator) -> int:
+        """An instance given as a flat op array (aux = parameter slot, angle = its initial value)."""
+        ops = np.ascontiguousarray(ops)
+        offs, qs, ks, ph, co = hamiltonian.packed()
+        idx = ctypes.c_size_t()
+        check(load().kq_batch_add(self._h, int(n_qubits), ptr(ops), len(ops), int(n_params), hamiltonian.size(),
+                                  ptr(offs), ptr(qs), ptr(ks), ptr(ph), ptr(co), ctypes.byref(idx)))
+        self._n_params.append(int(n_params))
+        return idx.value
+
+    def set_parameters(self, index: int, values) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.size != self._n_params[index]:
+            raise ValueError(f"instance {index} has {self._n_params[index]} parameters, got {v.size}")
+        check(load().kq_batch_set_parameters(self._h, int(index), ptr(v)))
+
+    def run(self, use_graph: bool = True) -> BatchResult:
+        n = len(self._n_params)
+        energies = np.zeros(n)
+        drift = np.zeros(n)
+        grads = np.zeros(max(sum(self._n_params), 1))
+        check(load().kq_batch_run(self._h, int(bool(use_graph)), ptr(energies), ptr(grads), ptr(drift)))
+        out = []
+        o = 0
+        for p in self._n_params:
+            out.append(grads[o:o + p].copy())
+            o += p
+        return BatchResult(energies, out, drift)
+
+    def info(self) -> dict:
+        a, b, c, d = (ctypes.c_size_t() for _ in range(4))
+        check(load().kq_batch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))
+        return {"instances": a.value, "parameters": b.value, "device_bytes": c.value, "launches": d.value}
+
+
+__all__ = ["EnergyGradientBatch", "BatchResult", "_lib"]

@@ -22,7 +22,7 @@ BUILD_DIR = ROOT / "kettle_amd" / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp"]
+KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_batch.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -520,6 +520,8 @@ Canon canonical(const uint32_t* q, const uint8_t* k, size_t nt, uint32_t n)
 
 const std::complex<double> PHASES[4] = {{1.0, 0.0}, {0.0, 1.0}, {-1.0, 0.0}, {0.0, -1.0}};
 
+}  // namespace
+
 void build_pauli(uint32_t n, size_t n_strings, const uint32_t* off, const uint32_t* q, const uint8_t* k,
                  const uint8_t* phases, const double* coeffs, std::vector<PauliGroup>& groups,
                  std::vector<PauliZ>& strings)
@@ -600,6 +602,9 @@ bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d)
     }
     return false;
 }
+
+namespace
+{
 
 // One segment of the reverse sweep through fused adjoint tile passes.  inv[i] is the inverse of the
 // i-th gate in reverse order and deriv[i] its derivative (or null); values[i] receives
@@ -1365,6 +1370,71 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
         KQ_HIP_CHECK(hipDeviceSynchronize());
         KQ_HIP_CHECK(hipMemcpy(out, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
         KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_batch_create(kq_batch** out, int precision)
+{
+    return guarded([&] {
+        if (!out || (precision != KQ_C128 && precision != KQ_C64)) {
+            throw InvalidArg {"batch: bad arguments"};
+        }
+        ensure_init();
+        hipStream_t st = nullptr;
+        KQ_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        *out = batch_create(precision, st);
+    });
+}
+
+int kq_batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params, size_t n_strings,
+                 const uint32_t* term_offsets, const uint32_t* term_qubits, const uint8_t* term_kinds, const uint8_t* phases,
+                 const double* coeffs, size_t* index)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_add(b, n_qubits, ops, n_ops, n_params, n_strings, term_offsets, term_qubits, term_kinds, phases, coeffs, index);
+    });
+}
+
+int kq_batch_set_parameters(kq_batch* b, size_t index, const double* values)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_set_parameters(b, index, values);
+    });
+}
+
+int kq_batch_run(kq_batch* b, int use_graph, double* energies, double* grads, double* drift)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        ensure_init();
+        batch_run(b, use_graph != 0, energies, grads, drift);
+    });
+}
+
+int kq_batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params_total, size_t* device_bytes, size_t* n_launches)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_info(b, n_instances, n_params_total, device_bytes, n_launches);
+    });
+}
+
+int kq_batch_destroy(kq_batch* b)
+{
+    return guarded([&] {
+        if (b) {
+            batch_destroy(b);
+        }
     });
 }
 

@@ -1,0 +1,661 @@
+// Batched energy + adjoint gradient of many independent parameterised circuits of uneven qubit
+// counts (the VQE inner loop of kettle's optimize/ drivers, run for thousands of instances at once).
+//
+// Every instance is planned once into fused tile passes (forward) and adjoint tile passes (reverse
+// sweep with derivative partials), with angle-independent gate kinds so that new parameter values
+// only re-lower coefficients.  Execution is a fixed sequence of launches over all instances:
+//   reset every state to |0...0>
+//   forward round r of each tile-size group: pass r of every instance that has one, in one launch
+//   lambda = H psi and the energy partials of every instance, in one launch
+//   reverse round r of each group, then one FP64 row sum of that round's derivative partials
+//   distance of every uncomputed state from |0...0> (inverse-gate drift), in one launch
+// The sequence contains no allocation, no host synchronisation and no memset, so after a first
+// eager run it is captured in a hipGraph and replayed by later calls.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+
+struct BatchInstance
+{
+    uint32_t n = 0;                    // qubits as given
+    uint32_t nv = 0;                   // qubits simulated (at least 4: one register block)
+    std::vector<kq_op> ops;
+    size_t n_params = 0;
+    std::vector<double> params;
+    std::vector<PauliGroup> groups;
+    std::vector<PauliZ> strings;
+    // plan
+    int k = 0;
+    std::vector<TilePass> fwd, bwd;
+    std::vector<uint32_t> bwd_gate;    // reversed position -> gate index
+    std::vector<char> has_d;
+    // lowered programs and their byte offsets in the program arena
+    std::vector<TileProgram> fwd_prog, bwd_prog;
+    std::vector<size_t> fwd_off, bwd_off;
+    uint64_t amp_off = 0;              // offset of psi / lambda in the arenas (amplitudes)
+    uint64_t n_amps = 0;
+    size_t grad_off = 0;               // first gradient entry of the instance in the output
+    size_t pauli_off = 0;              // byte offset of the groups (strings follow) in the Pauli arena
+};
+
+struct BatchRound
+{
+    bool adjoint = false;
+    int k = 0;
+    uint32_t n_items = 0;
+    uint32_t n_blocks = 0;
+    size_t items_off = 0;              // byte offset of the BatchTileItem array in the table arena
+    // adjoint rounds: one row per derivative slot
+    uint32_t n_rows = 0;
+    size_t row_off_off = 0;            // byte offsets of the uint64 row offsets / uint32 row counts
+    size_t row_cnt_off = 0;
+    size_t row_base = 0;               // first global row of the round
+};
+
+}  // namespace kq
+
+struct kq_batch
+{
+    int precision = KQ_C64;
+    hipStream_t stream = nullptr;
+    std::vector<kq::BatchInstance> inst;
+    bool planned = false;
+    bool coeffs_dirty = true;
+    // plan-wide tables
+    std::vector<kq::BatchRound> rounds;
+    std::vector<std::pair<uint32_t, uint32_t>> rows;   // global derivative row -> (instance, gate index)
+    uint64_t total_amps = 0;
+    uint64_t part_doubles = 0;          // largest adjoint round's partials
+    uint32_t pauli_blocks = 0;
+    size_t pauli_items_off = 0;
+    size_t e_row_off_off = 0, e_row_cnt_off = 0;
+    size_t ones_off = 0;
+    size_t prog_bytes = 0;
+    size_t table_bytes = 0;
+    size_t pauli_bytes = 0;
+    // device memory
+    void* psi = nullptr;
+    void* lam = nullptr;
+    void* prog = nullptr;
+    void* tables = nullptr;
+    void* pauli = nullptr;
+    double* part = nullptr;             // derivative partials (largest round), then energy partials
+    double* slot_values = nullptr;
+    double* energies = nullptr;
+    double* drift = nullptr;
+    // host staging of the program arena
+    std::vector<unsigned char> prog_host;
+    // captured launch sequence
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    size_t n_launches = 0;
+};
+
+namespace kq
+{
+namespace
+{
+
+size_t align256(size_t x) { return (x + 255) & ~size_t {255}; }
+
+size_t program_bytes(const TileProgram& tp)
+{
+    return align256(tp.gates.size() * sizeof(TileGate)) + align256(tp.steps.size() * sizeof(TileStep))
+         + align256(tp.layouts.size() * sizeof(TileLayout)) + align256(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+}
+
+void write_program(const TileProgram& tp, unsigned char* dst)
+{
+    std::memcpy(dst, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+    dst += align256(tp.gates.size() * sizeof(TileGate));
+    std::memcpy(dst, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+    dst += align256(tp.steps.size() * sizeof(TileStep));
+    std::memcpy(dst, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+    dst += align256(tp.layouts.size() * sizeof(TileLayout));
+    std::memcpy(dst, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
+}
+
+// records of the instance's current angles: forward records, inverse records in reverse order and
+// derivative records (indexed like the reversed list)
+void compile_instance(const BatchInstance& in, std::vector<GateRec>& recs, std::vector<GateRec>& inv,
+                      std::vector<GateRec>& derivs, std::vector<char>& has_d)
+{
+    const size_t G = in.ops.size();
+    recs.clear();
+    recs.reserve(G);
+    for (const kq_op& op0 : in.ops) {
+        kq_op op = op0;
+        if (op.aux >= 0) {
+            op.angle = in.params[static_cast<size_t>(op.aux)];
+        }
+        compile_gate_stable(op, in.nv, recs);
+    }
+    inv.resize(G);
+    derivs.assign(G, GateRec {});
+    has_d.assign(G, 0);
+    for (size_t i = 0; i < G; ++i) {
+        const size_t gi = G - 1 - i;
+        inv[i] = inverse_of(recs[gi]);
+        kq_op op = in.ops[gi];
+        if (op.aux >= 0) {
+            op.angle = in.params[static_cast<size_t>(op.aux)];
+            has_d[i] = derivative_of(op, recs[gi], derivs[i]) ? 1 : 0;
+        }
+    }
+}
+
+void plan_instance(BatchInstance& in, int precision)
+{
+    std::vector<GateRec> recs, inv, derivs;
+    std::vector<char> has_d;
+    compile_instance(in, recs, inv, derivs, has_d);
+    in.k = fusion_max_k(precision, in.nv);
+    if (in.k < register_qubits(precision)) {
+        throw InvalidArg {"batch: the tile does not hold one register block"};
+    }
+    in.fwd = plan_passes(recs, in.nv, in.k, 5, fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
+    in.bwd = plan_passes(inv, in.nv, in.k, 5, fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
+    in.has_d = has_d;
+    in.bwd_gate.resize(inv.size());
+    for (size_t i = 0; i < inv.size(); ++i) {
+        in.bwd_gate[i] = static_cast<uint32_t>(inv.size() - 1 - i);
+    }
+}
+
+// lower the instance's passes with its current angles (the plan and the program sizes are fixed)
+void lower_instance(BatchInstance& in, int precision)
+{
+    std::vector<GateRec> recs, inv, derivs;
+    std::vector<char> has_d;
+    compile_instance(in, recs, inv, derivs, has_d);
+    std::vector<const GateRec*> dptr(inv.size(), nullptr);
+    for (size_t i = 0; i < inv.size(); ++i) {
+        if (has_d[i]) {
+            dptr[i] = &derivs[i];
+        }
+    }
+    const auto patch = [](TilePass& p, const std::vector<GateRec>& src) {
+        for (size_t j = 0; j < p.gates.size(); ++j) {
+            std::memcpy(p.gates[j].m, src[p.src[j]].m, sizeof p.gates[j].m);
+        }
+    };
+    in.fwd_prog.resize(in.fwd.size());
+    for (size_t p = 0; p < in.fwd.size(); ++p) {
+        patch(in.fwd[p], recs);
+        lower_tile_pass(in.fwd[p], precision, in.nv, in.fwd_prog[p]);
+    }
+    in.bwd_prog.resize(in.bwd.size());
+    for (size_t p = 0; p < in.bwd.size(); ++p) {
+        patch(in.bwd[p], inv);
+        lower_tile_pass(in.bwd[p], precision, in.nv, in.bwd_prog[p], &dptr);
+    }
+}
+
+void* balloc(size_t bytes)
+{
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess || p == nullptr) {
+        (void) hipGetLastError();
+        throw AllocError {"batch: hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    return p;
+}
+
+void release(kq_batch* b)
+{
+    if (b->graph_exec) {
+        (void) hipGraphExecDestroy(b->graph_exec);
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        (void) hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    for (void** p : {&b->psi, &b->lam, &b->prog, &b->tables, &b->pauli}) {
+        if (*p) {
+            (void) hipFree(*p);
+            *p = nullptr;
+        }
+    }
+    for (double** p : {&b->part, &b->slot_values, &b->energies, &b->drift}) {
+        if (*p) {
+            (void) hipFree(*p);
+            *p = nullptr;
+        }
+    }
+}
+
+uint64_t free_mask_of(const TileProgram& tp)
+{
+    uint64_t m = 0;
+    for (const uint8_t q : tp.freepos) {
+        m |= uint64_t {1} << q;
+    }
+    return m;
+}
+
+// plan every instance, place states, programs and tables, allocate device memory
+void build(kq_batch* b)
+{
+    release(b);
+    b->rounds.clear();
+    b->rows.clear();
+    const size_t ab = amp_bytes(b->precision);
+    // instances: plan, place states and programs
+    uint64_t amps = 0;
+    size_t prog = 0, pauli = 0, grad = 0;
+    for (BatchInstance& in : b->inst) {
+        plan_instance(in, b->precision);
+        lower_instance(in, b->precision);
+        in.amp_off = amps;
+        in.n_amps = uint64_t {1} << in.nv;
+        amps += in.n_amps;
+        in.fwd_off.clear();
+        in.bwd_off.clear();
+        for (const TileProgram& tp : in.fwd_prog) {
+            in.fwd_off.push_back(prog);
+            prog += program_bytes(tp);
+        }
+        for (const TileProgram& tp : in.bwd_prog) {
+            in.bwd_off.push_back(prog);
+            prog += program_bytes(tp);
+        }
+        in.pauli_off = pauli;
+        pauli += align256(in.groups.size() * sizeof(PauliGroup)) + align256(in.strings.size() * sizeof(PauliZ));
+        in.grad_off = grad;
+        grad += in.n_params;
+    }
+    b->total_amps = amps;
+    b->prog_bytes = std::max<size_t>(prog, 256);
+    b->pauli_bytes = std::max<size_t>(pauli, 256);
+    // rounds, per tile size (which fixes the workgroup size)
+    std::vector<int> ks;
+    for (const BatchInstance& in : b->inst) {
+        ks.push_back(in.k);
+    }
+    std::sort(ks.begin(), ks.end());
+    ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    std::vector<unsigned char> tables;
+    const auto put = [&](const void* src, size_t bytes) {
+        const size_t off = tables.size();
+        tables.resize(off + align256(std::max<size_t>(bytes, 1)), 0);
+        std::memcpy(tables.data() + off, src, bytes);
+        return off;
+    };
+    std::vector<std::vector<BatchTileItem>> round_items;
+    std::vector<std::vector<uint64_t>> round_row_off;
+    std::vector<std::vector<uint32_t>> round_row_cnt;
+    b->part_doubles = 1;
+    const auto add_rounds = [&](bool adjoint) {
+        for (const int k : ks) {
+            size_t n_rounds = 0;
+            for (const BatchInstance& in : b->inst) {
+                if (in.k == k) {
+                    n_rounds = std::max(n_rounds, adjoint ? in.bwd_prog.size() : in.fwd_prog.size());
+                }
+            }
+            for (size_t r = 0; r < n_rounds; ++r) {
+                BatchRound rd;
+                rd.adjoint = adjoint;
+                rd.k = k;
+                std::vector<BatchTileItem> items;
+                std::vector<uint64_t> roff;
+                std::vector<uint32_t> rcnt;
+                uint32_t blocks = 0;
+                uint64_t part = 0;
+                for (uint32_t ii = 0; ii < b->inst.size(); ++ii) {
+                    const BatchInstance& in = b->inst[ii];
+                    const auto& progs = adjoint ? in.bwd_prog : in.fwd_prog;
+                    if (in.k != k || r >= progs.size()) {
+                        continue;
+                    }
+                    const TileProgram& tp = progs[r];
+                    const size_t off = adjoint ? in.bwd_off[r] : in.fwd_off[r];
+                    BatchTileItem it {};
+                    it.a_off = in.amp_off;
+                    it.b_off = in.amp_off;
+                    // device addresses are filled in once the program arena exists (offsets here)
+                    it.gates = off;
+                    it.steps = off + align256(tp.gates.size() * sizeof(TileGate));
+                    it.layouts = it.steps + align256(tp.steps.size() * sizeof(TileStep));
+                    it.derivs = it.layouts + align256(tp.layouts.size() * sizeof(TileLayout));
+                    it.free_mask = free_mask_of(tp);
+                    it.n_thr_bits = tp.n_thr_bits;
+                    it.n_steps = static_cast<int32_t>(tp.steps.size());
+                    it.out_layout = tp.out_layout;
+                    it.block_begin = blocks;
+                    const uint32_t nb = tile_pass_blocks(tp);
+                    it.n_part = nb * tile_pass_waves(tp);
+                    it.partials = part;   // offset in doubles, made an address later
+                    for (size_t sl = 0; sl < tp.derivs.size(); ++sl) {
+                        roff.push_back(part + sl * it.n_part);
+                        rcnt.push_back(it.n_part);
+                        b->rows.emplace_back(ii, in.bwd_gate[tp.deriv_src[sl]]);
+                    }
+                    part += tp.derivs.size() * it.n_part;
+                    blocks += nb;
+                    items.push_back(it);
+                }
+                rd.n_items = static_cast<uint32_t>(items.size());
+                rd.n_blocks = blocks;
+                rd.n_rows = static_cast<uint32_t>(roff.size());
+                rd.row_base = b->rows.size() - roff.size();
+                b->part_doubles = std::max<uint64_t>(b->part_doubles, part);
+                b->rounds.push_back(rd);
+                round_items.push_back(std::move(items));
+                round_row_off.push_back(std::move(roff));
+                round_row_cnt.push_back(std::move(rcnt));
+            }
+        }
+    };
+    add_rounds(false);
+    add_rounds(true);
+    // Pauli / read-out items and energy rows
+    std::vector<BatchPauliItem> pitems;
+    std::vector<uint64_t> e_off;
+    std::vector<uint32_t> e_cnt;
+    std::vector<uint64_t> ones;
+    uint32_t pblocks = 0;
+    for (const BatchInstance& in : b->inst) {
+        BatchPauliItem it {};
+        it.a_off = in.amp_off;
+        it.b_off = in.amp_off;
+        it.n_amps = in.n_amps;
+        it.groups = in.pauli_off;
+        it.zs = in.pauli_off + align256(in.groups.size() * sizeof(PauliGroup));
+        it.n_groups = static_cast<uint32_t>(in.groups.size());
+        it.block_begin = pblocks;
+        it.part_off = pblocks;
+        const uint32_t nb = batch_blocks(in.n_amps);
+        e_off.push_back(pblocks);
+        e_cnt.push_back(nb);
+        pblocks += nb;
+        pitems.push_back(it);
+        ones.push_back(in.amp_off);
+    }
+    b->pauli_blocks = pblocks;
+    const uint64_t part_total = b->part_doubles + pblocks;
+    // device memory
+    b->psi = balloc(b->total_amps * ab);
+    b->lam = balloc(b->total_amps * ab);
+    b->prog = balloc(b->prog_bytes);
+    b->pauli = balloc(b->pauli_bytes);
+    b->part = static_cast<double*>(balloc(part_total * sizeof(double)));
+    b->slot_values = static_cast<double*>(balloc(std::max<size_t>(b->rows.size(), 1) * sizeof(double)));
+    b->energies = static_cast<double*>(balloc(b->inst.size() * sizeof(double)));
+    b->drift = static_cast<double*>(balloc(b->inst.size() * sizeof(double)));
+    // fix up device addresses and write the tables
+    const auto prog_base = reinterpret_cast<uint64_t>(b->prog);
+    const auto part_base = reinterpret_cast<uint64_t>(b->part);
+    for (size_t ri = 0; ri < b->rounds.size(); ++ri) {
+        for (BatchTileItem& it : round_items[ri]) {
+            it.gates += prog_base;
+            it.steps += prog_base;
+            it.layouts += prog_base;
+            it.derivs += prog_base;
+            it.partials = part_base + it.partials * sizeof(double);
+        }
+        b->rounds[ri].items_off = put(round_items[ri].data(), round_items[ri].size() * sizeof(BatchTileItem));
+        b->rounds[ri].row_off_off = put(round_row_off[ri].data(), round_row_off[ri].size() * sizeof(uint64_t));
+        b->rounds[ri].row_cnt_off = put(round_row_cnt[ri].data(), round_row_cnt[ri].size() * sizeof(uint32_t));
+    }
+    const auto pauli_base = reinterpret_cast<uint64_t>(b->pauli);
+    for (BatchPauliItem& it : pitems) {
+        it.groups += pauli_base;
+        it.zs += pauli_base;
+        it.part_off += b->part_doubles;   // energy partials follow the derivative partials
+    }
+    for (uint64_t& o : e_off) {
+        o += b->part_doubles;
+    }
+    b->pauli_items_off = put(pitems.data(), pitems.size() * sizeof(BatchPauliItem));
+    b->e_row_off_off = put(e_off.data(), e_off.size() * sizeof(uint64_t));
+    b->e_row_cnt_off = put(e_cnt.data(), e_cnt.size() * sizeof(uint32_t));
+    b->ones_off = put(ones.data(), ones.size() * sizeof(uint64_t));
+    b->table_bytes = tables.size();
+    b->tables = balloc(b->table_bytes);
+    KQ_HIP_CHECK(hipMemcpyAsync(b->tables, tables.data(), tables.size(), hipMemcpyHostToDevice, b->stream));
+    // Pauli tables
+    std::vector<unsigned char> ph(b->pauli_bytes, 0);
+    for (const BatchInstance& in : b->inst) {
+        std::memcpy(ph.data() + in.pauli_off, in.groups.data(), in.groups.size() * sizeof(PauliGroup));
+        std::memcpy(ph.data() + in.pauli_off + align256(in.groups.size() * sizeof(PauliGroup)), in.strings.data(),
+                    in.strings.size() * sizeof(PauliZ));
+    }
+    KQ_HIP_CHECK(hipMemcpyAsync(b->pauli, ph.data(), ph.size(), hipMemcpyHostToDevice, b->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    b->prog_host.assign(b->prog_bytes, 0);
+    b->planned = true;
+    b->coeffs_dirty = true;
+}
+
+void upload_programs(kq_batch* b, bool relower)
+{
+    for (BatchInstance& in : b->inst) {
+        if (relower) {
+            lower_instance(in, b->precision);
+        }
+        for (size_t p = 0; p < in.fwd_prog.size(); ++p) {
+            write_program(in.fwd_prog[p], b->prog_host.data() + in.fwd_off[p]);
+        }
+        for (size_t p = 0; p < in.bwd_prog.size(); ++p) {
+            write_program(in.bwd_prog[p], b->prog_host.data() + in.bwd_off[p]);
+        }
+    }
+    KQ_HIP_CHECK(hipMemcpyAsync(b->prog, b->prog_host.data(), b->prog_bytes, hipMemcpyHostToDevice, b->stream));
+    b->coeffs_dirty = false;
+}
+
+// the launch sequence (no allocation, no synchronisation: capturable)
+void enqueue(kq_batch* b)
+{
+    auto* tab = static_cast<unsigned char*>(b->tables);
+    const uint32_t n = static_cast<uint32_t>(b->inst.size());
+    size_t launches = 0;
+    launch_batch_reset(b->stream, b->precision, b->psi, b->total_amps, reinterpret_cast<const uint64_t*>(tab + b->ones_off), n);
+    launches += 2;
+    for (const BatchRound& rd : b->rounds) {
+        if (rd.adjoint) {
+            continue;
+        }
+        launch_tile_batch(b->stream, b->precision, false, b->psi, nullptr,
+                          reinterpret_cast<const BatchTileItem*>(tab + rd.items_off), rd.n_items, rd.n_blocks, rd.k);
+        ++launches;
+    }
+    launch_pauli_batch(b->stream, b->precision, b->psi, b->lam, reinterpret_cast<const BatchPauliItem*>(tab + b->pauli_items_off),
+                       n, b->pauli_blocks, b->part);
+    launch_sum_rows(b->stream, b->part, reinterpret_cast<const uint64_t*>(tab + b->e_row_off_off),
+                    reinterpret_cast<const uint32_t*>(tab + b->e_row_cnt_off), n, b->energies);
+    launches += 2;
+    for (const BatchRound& rd : b->rounds) {
+        if (!rd.adjoint) {
+            continue;
+        }
+        launch_tile_batch(b->stream, b->precision, true, b->psi, b->lam,
+                          reinterpret_cast<const BatchTileItem*>(tab + rd.items_off), rd.n_items, rd.n_blocks, rd.k);
+        launch_sum_rows(b->stream, b->part, reinterpret_cast<const uint64_t*>(tab + rd.row_off_off),
+                        reinterpret_cast<const uint32_t*>(tab + rd.row_cnt_off), rd.n_rows, b->slot_values + rd.row_base);
+        launches += rd.n_rows ? 2 : 1;
+    }
+    launch_basis_distance_batch(b->stream, b->precision, b->psi, reinterpret_cast<const BatchPauliItem*>(tab + b->pauli_items_off),
+                                n, b->pauli_blocks, b->part);
+    launch_max_rows(b->stream, b->part, reinterpret_cast<const uint64_t*>(tab + b->e_row_off_off),
+                    reinterpret_cast<const uint32_t*>(tab + b->e_row_cnt_off), n, b->drift);
+    launches += 2;
+    b->n_launches = launches;
+}
+
+}  // namespace
+
+void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params, size_t n_strings,
+               const uint32_t* off, const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs,
+               size_t* index)
+{
+    if (n_qubits == 0 || n_qubits > 34) {
+        throw InvalidArg {"batch: n_qubits must be 1..34"};
+    }
+    if (n_ops > 0 && ops == nullptr) {
+        throw InvalidArg {"batch: null op array"};
+    }
+    BatchInstance in;
+    in.n = n_qubits;
+    in.nv = std::max<uint32_t>(n_qubits, 4);
+    in.ops.assign(ops, ops + n_ops);
+    in.n_params = n_params;
+    in.params.assign(n_params, 0.0);
+    std::vector<char> seen(n_params, 0);
+    for (size_t i = 0; i < n_ops; ++i) {
+        const kq_op& op = ops[i];
+        if (op.gate < 0 || op.gate >= KQ_M) {
+            throw InvalidArg {"batch: op " + std::to_string(i) + " is not a unitary gate"};
+        }
+        const bool ctrl = (op.gate >= KQ_CH && op.gate <= KQ_CP) || op.gate == KQ_CU;
+        const int32_t t = ctrl ? op.q1 : op.q0;
+        if (t < 0 || static_cast<uint32_t>(t) >= n_qubits || (ctrl && (op.q0 < 0 || static_cast<uint32_t>(op.q0) >= n_qubits))) {
+            throw InvalidArg {"batch: gate qubit index out of range"};
+        }
+        if (op.aux >= 0) {
+            if (static_cast<size_t>(op.aux) >= n_params) {
+                throw InvalidArg {"batch: parameter index out of range"};
+            }
+            if (!seen[static_cast<size_t>(op.aux)]) {
+                in.params[static_cast<size_t>(op.aux)] = op.angle;
+                seen[static_cast<size_t>(op.aux)] = 1;
+            }
+        }
+    }
+    build_pauli(n_qubits, n_strings, off, q, k, phases, coeffs, in.groups, in.strings);
+    b->inst.push_back(std::move(in));
+    b->planned = false;
+    if (index) {
+        *index = b->inst.size() - 1;
+    }
+}
+
+void batch_set_parameters(kq_batch* b, size_t index, const double* values)
+{
+    if (index >= b->inst.size()) {
+        throw InvalidArg {"batch: instance index out of range"};
+    }
+    BatchInstance& in = b->inst[index];
+    if (in.n_params > 0 && values == nullptr) {
+        throw InvalidArg {"batch: null parameter values"};
+    }
+    in.params.assign(values, values + in.n_params);
+    b->coeffs_dirty = true;
+}
+
+void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, double* drift)
+{
+    if (b->inst.empty()) {
+        throw InvalidArg {"batch: no instances"};
+    }
+    const bool fresh = !b->planned;
+    if (fresh) {
+        build(b);
+    }
+    if (b->coeffs_dirty) {
+        upload_programs(b, !fresh);
+    }
+    if (use_graph && b->graph_exec == nullptr && !fresh) {
+        KQ_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueue(b);
+        }
+        catch (...) {
+            hipGraph_t g = nullptr;
+            (void) hipStreamEndCapture(b->stream, &g);
+            if (g) {
+                (void) hipGraphDestroy(g);
+            }
+            throw;
+        }
+        KQ_HIP_CHECK(hipStreamEndCapture(b->stream, &b->graph));
+        KQ_HIP_CHECK(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    }
+    if (use_graph && b->graph_exec != nullptr) {
+        KQ_HIP_CHECK(hipGraphLaunch(b->graph_exec, b->stream));
+    }
+    else {
+        enqueue(b);   // first run eager: kernel attributes are set before any capture
+    }
+    const size_t n = b->inst.size();
+    std::vector<double> e(n), dr(n), sv(b->rows.size());
+    KQ_HIP_CHECK(hipMemcpyAsync(e.data(), b->energies, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    KQ_HIP_CHECK(hipMemcpyAsync(dr.data(), b->drift, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    if (!sv.empty()) {
+        KQ_HIP_CHECK(hipMemcpyAsync(sv.data(), b->slot_values, sv.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(b->stream));
+    if (energies) {
+        std::copy(e.begin(), e.end(), energies);
+    }
+    if (drift) {
+        std::copy(dr.begin(), dr.end(), drift);
+    }
+    if (grads) {
+        size_t total = 0;
+        for (const BatchInstance& in : b->inst) {
+            total += in.n_params;
+        }
+        std::fill(grads, grads + total, 0.0);
+        // rows are in a fixed order, so the sums over gates sharing a parameter are deterministic
+        for (size_t r = 0; r < b->rows.size(); ++r) {
+            const BatchInstance& in = b->inst[b->rows[r].first];
+            const kq_op& op = in.ops[b->rows[r].second];
+            grads[in.grad_off + static_cast<size_t>(op.aux)] += 2.0 * sv[r];
+        }
+    }
+}
+
+kq_batch* batch_create(int precision, hipStream_t stream)
+{
+    auto* b = new kq_batch;
+    b->precision = precision;
+    b->stream = stream;
+    return b;
+}
+
+void batch_destroy(kq_batch* b)
+{
+    if (b->stream) {
+        (void) hipStreamSynchronize(b->stream);
+    }
+    release(b);
+    if (b->stream) {
+        (void) hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+void batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params, size_t* device_bytes, size_t* n_launches)
+{
+    size_t np = 0;
+    for (const BatchInstance& in : b->inst) {
+        np += in.n_params;
+    }
+    if (n_instances) {
+        *n_instances = b->inst.size();
+    }
+    if (n_params) {
+        *n_params = np;
+    }
+    if (device_bytes) {
+        *device_bytes = b->planned ? 2 * b->total_amps * amp_bytes(b->precision) + b->prog_bytes + b->table_bytes + b->pauli_bytes
+                                         + (b->part_doubles + b->pauli_blocks + b->rows.size() + 2 * b->inst.size()) * sizeof(double)
+                                   : 0;
+    }
+    if (n_launches) {
+        *n_launches = b->n_launches;
+    }
+}
+
+}  // namespace kq

@@ -127,6 +127,21 @@ struct TileStep
     uint32_t pad;
 };
 
+// One state of a batched tile launch (device memory; pointers are device addresses).
+struct BatchTileItem
+{
+    uint64_t a_off;          // amplitude offset of the state (psi) in the first arena
+    uint64_t b_off;          // amplitude offset of lambda in the second arena (adjoint launches)
+    uint64_t steps, layouts, gates, derivs, partials;
+    uint64_t free_mask;
+    int32_t n_thr_bits;
+    int32_t n_steps;
+    uint32_t out_layout;
+    uint32_t n_part;
+    uint32_t block_begin;
+    uint32_t pad;
+};
+
 struct TileProgram
 {
     int k = 0;
@@ -217,6 +232,8 @@ void profiler_collect();   // read and release pending events (after a stream sy
 // ---- gate compilation and planning (kq_plan.cpp) --------------------------------------------
 // kettle gate (kq_op) -> one (statevector) or two (density matrix) device records
 void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>& out);
+// statevector record whose kind depends on the gate type only for parameterised gates (aux >= 0)
+void compile_gate_stable(const kq_op& op, uint32_t n, std::vector<GateRec>& out);
 bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
 // allow_sweep: a pass holding a single diagonal gate may run as a plain sweep (off for adjoint plans)
@@ -245,6 +262,9 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
 // adjoint reverse-sweep pass over (psi, lam); partials: [slot][blocks * waves] doubles
 void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgram& prog, const void* d_gates,
                               const void* d_steps, const void* d_layouts, const void* d_derivs, double* d_partials);
+// many states in one launch (k: tile qubits of every item; items sorted by block_begin)
+void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* arena_a, void* arena_b,
+                       const BatchTileItem* d_items, uint32_t n_items, uint32_t n_blocks, int k);
 // out[i] = sum of rows[offs[i] .. offs[i] + counts[i]) in a fixed order (one row per slot)
 void launch_sum_rows(hipStream_t st, const double* rows, const uint64_t* d_offs, const uint32_t* d_counts, uint32_t n,
                      double* d_out);
@@ -273,6 +293,37 @@ struct PauliZ
     uint64_t z;          // sign mask
     double w[2];         // total complex weight (coefficient * phase * canonical factor)
 };
+// Pauli operator in device form: strings grouped by X/Y flip mask (kq_pauli_expectation's packing)
+void build_pauli(uint32_t n, size_t n_strings, const uint32_t* off, const uint32_t* q, const uint8_t* k,
+                 const uint8_t* phases, const double* coeffs, std::vector<PauliGroup>& groups,
+                 std::vector<PauliZ>& strings);
+// adjoint helpers: G^dagger, and dG/dtheta of an angle gate (false for gates without an angle)
+GateRec inverse_of(const GateRec& g);
+bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d);
+
+// One state of a batched Pauli / read-out launch (device memory).
+struct BatchPauliItem
+{
+    uint64_t a_off;          // amplitude offset of psi
+    uint64_t b_off;          // amplitude offset of lambda
+    uint64_t n_amps;
+    uint64_t groups, zs;     // device addresses of the state's PauliGroup / PauliZ tables
+    uint64_t part_off;       // first partial of the state
+    uint32_t n_groups;
+    uint32_t block_begin;
+};
+uint32_t batch_blocks(uint64_t n_amps);   // blocks of one state in a batched Pauli / read-out launch
+// lambda = H psi per state and per-block partials of Re <psi|lambda>
+void launch_pauli_batch(hipStream_t st, int precision, const void* arena_a, void* arena_b, const BatchPauliItem* d_items,
+                        uint32_t n_items, uint32_t n_blocks, double* d_part);
+// per-block partials of max |psi_i - delta_{i0}|
+void launch_basis_distance_batch(hipStream_t st, int precision, const void* arena, const BatchPauliItem* d_items,
+                                 uint32_t n_items, uint32_t n_blocks, double* d_part);
+// every state of the arena to |0...0> (zero fill, then a one at each state's offset)
+void launch_batch_reset(hipStream_t st, int precision, void* arena, uint64_t n_amps, const uint64_t* d_offs, uint32_t n);
+void launch_max_rows(hipStream_t st, const double* rows, const uint64_t* d_offs, const uint32_t* d_counts, uint32_t n,
+                     double* d_out);
+
 // sum_i conj(a_i) * sum_g (sum_{s in g} w_s (-1)^{|i & z_s|}) a_{i ^ x_g}
 void launch_pauli_expectation(kq_state* s, const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings,
                               double out[2]);
@@ -298,5 +349,15 @@ void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
+
+// ---- batched energy + adjoint gradient (kq_batch.cpp) ------------------------------------------------
+kq_batch* batch_create(int precision, hipStream_t stream);
+void batch_destroy(kq_batch* b);
+void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params, size_t n_strings,
+               const uint32_t* off, const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs,
+               size_t* index);
+void batch_set_parameters(kq_batch* b, size_t index, const double* values);
+void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, double* drift);
+void batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params, size_t* device_bytes, size_t* n_launches);
 
 }  // namespace kq

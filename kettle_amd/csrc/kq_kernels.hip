@@ -757,6 +757,147 @@ __global__ __launch_bounds__(BLOCK) void k_dot(const Amp<R>* a, const Amp<R>* b,
     }
 }
 
+// ---- batched states -------------------------------------------------------------------------------
+// lambda = H psi for many states in one launch, and per-block partials of Re <psi|lambda>.  Item i
+// covers blocks [block_begin, next block_begin), BATCH_AMPS amplitudes per block.
+constexpr uint32_t BATCH_AMPS = 4 * BLOCK;
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict__ arena_a, Amp<R>* __restrict__ arena_b,
+                                                       const BatchPauliItem* __restrict__ items, uint32_t n_items,
+                                                       double* __restrict__ part)
+{
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (items[mid].block_begin <= blockIdx.x) {
+            lo = mid;
+        }
+        else {
+            hi = mid;
+        }
+    }
+    const BatchPauliItem& it = items[lo];
+    const uint32_t local = blockIdx.x - it.block_begin;
+    const Amp<R>* a = arena_a + it.a_off;
+    Amp<R>* b = arena_b + it.b_off;
+    const auto* groups = reinterpret_cast<const PauliGroup*>(it.groups);
+    const auto* zs = reinterpret_cast<const PauliZ*>(it.zs);
+    const uint64_t end = min(it.n_amps, static_cast<uint64_t>(local + 1) * BATCH_AMPS);
+    double e = 0.0;
+    for (uint64_t i = static_cast<uint64_t>(local) * BATCH_AMPS + threadIdx.x; i < end; i += BLOCK) {
+        double hr = 0.0, hi_ = 0.0;
+        for (uint32_t gi = 0; gi < it.n_groups; ++gi) {
+            const PauliGroup g = groups[gi];
+            double cr, ci;
+            pauli_coef<R>(i, g, zs, cr, ci);
+            const Amp<R> u = a[i ^ g.x];
+            const double br = u.x, bi = u.y;
+            hr += cr * br - ci * bi;
+            hi_ += cr * bi + ci * br;
+        }
+        const Amp<R> v = a[i];
+        e += static_cast<double>(v.x) * hr + static_cast<double>(v.y) * hi_;
+        Amp<R> w;
+        w.x = static_cast<R>(hr);
+        w.y = static_cast<R>(hi_);
+        b[i] = w;
+    }
+    const double2 r = block_sum2(e, 0.0);
+    if (threadIdx.x == 0) {
+        part[it.part_off + local] = r.x;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_fill_zero(Amp<R>* __restrict__ a, uint64_t n)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        Amp<R> z;
+        z.x = 0;
+        z.y = 0;
+        a[i] = z;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_set_ones(Amp<R>* __restrict__ a, const uint64_t* __restrict__ offs, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) {
+        Amp<R> one;
+        one.x = 1;
+        one.y = 0;
+        a[offs[i]] = one;
+    }
+}
+
+// max over |a_i - delta_{i,0}| per state: the distance of each state from |0...0>, per block
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_basis_distance_batch(const Amp<R>* __restrict__ arena,
+                                                                const BatchPauliItem* __restrict__ items, uint32_t n_items,
+                                                                double* __restrict__ part)
+{
+    __shared__ double red[BLOCK];
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (items[mid].block_begin <= blockIdx.x) {
+            lo = mid;
+        }
+        else {
+            hi = mid;
+        }
+    }
+    const BatchPauliItem& it = items[lo];
+    const uint32_t local = blockIdx.x - it.block_begin;
+    const Amp<R>* a = arena + it.a_off;
+    const uint64_t end = min(it.n_amps, static_cast<uint64_t>(local + 1) * BATCH_AMPS);
+    double m = 0.0;
+    for (uint64_t i = static_cast<uint64_t>(local) * BATCH_AMPS + threadIdx.x; i < end; i += BLOCK) {
+        const Amp<R> v = a[i];
+        const double re = static_cast<double>(v.x) - (i == 0 ? 1.0 : 0.0);
+        m = fmax(m, sqrt(re * re + static_cast<double>(v.y) * static_cast<double>(v.y)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[it.part_off + local] = red[0];
+    }
+}
+
+// out[i] = max of rows[offs[i] .. offs[i] + counts[i])
+__global__ __launch_bounds__(BLOCK) void k_max_rows(const double* __restrict__ rows, const uint64_t* __restrict__ offs,
+                                                    const uint32_t* __restrict__ counts, double* __restrict__ out)
+{
+    __shared__ double red[BLOCK];
+    const uint32_t row = blockIdx.x;
+    const double* p = rows + offs[row];
+    const uint32_t n = counts[row];
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        acc = fmax(acc, p[i]);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = BLOCK / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[row] = red[0];
+    }
+}
+
 // out[i] = sum of rows[offs[i] .. offs[i] + counts[i]) (one block per row, fixed summation order)
 __global__ __launch_bounds__(BLOCK) void k_sum_rows(const double* __restrict__ rows, const uint64_t* __restrict__ offs,
                                                     const uint32_t* __restrict__ counts, double* __restrict__ out)
@@ -1134,6 +1275,65 @@ void launch_sum_rows(hipStream_t st, const double* rows, const uint64_t* d_offs,
         return;
     }
     k_sum_rows<<<n, BLOCK, 0, st>>>(rows, d_offs, d_counts, d_out);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+uint32_t batch_blocks(uint64_t n_amps) { return static_cast<uint32_t>((n_amps + BATCH_AMPS - 1) / BATCH_AMPS); }
+
+void launch_pauli_batch(hipStream_t st, int precision, const void* arena_a, void* arena_b, const BatchPauliItem* d_items,
+                        uint32_t n_items, uint32_t n_blocks, double* d_part)
+{
+    if (n_items == 0 || n_blocks == 0) {
+        return;
+    }
+    if (precision == KQ_C64) {
+        k_pauli_batch<float><<<n_blocks, BLOCK, 0, st>>>(static_cast<const float2*>(arena_a), static_cast<float2*>(arena_b),
+                                                         d_items, n_items, d_part);
+    }
+    else {
+        k_pauli_batch<double><<<n_blocks, BLOCK, 0, st>>>(static_cast<const double2*>(arena_a), static_cast<double2*>(arena_b),
+                                                          d_items, n_items, d_part);
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_basis_distance_batch(hipStream_t st, int precision, const void* arena, const BatchPauliItem* d_items,
+                                 uint32_t n_items, uint32_t n_blocks, double* d_part)
+{
+    if (n_items == 0 || n_blocks == 0) {
+        return;
+    }
+    if (precision == KQ_C64) {
+        k_basis_distance_batch<float><<<n_blocks, BLOCK, 0, st>>>(static_cast<const float2*>(arena), d_items, n_items, d_part);
+    }
+    else {
+        k_basis_distance_batch<double><<<n_blocks, BLOCK, 0, st>>>(static_cast<const double2*>(arena), d_items, n_items, d_part);
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_batch_reset(hipStream_t st, int precision, void* arena, uint64_t n_amps, const uint64_t* d_offs, uint32_t n)
+{
+    const unsigned blocks = grid_for(n_amps, 4);
+    const unsigned ob = (n + BLOCK - 1) / BLOCK;
+    if (precision == KQ_C64) {
+        k_fill_zero<float><<<blocks, BLOCK, 0, st>>>(static_cast<float2*>(arena), n_amps);
+        k_set_ones<float><<<ob, BLOCK, 0, st>>>(static_cast<float2*>(arena), d_offs, n);
+    }
+    else {
+        k_fill_zero<double><<<blocks, BLOCK, 0, st>>>(static_cast<double2*>(arena), n_amps);
+        k_set_ones<double><<<ob, BLOCK, 0, st>>>(static_cast<double2*>(arena), d_offs, n);
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_max_rows(hipStream_t st, const double* rows, const uint64_t* d_offs, const uint32_t* d_counts, uint32_t n,
+                     double* d_out)
+{
+    if (n == 0) {
+        return;
+    }
+    k_max_rows<<<n, BLOCK, 0, st>>>(rows, d_offs, d_counts, d_out);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

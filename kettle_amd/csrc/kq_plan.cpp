@@ -196,6 +196,25 @@ void compile_gate(const kq_op& op, uint32_t n, bool is_dm, std::vector<GateRec>&
     }
 }
 
+void compile_gate_stable(const kq_op& op, uint32_t n, std::vector<GateRec>& out)
+{
+    const size_t first = out.size();
+    compile_gate(op, n, false, out);
+    if (op.aux < 0) {
+        return;
+    }
+    // a parameterised gate keeps the arithmetic form of its gate type for every angle, so a plan made
+    // for one set of angles stays valid for any other
+    GateRec& g = out[first];
+    switch (op.gate) {
+        case KQ_RX: case KQ_CRX: g.kind = GK_AXIS; break;
+        case KQ_RY: case KQ_CRY: g.kind = GK_REAL; break;
+        case KQ_RZ: case KQ_CRZ: g.kind = GK_DIAG; break;
+        case KQ_P: case KQ_CP: g.kind = GK_DIAG1; break;
+        default: break;
+    }
+}
+
 namespace
 {
 

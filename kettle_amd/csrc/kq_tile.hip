@@ -308,12 +308,10 @@ __device__ __forceinline__ double wave_sum(double x, int width)
 // tile is reduced per wavefront into partials[slot * n_part + block * waves + wave]; then
 // lambda <- G^dag lambda.
 template <typename R, bool ADJ>
-__global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs d,
-                                                     const TileStep* __restrict__ steps,
-                                                     const TileLayout* __restrict__ layouts,
-                                                     const TileGate* __restrict__ gates,
-                                                     const TileGate* __restrict__ derivs, double* __restrict__ partials,
-                                                     uint32_t n_part)
+__device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs& d,
+                                          const TileStep* __restrict__ steps, const TileLayout* __restrict__ layouts,
+                                          const TileGate* __restrict__ gates, const TileGate* __restrict__ derivs,
+                                          double* __restrict__ partials, uint32_t n_part, uint32_t block)
 {
     extern __shared__ unsigned char smem_raw[];
     const uint32_t tid = threadIdx.x;
@@ -322,7 +320,7 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
     // deposit the block index into the free qubits (uniform: scalar instructions only)
     uint64_t base = 0;
     {
-        uint64_t blk = blockIdx.x;
+        uint64_t blk = block;
         uint64_t mask = d.free_mask;
         while (mask != 0) {
             const int q = __builtin_ctzll(mask);
@@ -376,7 +374,7 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
 
     const int width = (1 << tb) < 64 ? (1 << tb) : 64;
     const uint32_t waves = (blockDim.x + 63) / 64;
-    const uint32_t part_col = blockIdx.x * waves + tid / 64;
+    const uint32_t part_col = block * waves + tid / 64;
 
     for (int si = 0; si < d.n_steps; ++si) {
         const TileStep st = steps[si];
@@ -395,13 +393,13 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
             const Coef<R> m = coef_of<R>(g);
             const uint32_t code = g.code;
             const bool glb_ok = (base & g.glb_ctrl) == g.glb_ctrl;
-            const bool lane_ok = (tid & g.lane_ctrl) == g.lane_ctrl;
-            R fr = 0, fi = 0;
-            if (code >= TV_UNIFORM_BASE) {
+            // lane condition only for controls on thread bits (uniform test first: no exec masking)
+            const bool lane_ok = g.lane_ctrl == 0 || (tid & g.lane_ctrl) == g.lane_ctrl;
+            const auto uniform_factor = [&](R& fr, R& fi) {
                 const bool sel = g.lane_sel ? ((tid & g.lane_sel) != 0) : ((base & g.glb_sel) != 0);
                 fr = sel ? m.m[6] : m.m[0];
                 fi = sel ? m.m[7] : m.m[1];
-            }
+            };
             if constexpr (!ADJ) {
                 if (!glb_ok) {
                     continue;   // a control (or a d0 = 1 diagonal's target) outside the tile is 0
@@ -410,6 +408,8 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
                     continue;   // a control on a thread bit is 0 for this lane
                 }
                 if (code >= TV_UNIFORM_BASE) {
+                    R fr, fi;
+                    uniform_factor(fr, fi);
                     apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
                 }
                 else {
@@ -418,6 +418,10 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
             }
             else {
                 const bool ok = glb_ok && lane_ok;
+                R fr = 0, fi = 0;
+                if (code >= TV_UNIFORM_BASE) {
+                    uniform_factor(fr, fi);
+                }
                 if (ok) {
                     if (code >= TV_UNIFORM_BASE) {
                         apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
@@ -479,12 +483,54 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
 }
 
 template <typename R, bool ADJ>
+__global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs d,
+                                                     const TileStep* __restrict__ steps,
+                                                     const TileLayout* __restrict__ layouts,
+                                                     const TileGate* __restrict__ gates,
+                                                     const TileGate* __restrict__ derivs, double* __restrict__ partials,
+                                                     uint32_t n_part)
+{
+    tile_body<R, ADJ>(a, b, d, steps, layouts, gates, derivs, partials, n_part, blockIdx.x);
+}
+
+// Many states in one launch: item i covers blocks [block_begin, next item's block_begin) and runs its
+// own pass program on its own state (and lambda) in the arenas.  Items are sorted by block_begin.
+template <typename R, bool ADJ>
+__global__ __launch_bounds__(TILE_BLOCK) void k_tile_batch(Amp<R>* __restrict__ arena_a, Amp<R>* __restrict__ arena_b,
+                                                           const BatchTileItem* __restrict__ items, uint32_t n_items)
+{
+    uint32_t lo = 0, hi = n_items;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (items[mid].block_begin <= blockIdx.x) {
+            lo = mid;
+        }
+        else {
+            hi = mid;
+        }
+    }
+    const BatchTileItem& it = items[lo];
+    const TileArgs d {it.free_mask, it.n_thr_bits, it.n_steps, it.out_layout, 0};
+    // the tables live in global memory: address-space-1 pointers keep their reads scalar
+    using GStep = const __attribute__((address_space(1))) TileStep;
+    using GLayout = const __attribute__((address_space(1))) TileLayout;
+    using GGate = const __attribute__((address_space(1))) TileGate;
+    using GDouble = __attribute__((address_space(1))) double;
+    tile_body<R, ADJ>(arena_a + it.a_off, ADJ ? arena_b + it.b_off : nullptr, d,
+                      (const TileStep*) (GStep*) it.steps, (const TileLayout*) (GLayout*) it.layouts,
+                      (const TileGate*) (GGate*) it.gates, (const TileGate*) (GGate*) it.derivs,
+                      (double*) (GDouble*) it.partials, it.n_part, blockIdx.x - it.block_begin);
+}
+
+template <typename R, bool ADJ>
 void set_lds_attr()
 {
     static bool done = false;
     if (!done) {
         const int maxdyn = static_cast<int>(device_lds_per_block() - TILE_STATIC_LDS);
         (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<R, ADJ>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
+        (void) hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_batch<R, ADJ>),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, maxdyn);
         (void) hipGetLastError();
         done = true;
@@ -556,6 +602,43 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
         set_lds_attr<double, false>();
         k_tile<double, false><<<blocks, threads, lds, s->stream>>>(static_cast<double2*>(s->amps), nullptr, d, st, ly, g,
                                                                     nullptr, nullptr, 0);
+    }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* arena_a, void* arena_b,
+                       const BatchTileItem* d_items, uint32_t n_items, uint32_t n_blocks, int k)
+{
+    if (k < 4 || k > 12 || n_items == 0 || n_blocks == 0) {
+        throw InvalidArg {"tile batch: bad launch shape"};
+    }
+    const unsigned threads = 1u << (k - 4);
+    const size_t lds = (size_t {1} << k) * amp_bytes(precision);
+    if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
+        throw InvalidArg {"tile batch: tile does not fit the workgroup's LDS"};
+    }
+    if (precision == KQ_C64) {
+        if (adjoint) {
+            set_lds_attr<float, true>();
+            k_tile_batch<float, true><<<n_blocks, threads, lds, stream>>>(static_cast<float2*>(arena_a), static_cast<float2*>(arena_b),
+                                                                          d_items, n_items);
+        }
+        else {
+            set_lds_attr<float, false>();
+            k_tile_batch<float, false><<<n_blocks, threads, lds, stream>>>(static_cast<float2*>(arena_a), nullptr, d_items, n_items);
+        }
+    }
+    else {
+        if (adjoint) {
+            set_lds_attr<double, true>();
+            k_tile_batch<double, true><<<n_blocks, threads, lds, stream>>>(static_cast<double2*>(arena_a),
+                                                                           static_cast<double2*>(arena_b), d_items, n_items);
+        }
+        else {
+            set_lds_attr<double, false>();
+            k_tile_batch<double, false><<<n_blocks, threads, lds, stream>>>(static_cast<double2*>(arena_a), nullptr, d_items,
+                                                                            n_items);
+        }
     }
     KQ_HIP_CHECK(hipGetLastError());
 }

@@ -12,7 +12,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .api import PauliOperator, SparsePauliString
+from ._lib import OP_DTYPE
+
+from .api import PackedPauliOperator, PauliOperator, SparsePauliString
 from .circuit import Gate, Matrix2X2, QuantumCircuit
 
 
@@ -70,14 +72,20 @@ def random_circuit(n: int, depth: int, seed: int = 0) -> QuantumCircuit:
     return c
 
 
+def random_pauli_packed(n: int, n_strings: int, rng: np.random.Generator) -> PackedPauliOperator:
+    """Each site I/X/Y/Z with p = .7/.1/.1/.1; real weights N(0, 1) (all sites drawn first, then
+    all weights)."""
+    draws = rng.choice(4, size=(n_strings, n), p=[0.7, 0.1, 0.1, 0.1])
+    weights = rng.normal(size=n_strings)
+    rows, cols = np.nonzero(draws)
+    offsets = np.concatenate([[0], np.cumsum(np.count_nonzero(draws, axis=1))])
+    coeffs = np.stack([weights, np.zeros(n_strings)], axis=1).reshape(-1)
+    return PackedPauliOperator(n, offsets, cols, draws[rows, cols], np.zeros(n_strings), coeffs)
+
+
 def random_pauli_operator(n: int, n_strings: int, rng: np.random.Generator) -> PauliOperator:
-    """Each site I/X/Y/Z with p = .7/.1/.1/.1; real weights N(0, 1)."""
-    op = PauliOperator(n_qubits=n)
-    for _ in range(n_strings):
-        draws = rng.choice(4, size=n, p=[0.7, 0.1, 0.1, 0.1])
-        terms = [(q, int(t)) for q, t in enumerate(draws) if t != 0]
-        op.add(float(rng.normal()), SparsePauliString(terms, n))
-    return op
+    """``random_pauli_packed`` as a PauliOperator."""
+    return random_pauli_packed(n, n_strings, rng).to_operator()
 
 
 @dataclass
@@ -204,3 +212,61 @@ def vqe(n: int = 24, layers: int = 20, n_random: int = 200, seed: int = 0) -> VQ
     for pid, t in zip(ids, theta):
         c.set_parameter_value(pid, float(t))
     return VQEWorkload(c, ids, tfim_plus_random(n, n_random, rng), theta)
+
+
+@dataclass
+class BatchedVQEInstance:
+    n: int
+    depth: int
+    theta: np.ndarray                 # parameter values in n_local creation order
+    hamiltonian: PackedPauliOperator
+
+    def ops(self) -> np.ndarray:
+        """The n_local(n, {RY, RZ}, {CX}, LINEAR, depth, skip last) gate array with parameter slots."""
+        return n_local_ry_rz_cx_ops(self.n, self.depth, self.theta)
+
+    def circuit(self) -> tuple[QuantumCircuit, list[str]]:
+        """The same instance as a parameterised QuantumCircuit (kettle's n_local)."""
+        c, ids = n_local(self.n, [Gate.RY, Gate.RZ], [Gate.CX], NLocalEntanglement.LINEAR, self.depth, True)
+        for pid, t in zip(ids, self.theta):
+            c.set_parameter_value(pid, float(t))
+        return c, ids
+
+
+def n_local_ry_rz_cx_ops(n: int, depth: int, theta: np.ndarray) -> np.ndarray:
+    """Op array of n_local(n, {RY, RZ}, {CX}, LINEAR, depth, skip last rotation layer): per
+    repetition RY on every qubit, RZ on every qubit (parameter slots in that order), then CX(q, q+1)."""
+    per = 3 * n - 1
+    ops = np.zeros(depth * per, dtype=OP_DTYPE)
+    q = np.arange(n, dtype=np.int32)
+    for r in range(depth):
+        blk = ops[r * per:(r + 1) * per]
+        blk["gate"][:n] = int(Gate.RY)
+        blk["gate"][n:2 * n] = int(Gate.RZ)
+        blk["gate"][2 * n:] = int(Gate.CX)
+        blk["q0"][:n] = q
+        blk["q0"][n:2 * n] = q
+        blk["q0"][2 * n:] = q[:-1]
+        blk["q1"][2 * n:] = q[1:]
+        slots = 2 * n * r + np.arange(2 * n)
+        blk["aux"][:2 * n] = slots
+        blk["aux"][2 * n:] = -1
+        blk["angle"][:2 * n] = theta[slots]
+    return ops
+
+
+def batched_vqe(n_instances: int = 4096, n_range=(8, 22), depth_range=(4, 16), strings_range=(32, 256),
+                seed: int = 0) -> list[BatchedVQEInstance]:
+    """Uneven VQE instances: qubits uniform in n_range, ansatz depth uniform in depth_range of
+    n_local(n, {RY, RZ}, {CX}, LINEAR, depth, skip last), parameters uniform in [-pi, pi], and a
+    Hamiltonian of U{strings_range} random weighted Pauli strings (I/X/Y/Z with p = .7/.1/.1/.1,
+    weights N(0, 1)).  Instance i draws from its own generator seeded 0x9E3779B9 * i + seed."""
+    out = []
+    for i in range(n_instances):
+        rng = np.random.default_rng((0x9E3779B9 * i + seed) & 0xFFFFFFFFFFFFFFFF)
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        depth = int(rng.integers(depth_range[0], depth_range[1] + 1))
+        n_strings = int(rng.integers(strings_range[0], strings_range[1] + 1))
+        theta = rng.uniform(-math.pi, math.pi, size=2 * n * depth)
+        out.append(BatchedVQEInstance(n, depth, theta, random_pauli_packed(n, n_strings, rng)))
+    return out
