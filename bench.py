@@ -35,6 +35,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per tile-pass launch over algorithmic bytes (one read + one write of the state), from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the 28-qubit version of this workload, FETCH_SIZE scaled
+# by 2.0 and WRITE_SIZE by 1.0 (the factors a plain copy kernel of known size reads):
+# (2 * 2.09769e6 + 4.19430e6) KiB / (2 * 16 * 2^28 B) per dispatch
+TRAFFIC_PER_ALGORITHMIC_BYTE = 1.00013
 
 
 def log(msg: str) -> None:
@@ -109,27 +114,33 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
     from kettle_amd import _lib, workloads
     from kettle_amd.batch import EnergyGradientBatch
 
-    t0 = time.perf_counter()
     insts = workloads.batched_vqe(n_instances)
-    b = EnergyGradientBatch(ka.KQ_C64)
-    for inst in insts:
-        b.add_ops(inst.n, inst.ops(), len(inst.theta), inst.hamiltonian)
-    t_setup = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    r = b.run(use_graph=False)            # plans, lowers, uploads, runs eagerly
-    t_first = time.perf_counter() - t0
-    b.run(use_graph=True)                 # captures the graph
-    log(f"batch: {n_instances} instances, setup {t_setup:.1f} s, first run {t_first:.1f} s")
-    t0 = time.perf_counter()
-    for i in range(reps):
-        b.set_parameters(i % n_instances, insts[i % n_instances].theta)   # an optimiser step touches instances
-        r = b.run(use_graph=True)
-    t_run = (time.perf_counter() - t0) / reps
-    info = b.info()
+
+    def timed(forward_precision):
+        t0 = time.perf_counter()
+        b = EnergyGradientBatch(ka.KQ_C64, forward_precision)
+        for inst in insts:
+            b.add_ops(inst.n, inst.ops(), len(inst.theta), inst.hamiltonian)
+        r = b.run(use_graph=False)            # plans, lowers, uploads, runs eagerly
+        b.run(use_graph=True)                 # captures the graph
+        log(f"batch: {n_instances} instances, forward precision {forward_precision}: setup + first runs "
+            f"{time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        for i in range(reps):
+            b.set_parameters(i % n_instances, insts[i % n_instances].theta)   # an optimiser step touches instances
+            r = b.run(use_graph=True)
+        t = (time.perf_counter() - t0) / reps
+        info = b.info()
+        b.close()
+        return r, t, info
+
+    r64, t64, _ = timed(ka.KQ_C64)
+    r, t_run, info = timed(ka.KQ_C128)
     # plain per-instance FP64 path on a sample, scaled to the whole batch
     idx = np.unique(np.linspace(0, n_instances - 1, plain_sample).astype(int))
     t_plain_sample = 0.0
     worst = 0.0
+    worst64 = 0.0
     for i in idx:
         inst = insts[i]
         ops = inst.ops()
@@ -142,16 +153,19 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
         t_e = (time.perf_counter() - t1) / 2
         t_plain_sample += (2 * len(inst.theta) + 1) * t_e
         worst = max(worst, abs(r.energies[i] - e))
+        worst64 = max(worst64, abs(r64.energies[i] - e))
     t_plain = t_plain_sample * n_instances / len(idx)
-    b.close()
     return {
         "config": f"{n_instances} instances, qubits U[8,22], n_local(RY,RZ; CX linear) depth U[4,16], "
-                  f"U[32,256] Pauli strings, complex64 states with FP64 reductions, hipGraph replay",
+                  f"U[32,256] Pauli strings; complex128 forward sweep (FP64 energies), complex64 reverse "
+                  f"sweep with FP64 reductions, hipGraph replay",
         "batched_adjoint_c64_evals_per_s": round(n_instances / t_run, 2),
         "ms_per_batch": round(t_run * 1e3, 2),
         "plain_fp64_param_shift_evals_per_s": round(n_instances / t_plain, 4),
         "speedup": round(t_plain / t_run, 1),
         "energy_err_vs_fp64_sample": worst,
+        "all_c64_evals_per_s": round(n_instances / t64, 2),
+        "all_c64_energy_err_vs_fp64_sample": worst64,
         "max_inverse_drift": float(np.max(r.drift)),
         "launches_per_batch": info["launches"],
         "device_bytes": info["device_bytes"],
@@ -256,7 +270,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": round(TRAFFIC_PER_ALGORITHMIC_BYTE * 2 * 16 * (1 << n)),
         "bytes_per_launch": int(2 * 16 * (1 << n)),
         "avg_launch_ms": round(float(tile["ms"]) / max(int(tile["launches"]), 1), 4),
     }

@@ -202,9 +202,12 @@ int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops,
  * gradient at grads + (sum of n_params of the instances before i); drift[i] (nullable) is the
  * max-abs distance of the uncomputed state from |0...0> after the reverse sweep, a measure of
  * inverse-gate drift.  use_graph != 0 replays the launch sequence as a captured hipGraph from the
- * second run on.  A batch owns its stream and device memory; calls on one batch are serialised. */
+ * second run on.  A batch owns its stream and device memory; calls on one batch are serialised.
+ * precision is that of the reverse sweep and of lambda; forward_precision that of the forward
+ * sweep and the energies: equal to precision, or KQ_C128 with KQ_C64 (FP64-exact energies, the
+ * gradient's reverse sweep at complex64 traffic). */
 typedef struct kq_batch kq_batch;
-int kq_batch_create(kq_batch** out, int precision);
+int kq_batch_create(kq_batch** out, int precision, int forward_precision);
 int kq_batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params,
                  size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
                  const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs, size_t* index);

@@ -31,10 +31,14 @@ class BatchResult:
 
 
 class EnergyGradientBatch:
-    def __init__(self, precision: int = KQ_C64):
+    """precision: the reverse sweep's; forward_precision: the forward sweep's and the energies'
+    (default: the same; KQ_C128 with KQ_C64 gives FP64-exact energies)."""
+
+    def __init__(self, precision: int = KQ_C64, forward_precision: int | None = None):
         self.precision = precision
+        self.forward_precision = precision if forward_precision is None else forward_precision
         self._h = ctypes.c_void_p()
-        check(load().kq_batch_create(ctypes.byref(self._h), int(precision)))
+        check(load().kq_batch_create(ctypes.byref(self._h), int(precision), int(self.forward_precision)))
         self._n_params: list[int] = []
 
     def __del__(self):

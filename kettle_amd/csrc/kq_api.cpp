@@ -1373,16 +1373,23 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
     });
 }
 
-int kq_batch_create(kq_batch** out, int precision)
+int kq_batch_create(kq_batch** out, int precision, int forward_precision)
 {
     return guarded([&] {
-        if (!out || (precision != KQ_C128 && precision != KQ_C64)) {
+        if (!out || (precision != KQ_C128 && precision != KQ_C64)
+            || (forward_precision != KQ_C128 && forward_precision != KQ_C64)) {
             throw InvalidArg {"batch: bad arguments"};
         }
         ensure_init();
         hipStream_t st = nullptr;
         KQ_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        *out = batch_create(precision, st);
+        try {
+            *out = batch_create(precision, forward_precision, st);
+        }
+        catch (...) {
+            (void) hipStreamDestroy(st);
+            throw;
+        }
     });
 }
 

@@ -6,7 +6,9 @@
 // only re-lower coefficients.  Execution is a fixed sequence of launches over all instances:
 //   reset every state to |0...0>
 //   forward round r of each tile-size group: pass r of every instance that has one, in one launch
-//   lambda = H psi and the energy partials of every instance, in one launch
+//   lambda = H psi and the energy partials of every instance, in one launch (with a complex128
+//   forward sweep and a complex64 reverse sweep: energies from the complex128 states, then psi and
+//   lambda narrowed to complex64)
 //   reverse round r of each group, then one FP64 row sum of that round's derivative partials
 //   distance of every uncomputed state from |0...0> (inverse-gate drift), in one launch
 // The sequence contains no allocation, no host synchronisation and no memset, so after a first
@@ -63,7 +65,8 @@ struct BatchRound
 
 struct kq_batch
 {
-    int precision = KQ_C64;
+    int precision = KQ_C64;             // reverse sweep and lambda
+    int fwd_precision = KQ_C64;         // forward sweep and energy
     hipStream_t stream = nullptr;
     std::vector<kq::BatchInstance> inst;
     bool planned = false;
@@ -81,6 +84,7 @@ struct kq_batch
     size_t table_bytes = 0;
     size_t pauli_bytes = 0;
     // device memory
+    void* psi_fwd = nullptr;            // forward states when the forward precision differs
     void* psi = nullptr;
     void* lam = nullptr;
     void* prog = nullptr;
@@ -151,12 +155,12 @@ void compile_instance(const BatchInstance& in, std::vector<GateRec>& recs, std::
     }
 }
 
-void plan_instance(BatchInstance& in, int precision)
+void plan_instance(BatchInstance& in, int precision, int fwd_precision)
 {
     std::vector<GateRec> recs, inv, derivs;
     std::vector<char> has_d;
     compile_instance(in, recs, inv, derivs, has_d);
-    in.k = fusion_max_k(precision, in.nv);
+    in.k = std::min(fusion_max_k(precision, in.nv), fusion_max_k(fwd_precision, in.nv));
     if (in.k < register_qubits(precision)) {
         throw InvalidArg {"batch: the tile does not hold one register block"};
     }
@@ -170,7 +174,7 @@ void plan_instance(BatchInstance& in, int precision)
 }
 
 // lower the instance's passes with its current angles (the plan and the program sizes are fixed)
-void lower_instance(BatchInstance& in, int precision)
+void lower_instance(BatchInstance& in, int precision, int fwd_precision)
 {
     std::vector<GateRec> recs, inv, derivs;
     std::vector<char> has_d;
@@ -189,7 +193,7 @@ void lower_instance(BatchInstance& in, int precision)
     in.fwd_prog.resize(in.fwd.size());
     for (size_t p = 0; p < in.fwd.size(); ++p) {
         patch(in.fwd[p], recs);
-        lower_tile_pass(in.fwd[p], precision, in.nv, in.fwd_prog[p]);
+        lower_tile_pass(in.fwd[p], fwd_precision, in.nv, in.fwd_prog[p]);
     }
     in.bwd_prog.resize(in.bwd.size());
     for (size_t p = 0; p < in.bwd.size(); ++p) {
@@ -219,7 +223,7 @@ void release(kq_batch* b)
         (void) hipGraphDestroy(b->graph);
         b->graph = nullptr;
     }
-    for (void** p : {&b->psi, &b->lam, &b->prog, &b->tables, &b->pauli}) {
+    for (void** p : {&b->psi_fwd, &b->psi, &b->lam, &b->prog, &b->tables, &b->pauli}) {
         if (*p) {
             (void) hipFree(*p);
             *p = nullptr;
@@ -253,8 +257,8 @@ void build(kq_batch* b)
     uint64_t amps = 0;
     size_t prog = 0, pauli = 0, grad = 0;
     for (BatchInstance& in : b->inst) {
-        plan_instance(in, b->precision);
-        lower_instance(in, b->precision);
+        plan_instance(in, b->precision, b->fwd_precision);
+        lower_instance(in, b->precision, b->fwd_precision);
         in.amp_off = amps;
         in.n_amps = uint64_t {1} << in.nv;
         amps += in.n_amps;
@@ -385,6 +389,9 @@ void build(kq_batch* b)
     const uint64_t part_total = b->part_doubles + pblocks;
     // device memory
     b->psi = balloc(b->total_amps * ab);
+    if (b->fwd_precision != b->precision) {
+        b->psi_fwd = balloc(b->total_amps * amp_bytes(b->fwd_precision));
+    }
     b->lam = balloc(b->total_amps * ab);
     b->prog = balloc(b->prog_bytes);
     b->pauli = balloc(b->pauli_bytes);
@@ -441,7 +448,7 @@ void upload_programs(kq_batch* b, bool relower)
 {
     for (BatchInstance& in : b->inst) {
         if (relower) {
-            lower_instance(in, b->precision);
+            lower_instance(in, b->precision, b->fwd_precision);
         }
         for (size_t p = 0; p < in.fwd_prog.size(); ++p) {
             write_program(in.fwd_prog[p], b->prog_host.data() + in.fwd_off[p]);
@@ -460,21 +467,27 @@ void enqueue(kq_batch* b)
     auto* tab = static_cast<unsigned char*>(b->tables);
     const uint32_t n = static_cast<uint32_t>(b->inst.size());
     size_t launches = 0;
-    launch_batch_reset(b->stream, b->precision, b->psi, b->total_amps, reinterpret_cast<const uint64_t*>(tab + b->ones_off), n);
+    const bool mixed = b->fwd_precision != b->precision;
+    void* fwd = mixed ? b->psi_fwd : b->psi;
+    launch_batch_reset(b->stream, b->fwd_precision, fwd, b->total_amps, reinterpret_cast<const uint64_t*>(tab + b->ones_off), n);
     launches += 2;
     for (const BatchRound& rd : b->rounds) {
         if (rd.adjoint) {
             continue;
         }
-        launch_tile_batch(b->stream, b->precision, false, b->psi, nullptr,
+        launch_tile_batch(b->stream, b->fwd_precision, false, fwd, nullptr,
                           reinterpret_cast<const BatchTileItem*>(tab + rd.items_off), rd.n_items, rd.n_blocks, rd.k);
         ++launches;
     }
-    launch_pauli_batch(b->stream, b->precision, b->psi, b->lam, reinterpret_cast<const BatchPauliItem*>(tab + b->pauli_items_off),
-                       n, b->pauli_blocks, b->part);
+    launch_pauli_batch(b->stream, b->fwd_precision, b->precision, fwd, b->lam,
+                       reinterpret_cast<const BatchPauliItem*>(tab + b->pauli_items_off), n, b->pauli_blocks, b->part);
     launch_sum_rows(b->stream, b->part, reinterpret_cast<const uint64_t*>(tab + b->e_row_off_off),
                     reinterpret_cast<const uint32_t*>(tab + b->e_row_cnt_off), n, b->energies);
     launches += 2;
+    if (mixed) {
+        launch_narrow(b->stream, b->psi_fwd, b->psi, b->total_amps);
+        ++launches;
+    }
     for (const BatchRound& rd : b->rounds) {
         if (!rd.adjoint) {
             continue;
@@ -616,10 +629,14 @@ void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, dou
     }
 }
 
-kq_batch* batch_create(int precision, hipStream_t stream)
+kq_batch* batch_create(int precision, int forward_precision, hipStream_t stream)
 {
+    if (forward_precision != precision && !(precision == KQ_C64 && forward_precision == KQ_C128)) {
+        throw InvalidArg {"batch: the forward precision must equal the precision, or be complex128 with complex64"};
+    }
     auto* b = new kq_batch;
     b->precision = precision;
+    b->fwd_precision = forward_precision;
     b->stream = stream;
     return b;
 }
@@ -649,7 +666,8 @@ void batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params, size_t
         *n_params = np;
     }
     if (device_bytes) {
-        *device_bytes = b->planned ? 2 * b->total_amps * amp_bytes(b->precision) + b->prog_bytes + b->table_bytes + b->pauli_bytes
+        const size_t fwd_bytes = b->fwd_precision != b->precision ? b->total_amps * amp_bytes(b->fwd_precision) : 0;
+        *device_bytes = b->planned ? 2 * b->total_amps * amp_bytes(b->precision) + fwd_bytes + b->prog_bytes + b->table_bytes + b->pauli_bytes
                                          + (b->part_doubles + b->pauli_blocks + b->rows.size() + 2 * b->inst.size()) * sizeof(double)
                                    : 0;
     }

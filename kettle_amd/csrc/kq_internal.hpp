@@ -314,8 +314,10 @@ struct BatchPauliItem
 };
 uint32_t batch_blocks(uint64_t n_amps);   // blocks of one state in a batched Pauli / read-out launch
 // lambda = H psi per state and per-block partials of Re <psi|lambda>
-void launch_pauli_batch(hipStream_t st, int precision, const void* arena_a, void* arena_b, const BatchPauliItem* d_items,
-                        uint32_t n_items, uint32_t n_blocks, double* d_part);
+void launch_pauli_batch(hipStream_t st, int in_precision, int out_precision, const void* arena_a, void* arena_b,
+                        const BatchPauliItem* d_items, uint32_t n_items, uint32_t n_blocks, double* d_part);
+// complex128 -> complex64 copy of n amplitudes
+void launch_narrow(hipStream_t st, const void* in, void* out, uint64_t n);
 // per-block partials of max |psi_i - delta_{i0}|
 void launch_basis_distance_batch(hipStream_t st, int precision, const void* arena, const BatchPauliItem* d_items,
                                  uint32_t n_items, uint32_t n_blocks, double* d_part);
@@ -351,7 +353,9 @@ double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
 
 // ---- batched energy + adjoint gradient (kq_batch.cpp) ------------------------------------------------
-kq_batch* batch_create(int precision, hipStream_t stream);
+// precision: the reverse sweep's (and lambda's); forward_precision: the forward sweep's and the
+// energy's (complex128 forward with a complex64 reverse sweep gives FP64-exact energies)
+kq_batch* batch_create(int precision, int forward_precision, hipStream_t stream);
 void batch_destroy(kq_batch* b);
 void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, size_t n_params, size_t n_strings,
                const uint32_t* off, const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs,

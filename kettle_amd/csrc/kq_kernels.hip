@@ -762,8 +762,8 @@ __global__ __launch_bounds__(BLOCK) void k_dot(const Amp<R>* a, const Amp<R>* b,
 // covers blocks [block_begin, next block_begin), BATCH_AMPS amplitudes per block.
 constexpr uint32_t BATCH_AMPS = 4 * BLOCK;
 
-template <typename R>
-__global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict__ arena_a, Amp<R>* __restrict__ arena_b,
+template <typename R, typename RO>
+__global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict__ arena_a, Amp<RO>* __restrict__ arena_b,
                                                        const BatchPauliItem* __restrict__ items, uint32_t n_items,
                                                        double* __restrict__ part)
 {
@@ -780,9 +780,11 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict_
     const BatchPauliItem& it = items[lo];
     const uint32_t local = blockIdx.x - it.block_begin;
     const Amp<R>* a = arena_a + it.a_off;
-    Amp<R>* b = arena_b + it.b_off;
-    const auto* groups = reinterpret_cast<const PauliGroup*>(it.groups);
-    const auto* zs = reinterpret_cast<const PauliZ*>(it.zs);
+    Amp<RO>* b = arena_b + it.b_off;
+    using GGroup = const __attribute__((address_space(1))) PauliGroup;
+    using GZ = const __attribute__((address_space(1))) PauliZ;
+    const auto* groups = (const PauliGroup*) (GGroup*) it.groups;
+    const auto* zs = (const PauliZ*) (GZ*) it.zs;
     const uint64_t end = min(it.n_amps, static_cast<uint64_t>(local + 1) * BATCH_AMPS);
     double e = 0.0;
     for (uint64_t i = static_cast<uint64_t>(local) * BATCH_AMPS + threadIdx.x; i < end; i += BLOCK) {
@@ -798,14 +800,24 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict_
         }
         const Amp<R> v = a[i];
         e += static_cast<double>(v.x) * hr + static_cast<double>(v.y) * hi_;
-        Amp<R> w;
-        w.x = static_cast<R>(hr);
-        w.y = static_cast<R>(hi_);
+        Amp<RO> w;
+        w.x = static_cast<RO>(hr);
+        w.y = static_cast<RO>(hi_);
         b[i] = w;
     }
     const double2 r = block_sum2(e, 0.0);
     if (threadIdx.x == 0) {
         part[it.part_off + local] = r.x;
+    }
+}
+
+// complex128 -> complex64 copy of a whole arena
+__global__ __launch_bounds__(BLOCK) void k_narrow(const double2* __restrict__ in, float2* __restrict__ out, uint64_t n)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        const double2 v = in[i];
+        out[i] = make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
     }
 }
 
@@ -1280,20 +1292,34 @@ void launch_sum_rows(hipStream_t st, const double* rows, const uint64_t* d_offs,
 
 uint32_t batch_blocks(uint64_t n_amps) { return static_cast<uint32_t>((n_amps + BATCH_AMPS - 1) / BATCH_AMPS); }
 
-void launch_pauli_batch(hipStream_t st, int precision, const void* arena_a, void* arena_b, const BatchPauliItem* d_items,
-                        uint32_t n_items, uint32_t n_blocks, double* d_part)
+void launch_pauli_batch(hipStream_t st, int in_precision, int out_precision, const void* arena_a, void* arena_b,
+                        const BatchPauliItem* d_items, uint32_t n_items, uint32_t n_blocks, double* d_part)
 {
     if (n_items == 0 || n_blocks == 0) {
         return;
     }
-    if (precision == KQ_C64) {
-        k_pauli_batch<float><<<n_blocks, BLOCK, 0, st>>>(static_cast<const float2*>(arena_a), static_cast<float2*>(arena_b),
-                                                         d_items, n_items, d_part);
+    if (in_precision == KQ_C64 && out_precision == KQ_C64) {
+        k_pauli_batch<float, float><<<n_blocks, BLOCK, 0, st>>>(static_cast<const float2*>(arena_a), static_cast<float2*>(arena_b),
+                                                                d_items, n_items, d_part);
+    }
+    else if (in_precision == KQ_C128 && out_precision == KQ_C64) {
+        k_pauli_batch<double, float><<<n_blocks, BLOCK, 0, st>>>(static_cast<const double2*>(arena_a), static_cast<float2*>(arena_b),
+                                                                 d_items, n_items, d_part);
+    }
+    else if (in_precision == KQ_C128 && out_precision == KQ_C128) {
+        k_pauli_batch<double, double><<<n_blocks, BLOCK, 0, st>>>(static_cast<const double2*>(arena_a), static_cast<double2*>(arena_b),
+                                                                  d_items, n_items, d_part);
     }
     else {
-        k_pauli_batch<double><<<n_blocks, BLOCK, 0, st>>>(static_cast<const double2*>(arena_a), static_cast<double2*>(arena_b),
-                                                          d_items, n_items, d_part);
+        throw InvalidArg {"batch: unsupported precision pair"};
     }
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
+void launch_narrow(hipStream_t st, const void* in, void* out, uint64_t n)
+{
+    const unsigned blocks = grid_for(n, 4);
+    k_narrow<<<blocks, BLOCK, 0, st>>>(static_cast<const double2*>(in), static_cast<float2*>(out), n);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

@@ -27,8 +27,8 @@ def single(inst, precision):
     return ka.energy_and_gradient(c, inst.hamiltonian, ids, precision=precision)
 
 
-def make_batch(insts, precision):
-    b = EnergyGradientBatch(precision)
+def make_batch(insts, precision, forward_precision=None):
+    b = EnergyGradientBatch(precision, forward_precision)
     for inst in insts:
         b.add_ops(inst.n, inst.ops(), len(inst.theta), inst.hamiltonian)
     return b
@@ -56,6 +56,24 @@ def test_batch_matches_single_instance_and_oracle(precision):
             assert np.max(np.abs(r.gradients[i] - one.gradient), initial=0.0) <= 1e-4, i
             assert r.drift[i] <= 1e-5, i
     b.close()
+
+
+def test_mixed_precision_batch_energies_are_fp64():
+    """complex128 forward sweep, complex64 reverse sweep: energies FP64-exact, gradients at complex64
+    accuracy."""
+    insts = small_instances(12, 11)
+    b = make_batch(insts, ka.KQ_C64, ka.KQ_C128)
+    r = b.run(use_graph=False)
+    for i, inst in enumerate(insts):
+        one = single(inst, ka.KQ_C128)
+        assert abs(r.energies[i] - oracle_energy(inst)) <= 1e-12, i
+        assert np.max(np.abs(r.gradients[i] - one.gradient), initial=0.0) <= 1e-4, i
+        assert r.drift[i] <= 1e-5, i
+    rep = b.run(use_graph=True)
+    assert np.array_equal(rep.energies, r.energies)
+    b.close()
+    with pytest.raises(ka.KqError):
+        EnergyGradientBatch(ka.KQ_C128, ka.KQ_C64)
 
 
 def test_batch_gradient_against_parameter_shift():
@@ -123,16 +141,17 @@ def test_batch_errors():
 
 
 def test_batched_vqe_4096_instances_sample_check():
-    """The full 4096-instance uneven batch (8..22 qubits, complex64): every energy finite, the drift
-    small, and a sample of instances against the complex128 single-instance path."""
+    """The full 4096-instance uneven batch (8..22 qubits, complex128 forward, complex64 reverse
+    sweep): every energy finite, the drift small, and a sample of instances against the complex128
+    single-instance path."""
     insts = workloads.batched_vqe(4096)
-    b = make_batch(insts, ka.KQ_C64)
+    b = make_batch(insts, ka.KQ_C64, ka.KQ_C128)
     r = b.run(use_graph=False)
     assert np.all(np.isfinite(r.energies))
     assert np.max(r.drift) <= 1e-3
     for i in (0, 1, 2047, 4095):
         one = single(insts[i], ka.KQ_C128)
-        assert abs(r.energies[i] - one.energy) <= 1e-6 * max(1.0, abs(one.energy)), i
+        assert abs(r.energies[i] - one.energy) <= 1e-10, i
         assert np.max(np.abs(r.gradients[i] - one.gradient)) <= 1e-4 * max(1.0, np.max(np.abs(one.gradient))), i
     r2 = b.run(use_graph=True)
     assert np.array_equal(r2.energies, r.energies)
