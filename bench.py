@@ -188,6 +188,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--jit", type=int, default=1,
+                    help="run-time compiled tile passes: 0 off, 1 states of >= 18 qubits, 2 all")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
@@ -230,9 +232,11 @@ def main() -> int:
         sv.set_basis(0)
         ka.apply_gates(sv, ops)
 
+    ka.set_jit(args.jit)
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
+    jit_warm = ka.jit_info()
     _, n_passes, n_tiles = ka.plan_stats()
     _lib.check(lib.kq_synchronize())
     stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
@@ -263,9 +267,11 @@ def main() -> int:
     tile["launches"] = stats[0]["launches"] + stats[1]["launches"]
     kernel_ms = float(stats["ms"].sum()) / args.steps
     achieved = float(tile["bytes"]) / (float(tile["ms"]) * 1e-3) / 1e9 if tile["ms"] else 0.0
+    jit_end = ka.jit_info()
+    jit_used = jit_end["cache_hits"] > jit_warm["cache_hits"]
     roofline = {
         "bound": "hbm",
-        "kernel": "k_tile<double>",
+        "kernel": "kq_pass (run-time compiled tile pass, double)" if jit_used else "k_tile<double>",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -298,6 +304,9 @@ def main() -> int:
         },
         "roofline": roofline,
         "passes": passes,
+        "jit": {"mode": args.jit, "kernels_compiled": jit_end["compiled"], "failures": jit_end["failures"],
+                "compile_s_outside_timed_region": round(jit_warm["compile_seconds"], 2),
+                "compiles_inside_timed_region": jit_end["compiled"] - jit_warm["compiled"]},
     }
     del sv
     if not args.no_cpu:

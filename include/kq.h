@@ -137,6 +137,12 @@ int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages
 /* max_cost_per_pass bounds a pass's arithmetic in units of one general complex 2x2 gate (real or
  * axis-rotation matrices count 1/2, one-sided diagonals 1/4, swaps 1/10): above roughly the HBM
  * time of one sweep a pass stops being memory-bound. */
+/* Tile passes compiled at run time for their exact gate structure (straight-line code, no per-gate
+ * dispatch; coefficients stay data, so repeated structures with other angles reuse the kernel):
+ * mode 0 = off (every pass interpreted), 1 = states of >= 18 qubits (default), 2 = every tile pass.
+ * A pass whose kernel fails to compile runs interpreted; kq_jit_info counts it. */
+int kq_set_jit(int mode);
+int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds);
 /* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
 int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);
 
@@ -249,6 +255,14 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
  * units of kq_set_fusion's max_cost_per_pass.  order and pass_of hold one entry per record,
  * pass_kind and pass_cost at least as many; lds_bytes is the per-workgroup LDS the tile is
  * fitted to (0 = the value of the initialised device, 64 KiB before kq_init). */
+/* Host-only: generate and compile (without loading) the run-time pass kernels of a straight-line
+ * gate list; fails with the compiler log if any does not compile.  No GPU needed. */
+int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                         size_t* n_kernels, size_t* failures);
+/* Host-only: the generated source of tile pass pass_index (NUL-terminated, truncated to cap - 1);
+ * *len is the full length and *n_kernels the number of tile passes. */
+int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                        size_t pass_index, char* buf, size_t cap, size_t* len, size_t* n_kernels);
 int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op* ops, size_t n_ops,
                   size_t lds_bytes, uint32_t* order, uint32_t* pass_of, int32_t* pass_kind,
                   double* pass_cost, size_t* n_passes);

@@ -217,6 +217,40 @@ def debug_plan(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, is_dens
     return FusionPlan(order[:n_rec], pass_of[:n_rec], kind[:npass], cost[:npass])
 
 
+def set_jit(mode: int) -> None:
+    """Run-time compiled tile passes: 0 off, 1 states of >= 18 qubits (default), 2 every tile pass."""
+    check(load().kq_set_jit(int(mode)))
+
+
+def jit_info() -> dict:
+    c = _lib.ctypes
+    mode, comp, hits, fails, secs = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double()
+    check(load().kq_jit_info(c.byref(mode), c.byref(comp), c.byref(hits), c.byref(fails), c.byref(secs)))
+    return {"mode": mode.value, "compiled": comp.value, "cache_hits": hits.value, "failures": fails.value,
+            "compile_seconds": secs.value}
+
+
+def debug_jit_compile(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, lds_bytes: int = 160 * 1024) -> int:
+    """Host-only: generate and compile the run-time pass kernels of a gate list; returns their count."""
+    c = _lib.ctypes
+    nk, nf = c.c_size_t(), c.c_size_t()
+    check(load().kq_debug_jit_compile(int(n_qubits), int(precision), ptr(ops), len(ops), int(lds_bytes), c.byref(nk),
+                                      c.byref(nf)))
+    return nk.value
+
+
+def debug_jit_source(ops: np.ndarray, n_qubits: int, pass_index: int = 0, precision: int = KQ_C128,
+                     lds_bytes: int = 160 * 1024) -> tuple[str, int]:
+    """Host-only: (generated source of tile pass pass_index, number of tile passes)."""
+    c = _lib.ctypes
+    ln, nk = c.c_size_t(), c.c_size_t()
+    args = (int(n_qubits), int(precision), ptr(ops), len(ops), int(lds_bytes), int(pass_index))
+    check(load().kq_debug_jit_source(*args, None, 0, c.byref(ln), c.byref(nk)))
+    buf = c.create_string_buffer(ln.value + 1)
+    check(load().kq_debug_jit_source(*args, buf, ln.value + 1, c.byref(ln), c.byref(nk)))
+    return buf.value.decode(), nk.value
+
+
 def set_fusion(max_tile_qubits: int, max_gates_per_pass: int = 0, max_stages_per_pass: int = 0,
                max_cost_per_pass: float = 0.0) -> None:
     """Tile size 2^max_tile_qubits (0: no fusion, -1: largest that fits LDS); <= 0 keeps a bound."""

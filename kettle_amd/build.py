@@ -22,7 +22,7 @@ BUILD_DIR = ROOT / "kettle_amd" / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_batch.cpp"]
+KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_batch.cpp", "kq_jit.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -52,22 +52,33 @@ def _run(cmd: list[str]) -> None:
 EXTRA_FLAGS = {"kq_tile.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
 
 
+def embed_device_header() -> Path:
+    """The device templates as a C string for the run-time compiler (kq_jit.cpp)."""
+    text = (CSRC / "kq_tile_device.hpp").read_text().replace("#pragma once\n", "")
+    out = BUILD_DIR / "kq_tile_device.inc"
+    out.write_text('R"KQDEV(' + text + ')KQDEV"\n')
+    return out
+
+
 def build_kq(jobs: int = 4) -> Path:
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     LIB_DIR.mkdir(parents=True, exist_ok=True)
+    embed_device_header()
     objs = []
     cmds = []
     for src in KQ_SOURCES:
         obj = BUILD_DIR / (src + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmds.append([HIPCC, *HIP_FLAGS, *EXTRA_FLAGS.get(src, []), *lang, "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([HIPCC, *HIP_FLAGS, "-I", str(BUILD_DIR), *EXTRA_FLAGS.get(src, []), *lang, "-c", str(CSRC / src),
+                     "-o", str(obj)])
         objs.append(str(obj))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for fut in [ex.submit(_run, c) for c in cmds]:
             fut.result()
     out = LIB_DIR / "libkq.so"
     tmp = LIB_DIR / "libkq.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs, "-L/opt/rocm/lib", "-lhiprtc",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
 

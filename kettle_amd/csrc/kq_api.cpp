@@ -185,20 +185,46 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
     }
     const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
                                     fusion_max_stages());
-    // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts]
+    // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts][terms]
+    // (interpreted) or [coefficients] (compiled for its structure)
     std::vector<TileProgram> progs(passes.size());
     size_t total = 0;
     const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
     std::vector<size_t> offs(passes.size());
+    const bool use_jit = jit_wanted(s) && jit_available();
+    std::vector<std::vector<double>> jcoef(passes.size());
+    std::vector<std::string> jsrc;
+    std::vector<size_t> jpass;
     for (size_t i = 0; i < passes.size(); ++i) {
         if (passes[i].sweep) {
             continue;
         }
         lower_tile_pass(passes[i], s->precision, s->nvec, progs[i]);
+        if (use_jit) {
+            jsrc.push_back(jit_source(progs[i], s->precision, jcoef[i]));
+            jpass.push_back(i);
+        }
+    }
+    std::vector<hipFunction_t> jfn(passes.size(), nullptr);
+    if (use_jit) {
+        const auto fns = jit_kernels(jsrc);
+        for (size_t j = 0; j < jpass.size(); ++j) {
+            jfn[jpass[j]] = fns[j];
+        }
+    }
+    for (size_t i = 0; i < passes.size(); ++i) {
+        if (passes[i].sweep) {
+            continue;
+        }
         offs[i] = total;
         const TileProgram& tp = progs[i];
-        total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
-               + align(tp.layouts.size() * sizeof(TileLayout));
+        if (jfn[i]) {
+            total += align(std::max<size_t>(jcoef[i].size(), 1) * sizeof(double));
+        }
+        else {
+            total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
+                   + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
+        }
     }
     std::vector<unsigned char> blob(std::max<size_t>(total, 256), 0);
     for (size_t i = 0; i < passes.size(); ++i) {
@@ -207,11 +233,17 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         }
         const TileProgram& tp = progs[i];
         size_t o = offs[i];
+        if (jfn[i]) {
+            std::memcpy(blob.data() + o, jcoef[i].data(), jcoef[i].size() * sizeof(double));
+            continue;
+        }
         std::memcpy(blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
         o += align(tp.gates.size() * sizeof(TileGate));
         std::memcpy(blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
         o += align(tp.steps.size() * sizeof(TileStep));
         std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+        o += align(tp.layouts.size() * sizeof(TileLayout));
+        std::memcpy(blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
     }
     auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
     KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
@@ -224,11 +256,18 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         else {
             const TileProgram& tp = progs[i];
             const unsigned char* g = d + offs[i];
-            const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
-            const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
             const int cls = p.c >= p.k ? 0 : 1;
-            profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                     [&] { launch_tile_pass(s, tp, g, sp, ly); });
+            if (jfn[i]) {
+                profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
+                         [&] { launch_jit_pass(s, jfn[i], tp, reinterpret_cast<const double*>(g)); });
+            }
+            else {
+                const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
+                const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
+                const unsigned char* tm = ly + align(tp.layouts.size() * sizeof(TileLayout));
+                profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
+                         [&] { launch_tile_pass(s, tp, g, sp, ly, tm); });
+            }
             ++st.n_tile_passes;
         }
         ++st.n_passes;
@@ -629,7 +668,8 @@ void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<Ga
         const TileProgram& tp = progs[i];
         offs[i] = total;
         total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
-               + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+               + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate))
+               + align(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
         const uint64_t n_part = uint64_t {tile_pass_blocks(tp)} * tile_pass_waves(tp);
         part_off[i] = part_total;
         for (size_t k = 0; k < tp.derivs.size(); ++k) {
@@ -651,6 +691,8 @@ void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<Ga
         std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
         o += align(tp.layouts.size() * sizeof(TileLayout));
         std::memcpy(blob.data() + o, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
+        o += align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+        std::memcpy(blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
     }
     std::memcpy(blob.data() + total, row_off.data(), row_off.size() * sizeof(uint64_t));
     std::memcpy(blob.data() + total + align(row_off.size() * sizeof(uint64_t)), row_cnt.data(),
@@ -669,8 +711,9 @@ void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<Ga
         const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
         const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
         const unsigned char* dv = ly + align(tp.layouts.size() * sizeof(TileLayout));
+        const unsigned char* tm = dv + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
         profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
-                 [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, d_part + part_off[i]); });
+                 [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, tm, d_part + part_off[i]); });
     }
     std::vector<double> sums(n_rows_total);
     if (n_rows_total) {
@@ -1370,6 +1413,111 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
         KQ_HIP_CHECK(hipDeviceSynchronize());
         KQ_HIP_CHECK(hipMemcpy(out, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
         KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_set_jit(int mode)
+{
+    return guarded([&] { set_jit_mode(mode); });
+}
+
+int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds)
+{
+    return guarded([&] {
+        const JitStats& st = jit_stats();
+        if (mode) {
+            *mode = jit_mode();
+        }
+        if (compiled) {
+            *compiled = st.compiled;
+        }
+        if (cache_hits) {
+            *cache_hits = st.hits;
+        }
+        if (failures) {
+            *failures = st.failures;
+        }
+        if (compile_seconds) {
+            *compile_seconds = st.compile_seconds;
+        }
+    });
+}
+
+namespace
+{
+
+// run-time pass kernel sources of a straight-line gate list (host only)
+std::vector<std::string> debug_jit_sources(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops,
+                                           size_t lds_bytes)
+{
+    if (n_qubits < 4 || n_qubits > 40 || (precision != KQ_C128 && precision != KQ_C64) || (!ops && n_ops)) {
+        throw InvalidArg {"debug jit: bad arguments"};
+    }
+    std::vector<GateRec> recs;
+    for (size_t i = 0; i < n_ops; ++i) {
+        if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+            throw InvalidArg {"debug jit: op " + std::to_string(i) + " is not a unitary gate"};
+        }
+        compile_gate(ops[i], n_qubits, false, recs);
+    }
+    const int max_k = fusion_max_k_for(precision, n_qubits, lds_bytes ? lds_bytes : device_lds_per_block());
+    if (max_k < register_qubits(precision)) {
+        throw InvalidArg {"debug jit: state too small for a tile"};
+    }
+    const auto passes = plan_passes(recs, n_qubits, max_k, 5, fusion_max_gates(), register_qubits(precision),
+                                    fusion_max_stages());
+    std::vector<std::string> srcs;
+    for (const auto& p : passes) {
+        if (p.sweep) {
+            continue;
+        }
+        TileProgram tp;
+        lower_tile_pass(p, precision, n_qubits, tp);
+        std::vector<double> coefs;
+        srcs.push_back(jit_source(tp, precision, coefs));
+    }
+    return srcs;
+}
+
+}  // namespace
+
+int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                         size_t* n_kernels, size_t* failures)
+{
+    return guarded([&] {
+        if (!n_kernels || !failures) {
+            throw InvalidArg {"debug jit: null output"};
+        }
+        const auto srcs = debug_jit_sources(n_qubits, precision, ops, n_ops, lds_bytes);
+        std::string log;
+        *failures = jit_compile_only(srcs, &log);
+        *n_kernels = srcs.size();
+        if (*failures) {
+            throw InvalidArg {"debug jit: " + std::to_string(*failures) + " of " + std::to_string(srcs.size())
+                              + " pass kernels failed to compile:\n" + log.substr(0, 4000)};
+        }
+    });
+}
+
+int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                        size_t pass_index, char* buf, size_t cap, size_t* len, size_t* n_kernels)
+{
+    return guarded([&] {
+        if (!len || !n_kernels) {
+            throw InvalidArg {"debug jit: null output"};
+        }
+        const auto srcs = debug_jit_sources(n_qubits, precision, ops, n_ops, lds_bytes);
+        *n_kernels = srcs.size();
+        if (pass_index >= srcs.size()) {
+            throw InvalidArg {"debug jit: pass index out of range"};
+        }
+        const std::string& src = srcs[pass_index];
+        *len = src.size();
+        if (buf && cap) {
+            const size_t n = std::min(cap - 1, src.size());
+            std::memcpy(buf, src.data(), n);
+            buf[n] = '\0';
+        }
     });
 }
 

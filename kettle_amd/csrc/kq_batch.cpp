@@ -112,7 +112,8 @@ size_t align256(size_t x) { return (x + 255) & ~size_t {255}; }
 size_t program_bytes(const TileProgram& tp)
 {
     return align256(tp.gates.size() * sizeof(TileGate)) + align256(tp.steps.size() * sizeof(TileStep))
-         + align256(tp.layouts.size() * sizeof(TileLayout)) + align256(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+         + align256(tp.layouts.size() * sizeof(TileLayout)) + align256(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate))
+         + align256(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
 }
 
 void write_program(const TileProgram& tp, unsigned char* dst)
@@ -124,6 +125,8 @@ void write_program(const TileProgram& tp, unsigned char* dst)
     std::memcpy(dst, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
     dst += align256(tp.layouts.size() * sizeof(TileLayout));
     std::memcpy(dst, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
+    dst += align256(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+    std::memcpy(dst, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
 }
 
 // records of the instance's current angles: forward records, inverse records in reverse order and
@@ -331,6 +334,7 @@ void build(kq_batch* b)
                     it.steps = off + align256(tp.gates.size() * sizeof(TileGate));
                     it.layouts = it.steps + align256(tp.steps.size() * sizeof(TileStep));
                     it.derivs = it.layouts + align256(tp.layouts.size() * sizeof(TileLayout));
+                    it.terms = it.derivs + align256(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
                     it.free_mask = free_mask_of(tp);
                     it.n_thr_bits = tp.n_thr_bits;
                     it.n_steps = static_cast<int32_t>(tp.steps.size());
@@ -408,6 +412,7 @@ void build(kq_batch* b)
             it.steps += prog_base;
             it.layouts += prog_base;
             it.derivs += prog_base;
+            it.terms += prog_base;
             it.partials = part_base + it.partials * sizeof(double);
         }
         b->rounds[ri].items_off = put(round_items[ri].data(), round_items[ri].size() * sizeof(BatchTileItem));

@@ -96,15 +96,33 @@ struct TilePass
 enum TileKind : uint32_t { TK_DENSE = 0, TK_REAL, TK_AXIS, TK_DIAG, TK_DIAG1, TK_SWAP, TK_COUNT };
 constexpr uint32_t tile_code(uint32_t kind, uint32_t pos, uint32_t ctrl) { return (kind * 4 + pos) * 5 + ctrl; }
 constexpr uint32_t TV_UNIFORM_BASE = TK_COUNT * 4 * 5;
+// Merged diagonal run: variant TV_MERGE_BASE + t * 5 + C multiplies, per lane, the factors of a list
+// of diagonal terms (TileTerm) sharing the target t (register position 0..3, or 4 = a thread or
+// outside bit, i.e. every slot) and the register control C, then applies them in one step.
+constexpr uint32_t TV_MERGE_BASE = TV_UNIFORM_BASE + 5;
+constexpr uint32_t TV_END = TV_MERGE_BASE + 25;
+
+// one diagonal gate inside a merged run: its conditions, selection and the two diagonal entries
+struct TileTerm
+{
+    uint32_t lane_ctrl;
+    uint32_t lane_sel;
+    uint64_t glb_ctrl;
+    uint64_t glb_sel;
+    double d[4];             // d0 (re, im), d1 (re, im): doubles, or 4 floats first (complex64)
+    uint64_t pad;
+};
 
 struct TileGate
 {
     uint32_t code;
     uint32_t lane_ctrl;      // thread-index bits that must all be 1 (controls on thread bits)
-    uint32_t lane_sel;       // uniform-diagonal: the thread-index bit selecting d1 (0: see glb_sel)
-    uint32_t pad;
+    uint32_t lane_sel;       // uniform-diagonal: the thread-index bit selecting d1 (0: see glb_sel);
+                             // merged run: index of its first term
+    uint32_t pad;            // adjoint: derivative record index + 1 (0: none)
     uint64_t glb_ctrl;       // amplitude-index bits outside the tile that must all be 1
-    uint64_t glb_sel;        // uniform-diagonal: the index bit outside the tile selecting d1
+    uint64_t glb_sel;        // uniform-diagonal: the index bit outside the tile selecting d1;
+                             // merged run: number of terms
     double m[8];             // coefficients: 8 doubles (complex128) or 8 floats first (complex64)
 };
 
@@ -132,7 +150,7 @@ struct BatchTileItem
 {
     uint64_t a_off;          // amplitude offset of the state (psi) in the first arena
     uint64_t b_off;          // amplitude offset of lambda in the second arena (adjoint launches)
-    uint64_t steps, layouts, gates, derivs, partials;
+    uint64_t steps, layouts, gates, derivs, partials, terms;
     uint64_t free_mask;
     int32_t n_thr_bits;
     int32_t n_steps;
@@ -152,6 +170,7 @@ struct TileProgram
     std::vector<TileStep> steps;
     std::vector<TileLayout> layouts;
     std::vector<TileGate> derivs;     // adjoint passes: derivative records (pad = partial slot)
+    std::vector<TileTerm> terms;      // terms of merged diagonal runs
     std::vector<uint32_t> deriv_src;  // input record index of each derivative slot
 };
 
@@ -160,6 +179,9 @@ struct TileProgram
 // gets TileGate::pad = derivative index + 1 and its derivative record the next partial slot.
 void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out,
                      const std::vector<const GateRec*>* deriv_of_src = nullptr);
+// merge runs of commuting diagonal gates of one step (appends to out / terms)
+void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, std::vector<TileGate>& out,
+                         std::vector<TileTerm>& terms);
 uint32_t tile_pass_blocks(const TileProgram& prog);
 uint32_t tile_pass_waves(const TileProgram& prog);
 
@@ -256,12 +278,13 @@ int fusion_setting_k();
 
 // ---- kernel launchers (kq_kernels.hip) --------------------------------------------------------
 void launch_gate_sweep(kq_state* s, const GateRec& g);
-// d_prog: the uploaded [gates][steps][layouts] of `prog`
+// d_prog: the uploaded [gates][steps][layouts][terms] of `prog`
 void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
-                      const void* d_layouts);
+                      const void* d_layouts, const void* d_terms);
 // adjoint reverse-sweep pass over (psi, lam); partials: [slot][blocks * waves] doubles
 void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgram& prog, const void* d_gates,
-                              const void* d_steps, const void* d_layouts, const void* d_derivs, double* d_partials);
+                              const void* d_steps, const void* d_layouts, const void* d_derivs, const void* d_terms,
+                              double* d_partials);
 // many states in one launch (k: tile qubits of every item; items sorted by block_begin)
 void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* arena_a, void* arena_b,
                        const BatchTileItem* d_items, uint32_t n_items, uint32_t n_blocks, int k);
@@ -351,6 +374,28 @@ void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
+
+// ---- tile passes compiled at run time (kq_jit.cpp) ---------------------------------------------------
+struct JitStats
+{
+    size_t compiled = 0;       // kernels compiled
+    size_t hits = 0;           // passes served from the cache
+    size_t failures = 0;       // compilations that failed (the pass then runs interpreted)
+    double compile_seconds = 0.0;
+    std::string last_log;
+};
+JitStats& jit_stats();
+void set_jit_mode(int mode);   // 0 off, 1 states of >= 18 qubits, 2 every tile pass
+int jit_mode();
+bool jit_wanted(const kq_state* s);
+bool jit_available();
+// kernel source of one forward pass; coefs receives its coefficient buffer
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
+// compiled kernels for the sources (null where compilation failed); compiles missing ones in parallel
+std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
+// compile without loading (no device needed): number of failures, last log in *log
+size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
+void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);
 
 // ---- batched energy + adjoint gradient (kq_batch.cpp) ------------------------------------------------
 // precision: the reverse sweep's (and lambda's); forward_precision: the forward sweep's and the

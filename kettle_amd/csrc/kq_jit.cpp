@@ -1,0 +1,432 @@
+// Tile passes compiled at run time (hiprtc) for their exact gate structure.
+//
+// The interpreted tile kernel (kq_tile.hip) dispatches every gate through a switch over ~150 variants;
+// each dispatch costs register moves and branches comparable to the arithmetic of a cheap gate.  A
+// generated pass kernel has the pass's structure (layouts, remaps, gate variants, register positions,
+// controls, merged diagonal runs) as straight-line code with literal constants, so swaps become register
+// renaming and no dispatch remains.  Only the coefficients are data: they are read from a per-pass buffer
+// of doubles, so passes whose structure repeats with other angles (optimiser iterations) reuse the
+// compiled kernel.  Kernels are cached by their generated source; missing ones compile in parallel.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+namespace
+{
+
+// the device templates, embedded at build time from kq_tile_device.hpp
+const char* const DEVICE_HEADER =
+#include "kq_tile_device.inc"
+    ;
+
+struct JitCache
+{
+    std::mutex mu;
+    std::unordered_map<std::string, hipFunction_t> kernels;
+    std::vector<hipModule_t> modules;
+    JitStats stats {};
+    int mode = 1;            // 0 off, 1 large states, 2 every tile pass
+    int available = -1;      // hiprtc usable (-1 unknown)
+};
+
+JitCache& cache()
+{
+    static JitCache c;
+    return c;
+}
+
+std::string hex64(uint64_t x)
+{
+    std::ostringstream o;
+    o << "0x" << std::hex << x << "ull";
+    return o.str();
+}
+
+std::string hex32(uint32_t x)
+{
+    std::ostringstream o;
+    o << "0x" << std::hex << x << "u";
+    return o.str();
+}
+
+// block index deposited into the free qubits, as runs of consecutive bits
+std::string deposit_expr(const std::vector<uint8_t>& freepos)
+{
+    std::string e;
+    size_t i = 0, src = 0;
+    while (i < freepos.size()) {
+        size_t j = i;
+        while (j + 1 < freepos.size() && freepos[j + 1] == freepos[j] + 1) {
+            ++j;
+        }
+        const size_t len = j - i + 1;
+        if (!e.empty()) {
+            e += " | ";
+        }
+        e += "(((blk >> " + std::to_string(src) + ") & " + hex64((len >= 64) ? ~0ull : ((1ull << len) - 1)) + ") << "
+           + std::to_string(freepos[i]) + ")";
+        src += len;
+        i = j + 1;
+    }
+    return e.empty() ? std::string {"0ull"} : e;
+}
+
+std::string lds_thread_expr(const TileLayout& L, int nbits)
+{
+    std::string e = "0u";
+    for (int j = 0; j < nbits; ++j) {
+        e += " ^ ((tid & " + hex32(1u << j) + ") ? " + hex32(L.lds_thr[j]) + " : 0u)";
+    }
+    return e;
+}
+
+std::string hbm_thread_expr(const TileLayout& L, int nbits)
+{
+    std::string e = "0ull";
+    for (int j = 0; j < nbits; ++j) {
+        e += " | ((tid & " + hex32(1u << j) + ") ? " + hex64(L.hbm_thr[j]) + " : 0ull)";
+    }
+    return e;
+}
+
+std::string cond_expr(uint64_t glb_ctrl, uint32_t lane_ctrl)
+{
+    std::string c = "true";
+    if (glb_ctrl) {
+        c += " && ((base & " + hex64(glb_ctrl) + ") == " + hex64(glb_ctrl) + ")";
+    }
+    if (lane_ctrl) {
+        c += " && ((tid & " + hex32(lane_ctrl) + ") == " + hex32(lane_ctrl) + ")";
+    }
+    return c;
+}
+
+std::string sel_expr(uint32_t lane_sel, uint64_t glb_sel)
+{
+    if (lane_sel) {
+        return "((tid & " + hex32(lane_sel) + ") != 0u)";
+    }
+    return "((base & " + hex64(glb_sel) + ") != 0ull)";
+}
+
+}  // namespace
+
+JitStats& jit_stats() { return cache().stats; }
+
+void set_jit_mode(int mode)
+{
+    if (mode < 0 || mode > 2) {
+        throw InvalidArg {"jit mode must be 0 (off), 1 (large states) or 2 (every tile pass)"};
+    }
+    std::lock_guard<std::mutex> lk(cache().mu);
+    cache().mode = mode;
+}
+
+int jit_mode() { return cache().mode; }
+
+bool jit_wanted(const kq_state* s)
+{
+    const int mode = cache().mode;
+    return mode == 2 || (mode == 1 && s->nvec >= 18);
+}
+
+// Generate the kernel source of one forward tile pass and the layout of its coefficient buffer.
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
+{
+    coefs.clear();
+    const bool c64 = precision == KQ_C64;
+    const int nt = 1 << tp.n_thr_bits;
+    std::ostringstream o;
+    o << "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\ntypedef int int32_t;\n"
+      << DEVICE_HEADER << "\nusing R = " << (c64 ? "float" : "double") << ";\n"
+      << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void kq_pass(kq::dev::Amp<R>* __restrict__ a, "
+      << "const double* __restrict__ cf)\n{\n  using namespace kq::dev;\n";
+    bool remaps = tp.out_layout != 0;
+    for (const TileStep& st : tp.steps) {
+        remaps = remaps || st.layout != 0;
+    }
+    if (remaps) {
+        o << "  __shared__ Amp<R> tile[" << (1 << tp.k) << "];\n"
+          << "  char* const lds = reinterpret_cast<char*>(tile);\n";
+    }
+    o << "  const uint32_t tid = threadIdx.x;\n  const uint64_t blk = blockIdx.x;\n"
+      << "  const uint64_t base = " << deposit_expr(tp.freepos) << ";\n  Cx<R> v[16];\n";
+    const auto io = [&](const TileLayout& L, bool load) {
+        o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
+        for (int r = 0; r < 16; ++r) {
+            if (load) {
+                o << "    { const Amp<R> x = a[off + " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n";
+            }
+            else {
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off + " << hex64(L.hbm_slot[r])
+                  << "] = x; }\n";
+            }
+        }
+        o << "  }\n";
+    };
+    const auto remap = [&](const TileLayout& A, const TileLayout& B) {
+        o << "  {\n    const uint32_t xa = " << lds_thread_expr(A, tp.n_thr_bits) << ";\n"
+          << "    const uint32_t xb = " << lds_thread_expr(B, tp.n_thr_bits) << ";\n    __syncthreads();\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; *reinterpret_cast<Amp<R>*>(lds + (xa ^ "
+              << hex32(A.lds_slot[r]) << ")) = x; }\n";
+        }
+        o << "    __syncthreads();\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    { const Amp<R> x = *reinterpret_cast<const Amp<R>*>(lds + (xb ^ " << hex32(B.lds_slot[r])
+              << ")); v[" << r << "] = {x.x, x.y}; }\n";
+        }
+        o << "  }\n";
+    };
+    const auto push_gate_coefs = [&](const TileGate& g) {
+        const size_t off = coefs.size();
+        if (c64) {
+            for (int i = 0; i < 4; ++i) {
+                coefs.push_back(g.m[i]);   // 8 packed floats
+            }
+        }
+        else {
+            coefs.insert(coefs.end(), g.m, g.m + 8);
+        }
+        return off;
+    };
+    const auto push_term_coefs = [&](const TileTerm& t) {
+        const size_t off = coefs.size();
+        if (c64) {
+            coefs.push_back(t.d[0]);
+            coefs.push_back(t.d[1]);
+        }
+        else {
+            coefs.insert(coefs.end(), t.d, t.d + 4);
+        }
+        return off;
+    };
+    io(tp.layouts[0], true);
+    uint32_t cur = 0;
+    for (const TileStep& st : tp.steps) {
+        if (st.layout != cur) {
+            remap(tp.layouts[cur], tp.layouts[st.layout]);
+            cur = st.layout;
+        }
+        for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
+            const TileGate& g = tp.gates[gi];
+            if (g.code >= TV_MERGE_BASE) {
+                const uint32_t key = g.code - TV_MERGE_BASE;
+                const bool uniform = key >= 20;
+                o << "  {\n    R f0r = 1, f0i = 0, f1r = 1, f1i = 0;\n";
+                for (uint64_t j = 0; j < g.glb_sel; ++j) {
+                    const TileTerm& t = tp.terms[g.lane_sel + j];
+                    const size_t off = push_term_coefs(t);
+                    o << "    { R d0r, d0i, d1r, d1i; term_at<R>(cf + " << off << ", d0r, d0i, d1r, d1i); const bool ok = "
+                      << cond_expr(t.glb_ctrl, t.lane_ctrl) << ";";
+                    if (uniform) {
+                        o << " const bool sel = " << sel_expr(t.lane_sel, t.glb_sel)
+                          << "; mul_if<R>(ok, f1r, f1i, sel ? d1r : d0r, sel ? d1i : d0i); }\n";
+                    }
+                    else {
+                        o << " mul_if<R>(ok, f0r, f0i, d0r, d0i); mul_if<R>(ok, f1r, f1i, d1r, d1i); }\n";
+                    }
+                }
+                if (uniform) {
+                    o << "    scale<R, " << (key - 20) << ">(v, f1r, f1i);\n  }\n";
+                }
+                else {
+                    o << "    const Coef<R> m {{f0r, f0i, 0, 0, 0, 0, f1r, f1i}};\n    apply<R, K_DIAG, " << key / 5 << ", "
+                      << key % 5 << ">(v, m);\n  }\n";
+                }
+                continue;
+            }
+            const size_t off = push_gate_coefs(g);
+            o << "  if (" << cond_expr(g.glb_ctrl, g.lane_ctrl) << ") {\n    const Coef<R> m = coef_at<R>(cf + " << off
+              << ");\n";
+            if (g.code >= TV_UNIFORM_BASE) {
+                o << "    const bool sel = " << sel_expr(g.lane_sel, g.glb_sel) << ";\n    scale<R, "
+                  << (g.code - TV_UNIFORM_BASE) << ">(v, sel ? m.m[6] : m.m[0], sel ? m.m[7] : m.m[1]);\n  }\n";
+            }
+            else {
+                o << "    apply<R, " << g.code / 20 << ", " << (g.code / 5) % 4 << ", " << g.code % 5 << ">(v, m);\n  }\n";
+            }
+        }
+    }
+    if (tp.out_layout != cur) {
+        remap(tp.layouts[cur], tp.layouts[tp.out_layout]);
+        cur = tp.out_layout;
+    }
+    io(tp.layouts[cur], false);
+    o << "}\n";
+    return o.str();
+}
+
+bool jit_available()
+{
+    JitCache& c = cache();
+    if (c.available >= 0) {
+        return c.available == 1;
+    }
+    int major = 0, minor = 0;
+    c.available = hiprtcVersion(&major, &minor) == HIPRTC_SUCCESS ? 1 : 0;
+    return c.available == 1;
+}
+
+namespace
+{
+
+struct Compiled
+{
+    std::vector<char> code;
+    std::string log;
+    bool ok = false;
+};
+
+Compiled compile_one(const std::string& src)
+{
+    Compiled out;
+    hiprtcProgram prog = nullptr;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "kq_pass.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        out.log = "hiprtcCreateProgram failed";
+        return out;
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    const hiprtcResult r = hiprtcCompileProgram(prog, static_cast<int>(sizeof opts / sizeof opts[0]), opts);
+    size_t n = 0;
+    if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
+        out.log.resize(n);
+        (void) hiprtcGetProgramLog(prog, out.log.data());
+    }
+    if (r == HIPRTC_SUCCESS && hiprtcGetCodeSize(prog, &n) == HIPRTC_SUCCESS) {
+        out.code.resize(n);
+        out.ok = hiprtcGetCode(prog, out.code.data()) == HIPRTC_SUCCESS;
+    }
+    (void) hiprtcDestroyProgram(&prog);
+    return out;
+}
+
+}  // namespace
+
+namespace
+{
+
+// compile sources[idx[j]] for every j on up to 16 host threads
+std::vector<Compiled> compile_all(const std::vector<std::string>& sources, const std::vector<size_t>& idx)
+{
+    std::vector<Compiled> res(idx.size());
+    std::atomic<size_t> next {0};
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t workers = std::min<size_t>({idx.size(), hw, 16});
+    std::vector<std::thread> pool;
+    for (size_t w = 0; w < workers; ++w) {
+        pool.emplace_back([&] {
+            for (size_t j = next++; j < idx.size(); j = next++) {
+                res[j] = compile_one(sources[idx[j]]);
+            }
+        });
+    }
+    for (auto& t : pool) {
+        t.join();
+    }
+    return res;
+}
+
+}  // namespace
+
+// Kernels for the given sources (compiling the missing ones in parallel); null where compilation failed.
+std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources)
+{
+    JitCache& c = cache();
+    std::vector<hipFunction_t> out(sources.size(), nullptr);
+    std::vector<size_t> missing;
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        std::unordered_map<std::string, size_t> first;
+        for (size_t i = 0; i < sources.size(); ++i) {
+            auto it = c.kernels.find(sources[i]);
+            if (it != c.kernels.end()) {
+                out[i] = it->second;
+                ++c.stats.hits;
+            }
+            else if (first.emplace(sources[i], i).second) {
+                missing.push_back(i);
+            }
+        }
+    }
+    if (!missing.empty()) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const std::vector<Compiled> res = compile_all(sources, missing);
+        std::lock_guard<std::mutex> lk(c.mu);
+        for (size_t j = 0; j < missing.size(); ++j) {
+            if (!res[j].ok) {
+                ++c.stats.failures;
+                c.stats.last_log = res[j].log;
+                continue;
+            }
+            hipModule_t mod = nullptr;
+            hipFunction_t fn = nullptr;
+            if (hipModuleLoadData(&mod, res[j].code.data()) != hipSuccess
+                || hipModuleGetFunction(&fn, mod, "kq_pass") != hipSuccess) {
+                (void) hipGetLastError();
+                ++c.stats.failures;
+                c.stats.last_log = "module load failed";
+                continue;
+            }
+            c.modules.push_back(mod);
+            c.kernels.emplace(sources[missing[j]], fn);
+            ++c.stats.compiled;
+        }
+        c.stats.compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (size_t i = 0; i < sources.size(); ++i) {
+            if (!out[i]) {
+                auto it = c.kernels.find(sources[i]);
+                out[i] = it == c.kernels.end() ? nullptr : it->second;
+            }
+        }
+    }
+    return out;
+}
+
+size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log)
+{
+    std::vector<size_t> idx(sources.size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+        idx[i] = i;
+    }
+    size_t failures = 0;
+    for (const Compiled& c : compile_all(sources, idx)) {
+        if (!c.ok) {
+            ++failures;
+            if (log) {
+                *log = c.log;
+            }
+        }
+    }
+    return failures;
+}
+
+void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs)
+{
+    if (tp.freepos.size() > 31 || tp.freepos.size() + tp.k != s->nvec) {
+        throw InvalidArg {"jit pass: bad program shape"};
+    }
+    void* amps = s->amps;
+    const double* cf = d_coefs;
+    void* args[] = {&amps, &cf};
+    const unsigned blocks = tile_pass_blocks(tp);
+    const unsigned threads = 1u << tp.n_thr_bits;
+    KQ_HIP_CHECK(hipModuleLaunchKernel(fn, blocks, 1, 1, threads, 1, 1, 0, s->stream, args, nullptr));
+}
+
+}  // namespace kq

@@ -554,6 +554,131 @@ TileGate lower_gate(const GateRec& g, int precision, bool derivative)
     return t;
 }
 
+// the coefficients of a lowered record as doubles
+void coef_values(const TileGate& t, int precision, double out[8])
+{
+    if (precision == KQ_C64) {
+        float f[8];
+        std::memcpy(f, t.m, sizeof f);
+        for (int i = 0; i < 8; ++i) {
+            out[i] = f[i];
+        }
+    }
+    else {
+        std::memcpy(out, t.m, 8 * sizeof(double));
+    }
+}
+
+void set_term_coefs(TileTerm& term, int precision, const double d[4])
+{
+    std::memset(term.d, 0, sizeof term.d);
+    if (precision == KQ_C64) {
+        const float f[4] = {static_cast<float>(d[0]), static_cast<float>(d[1]), static_cast<float>(d[2]),
+                            static_cast<float>(d[3])};
+        std::memcpy(term.d, f, sizeof f);
+    }
+    else {
+        std::memcpy(term.d, d, 4 * sizeof(double));
+    }
+}
+
+}  // namespace
+
+// Merge diagonal gates of one step.  Diagonal gates commute with each other, and with a non-diagonal
+// gate unless it targets their target or register-control position, so every diagonal gate without
+// a derivative joins the open run of its key (target position or "uniform", register control);
+// a non-diagonal gate on position P first emits the runs that touch P.  A run of one gate is emitted
+// unchanged; a longer run becomes one merged record whose terms the kernel multiplies per lane.
+void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, std::vector<TileGate>& out,
+                         std::vector<TileTerm>& terms)
+{
+    struct Run
+    {
+        uint32_t key;
+        std::vector<const TileGate*> gates;
+    };
+    std::vector<Run> open;
+    const auto emit = [&](const Run& r) {
+        if (r.gates.size() == 1) {
+            out.push_back(*r.gates[0]);
+            return;
+        }
+        TileGate m {};
+        m.code = TV_MERGE_BASE + r.key;
+        m.lane_sel = static_cast<uint32_t>(terms.size());
+        m.glb_sel = r.gates.size();
+        const bool uniform = r.key / 5 == 4;
+        for (const TileGate* g : r.gates) {
+            TileTerm term {};
+            term.lane_ctrl = g->lane_ctrl;
+            term.glb_ctrl = g->glb_ctrl;
+            double c[8];
+            coef_values(*g, precision, c);
+            double d[4] = {c[0], c[1], c[6], c[7]};
+            if (uniform) {
+                term.lane_sel = g->lane_sel;
+                term.glb_sel = g->glb_sel;
+            }
+            else if (g->code / 20 == TK_DIAG1) {
+                d[0] = 1.0;
+                d[1] = 0.0;
+            }
+            set_term_coefs(term, precision, d);
+            terms.push_back(term);
+        }
+        out.push_back(m);
+    };
+    const auto flush_if = [&](auto pred) {
+        std::vector<Run> keep;
+        for (Run& r : open) {
+            if (pred(r.key)) {
+                emit(r);
+            }
+            else {
+                keep.push_back(std::move(r));
+            }
+        }
+        open = std::move(keep);
+    };
+    for (uint32_t i = 0; i < count; ++i) {
+        const TileGate& g = gates[i];
+        int key = -1;
+        if (g.pad == 0) {
+            if (g.code < TV_UNIFORM_BASE) {
+                const uint32_t kind = g.code / 20;
+                if (kind == TK_DIAG || kind == TK_DIAG1) {
+                    key = static_cast<int>(((g.code / 5) % 4) * 5 + g.code % 5);
+                }
+            }
+            else if (g.code < TV_MERGE_BASE) {
+                key = static_cast<int>(4 * 5 + (g.code - TV_UNIFORM_BASE));
+            }
+        }
+        if (key >= 0) {
+            auto it = std::find_if(open.begin(), open.end(), [&](const Run& r) { return r.key == static_cast<uint32_t>(key); });
+            if (it == open.end()) {
+                open.push_back(Run {static_cast<uint32_t>(key), {&g}});
+            }
+            else {
+                it->gates.push_back(&g);
+            }
+            continue;
+        }
+        if (g.code < TV_UNIFORM_BASE) {
+            const uint32_t kind = g.code / 20;
+            if (kind != TK_DIAG && kind != TK_DIAG1) {
+                const uint32_t pos = (g.code / 5) % 4;
+                flush_if([&](uint32_t k) { return k / 5 == pos || (k % 5 != 0 && k % 5 - 1 == pos); });
+            }
+        }
+        out.push_back(g);
+    }
+    flush_if([](uint32_t) { return true; });
+}
+
+namespace
+{
+
 }  // namespace
 
 // Turn the planner's bit encoding (enc = idx << 2 | src; src 0 register position, 1 thread bit,
@@ -633,7 +758,8 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
         }
     }
     // gates (and derivative records)
-    out.gates.reserve(p.gates.size());
+    std::vector<TileGate> lowered;
+    lowered.reserve(p.gates.size());
     for (size_t i = 0; i < p.gates.size(); ++i) {
         TileGate t = lower_gate(p.gates[i], precision, false);
         const GateRec* dr = deriv_of_src ? (*deriv_of_src)[p.src[i]] : nullptr;
@@ -650,7 +776,15 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
             out.deriv_src.push_back(p.src[i]);
             t.pad = static_cast<uint32_t>(out.derivs.size());
         }
-        out.gates.push_back(t);
+        lowered.push_back(t);
+    }
+    // per step: merge runs of commuting diagonal gates
+    out.gates.reserve(lowered.size());
+    for (size_t si = 0; si < p.stages.size(); ++si) {
+        const uint32_t begin = static_cast<uint32_t>(out.gates.size());
+        merge_diagonal_runs(lowered.data() + p.stages[si].g_begin, p.stages[si].g_count, precision, out.gates, out.terms);
+        out.steps[si].g_begin = begin;
+        out.steps[si].g_count = static_cast<uint32_t>(out.gates.size()) - begin;
     }
 }
 

@@ -28,19 +28,29 @@
 #include <cstring>
 
 #include "kq_internal.hpp"
+#include "kq_tile_device.hpp"
 
 namespace kq
 {
 namespace
 {
 
+using dev::Amp;
+using dev::apply;
+using dev::Coef;
+using dev::Cx;
+using dev::cmul;
+using dev::ddot;
+using dev::ddot_uniform;
+using dev::scale;
+using dev::Unroll;
+static_assert(int {dev::K_DENSE} == int {TK_DENSE} && int {dev::K_REAL} == int {TK_REAL} && int {dev::K_AXIS} == int {TK_AXIS}
+                  && int {dev::K_DIAG} == int {TK_DIAG} && int {dev::K_DIAG1} == int {TK_DIAG1}
+                  && int {dev::K_SWAP} == int {TK_SWAP},
+              "gate kind values");
+
 constexpr int TILE_BLOCK = 256;
 constexpr int MAX_FREE_Q = 48;
-
-template <typename R> struct AmpT;
-template <> struct AmpT<double> { using T = double2; };
-template <> struct AmpT<float> { using T = float2; };
-template <typename R> using Amp = typename AmpT<R>::T;
 
 struct TileArgs
 {
@@ -49,25 +59,6 @@ struct TileArgs
     int n_steps;
     uint32_t out_layout;
     uint32_t pad;
-};
-
-template <typename R>
-struct Cx
-{
-    R re, im;
-};
-
-template <typename R>
-__device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
-{
-    return {a.re * br - a.im * bi, a.re * bi + a.im * br};
-}
-
-// the eight coefficients of a gate record, already in registers
-template <typename R>
-struct Coef
-{
-    R m[8];
 };
 
 // a gate record's coefficients: doubles (complex128) or the floats packed in its first 32 bytes
@@ -92,81 +83,6 @@ __device__ __forceinline__ Coef<R> coef_of(const TileGate& g)
     return c;
 }
 
-// compile-time loop over register slots
-template <int I, int N>
-struct Unroll
-{
-    template <typename F>
-    __device__ __forceinline__ static void run(F&& f)
-    {
-        f.template operator()<I>();
-        Unroll<I + 1, N>::run(f);
-    }
-};
-template <int N>
-struct Unroll<N, N>
-{
-    template <typename F>
-    __device__ __forceinline__ static void run(F&&)
-    {
-    }
-};
-
-// Variant (KIND, P, C): pairs (v[r], v[r | 1 << P]) for every r with bit P clear and, if C > 0,
-// bit C - 1 set.  The zero entries of real, axis-rotation and diagonal matrices are skipped.
-template <typename R, int KIND, int P, int C>
-__device__ __forceinline__ void apply(Cx<R> (&v)[16], const Coef<R>& m)
-{
-    if constexpr (C == P + 1) {
-        return;   // never emitted by the host
-    }
-    else {
-        const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
-        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1)) {
-                constexpr int r1 = r | (1 << P);
-                const Cx<R> a = v[r];
-                const Cx<R> b = v[r1];
-                if constexpr (KIND == TK_SWAP) {
-                    v[r] = b;
-                    v[r1] = a;
-                }
-                else if constexpr (KIND == TK_REAL) {
-                    v[r] = {a.re * m0 + b.re * m2, a.im * m0 + b.im * m2};
-                    v[r1] = {a.re * m4 + b.re * m6, a.im * m4 + b.im * m6};
-                }
-                else if constexpr (KIND == TK_AXIS) {
-                    // m00 = m0, m01 = i m3, m10 = i m5, m11 = m6
-                    v[r] = {a.re * m0 - b.im * m3, a.im * m0 + b.re * m3};
-                    v[r1] = {b.re * m6 - a.im * m5, b.im * m6 + a.re * m5};
-                }
-                else if constexpr (KIND == TK_DIAG1) {
-                    v[r1] = cmul(b, m6, m7);
-                }
-                else if constexpr (KIND == TK_DIAG) {
-                    v[r] = cmul(a, m0, m1);
-                    v[r1] = cmul(b, m6, m7);
-                }
-                else {
-                    v[r] = {a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3, a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2};
-                    v[r1] = {a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7, a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6};
-                }
-            }
-        });
-    }
-}
-
-// Uniform diagonal: every slot (C = 0) or the slots with register bit C - 1 set, times (fr, fi).
-template <typename R, int C>
-__device__ __forceinline__ void scale(Cx<R> (&v)[16], R fr, R fi)
-{
-    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
-            v[r] = cmul(v[r], fr, fi);
-        }
-    });
-}
-
 #define KQ_TILE_CASE(K, P, C) \
     case tile_code(K, P, C): apply<R, K, P, C>(v, m); break;
 #define KQ_TILE_CASES_C(K, P) \
@@ -184,7 +100,9 @@ __device__ __forceinline__ void apply_gate(Cx<R> (&v)[16], uint32_t code, const 
         KQ_TILE_CASES_P(TK_DIAG)
         KQ_TILE_CASES_P(TK_DIAG1)
         KQ_TILE_CASES_P(TK_SWAP)
-        default: break;
+        // the host emits only the codes above: no fall-through edge, so no path merges the
+        // unchanged amplitudes with a changed copy (which would cost register moves per gate)
+        default: __builtin_unreachable();
     }
 }
 
@@ -197,44 +115,8 @@ __device__ __forceinline__ void apply_uniform(Cx<R> (&v)[16], uint32_t c, R fr, 
         case 2: scale<R, 2>(v, fr, fi); break;
         case 3: scale<R, 3>(v, fr, fi); break;
         case 4: scale<R, 4>(v, fr, fi); break;
-        default: break;
+        default: __builtin_unreachable();
     }
-}
-
-// <w| dG |v> restricted to the slots a derivative record touches, real part, FP64 accumulation.
-// Register target P (any matrix, dense formula) with register control C.
-template <typename R, int P, int C>
-__device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16], const Coef<R>& m)
-{
-    const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
-    double acc = 0.0;
-    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-        if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1) && C != P + 1) {
-            constexpr int r1 = r | (1 << P);
-            const Cx<R> a = v[r];
-            const Cx<R> b = v[r1];
-            const R d0r = a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3;
-            const R d0i = a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2;
-            const R d1r = a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7;
-            const R d1i = a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6;
-            acc += static_cast<double>(w[r].re * d0r + w[r].im * d0i) + static_cast<double>(w[r1].re * d1r + w[r1].im * d1i);
-        }
-    });
-    return acc;
-}
-
-// uniform diagonal derivative: sum over the slots (with register control C) of Re(conj(w) f v)
-template <typename R, int C>
-__device__ __forceinline__ double ddot_uniform(const Cx<R> (&v)[16], const Cx<R> (&w)[16], R fr, R fi)
-{
-    double acc = 0.0;
-    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
-            const Cx<R> d = cmul(v[r], fr, fi);
-            acc += static_cast<double>(w[r].re * d.re + w[r].im * d.im);
-        }
-    });
-    return acc;
 }
 
 #define KQ_DDOT_CASE(P, C) \
@@ -265,6 +147,98 @@ __device__ __forceinline__ double ddot_uniform_gate(const Cx<R> (&v)[16], const 
         default: break;
     }
     return 0.0;
+}
+
+// the four coefficients of a merged-run term (d0, d1), doubles or the floats packed first
+template <typename R>
+__device__ __forceinline__ void term_coefs(const TileTerm& t, R& d0r, R& d0i, R& d1r, R& d1i)
+{
+    if constexpr (sizeof(R) == 8) {
+        d0r = t.d[0];
+        d0i = t.d[1];
+        d1r = t.d[2];
+        d1i = t.d[3];
+    }
+    else {
+        const float2 a = __builtin_bit_cast(float2, t.d[0]);
+        const float2 b = __builtin_bit_cast(float2, t.d[1]);
+        d0r = a.x;
+        d0i = a.y;
+        d1r = b.x;
+        d1i = b.y;
+    }
+}
+
+// Per-lane product of a merged run's terms: f1 for slots whose target bit is 1 (or every slot for a
+// uniform target), f0 for target bit 0.
+template <typename R>
+__device__ __forceinline__ void merged_factors(const TileTerm* __restrict__ terms, uint32_t first, uint32_t count,
+                                               bool uniform, uint64_t base, uint32_t tid, R& f0r, R& f0i, R& f1r,
+                                               R& f1i)
+{
+    f0r = 1;
+    f0i = 0;
+    f1r = 1;
+    f1i = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        const TileTerm t = terms[first + j];
+        R d0r, d0i, d1r, d1i;
+        term_coefs<R>(t, d0r, d0i, d1r, d1i);
+        const bool ok = (base & t.glb_ctrl) == t.glb_ctrl && (t.lane_ctrl == 0 || (tid & t.lane_ctrl) == t.lane_ctrl);
+        if (uniform) {
+            const bool sel = t.lane_sel ? ((tid & t.lane_sel) != 0) : ((base & t.glb_sel) != 0);
+            const R fr = sel ? d1r : d0r;
+            const R fi = sel ? d1i : d0i;
+            const R nr = f1r * fr - f1i * fi;
+            const R ni = f1r * fi + f1i * fr;
+            f1r = ok ? nr : f1r;
+            f1i = ok ? ni : f1i;
+        }
+        else {
+            const R n0r = f0r * d0r - f0i * d0i;
+            const R n0i = f0r * d0i + f0i * d0r;
+            const R n1r = f1r * d1r - f1i * d1i;
+            const R n1i = f1r * d1i + f1i * d1r;
+            f0r = ok ? n0r : f0r;
+            f0i = ok ? n0i : f0i;
+            f1r = ok ? n1r : f1r;
+            f1i = ok ? n1i : f1i;
+        }
+    }
+}
+
+#define KQ_DIAG_CASE(P, C) \
+    case (P) * 5 + (C): apply<R, TK_DIAG, P, C>(v, m); break;
+
+// a diagonal with per-lane factors on register position P with register control C (key = P * 5 + C)
+template <typename R>
+__device__ __forceinline__ void apply_diag_key(Cx<R> (&v)[16], uint32_t key, const Coef<R>& m)
+{
+    switch (key) {
+        KQ_DIAG_CASE(0, 0) KQ_DIAG_CASE(0, 2) KQ_DIAG_CASE(0, 3) KQ_DIAG_CASE(0, 4)
+        KQ_DIAG_CASE(1, 0) KQ_DIAG_CASE(1, 1) KQ_DIAG_CASE(1, 3) KQ_DIAG_CASE(1, 4)
+        KQ_DIAG_CASE(2, 0) KQ_DIAG_CASE(2, 1) KQ_DIAG_CASE(2, 2) KQ_DIAG_CASE(2, 4)
+        KQ_DIAG_CASE(3, 0) KQ_DIAG_CASE(3, 1) KQ_DIAG_CASE(3, 2) KQ_DIAG_CASE(3, 3)
+        default: __builtin_unreachable();
+    }
+}
+
+// apply a merged diagonal run to one register block
+template <typename R>
+__device__ __forceinline__ void apply_merged(Cx<R> (&v)[16], uint32_t key, R f0r, R f0i, R f1r, R f1i)
+{
+    if (key >= 20) {
+        apply_uniform<R>(v, key - 20, f1r, f1i);
+    }
+    else {
+        Coef<R> m;
+        m.m[0] = f0r;
+        m.m[1] = f0i;
+        m.m[2] = m.m[3] = m.m[4] = m.m[5] = 0;
+        m.m[6] = f1r;
+        m.m[7] = f1i;
+        apply_diag_key<R>(v, key, m);
+    }
 }
 
 // Thread parts of a layout's addresses.  The per-bit tables are read unconditionally (one batch of
@@ -311,7 +285,8 @@ template <typename R, bool ADJ>
 __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs& d,
                                           const TileStep* __restrict__ steps, const TileLayout* __restrict__ layouts,
                                           const TileGate* __restrict__ gates, const TileGate* __restrict__ derivs,
-                                          double* __restrict__ partials, uint32_t n_part, uint32_t block)
+                                          const TileTerm* __restrict__ terms, double* __restrict__ partials,
+                                          uint32_t n_part, uint32_t block)
 {
     extern __shared__ unsigned char smem_raw[];
     const uint32_t tid = threadIdx.x;
@@ -407,7 +382,14 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                 if (!lane_ok) {
                     continue;   // a control on a thread bit is 0 for this lane
                 }
-                if (code >= TV_UNIFORM_BASE) {
+                if (code >= TV_MERGE_BASE) {
+                    const uint32_t key = code - TV_MERGE_BASE;
+                    R f0r, f0i, f1r, f1i;
+                    merged_factors<R>(terms, g.lane_sel, static_cast<uint32_t>(g.glb_sel), key >= 20, base, tid, f0r, f0i,
+                                      f1r, f1i);
+                    apply_merged<R>(v, key, f0r, f0i, f1r, f1i);
+                }
+                else if (code >= TV_UNIFORM_BASE) {
                     R fr, fi;
                     uniform_factor(fr, fi);
                     apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
@@ -418,12 +400,19 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
             }
             else {
                 const bool ok = glb_ok && lane_ok;
-                R fr = 0, fi = 0;
-                if (code >= TV_UNIFORM_BASE) {
+                R fr = 0, fi = 0, f0r = 0, f0i = 0;
+                if (code >= TV_MERGE_BASE) {
+                    merged_factors<R>(terms, g.lane_sel, static_cast<uint32_t>(g.glb_sel), code - TV_MERGE_BASE >= 20, base,
+                                      tid, f0r, f0i, fr, fi);
+                }
+                else if (code >= TV_UNIFORM_BASE) {
                     uniform_factor(fr, fi);
                 }
                 if (ok) {
-                    if (code >= TV_UNIFORM_BASE) {
+                    if (code >= TV_MERGE_BASE) {
+                        apply_merged<R>(v, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
+                    }
+                    else if (code >= TV_UNIFORM_BASE) {
                         apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
                     }
                     else {
@@ -450,7 +439,10 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                     }
                 }
                 if (ok) {
-                    if (code >= TV_UNIFORM_BASE) {
+                    if (code >= TV_MERGE_BASE) {
+                        apply_merged<R>(w, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
+                    }
+                    else if (code >= TV_UNIFORM_BASE) {
                         apply_uniform<R>(w, code - TV_UNIFORM_BASE, fr, fi);
                     }
                     else {
@@ -487,10 +479,10 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile(Amp<R>* __restrict__ a, Amp
                                                      const TileStep* __restrict__ steps,
                                                      const TileLayout* __restrict__ layouts,
                                                      const TileGate* __restrict__ gates,
-                                                     const TileGate* __restrict__ derivs, double* __restrict__ partials,
-                                                     uint32_t n_part)
+                                                     const TileGate* __restrict__ derivs, const TileTerm* __restrict__ terms,
+                                                     double* __restrict__ partials, uint32_t n_part)
 {
-    tile_body<R, ADJ>(a, b, d, steps, layouts, gates, derivs, partials, n_part, blockIdx.x);
+    tile_body<R, ADJ>(a, b, d, steps, layouts, gates, derivs, terms, partials, n_part, blockIdx.x);
 }
 
 // Many states in one launch: item i covers blocks [block_begin, next item's block_begin) and runs its
@@ -515,11 +507,13 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile_batch(Amp<R>* __restrict__ 
     using GStep = const __attribute__((address_space(1))) TileStep;
     using GLayout = const __attribute__((address_space(1))) TileLayout;
     using GGate = const __attribute__((address_space(1))) TileGate;
+    using GTerm = const __attribute__((address_space(1))) TileTerm;
     using GDouble = __attribute__((address_space(1))) double;
     tile_body<R, ADJ>(arena_a + it.a_off, ADJ ? arena_b + it.b_off : nullptr, d,
                       (const TileStep*) (GStep*) it.steps, (const TileLayout*) (GLayout*) it.layouts,
                       (const TileGate*) (GGate*) it.gates, (const TileGate*) (GGate*) it.derivs,
-                      (double*) (GDouble*) it.partials, it.n_part, blockIdx.x - it.block_begin);
+                      (const TileTerm*) (GTerm*) it.terms, (double*) (GDouble*) it.partials, it.n_part,
+                      blockIdx.x - it.block_begin);
 }
 
 template <typename R, bool ADJ>
@@ -555,6 +549,12 @@ TileArgs check_program(const kq_state* s, const TileProgram& prog)
         if (g.pad > prog.derivs.size()) {
             throw InvalidArg {"tile pass: derivative index out of range"};
         }
+        if (g.code >= TV_END || (g.code < TV_UNIFORM_BASE && g.code % 5 == (g.code / 5) % 4 + 1)) {
+            throw InvalidArg {"tile pass: unknown gate variant"};
+        }
+        if (g.code >= TV_MERGE_BASE && (static_cast<uint64_t>(g.lane_sel) + g.glb_sel > prog.terms.size())) {
+            throw InvalidArg {"tile pass: merged run out of range"};
+        }
     }
     TileArgs d {};
     d.n_thr_bits = prog.n_thr_bits;
@@ -581,7 +581,7 @@ uint32_t tile_pass_blocks(const TileProgram& prog) { return 1u << prog.freepos.s
 uint32_t tile_pass_waves(const TileProgram& prog) { return ((1u << prog.n_thr_bits) + 63) / 64; }
 
 void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates, const void* d_steps,
-                      const void* d_layouts)
+                      const void* d_layouts, const void* d_terms)
 {
     const TileArgs d = check_program(s, prog);
     if (!prog.derivs.empty()) {
@@ -593,15 +593,16 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
     const auto* g = static_cast<const TileGate*>(d_gates);
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
+    const auto* tm = static_cast<const TileTerm*>(d_terms);
     if (s->precision == KQ_C64) {
         set_lds_attr<float, false>();
         k_tile<float, false><<<blocks, threads, lds, s->stream>>>(static_cast<float2*>(s->amps), nullptr, d, st, ly, g,
-                                                                   nullptr, nullptr, 0);
+                                                                   nullptr, tm, nullptr, 0);
     }
     else {
         set_lds_attr<double, false>();
         k_tile<double, false><<<blocks, threads, lds, s->stream>>>(static_cast<double2*>(s->amps), nullptr, d, st, ly, g,
-                                                                    nullptr, nullptr, 0);
+                                                                    nullptr, tm, nullptr, 0);
     }
     KQ_HIP_CHECK(hipGetLastError());
 }
@@ -644,7 +645,8 @@ void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* ar
 }
 
 void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgram& prog, const void* d_gates,
-                              const void* d_steps, const void* d_layouts, const void* d_derivs, double* d_partials)
+                              const void* d_steps, const void* d_layouts, const void* d_derivs, const void* d_terms,
+                              double* d_partials)
 {
     const TileArgs d = check_program(s, prog);
     const unsigned blocks = tile_pass_blocks(prog);
@@ -655,15 +657,16 @@ void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgr
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
     const auto* dv = static_cast<const TileGate*>(d_derivs);
+    const auto* tm = static_cast<const TileTerm*>(d_terms);
     if (s->precision == KQ_C64) {
         set_lds_attr<float, true>();
         k_tile<float, true><<<blocks, threads, lds, s->stream>>>(static_cast<float2*>(psi), static_cast<float2*>(lam), d, st,
-                                                                  ly, g, dv, d_partials, n_part);
+                                                                  ly, g, dv, tm, d_partials, n_part);
     }
     else {
         set_lds_attr<double, true>();
         k_tile<double, true><<<blocks, threads, lds, s->stream>>>(static_cast<double2*>(psi), static_cast<double2*>(lam), d,
-                                                                   st, ly, g, dv, d_partials, n_part);
+                                                                   st, ly, g, dv, tm, d_partials, n_part);
     }
     KQ_HIP_CHECK(hipGetLastError());
 }

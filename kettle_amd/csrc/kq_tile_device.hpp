@@ -1,0 +1,211 @@
+// Device templates of the tile pass, shared by the compiled tile kernels (kq_tile.hip) and the pass
+// kernels generated at run time (kq_jit.cpp).  Self-contained: it compiles under hipcc after the HIP
+// runtime header and under hiprtc, whose prelude supplies the fixed-width integer typedefs.
+#pragma once
+
+namespace kq
+{
+namespace dev
+{
+
+// gate kinds (the values of kq::TileKind)
+enum : int
+{
+    K_DENSE = 0,
+    K_REAL = 1,
+    K_AXIS = 2,
+    K_DIAG = 3,
+    K_DIAG1 = 4,
+    K_SWAP = 5
+};
+
+template <typename R> struct AmpT;
+template <> struct AmpT<double> { using T = double2; };
+template <> struct AmpT<float> { using T = float2; };
+template <typename R> using Amp = typename AmpT<R>::T;
+
+template <typename R>
+struct Cx
+{
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
+{
+    return {a.re * br - a.im * bi, a.re * bi + a.im * br};
+}
+
+// the eight coefficients of a gate record, already in registers
+template <typename R>
+struct Coef
+{
+    R m[8];
+};
+
+// compile-time loop over register slots
+template <int I, int N>
+struct Unroll
+{
+    template <typename F>
+    __device__ __forceinline__ static void run(F&& f)
+    {
+        f.template operator()<I>();
+        Unroll<I + 1, N>::run(f);
+    }
+};
+template <int N>
+struct Unroll<N, N>
+{
+    template <typename F>
+    __device__ __forceinline__ static void run(F&&)
+    {
+    }
+};
+
+// Variant (KIND, P, C): pairs (v[r], v[r | 1 << P]) for every r with bit P clear and, if C > 0,
+// bit C - 1 set.  The zero entries of real, axis-rotation and diagonal matrices are skipped.
+template <typename R, int KIND, int P, int C>
+__device__ __forceinline__ void apply(Cx<R> (&v)[16], const Coef<R>& m)
+{
+    if constexpr (C == P + 1) {
+        __builtin_unreachable();   // never emitted by the host
+    }
+    else {
+        const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1)) {
+                constexpr int r1 = r | (1 << P);
+                const Cx<R> a = v[r];
+                const Cx<R> b = v[r1];
+                if constexpr (KIND == K_SWAP) {
+                    v[r] = b;
+                    v[r1] = a;
+                }
+                else if constexpr (KIND == K_REAL) {
+                    v[r] = {a.re * m0 + b.re * m2, a.im * m0 + b.im * m2};
+                    v[r1] = {a.re * m4 + b.re * m6, a.im * m4 + b.im * m6};
+                }
+                else if constexpr (KIND == K_AXIS) {
+                    // m00 = m0, m01 = i m3, m10 = i m5, m11 = m6
+                    v[r] = {a.re * m0 - b.im * m3, a.im * m0 + b.re * m3};
+                    v[r1] = {b.re * m6 - a.im * m5, b.im * m6 + a.re * m5};
+                }
+                else if constexpr (KIND == K_DIAG1) {
+                    v[r1] = cmul(b, m6, m7);
+                }
+                else if constexpr (KIND == K_DIAG) {
+                    v[r] = cmul(a, m0, m1);
+                    v[r1] = cmul(b, m6, m7);
+                }
+                else {
+                    v[r] = {a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3, a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2};
+                    v[r1] = {a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7, a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6};
+                }
+            }
+        });
+    }
+}
+
+// Uniform diagonal: every slot (C = 0) or the slots with register bit C - 1 set, times (fr, fi).
+template <typename R, int C>
+__device__ __forceinline__ void scale(Cx<R> (&v)[16], R fr, R fi)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
+            v[r] = cmul(v[r], fr, fi);
+        }
+    });
+}
+
+// <w| dG |v> restricted to the slots a derivative record touches, real part, FP64 accumulation.
+// Register target P (any matrix, dense formula) with register control C.
+template <typename R, int P, int C>
+__device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16], const Coef<R>& m)
+{
+    const R m0 = m.m[0], m1 = m.m[1], m2 = m.m[2], m3 = m.m[3], m4 = m.m[4], m5 = m.m[5], m6 = m.m[6], m7 = m.m[7];
+    double acc = 0.0;
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1) && C != P + 1) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r];
+            const Cx<R> b = v[r1];
+            const R d0r = a.re * m0 - a.im * m1 + b.re * m2 - b.im * m3;
+            const R d0i = a.re * m1 + a.im * m0 + b.re * m3 + b.im * m2;
+            const R d1r = a.re * m4 - a.im * m5 + b.re * m6 - b.im * m7;
+            const R d1i = a.re * m5 + a.im * m4 + b.re * m7 + b.im * m6;
+            acc += static_cast<double>(w[r].re * d0r + w[r].im * d0i) + static_cast<double>(w[r1].re * d1r + w[r1].im * d1i);
+        }
+    });
+    return acc;
+}
+
+// uniform diagonal derivative: sum over the slots (with register control C) of Re(conj(w) f v)
+template <typename R, int C>
+__device__ __forceinline__ double ddot_uniform(const Cx<R> (&v)[16], const Cx<R> (&w)[16], R fr, R fi)
+{
+    double acc = 0.0;
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (C == 0 || ((r >> (C - 1)) & 1) == 1) {
+            const Cx<R> d = cmul(v[r], fr, fi);
+            acc += static_cast<double>(w[r].re * d.re + w[r].im * d.im);
+        }
+    });
+    return acc;
+}
+
+// Run-time generated passes read coefficients from a per-pass buffer of doubles: 8 per gate
+// (complex128), or 4 doubles holding 8 packed floats (complex64).
+template <typename R>
+__device__ __forceinline__ Coef<R> coef_at(const double* __restrict__ cf)
+{
+    Coef<R> c;
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            c.m[i] = cf[i];
+        }
+    }
+    else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float2 f = __builtin_bit_cast(float2, cf[i]);
+            c.m[2 * i] = f.x;
+            c.m[2 * i + 1] = f.y;
+        }
+    }
+    return c;
+}
+
+// diagonal term (d0, d1) from the buffer: 4 doubles (complex128) or 2 doubles of packed floats
+template <typename R>
+__device__ __forceinline__ void term_at(const double* __restrict__ cf, R& d0r, R& d0i, R& d1r, R& d1i)
+{
+    if constexpr (sizeof(R) == 8) {
+        d0r = cf[0];
+        d0i = cf[1];
+        d1r = cf[2];
+        d1i = cf[3];
+    }
+    else {
+        const float2 a = __builtin_bit_cast(float2, cf[0]);
+        const float2 b = __builtin_bit_cast(float2, cf[1]);
+        d0r = a.x;
+        d0i = a.y;
+        d1r = b.x;
+        d1i = b.y;
+    }
+}
+
+// f *= d where the condition holds
+template <typename R>
+__device__ __forceinline__ void mul_if(bool ok, R& fr, R& fi, R dr, R di)
+{
+    const R nr = fr * dr - fi * di;
+    const R ni = fr * di + fi * dr;
+    fr = ok ? nr : fr;
+    fi = ok ? ni : fi;
+}
+
+}  // namespace dev
+}  // namespace kq

@@ -1,0 +1,158 @@
+"""Run-time compiled tile passes: generated kernels against the interpreted tile kernel and the oracle.
+
+The generated pass kernel holds a pass's structure as straight-line code and reads only the gate
+coefficients from memory; these tests check that it gives the oracle's amplitudes (complex128 within
+1e-12, complex64 within 2e-5), that a structure repeated with other angles reuses the compiled kernel,
+and (host only) that the generated sources of every gate kind compile for gfx950."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import kettle_amd as ka
+from kat_cases import basis, random_gate_circuit
+from oracle import oracle as orc
+
+TOL128 = 1e-12
+TOL64 = 2e-5
+
+
+@pytest.fixture(autouse=True)
+def _restore():
+    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    yield
+    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_jit(1)
+
+
+def ops_of(circuit):
+    ops, _, _ = circuit.flatten()
+    return ops
+
+
+# ---- host only ---------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_generated_sources_compile(precision):
+    """Every gate kind (controls on thread, register and block bits; merged diagonal runs)."""
+    rng = np.random.default_rng(7 + precision)
+    ops = ops_of(random_gate_circuit(12, 120, rng))
+    assert ka.debug_jit_compile(ops, 12, precision) >= 1
+
+
+def test_generated_source_is_straight_line():
+    rng = np.random.default_rng(3)
+    ops = ops_of(random_gate_circuit(12, 80, rng))
+    src, n_passes = ka.debug_jit_source(ops, 12, 0)
+    assert n_passes >= 1
+    assert 'extern "C" __global__' in src and "kq_pass" in src
+    assert "switch" not in src.split("using R =")[1]   # no gate dispatch in the kernel body
+    with pytest.raises(ka.KqError, match="out of range"):
+        ka.debug_jit_source(ops, 12, n_passes)
+
+
+def test_set_jit_rejects_bad_mode():
+    with pytest.raises(ka.KqError, match="jit mode"):
+        ka.set_jit(3)
+
+
+# ---- device ----------------------------------------------------------------------------------------------
+
+def run_sv(circuit, precision=ka.KQ_C128):
+    sv = ka.Statevector(circuit.n_qubits(), precision)
+    ka.simulate(circuit, sv)
+    return sv.download()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [5, 8, 12, 15])
+def test_jit_random_complex128(n):
+    ka.set_jit(2)
+    before = ka.jit_info()
+    c = random_gate_circuit(n, 200, np.random.default_rng(4000 + n))
+    out = run_sv(c)
+    info = ka.jit_info()
+    assert info["failures"] == before["failures"]
+    assert info["compiled"] + info["cache_hits"] > before["compiled"] + before["cache_hits"]
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(out - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [6, 13])
+def test_jit_random_complex64(n):
+    ka.set_jit(2)
+    before = ka.jit_info()
+    c = random_gate_circuit(n, 150, np.random.default_rng(5000 + n))
+    out = run_sv(c, ka.KQ_C64)
+    assert ka.jit_info()["failures"] == before["failures"]
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(out - ref)) <= TOL64
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("k", [4, 8, 12])
+def test_jit_matches_interpreter_bitwise_structure(k):
+    """Same plan, generated vs interpreted: equal to rounding (both FP64, same operation order)."""
+    n = 14
+    c = random_gate_circuit(n, 300, np.random.default_rng(6000 + k))
+    ka.set_fusion(k)
+    ka.set_jit(0)
+    interp = run_sv(c)
+    ka.set_jit(2)
+    jit = run_sv(c)
+    assert np.max(np.abs(jit - interp)) <= 1e-13
+
+
+@pytest.mark.gpu
+def test_jit_reuses_kernels_for_new_angles():
+    n = 12
+    ka.set_jit(2)
+
+    def layer_circuit(theta):
+        c = ka.QuantumCircuit(n)
+        for q in range(n):
+            c.add_h_gate(q)
+        for layer in range(3):
+            for q in range(n):
+                c.add_ry_gate(q, float(theta[layer, q]))
+                c.add_rz_gate(q, float(theta[layer, q] * 0.5))
+            for q in range(n - 1):
+                c.add_cx_gate(q, q + 1)
+            c.add_cp_gate(0, n - 1, float(theta[layer, 0]))
+        return c
+
+    rng = np.random.default_rng(11)
+    c1 = layer_circuit(rng.uniform(0, 2 * math.pi, size=(3, n)))
+    run_sv(c1)
+    after_first = ka.jit_info()
+    c2 = layer_circuit(rng.uniform(0, 2 * math.pi, size=(3, n)))
+    out = run_sv(c2)
+    after_second = ka.jit_info()
+    assert after_second["compiled"] == after_first["compiled"]
+    assert after_second["cache_hits"] > after_first["cache_hits"]
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c2))
+    assert np.max(np.abs(out - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+def test_jit_density_matrix():
+    n = 6
+    ka.set_jit(2)
+    c = random_gate_circuit(n, 60, np.random.default_rng(77))
+    rho0 = orc.sv_to_dm(basis("0" * n), n)
+    ref, _ = orc.dm_simulate(rho0, n, ops_of(c))
+    dm = ka.DensityMatrix(n)
+    ka.simulate(c, dm)
+    assert np.max(np.abs(dm.download() - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+def test_jit_mode_one_leaves_small_states_interpreted():
+    ka.set_jit(1)
+    before = ka.jit_info()
+    run_sv(random_gate_circuit(10, 50, np.random.default_rng(5)))
+    after = ka.jit_info()
+    assert after["compiled"] == before["compiled"] and after["cache_hits"] == before["cache_hits"]
