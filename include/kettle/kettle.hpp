@@ -13,6 +13,8 @@
 #include <kettle/gates/random_u_gates.hpp>
 #include <kettle/gradient/energy_gradient.hpp>
 #include <kettle/io/read_pauli_operator.hpp>
+#include <kettle/io/read_tangelo_file.hpp>
+#include <kettle/io/write_tangelo_file.hpp>
 #include <kettle/operator/pauli/pauli_operator.hpp>
 #include <kettle/operator/pauli/sparse_pauli_string.hpp>
 #include <kettle/optimize/n_local.hpp>

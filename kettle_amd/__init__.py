@@ -6,12 +6,13 @@ Importing the package does not touch the GPU; the first call that needs libkq lo
 from ._lib import KQ_C128, KQ_C64, KqError
 from .circuit import (ControlFlowBooleanKind, ControlFlowPredicate, Gate, Matrix2X2, ParameterID, QuantumCircuit,
                       extend_circuit)
-from .api import (DensityMatrix, EnergyGradient, FusionPlan, PackedPauliOperator, debug_jit_compile, debug_jit_source, debug_plan, jit_info,
-                  set_jit, PauliOperator, PauliPhase, PauliTerm, QuantumNoise,
+from .api import (DensityMatrix, EnergyGradient, FusionPlan, PackedPauliOperator, PauliOperator, PauliPhase,
+                  PauliTerm, QuantumNoise, debug_jit_compile, debug_jit_source, debug_plan, jit_info, set_jit,
                   SimulationResult, SparsePauliString, Statevector, apply_gates, calculate_probabilities_raw,
                   energy_and_gradient, expectation_value, partial_trace, perform_measurements_as_counts,
                   perform_measurements_as_counts_marginal, perform_measurements_as_counts_raw,
                   perform_measurements_as_memory, plan_stats, set_fusion, simulate, simulate_pauli_string,
                   state_index_to_bitstring, statevector_to_density_matrix)
+from .io import read_pauli_operator, read_tangelo_circuit, write_tangelo_circuit
 
 __all__ = [name for name in dir() if not name.startswith("_")]

@@ -1,8 +1,12 @@
-"""Hamiltonian text reader: kettle's ``read_pauli_operator``
-(io/read_pauli_operator.cpp:29-69 in the reference).
+"""Text formats at the boundary of the simulator.
 
-One weighted Pauli string per line, ``<re> <im> : (q, P) (q, P) ...`` with P in {X, Y, Z}; a line
-with no terms is the identity string.
+* kettle's Hamiltonian reader ``read_pauli_operator`` (io/read_pauli_operator.cpp:29-69 in the
+  reference): one weighted Pauli string per line, ``<re> <im> : (q, P) (q, P) ...`` with P in
+  {X, Y, Z}; a line with no terms is the identity string.
+* kettle's tangelo-style circuit reader and writer ``read_tangelo_circuit`` /
+  ``write_tangelo_circuit`` (io/read_tangelo_file.cpp:241-376, io/write_tangelo_file.cpp:24-213,
+  io/io_control_flow.cpp:13-139): one gate per line (``RX        target : [5]   parameter : 1.57``),
+  U / CU followed by two matrix lines, ``IF BITS[..] == [..]`` / ``ELSE`` blocks indented by 4 spaces.
 """
 from __future__ import annotations
 
@@ -12,6 +16,8 @@ import re
 from typing import TextIO
 
 from .api import PauliOperator, PauliTerm, SparsePauliString
+from .circuit import (CONTROLLED, CONTROLLED_ANGLE, ONE_TARGET, ONE_TARGET_ANGLE, ControlFlowBooleanKind,
+                      ControlFlowPredicate, Gate, Matrix2X2, QuantumCircuit, _GateEl, _IfEl, _LogEl)
 
 _TERMS = {"X": PauliTerm.X, "Y": PauliTerm.Y, "Z": PauliTerm.Z}
 _TERM_RE = re.compile(r"\(\s*(\d+)\s*,\s*(\S)\s*\)")
@@ -42,3 +48,190 @@ def _read(stream: TextIO | io.StringIO, n_qubits: int) -> PauliOperator:
             s.add(int(q), _TERMS[t])
         op.add(complex(float(re_part), float(im_part)), s)
     return op
+
+
+# ---- tangelo-style circuits ------------------------------------------------------------------------------
+
+CONTROL_FLOW_INDENT = 4                      # io_control_flow.hpp:19
+_TANGELO_NAMES = {"CPHASE": "CP", "CNOT": "CX", "PHASE": "P"}   # read_tangelo_file.cpp:23-37
+_INT_RE = re.compile(r"\d+")
+_FLOAT_RE = re.compile(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?")
+
+
+def _bracket_ints(text: str) -> list[int]:
+    """The integers inside each ``[...]`` of a gate line, in order."""
+    out = []
+    for group in re.findall(r"\[([^\]]*)\]", text):
+        out.extend(int(v) for v in _INT_RE.findall(group))
+    return out
+
+
+def _angle(text: str) -> float:
+    return float(_FLOAT_RE.findall(text.split("parameter", 1)[1])[0])
+
+
+def _matrix(lines: list[str], i: int) -> Matrix2X2:
+    a = [float(v) for v in _FLOAT_RE.findall(lines[i])]
+    b = [float(v) for v in _FLOAT_RE.findall(lines[i + 1])]
+    return Matrix2X2(complex(a[0], a[1]), complex(a[2], a[3]), complex(b[0], b[1]), complex(b[2], b[3]))
+
+
+def _index_list(text: str) -> tuple[list[int], str]:
+    """``[i, j, ...]`` at the start of ``text`` (after any prefix up to '['); returns the rest."""
+    start = text.index("[")
+    end = text.index("]", start)
+    body = text[start + 1:end]
+    return [int(v) for v in re.findall(r"-?\d+", body)], text[end + 1:]
+
+
+def _predicate(text: str) -> ControlFlowPredicate:
+    bits, rest = _index_list(text)
+    sign = rest.split()[0] if rest.split() else ""
+    if sign == "==":
+        kind = ControlFlowBooleanKind.IF
+    elif sign == "!=":
+        kind = ControlFlowBooleanKind.IF_NOT
+    else:
+        raise RuntimeError("Invalid comparison sign found; must be '==' or '!='\n")
+    expected, _ = _index_list(rest)
+    return ControlFlowPredicate(bits, expected, kind)
+
+
+def _read_block(n_qubits: int, lines: list[str], i: int, indent: int | None) -> tuple[QuantumCircuit, int]:
+    circuit = QuantumCircuit(n_qubits)
+    while i < len(lines):
+        line = lines[i]
+        if indent is not None and not line.startswith(" " * indent):
+            return circuit, i
+        i += 1
+        words = line.split()
+        if not words:
+            continue
+        name = words[0]
+        if name == "IF":
+            predicate = _predicate(line.split("IF", 1)[1])
+            body, i = _read_block(n_qubits, lines, i, CONTROL_FLOW_INDENT)
+            circuit.add_if_statement(predicate, body)
+            continue
+        if name == "ELSE":
+            els = circuit.elements
+            if not els or not isinstance(els[-1], _IfEl) or els[-1].else_body is not None:
+                raise RuntimeError(
+                    "ERROR: encountered an 'ELSE' statement, but no previous matching 'IF' statement was found.\n")
+            stmt = els[-1]
+            circuit.pop_back()
+            else_body, i = _read_block(n_qubits, lines, i, CONTROL_FLOW_INDENT)
+            circuit.add_if_else_statement(stmt.predicate, stmt.body, else_body)
+            continue
+        name = _TANGELO_NAMES.get(name, name)
+        if name == "SWAP":
+            q0, q1 = _bracket_ints(line)[:2]
+            circuit.add_swap_gate(q0, q1)
+            continue
+        try:
+            gate = Gate[name]
+        except KeyError:
+            raise RuntimeError(f"Unknown gate found in `read_tangelo_file()` : {name}\n") from None
+        ints = _bracket_ints(line)
+        if gate in ONE_TARGET:
+            circuit._one(gate, ints[0])
+        elif gate in ONE_TARGET_ANGLE:
+            circuit._one_angle(gate, ints[0], _angle(line))
+        elif gate in CONTROLLED:
+            circuit._ctrl(gate, ints[1], ints[0])
+        elif gate in CONTROLLED_ANGLE:
+            circuit._ctrl_angle(gate, ints[1], ints[0], _angle(line))
+        elif gate == Gate.M:
+            circuit.add_m_gate(ints[0], ints[1])
+        elif gate == Gate.U:
+            circuit.add_u_gate(_matrix(lines, i), ints[0])
+            i += 2
+        else:
+            circuit.add_cu_gate(_matrix(lines, i), ints[1], ints[0])
+            i += 2
+    return circuit, i
+
+
+def read_tangelo_circuit(n_qubits: int, source: str | os.PathLike | TextIO, n_skip_lines: int = 0) -> QuantumCircuit:
+    """``source``: a path, or an open text stream; the first ``n_skip_lines`` lines are skipped.
+    Tangelo's CNOT / CPHASE / PHASE are read as CX / CP / P, SWAP as three CX gates; an unknown
+    gate name raises RuntimeError."""
+    if isinstance(source, (str, os.PathLike)):
+        try:
+            with open(source, encoding="utf-8") as f:
+                text = f.read()
+        except FileNotFoundError as e:
+            raise OSError(f"ERROR: unable to read tangelo circuit from : '{os.fspath(source)}'\n") from e
+    else:
+        text = source.read()
+    lines = text.split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    circuit, _ = _read_block(int(n_qubits), lines, int(n_skip_lines), None)
+    return circuit
+
+
+def _matrix_element(x: float) -> str:
+    return format(x, " .17f")[:16]            # write_tangelo_file.cpp:24-32
+
+
+def _matrix_lines(m: Matrix2X2) -> str:
+    def pair(z: complex) -> str:
+        return f"[{_matrix_element(z.real)}, {_matrix_element(z.imag)}]"
+    return f"    {pair(m.elem00)}   {pair(m.elem01)}\n    {pair(m.elem10)}   {pair(m.elem11)}\n"
+
+
+def _predicate_text(p: ControlFlowPredicate) -> str:
+    sign = " == " if p.control_kind == ControlFlowBooleanKind.IF else " != "
+    def lst(v):
+        return "[" + ", ".join(str(int(x)) for x in v) + "]"
+    return "IF BITS" + lst(p.bit_indices_to_check) + sign + lst(p.expected_bits)
+
+
+def _write(circuit: QuantumCircuit, out: list[str], indent: int) -> None:
+    pad = " " * indent
+    for el in circuit.elements:
+        if isinstance(el, _LogEl):
+            continue
+        if isinstance(el, _IfEl):
+            out.append(_predicate_text(el.predicate) + "\n")
+            _write(el.body, out, CONTROL_FLOW_INDENT)
+            if el.else_body is not None:
+                out.append("ELSE\n")
+                _write(el.else_body, out, CONTROL_FLOW_INDENT)
+            continue
+        assert isinstance(el, _GateEl)
+        g = el.gate
+        head = pad + f"{g.name:<10}"
+        # a parameterised gate carries the placeholder angle 0.0, as in the reference (gate_create.cpp:27,84)
+        angle = 0.0 if el.param is not None else el.angle
+        if g in ONE_TARGET:
+            out.append(f"{head}target : [{el.q0}]\n")
+        elif g in ONE_TARGET_ANGLE:
+            out.append(f"{head}target : [{el.q0}]   parameter : {angle:.16f}\n")
+        elif g in CONTROLLED:
+            out.append(f"{head}target : [{el.q1}]   control : [{el.q0}]\n")
+        elif g in CONTROLLED_ANGLE:
+            out.append(f"{head}target : [{el.q1}]   control : [{el.q0}]   parameter : {angle:.16f}\n")
+        elif g == Gate.M:
+            out.append(f"{head}target : [{el.q0}]   bit : [{el.q1}]\n")
+        elif g == Gate.U:
+            out.append(f"{head}target : [{el.q0}]\n" + _matrix_lines(el.matrix))
+        else:
+            out.append(f"{head}target : [{el.q1}]   control : [{el.q0}]\n" + _matrix_lines(el.matrix))
+
+
+def write_tangelo_circuit(circuit: QuantumCircuit, dest: str | os.PathLike | TextIO | None = None) -> str:
+    """Writes to ``dest`` (a path or a text stream) and returns the text; circuit loggers are skipped."""
+    out: list[str] = []
+    _write(circuit, out, 0)
+    text = "".join(out)
+    if isinstance(dest, (str, os.PathLike)):
+        try:
+            with open(dest, "w", encoding="utf-8") as f:
+                f.write(text)
+        except OSError as e:
+            raise OSError(f"ERROR: unable to open file to write tangelo-style circuit: '{os.fspath(dest)}'\n") from e
+    elif dest is not None:
+        dest.write(text)
+    return text
