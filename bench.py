@@ -190,6 +190,8 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--jit", type=int, default=1,
                     help="run-time compiled tile passes: 0 off, 1 states of >= 18 qubits, 2 all")
+    ap.add_argument("--jit-tuning", default="0,0",
+                    help="min waves per SIMD, persistent blocks per CU of the run-time compiled passes")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
@@ -233,6 +235,8 @@ def main() -> int:
         ka.apply_gates(sv, ops)
 
     ka.set_jit(args.jit)
+    waves, per_cu = (int(x) for x in args.jit_tuning.split(","))
+    ka.set_jit_tuning(waves, per_cu)
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -304,7 +308,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "passes": passes,
-        "jit": {"mode": args.jit, "kernels_compiled": jit_end["compiled"], "failures": jit_end["failures"],
+        "jit": {"mode": args.jit, "tuning": args.jit_tuning, "kernels_compiled": jit_end["compiled"], "failures": jit_end["failures"],
                 "compile_s_outside_timed_region": round(jit_warm["compile_seconds"], 2),
                 "compiles_inside_timed_region": jit_end["compiled"] - jit_warm["compiled"]},
     }

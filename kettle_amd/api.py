@@ -222,6 +222,12 @@ def set_jit(mode: int) -> None:
     check(load().kq_set_jit(int(mode)))
 
 
+def set_jit_tuning(min_waves_per_simd: int = 0, blocks_per_cu: int = 0) -> None:
+    """Register cap (min waves per SIMD, 0 = compiler's choice) and persistent blocks per CU (0 = one
+    block per tile) of the run-time compiled pass kernels."""
+    check(load().kq_set_jit_tuning(int(min_waves_per_simd), int(blocks_per_cu)))
+
+
 def jit_info() -> dict:
     c = _lib.ctypes
     mode, comp, hits, fails, secs = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double()

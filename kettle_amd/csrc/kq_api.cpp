@@ -73,6 +73,7 @@ void ensure_init()
         hipDeviceProp_t prop;
         KQ_HIP_CHECK(hipGetDeviceProperties(&prop, g_device));
         device_lds_per_block() = prop.sharedMemPerBlock;
+        device_cus() = static_cast<uint32_t>(prop.multiProcessorCount);
         g_initialised = true;
     }
     else {
@@ -1419,6 +1420,11 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
 int kq_set_jit(int mode)
 {
     return guarded([&] { set_jit_mode(mode); });
+}
+
+int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu)
+{
+    return guarded([&] { set_jit_tuning(JitTuning {min_waves_per_simd, blocks_per_cu}); });
 }
 
 int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds)

@@ -387,6 +387,17 @@ struct JitStats
 JitStats& jit_stats();
 void set_jit_mode(int mode);   // 0 off, 1 states of >= 18 qubits, 2 every tile pass
 int jit_mode();
+// min_waves_per_simd > 0 caps the generated kernels' registers so that many waves fit per SIMD;
+// blocks_per_cu > 0 launches that many blocks per CU, each walking several tiles (0: one block per tile)
+struct JitTuning
+{
+    int min_waves_per_simd = 0;
+    int blocks_per_cu = 0;
+};
+void set_jit_tuning(const JitTuning& t);
+JitTuning jit_tuning();
+uint32_t device_cu_count();
+uint32_t& device_cus();
 bool jit_wanted(const kq_state* s);
 bool jit_available();
 // kernel source of one forward pass; coefs receives its coefficient buffer

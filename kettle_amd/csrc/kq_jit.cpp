@@ -41,6 +41,7 @@ struct JitCache
     JitStats stats {};
     int mode = 1;            // 0 off, 1 large states, 2 every tile pass
     int available = -1;      // hiprtc usable (-1 unknown)
+    JitTuning tuning {};
 };
 
 JitCache& cache()
@@ -138,6 +139,17 @@ void set_jit_mode(int mode)
 
 int jit_mode() { return cache().mode; }
 
+void set_jit_tuning(const JitTuning& t)
+{
+    if (t.min_waves_per_simd < 0 || t.min_waves_per_simd > 8 || t.blocks_per_cu < 0 || t.blocks_per_cu > 64) {
+        throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8] and blocks per CU in [0, 64]"};
+    }
+    std::lock_guard<std::mutex> lk(cache().mu);
+    cache().tuning = t;
+}
+
+JitTuning jit_tuning() { return cache().tuning; }
+
 bool jit_wanted(const kq_state* s)
 {
     const int mode = cache().mode;
@@ -151,10 +163,15 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     const bool c64 = precision == KQ_C64;
     const int nt = 1 << tp.n_thr_bits;
     std::ostringstream o;
+    const JitTuning tune = jit_tuning();
     o << "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\ntypedef int int32_t;\n"
       << DEVICE_HEADER << "\nusing R = " << (c64 ? "float" : "double") << ";\n"
-      << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void kq_pass(kq::dev::Amp<R>* __restrict__ a, "
-      << "const double* __restrict__ cf)\n{\n  using namespace kq::dev;\n";
+      << "extern \"C\" __global__ __launch_bounds__(" << nt << ")";
+    if (tune.min_waves_per_simd > 0) {
+        o << " __attribute__((amdgpu_waves_per_eu(" << tune.min_waves_per_simd << ")))";
+    }
+    o << " void kq_pass(kq::dev::Amp<R>* __restrict__ a, const double* __restrict__ cf, uint32_t n_blocks)\n{\n"
+      << "  using namespace kq::dev;\n";
     bool remaps = tp.out_layout != 0;
     for (const TileStep& st : tp.steps) {
         remaps = remaps || st.layout != 0;
@@ -163,8 +180,28 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         o << "  __shared__ Amp<R> tile[" << (1 << tp.k) << "];\n"
           << "  char* const lds = reinterpret_cast<char*>(tile);\n";
     }
-    o << "  const uint32_t tid = threadIdx.x;\n  const uint64_t blk = blockIdx.x;\n"
-      << "  const uint64_t base = " << deposit_expr(tp.freepos) << ";\n  Cx<R> v[16];\n";
+    // Blocks walk the tiles with a grid stride.  With more tiles than blocks, the next tile's amplitudes
+    // are loaded into registers before the current tile's gates run, so HBM reads overlap the arithmetic.
+    const bool persistent = tune.blocks_per_cu > 0;
+    o << "  const uint32_t tid = threadIdx.x;\n  Cx<R> v[16];\n";
+    if (persistent) {
+        o << "  const uint64_t thr_in = " << hbm_thread_expr(tp.layouts[0], tp.n_thr_bits) << ";\n"
+          << "  uint32_t b = blockIdx.x;\n  Amp<R> nx[16];\n"
+          << "  const auto fetch = [&](uint64_t blk) {\n    const uint64_t off = (" << deposit_expr(tp.freepos)
+          << ") | thr_in;\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    nx[" << r << "] = a[off + " << hex64(tp.layouts[0].hbm_slot[r]) << "];\n";
+        }
+        o << "  };\n  fetch(b);\n  for (; b < n_blocks; b += gridDim.x) {\n"
+          << "    const uint64_t blk = b;\n    const uint64_t base = " << deposit_expr(tp.freepos) << ";\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    v[" << r << "] = {nx[" << r << "].x, nx[" << r << "].y};\n";
+        }
+        o << "    if (b + gridDim.x < n_blocks) {\n      fetch(b + gridDim.x);\n    }\n";
+    }
+    else {
+        o << "  {\n    const uint64_t blk = blockIdx.x;\n    const uint64_t base = " << deposit_expr(tp.freepos) << ";\n";
+    }
     const auto io = [&](const TileLayout& L, bool load) {
         o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
         for (int r = 0; r < 16; ++r) {
@@ -215,7 +252,9 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
         return off;
     };
-    io(tp.layouts[0], true);
+    if (!persistent) {
+        io(tp.layouts[0], true);
+    }
     uint32_t cur = 0;
     for (const TileStep& st : tp.steps) {
         if (st.layout != cur) {
@@ -267,7 +306,7 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         cur = tp.out_layout;
     }
     io(tp.layouts[cur], false);
-    o << "}\n";
+    o << "  }\n}\n";
     return o.str();
 }
 
@@ -423,10 +462,15 @@ void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const
     }
     void* amps = s->amps;
     const double* cf = d_coefs;
-    void* args[] = {&amps, &cf};
-    const unsigned blocks = tile_pass_blocks(tp);
+    uint32_t n_blocks = tile_pass_blocks(tp);
+    void* args[] = {&amps, &cf, &n_blocks};
+    const JitTuning tune = jit_tuning();
+    unsigned grid = n_blocks;
+    if (tune.blocks_per_cu > 0) {
+        grid = static_cast<unsigned>(std::min<uint64_t>(n_blocks, uint64_t {device_cu_count()} * tune.blocks_per_cu));
+    }
     const unsigned threads = 1u << tp.n_thr_bits;
-    KQ_HIP_CHECK(hipModuleLaunchKernel(fn, blocks, 1, 1, threads, 1, 1, 0, s->stream, args, nullptr));
+    KQ_HIP_CHECK(hipModuleLaunchKernel(fn, grid, 1, 1, threads, 1, 1, 0, s->stream, args, nullptr));
 }
 
 }  // namespace kq

@@ -118,6 +118,14 @@ size_t& device_lds_per_block()
     return bytes;
 }
 
+uint32_t& device_cus()
+{
+    static uint32_t n = 256;
+    return n;
+}
+
+uint32_t device_cu_count() { return device_cus(); }
+
 double fusion_max_cost() { return g_max_cost; }
 
 void set_fusion(int max_k, int max_gates, int max_stages, double max_cost)

@@ -156,3 +156,22 @@ def test_jit_mode_one_leaves_small_states_interpreted():
     run_sv(random_gate_circuit(10, 50, np.random.default_rng(5)))
     after = ka.jit_info()
     assert after["compiled"] == before["compiled"] and after["cache_hits"] == before["cache_hits"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tuning", [(0, 1), (0, 2), (4, 0)], ids=["persistent1", "persistent2", "waves4"])
+def test_jit_tunings_match_the_oracle(tuning):
+    """Persistent blocks walking several tiles each (21 qubits: 512 tiles of 2^12 amplitudes), and a
+    register cap, against the oracle."""
+    n = 21
+    c = random_gate_circuit(n, 120, np.random.default_rng(8000 + sum(tuning)))
+    ka.set_jit(2)
+    ka.set_jit_tuning(*tuning)
+    try:
+        before = ka.jit_info()
+        out = run_sv(c)
+        assert ka.jit_info()["failures"] == before["failures"]
+    finally:
+        ka.set_jit_tuning(0, 0)
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(out - ref)) <= TOL128
