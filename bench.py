@@ -190,8 +190,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--jit", type=int, default=1,
                     help="run-time compiled tile passes: 0 off, 1 states of >= 18 qubits, 2 all")
-    ap.add_argument("--jit-tuning", default="0,0",
-                    help="min waves per SIMD, persistent blocks per CU of the run-time compiled passes")
+    ap.add_argument("--jit-tuning", default="0,0,1",
+                    help="min waves per SIMD, persistent blocks per CU, pivot normalisation (0/1) of the "
+                         "run-time compiled passes")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
@@ -235,8 +236,8 @@ def main() -> int:
         ka.apply_gates(sv, ops)
 
     ka.set_jit(args.jit)
-    waves, per_cu = (int(x) for x in args.jit_tuning.split(","))
-    ka.set_jit_tuning(waves, per_cu)
+    waves, per_cu, normalise = (int(x) for x in args.jit_tuning.split(","))
+    ka.set_jit_tuning(waves, per_cu, bool(normalise))
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")

@@ -144,8 +144,10 @@ int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages
 int kq_set_jit(int mode);
 /* Code shape of the run-time pass kernels: min_waves_per_simd > 0 caps their registers so that at
  * least that many waves fit per SIMD; blocks_per_cu > 0 launches that many blocks per CU, each walking
- * several tiles and loading the next tile while the current one computes (0: one block per tile). */
-int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu);
+ * several tiles and loading the next tile while the current one computes (0: one block per tile);
+ * normalise != 0 (default) divides each uncontrolled one-qubit gate by its larger first-row entry
+ * and multiplies the product of those pivots back in once per pass. */
+int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise);
 int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds);
 /* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
 int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);

@@ -1422,9 +1422,9 @@ int kq_set_jit(int mode)
     return guarded([&] { set_jit_mode(mode); });
 }
 
-int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu)
+int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise)
 {
-    return guarded([&] { set_jit_tuning(JitTuning {min_waves_per_simd, blocks_per_cu}); });
+    return guarded([&] { set_jit_tuning(JitTuning {min_waves_per_simd, blocks_per_cu, normalise != 0}); });
 }
 
 int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds)

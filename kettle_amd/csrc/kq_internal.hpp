@@ -393,6 +393,7 @@ struct JitTuning
 {
     int min_waves_per_simd = 0;
     int blocks_per_cu = 0;
+    bool normalise = true;   // uncontrolled one-qubit gates with their pivot divided out
 };
 void set_jit_tuning(const JitTuning& t);
 JitTuning jit_tuning();

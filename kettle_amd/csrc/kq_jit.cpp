@@ -11,7 +11,10 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
+#include <cmath>
+#include <complex>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -124,6 +127,100 @@ std::string sel_expr(uint32_t lane_sel, uint64_t glb_sel)
     return "((base & " + hex64(glb_sel) + ") != 0ull)";
 }
 
+// the 8 coefficients of a record as doubles (complex64 records hold 8 packed floats in m[0..3])
+std::array<double, 8> record_values(const TileGate& g, bool c64)
+{
+    std::array<double, 8> m {};
+    if (!c64) {
+        std::copy(g.m, g.m + 8, m.begin());
+        return m;
+    }
+    float f[8];
+    std::memcpy(f, g.m, sizeof f);
+    std::copy(f, f + 8, m.begin());
+    return m;
+}
+
+// Emits an uncontrolled one-qubit gate with its pivot entry divided out (orth / axis / dense1 / a
+// d0-normalised diagonal) and multiplies the pivot into the pass factor; false if the record has
+// another shape.  Which entry is the pivot (the larger in magnitude, for accuracy) is part of the
+// generated source; the normalised values are data.
+bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::vector<double>& coefs,
+                     std::complex<double>& factor, bool& factor_complex)
+{
+    if (g.code >= TV_UNIFORM_BASE || g.lane_ctrl != 0 || g.glb_ctrl != 0 || g.code % 5 != 0) {
+        return false;
+    }
+    const int kind = static_cast<int>(g.code / 20);
+    const int P = static_cast<int>((g.code / 5) % 4);
+    const auto m = record_values(g, c64);
+    const auto val = [&](double x) {
+        coefs.push_back(x);
+        return "static_cast<R>(cf[" + std::to_string(coefs.size() - 1) + "])";
+    };
+    if (kind == TK_REAL) {
+        const bool rotation = m[6] == m[0] && m[4] == -m[2];
+        const bool reflection = m[6] == -m[0] && m[4] == m[2];
+        if (!rotation && !reflection) {
+            return false;
+        }
+        const bool diag_pivot = std::abs(m[0]) >= std::abs(m[2]);
+        const double pivot = diag_pivot ? m[0] : m[2];
+        if (pivot == 0.0) {
+            return false;
+        }
+        const int form = (diag_pivot ? 0 : 2) + (rotation ? 0 : 1);
+        o << "  orth<R, " << P << ", " << form << ">(v, " << val((diag_pivot ? m[2] : m[0]) / pivot) << ");\n";
+        factor *= pivot;
+        return true;
+    }
+    if (kind == TK_AXIS) {
+        if (m[6] != m[0] || m[5] != m[3]) {
+            return false;
+        }
+        const bool diag_pivot = std::abs(m[0]) >= std::abs(m[3]);
+        const double pivot = diag_pivot ? m[0] : m[3];
+        if (pivot == 0.0) {
+            return false;
+        }
+        o << "  axis<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << val((diag_pivot ? m[3] : m[0]) / pivot)
+          << ");\n";
+        factor *= pivot;
+        return true;
+    }
+    if (kind == TK_DIAG) {
+        const std::complex<double> d0 {m[0], m[1]};
+        if (std::abs(d0) < 0.5) {
+            return false;
+        }
+        const std::complex<double> d = std::complex<double> {m[6], m[7]} / d0;
+        const auto dr = val(d.real());
+        const auto di = val(d.imag());
+        o << "  { const Coef<R> m {{0, 0, 0, 0, 0, 0, " << dr << ", " << di << "}}; apply<R, K_DIAG1, " << P
+          << ", 0>(v, m); }\n";
+        factor *= d0;
+        factor_complex = true;
+        return true;
+    }
+    if (kind == TK_DENSE) {
+        const std::complex<double> m00 {m[0], m[1]}, m01 {m[2], m[3]}, m10 {m[4], m[5]}, m11 {m[6], m[7]};
+        const bool diag_pivot = std::abs(m00) >= std::abs(m01);
+        const std::complex<double> pivot = diag_pivot ? m00 : m01;
+        if (std::abs(pivot) == 0.0) {
+            return false;
+        }
+        const std::complex<double> x = (diag_pivot ? m01 : m00) / pivot, y = m10 / pivot, z = m11 / pivot;
+        const auto xr = val(x.real()), xi = val(x.imag()), yr = val(y.real()), yi = val(y.imag());
+        const auto zr = val(z.real()), zi = val(z.imag());
+        o << "  dense1<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << xr << ", " << xi << ", " << yr << ", "
+          << yi << ", " << zr << ", " << zi << ");\n";
+        factor *= pivot;
+        factor_complex = true;
+        return true;
+    }
+    return false;
+}
+
 }  // namespace
 
 JitStats& jit_stats() { return cache().stats; }
@@ -202,22 +299,58 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     else {
         o << "  {\n    const uint64_t blk = blockIdx.x;\n    const uint64_t base = " << deposit_expr(tp.freepos) << ";\n";
     }
+    // Controlled X gates whose control is a thread or block bit (a register swap under a lane- or
+    // block-dependent condition) are not executed in registers: each stays pending until the next
+    // LDS or HBM write, where it becomes an XOR of the write address, unless a gate on its register
+    // bit comes first.  Register slot r of layout L sits at slot address L.*_slot[r], which is
+    // XOR-linear in r, so swapping slots r and r ^ (1 << P) is the address XOR L.*_slot[1 << P].
+    struct PendingX
+    {
+        int P;
+        std::string cond;
+    };
+    std::vector<PendingX> pending;
+    const auto linear = [](const auto& slots) {
+        for (int r = 0; r < 16; ++r) {
+            for (int q = 0; q < 16; ++q) {
+                if (slots[r ^ q] != (slots[r] ^ slots[q])) {
+                    return false;
+                }
+            }
+        }
+        return true;
+    };
+    const auto pending_xor = [&](const auto& slots, const char* zero) {
+        std::string e = zero;
+        for (const PendingX& x : pending) {
+            e += " ^ ((" + x.cond + ") ? " + (std::string {zero} == "0ull" ? hex64(slots[1 << x.P]) : hex32(static_cast<uint32_t>(slots[1 << x.P])))
+               + " : " + zero + ")";
+        }
+        return e;
+    };
     const auto io = [&](const TileLayout& L, bool load) {
-        o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
+        o << "  {\n    const uint64_t off = (base | (" << hbm_thread_expr(L, tp.n_thr_bits) << "))";
+        if (!load) {
+            o << " ^ (" << pending_xor(L.hbm_slot, "0ull") << ")";
+            pending.clear();
+        }
+        o << ";\n";
         for (int r = 0; r < 16; ++r) {
             if (load) {
-                o << "    { const Amp<R> x = a[off + " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n";
+                o << "    { const Amp<R> x = a[off ^ " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n";
             }
             else {
-                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off + " << hex64(L.hbm_slot[r])
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off ^ " << hex64(L.hbm_slot[r])
                   << "] = x; }\n";
             }
         }
         o << "  }\n";
     };
     const auto remap = [&](const TileLayout& A, const TileLayout& B) {
-        o << "  {\n    const uint32_t xa = " << lds_thread_expr(A, tp.n_thr_bits) << ";\n"
+        o << "  {\n    const uint32_t xa = (" << lds_thread_expr(A, tp.n_thr_bits) << ") ^ ("
+          << pending_xor(A.lds_slot, "0u") << ");\n"
           << "    const uint32_t xb = " << lds_thread_expr(B, tp.n_thr_bits) << ";\n    __syncthreads();\n";
+        pending.clear();
         for (int r = 0; r < 16; ++r) {
             o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; *reinterpret_cast<Amp<R>*>(lds + (xa ^ "
               << hex32(A.lds_slot[r]) << ")) = x; }\n";
@@ -255,6 +388,56 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     if (!persistent) {
         io(tp.layouts[0], true);
     }
+    // pivots divided out of normalised gates, multiplied back in once before the store
+    const bool normalise = tune.normalise;
+    std::complex<double> factor {1.0, 0.0};
+    bool factor_complex = false;
+    bool normalised = false;
+    bool all_linear = true;
+    for (const TileLayout& L : tp.layouts) {
+        all_linear = all_linear && linear(L.lds_slot) && linear(L.hbm_slot);
+    }
+    // register bits a record reads or writes (target and register control)
+    const auto touched = [](const TileGate& g) {
+        uint32_t bits = 0;
+        uint32_t P = 4, C = 0;
+        if (g.code >= TV_MERGE_BASE) {
+            const uint32_t key = g.code - TV_MERGE_BASE;
+            if (key >= 20) {
+                C = key - 20;
+            }
+            else {
+                P = key / 5;
+                C = key % 5;
+            }
+        }
+        else if (g.code >= TV_UNIFORM_BASE) {
+            C = g.code - TV_UNIFORM_BASE;
+        }
+        else {
+            P = (g.code / 5) % 4;
+            C = g.code % 5;
+        }
+        if (P < 4) {
+            bits |= 1u << P;
+        }
+        if (C > 0) {
+            bits |= 1u << (C - 1);
+        }
+        return bits;
+    };
+    const auto materialise = [&](uint32_t bits) {
+        for (size_t i = 0; i < pending.size();) {
+            if ((bits >> pending[i].P) & 1u) {
+                o << "  if (" << pending[i].cond << ") {\n    const Coef<R> m {};\n    apply<R, K_SWAP, " << pending[i].P
+                  << ", 0>(v, m);\n  }\n";
+                pending.erase(pending.begin() + static_cast<std::ptrdiff_t>(i));
+            }
+            else {
+                ++i;
+            }
+        }
+    };
     uint32_t cur = 0;
     for (const TileStep& st : tp.steps) {
         if (st.layout != cur) {
@@ -263,6 +446,12 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
         for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
             const TileGate& g = tp.gates[gi];
+            materialise(touched(g));
+            if (all_linear && g.code < TV_UNIFORM_BASE && g.code / 20 == TK_SWAP && g.code % 5 == 0
+                && (g.lane_ctrl != 0 || g.glb_ctrl != 0)) {
+                pending.push_back(PendingX {static_cast<int>((g.code / 5) % 4), cond_expr(g.glb_ctrl, g.lane_ctrl)});
+                continue;
+            }
             if (g.code >= TV_MERGE_BASE) {
                 const uint32_t key = g.code - TV_MERGE_BASE;
                 const bool uniform = key >= 20;
@@ -289,6 +478,10 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
                 }
                 continue;
             }
+            if (normalise && emit_normalised(o, g, c64, coefs, factor, factor_complex)) {
+                normalised = true;
+                continue;
+            }
             const size_t off = push_gate_coefs(g);
             o << "  if (" << cond_expr(g.glb_ctrl, g.lane_ctrl) << ") {\n    const Coef<R> m = coef_at<R>(cf + " << off
               << ");\n";
@@ -299,6 +492,18 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
             else {
                 o << "    apply<R, " << g.code / 20 << ", " << (g.code / 5) % 4 << ", " << g.code % 5 << ">(v, m);\n  }\n";
             }
+        }
+    }
+    if (normalised) {
+        if (factor_complex) {
+            coefs.push_back(factor.real());
+            coefs.push_back(factor.imag());
+            o << "  scale<R, 0>(v, static_cast<R>(cf[" << coefs.size() - 2 << "]), static_cast<R>(cf[" << coefs.size() - 1
+              << "]));\n";
+        }
+        else {
+            coefs.push_back(factor.real());
+            o << "  scale_real<R>(v, static_cast<R>(cf[" << coefs.size() - 1 << "]));\n";
         }
     }
     if (tp.out_layout != cur) {

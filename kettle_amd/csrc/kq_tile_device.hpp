@@ -107,6 +107,87 @@ __device__ __forceinline__ void apply(Cx<R> (&v)[16], const Coef<R>& m)
     }
 }
 
+// Uncontrolled one-qubit gates with a uniform factor taken out (the pass multiplies the factor back in
+// once, before its store), so that one entry of each row or column is 1 and its products disappear.
+// Pairs (v[r], v[r | 1 << P]) over the slots r with bit P clear.
+//   FORM 0: [[1, t], [-t, 1]]   FORM 1: [[1, t], [t, -1]]     (real rotation / reflection, pivot m00)
+//   FORM 2: [[t, 1], [-1, t]]   FORM 3: [[t, 1], [1, -t]]     (pivot m01)
+template <typename R, int P, int FORM>
+__device__ __forceinline__ void orth(Cx<R> (&v)[16], R t)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r];
+            const Cx<R> b = v[r1];
+            if constexpr (FORM == 0) {
+                v[r] = {a.re + t * b.re, a.im + t * b.im};
+                v[r1] = {b.re - t * a.re, b.im - t * a.im};
+            }
+            else if constexpr (FORM == 1) {
+                v[r] = {a.re + t * b.re, a.im + t * b.im};
+                v[r1] = {t * a.re - b.re, t * a.im - b.im};
+            }
+            else if constexpr (FORM == 2) {
+                v[r] = {t * a.re + b.re, t * a.im + b.im};
+                v[r1] = {t * b.re - a.re, t * b.im - a.im};
+            }
+            else {
+                v[r] = {t * a.re + b.re, t * a.im + b.im};
+                v[r1] = {a.re - t * b.re, a.im - t * b.im};
+            }
+        }
+    });
+}
+
+//   FORM 0: [[1, i t], [i t, 1]]   FORM 1: [[t, i], [i, t]]   (axis rotation, pivot m00 / m01)
+template <typename R, int P, int FORM>
+__device__ __forceinline__ void axis(Cx<R> (&v)[16], R t)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r];
+            const Cx<R> b = v[r1];
+            if constexpr (FORM == 0) {
+                v[r] = {a.re - t * b.im, a.im + t * b.re};
+                v[r1] = {b.re - t * a.im, b.im + t * a.re};
+            }
+            else {
+                v[r] = {t * a.re - b.im, t * a.im + b.re};
+                v[r1] = {t * b.re - a.im, t * b.im + a.re};
+            }
+        }
+    });
+}
+
+//   FORM 0: [[1, x], [y, z]]   FORM 1: [[x, 1], [y, z]]   (complex, pivot m00 / m01)
+template <typename R, int P, int FORM>
+__device__ __forceinline__ void dense1(Cx<R> (&v)[16], R xr, R xi, R yr, R yi, R zr, R zi)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r];
+            const Cx<R> b = v[r1];
+            if constexpr (FORM == 0) {
+                v[r] = {a.re + xr * b.re - xi * b.im, a.im + xr * b.im + xi * b.re};
+            }
+            else {
+                v[r] = {xr * a.re - xi * a.im + b.re, xr * a.im + xi * a.re + b.im};
+            }
+            v[r1] = {yr * a.re - yi * a.im + zr * b.re - zi * b.im, yr * a.im + yi * a.re + zr * b.im + zi * b.re};
+        }
+    });
+}
+
+// every slot times a real factor
+template <typename R>
+__device__ __forceinline__ void scale_real(Cx<R> (&v)[16], R f)
+{
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) { v[r] = {v[r].re * f, v[r].im * f}; });
+}
+
 // Uniform diagonal: every slot (C = 0) or the slots with register bit C - 1 set, times (fr, fi).
 template <typename R, int C>
 __device__ __forceinline__ void scale(Cx<R> (&v)[16], R fr, R fi)

@@ -159,12 +159,13 @@ def test_jit_mode_one_leaves_small_states_interpreted():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tuning", [(0, 1), (0, 2), (4, 0)], ids=["persistent1", "persistent2", "waves4"])
+@pytest.mark.parametrize("tuning", [(0, 1, True), (0, 2, False), (4, 0, True)],
+                         ids=["persistent1", "persistent2-plain", "waves4"])
 def test_jit_tunings_match_the_oracle(tuning):
     """Persistent blocks walking several tiles each (21 qubits: 512 tiles of 2^12 amplitudes), and a
     register cap, against the oracle."""
     n = 21
-    c = random_gate_circuit(n, 120, np.random.default_rng(8000 + sum(tuning)))
+    c = random_gate_circuit(n, 120, np.random.default_rng(8000 + tuning[0] + tuning[1]))
     ka.set_jit(2)
     ka.set_jit_tuning(*tuning)
     try:
@@ -172,6 +173,54 @@ def test_jit_tunings_match_the_oracle(tuning):
         out = run_sv(c)
         assert ka.jit_info()["failures"] == before["failures"]
     finally:
-        ka.set_jit_tuning(0, 0)
+        ka.set_jit_tuning()
     ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
     assert np.max(np.abs(out - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("normalise", [True, False], ids=["normalised", "plain"])
+def test_jit_layered_workload_20q(normalise):
+    """QFT + random one-qubit layers with CX brick ladders at 20 qubits: controlled X gates with
+    controls on thread and block bits (deferred to the next write as address XORs) and every
+    normalised one-qubit form, against the oracle."""
+    from kettle_amd import workloads
+    n = 20
+    c = workloads.qft_plus_layers(n=n, layers=6)
+    ka.set_jit(2)
+    ka.set_jit_tuning(0, 0, normalise)
+    try:
+        out = run_sv(c)
+    finally:
+        ka.set_jit_tuning()
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(out - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+def test_jit_normalised_forms_at_extreme_angles():
+    """Angles where a pivot entry is exactly 0 or where |cos| = |sin| (pivot choice ties), and U
+    matrices with a zero diagonal, against the oracle (complex128 and complex64)."""
+    from kettle_amd.circuit import Matrix2X2
+    n = 13
+    c = ka.QuantumCircuit(n)
+    for q in range(n):
+        c.add_h_gate(q)
+    angles = [0.0, math.pi, -math.pi, math.pi / 2, -math.pi / 2, 3 * math.pi / 2, 2 * math.pi, 1e-300, 0.3]
+    for i, theta in enumerate(angles * 3):
+        q = i % n
+        c.add_rx_gate(q, theta)
+        c.add_ry_gate((q + 1) % n, theta)
+        c.add_rz_gate((q + 2) % n, theta)
+        c.add_p_gate((q + 3) % n, theta)
+        c.add_cx_gate((q + 5) % n, (q + 4) % n)
+    c.add_u_gate(Matrix2X2(0, 1j, 1j, 0), 3)
+    c.add_u_gate(Matrix2X2(0, -1, 1, 0), 7)
+    c.add_u_gate(Matrix2X2(1, 0, 0, -1), 9)
+    for q in range(n):
+        c.add_sx_gate(q)
+        c.add_sxdag_gate((q + 6) % n)
+    ka.set_jit(2)
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(run_sv(c) - ref)) <= TOL128
+    assert np.max(np.abs(run_sv(c, ka.KQ_C64) - ref)) <= TOL64
