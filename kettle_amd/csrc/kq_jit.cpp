@@ -221,6 +221,81 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
     return false;
 }
 
+// the two diagonal entries of a merged-run term (complex64 terms hold packed floats in d[0..1])
+std::array<std::complex<double>, 2> term_values(const TileTerm& t, bool c64)
+{
+    if (!c64) {
+        return {std::complex<double> {t.d[0], t.d[1]}, std::complex<double> {t.d[2], t.d[3]}};
+    }
+    float f[4];
+    std::memcpy(f, t.d, sizeof f);
+    return {std::complex<double> {f[0], f[1]}, std::complex<double> {f[2], f[3]}};
+}
+
+// A merged diagonal run whose terms are all pure phases (|d0| = |d1| = 1, as for CP, RZ, T, S and
+// P) as a sum of FP64 angles selected by each term's condition and one sincos per factor, in place
+// of a complex product per term.  Entries exactly 1 are skipped (structure, like the conditions).
+bool emit_phase_run(std::ostringstream& o, const TileGate& g, const TileProgram& tp, bool c64,
+                    std::vector<double>& coefs)
+{
+    // one FP64 sincos costs about as much as the complex products of eight terms
+    constexpr uint64_t MIN_TERMS = 8;
+    if (g.glb_sel < MIN_TERMS) {
+        return false;
+    }
+    const uint32_t key = g.code - TV_MERGE_BASE;
+    const bool uniform = key >= 20;
+    bool any0 = false;
+    for (uint64_t j = 0; j < g.glb_sel; ++j) {
+        const auto d = term_values(tp.terms[g.lane_sel + j], c64);
+        if (std::abs(std::abs(d[0]) - 1.0) > 1.0e-13 || std::abs(std::abs(d[1]) - 1.0) > 1.0e-13) {
+            return false;
+        }
+        any0 = any0 || d[0] != std::complex<double> {1.0, 0.0};
+    }
+    const bool need0 = any0 && !uniform;
+    o << "  {\n    double a0 = 0.0, a1 = 0.0;\n";
+    for (uint64_t j = 0; j < g.glb_sel; ++j) {
+        const TileTerm& t = tp.terms[g.lane_sel + j];
+        const auto d = term_values(t, c64);
+        const std::string ok = cond_expr(t.glb_ctrl, t.lane_ctrl);
+        const auto angle = [&](std::complex<double> z) {
+            coefs.push_back(std::arg(z));
+            return "cf[" + std::to_string(coefs.size() - 1) + "]";
+        };
+        if (uniform) {
+            const std::string sel = sel_expr(t.lane_sel, t.glb_sel);
+            const bool one0 = d[0] == std::complex<double> {1.0, 0.0};
+            const std::string p1 = angle(d[1]);
+            const std::string p0 = one0 ? std::string {"0.0"} : angle(d[0]);
+            o << "    a1 += (" << ok << ") ? (" << sel << " ? " << p1 << " : " << p0 << ") : 0.0;\n";
+        }
+        else {
+            o << "    if (" << ok << ") {";
+            if (need0 && d[0] != std::complex<double> {1.0, 0.0}) {
+                o << " a0 += " << angle(d[0]) << ";";
+            }
+            o << " a1 += " << angle(d[1]) << "; }\n";
+        }
+    }
+    o << "    double s1, c1;\n    sincos(a1, &s1, &c1);\n";
+    if (uniform) {
+        o << "    scale<R, " << (key - 20) << ">(v, static_cast<R>(c1), static_cast<R>(s1));\n  }\n";
+        return true;
+    }
+    if (need0) {
+        o << "    double s0, c0;\n    sincos(a0, &s0, &c0);\n"
+          << "    const Coef<R> m {{static_cast<R>(c0), static_cast<R>(s0), 0, 0, 0, 0, static_cast<R>(c1), "
+             "static_cast<R>(s1)}};\n    apply<R, K_DIAG, "
+          << key / 5 << ", " << key % 5 << ">(v, m);\n  }\n";
+    }
+    else {
+        o << "    const Coef<R> m {{1, 0, 0, 0, 0, 0, static_cast<R>(c1), static_cast<R>(s1)}};\n    apply<R, K_DIAG1, "
+          << key / 5 << ", " << key % 5 << ">(v, m);\n  }\n";
+    }
+    return true;
+}
+
 }  // namespace
 
 JitStats& jit_stats() { return cache().stats; }
@@ -349,13 +424,13 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     const auto remap = [&](const TileLayout& A, const TileLayout& B) {
         o << "  {\n    const uint32_t xa = (" << lds_thread_expr(A, tp.n_thr_bits) << ") ^ ("
           << pending_xor(A.lds_slot, "0u") << ");\n"
-          << "    const uint32_t xb = " << lds_thread_expr(B, tp.n_thr_bits) << ";\n    __syncthreads();\n";
+          << "    const uint32_t xb = " << lds_thread_expr(B, tp.n_thr_bits) << ";\n    lds_barrier();\n";
         pending.clear();
         for (int r = 0; r < 16; ++r) {
             o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; *reinterpret_cast<Amp<R>*>(lds + (xa ^ "
               << hex32(A.lds_slot[r]) << ")) = x; }\n";
         }
-        o << "    __syncthreads();\n";
+        o << "    lds_barrier();\n";
         for (int r = 0; r < 16; ++r) {
             o << "    { const Amp<R> x = *reinterpret_cast<const Amp<R>*>(lds + (xb ^ " << hex32(B.lds_slot[r])
               << ")); v[" << r << "] = {x.x, x.y}; }\n";
@@ -450,6 +525,9 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
             if (all_linear && g.code < TV_UNIFORM_BASE && g.code / 20 == TK_SWAP && g.code % 5 == 0
                 && (g.lane_ctrl != 0 || g.glb_ctrl != 0)) {
                 pending.push_back(PendingX {static_cast<int>((g.code / 5) % 4), cond_expr(g.glb_ctrl, g.lane_ctrl)});
+                continue;
+            }
+            if (g.code >= TV_MERGE_BASE && emit_phase_run(o, g, tp, c64, coefs)) {
                 continue;
             }
             if (g.code >= TV_MERGE_BASE) {

@@ -36,6 +36,15 @@ __device__ __forceinline__ Cx<R> cmul(Cx<R> a, R br, R bi)
     return {a.re * br - a.im * bi, a.re * bi + a.im * br};
 }
 
+// Workgroup barrier ordering LDS accesses only: global loads still in flight (the next tile's
+// amplitudes) are not waited for, which a full __syncthreads() fence would do.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // the eight coefficients of a gate record, already in registers
 template <typename R>
 struct Coef
