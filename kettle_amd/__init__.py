@@ -14,6 +14,7 @@ from .api import (DensityMatrix, EnergyGradient, FusionPlan, PackedPauliOperator
                   perform_measurements_as_counts_marginal, perform_measurements_as_counts_raw,
                   perform_measurements_as_memory, plan_stats, set_fusion, simulate, simulate_pauli_string,
                   state_index_to_bitstring, statevector_to_density_matrix)
-from .io import read_pauli_operator, read_tangelo_circuit, write_tangelo_circuit
+from .io import (Endian, load_statevector, read_numpy_statevector, read_pauli_operator, read_tangelo_circuit,
+                 save_statevector, write_tangelo_circuit)
 
 __all__ = [name for name in dir() if not name.startswith("_")]

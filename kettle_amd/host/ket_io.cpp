@@ -1,6 +1,7 @@
 // Text formats at the boundary of the simulator: the Hamiltonian reader (io/read_pauli_operator.cpp:29-69
 // in the reference), which feeds expectation_value() and energy_and_gradient(), and the tangelo-style
-// circuit reader and writer (io/read_tangelo_file.cpp, io/write_tangelo_file.cpp, io/io_control_flow.cpp).
+// circuit reader and writer (io/read_tangelo_file.cpp, io/write_tangelo_file.cpp, io/io_control_flow.cpp),
+// and statevector snapshots (io/statevector.cpp, io/numpy_statevector.cpp).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -9,9 +10,12 @@
 #include <stdexcept>
 #include <string>
 
+#include "kettle/io/numpy_statevector.hpp"
 #include "kettle/io/read_pauli_operator.hpp"
 #include "kettle/io/read_tangelo_file.hpp"
+#include "kettle/io/statevector.hpp"
 #include "kettle/io/write_tangelo_file.hpp"
+#include "kettle/state/qubit_state_conversion.hpp"
 
 namespace ket
 {
@@ -447,5 +451,113 @@ void write_tangelo_circuit(const QuantumCircuit& circuit, const std::filesystem:
 }
 
 void print_tangelo_circuit(const QuantumCircuit& circuit) { write_tangelo_circuit(circuit, std::cout); }
+
+// ---- statevector snapshots ---------------------------------------------------------------------------
+
+namespace
+{
+
+// "%.14f" with a space in front of non-negative values (statevector.cpp:45-66)
+auto snapshot_number(double x) -> std::string
+{
+    char buf[400];
+    std::snprintf(buf, sizeof buf, "%.14f", x);
+    return (x >= 0.0 ? " " : "") + std::string {buf};
+}
+
+auto endian_name(Endian e) -> const char* { return e == Endian::BIG ? "BIG" : "LITTLE"; }
+
+auto endian_of(const std::string& name) -> Endian
+{
+    if (name == "BIG") {
+        return Endian::BIG;
+    }
+    if (name == "LITTLE") {
+        return Endian::LITTLE;
+    }
+    throw std::runtime_error {"UNREACHABLE: dev error in `string_to_endian()`"};
+}
+
+}  // namespace
+
+void save_statevector(std::ostream& outstream, const Statevector& state, Endian endian)
+{
+    outstream << "ENDIANNESS: " << endian_name(endian) << '\n';
+    outstream << "NUMBER OF STATES: " << state.n_states() << '\n';
+    const std::size_t n = state.n_qubits();
+    std::string line;
+    for (std::size_t i = 0; i < state.n_states(); ++i) {
+        const auto& z = state[endian == Endian::LITTLE ? i : endian_flip(i, n)];
+        line = snapshot_number(z.real());
+        line += "   ";
+        line += snapshot_number(z.imag());
+        line += '\n';
+        outstream << line;
+    }
+}
+
+void save_statevector(const std::filesystem::path& filepath, const Statevector& state, Endian endian)
+{
+    auto out = std::ofstream {filepath};
+    if (!out.is_open()) {
+        throw std::ios::failure {"ERROR: unable to open file to save statevector: \n'" + filepath.string() + "'\n"};
+    }
+    save_statevector(out, state, endian);
+}
+
+auto load_statevector(std::istream& instream) -> Statevector
+{
+    std::string word;
+    std::string endian;
+    instream >> word >> endian;
+    const Endian e = endian_of(endian);
+    std::size_t n_states = 0;
+    instream >> word >> word >> word >> n_states;
+    std::vector<std::complex<double>> amps;
+    amps.reserve(n_states);
+    for (std::size_t i = 0; i < n_states; ++i) {
+        double re = 0.0;
+        double im = 0.0;
+        instream >> re >> im;
+        amps.emplace_back(re, im);
+    }
+    return Statevector {std::move(amps), e};
+}
+
+auto load_statevector(const std::filesystem::path& filepath) -> Statevector
+{
+    auto in = std::ifstream {filepath};
+    if (!in.is_open()) {
+        throw std::ios::failure {"ERROR: unable to open file to load statevector: \n'" + filepath.string() + "'\n"};
+    }
+    return load_statevector(in);
+}
+
+auto read_numpy_statevector(std::istream& instream, Endian input_endian) -> Statevector
+{
+    std::size_t n_qubits = 0;
+    instream >> n_qubits;
+    const std::size_t n_states = std::size_t {1} << n_qubits;
+    std::vector<std::complex<double>> amps;
+    amps.reserve(n_states);
+    for (std::size_t i = 0; i < n_states; ++i) {
+        // "(re+imj)": the sign of the imaginary part starts the second number
+        char ch = 0;
+        double re = 0.0;
+        double im = 0.0;
+        instream >> ch >> re >> im >> ch >> ch;
+        amps.emplace_back(re, im);
+    }
+    return Statevector {std::move(amps), input_endian};
+}
+
+auto read_numpy_statevector(const std::filesystem::path& filepath, Endian input_endian) -> Statevector
+{
+    auto in = std::ifstream {filepath};
+    if (!in.is_open()) {
+        throw std::ios::failure {"ERROR: unable to open file for statevector: \n'" + filepath.string() + "'\n"};
+    }
+    return read_numpy_statevector(in, input_endian);
+}
 
 }  // namespace ket

@@ -7,13 +7,19 @@
   ``write_tangelo_circuit`` (io/read_tangelo_file.cpp:241-376, io/write_tangelo_file.cpp:24-213,
   io/io_control_flow.cpp:13-139): one gate per line (``RX        target : [5]   parameter : 1.57``),
   U / CU followed by two matrix lines, ``IF BITS[..] == [..]`` / ``ELSE`` blocks indented by 4 spaces.
+* kettle's statevector snapshots ``save_statevector`` / ``load_statevector``
+  (io/statevector.cpp:74-169) and the numpy text reader ``read_numpy_statevector``
+  (io/numpy_statevector.cpp:35-76).
 """
 from __future__ import annotations
 
+import enum
 import io
 import os
 import re
 from typing import TextIO
+
+import numpy as np
 
 from .api import PauliOperator, PauliTerm, SparsePauliString
 from .circuit import (CONTROLLED, CONTROLLED_ANGLE, ONE_TARGET, ONE_TARGET_ANGLE, ControlFlowBooleanKind,
@@ -235,3 +241,122 @@ def write_tangelo_circuit(circuit: QuantumCircuit, dest: str | os.PathLike | Tex
     elif dest is not None:
         dest.write(text)
     return text
+
+
+# ---- statevector snapshots -------------------------------------------------------------------------------
+
+class Endian(enum.Enum):
+    """kettle's ``Endian`` (include/kettle/state/endian.hpp)."""
+
+    LITTLE = "LITTLE"
+    BIG = "BIG"
+
+
+def bit_reversal(n_qubits: int) -> np.ndarray:
+    """Index i -> i with its n_qubits low bits reversed (kettle's ``endian_flip``)."""
+    idx = np.arange(1 << n_qubits, dtype=np.int64)
+    out = np.zeros_like(idx)
+    for b in range(n_qubits):
+        out |= ((idx >> b) & 1) << (n_qubits - 1 - b)
+    return out
+
+
+def _qubits_of(n_states: int) -> int:
+    n = int(n_states).bit_length() - 1
+    if n_states < 2 or (1 << n) != n_states:
+        raise RuntimeError("The provided coefficients must have a size equal to a power of 2.\n")
+    return n
+
+
+def _snapshot_number(x: float) -> str:
+    return (" " if x >= 0.0 else "") + f"{x:.14f}"      # statevector.cpp:45-66
+
+
+def statevector_text(amps: np.ndarray, endian: Endian = Endian.LITTLE) -> str:
+    """The text ``save_statevector`` writes for these (little-endian indexed) amplitudes."""
+    a = np.asarray(amps, dtype=np.complex128).reshape(-1)
+    n = _qubits_of(a.size)
+    if endian == Endian.BIG:
+        a = a[bit_reversal(n)]
+    lines = [f"ENDIANNESS: {endian.value}", f"NUMBER OF STATES: {a.size}"]
+    lines += [f"{_snapshot_number(z.real)}   {_snapshot_number(z.imag)}" for z in a.tolist()]
+    return "\n".join(lines) + "\n"
+
+
+def parse_statevector_text(text: str) -> np.ndarray:
+    """Little-endian indexed amplitudes of a ``save_statevector`` text."""
+    words = text.split()
+    if len(words) < 6:
+        raise RuntimeError("ERROR: statevector text is truncated\n")
+    try:
+        endian = Endian(words[1])
+    except ValueError:
+        raise RuntimeError("UNREACHABLE: dev error in `string_to_endian()`") from None
+    n_states = int(words[5])
+    vals = np.array(words[6:6 + 2 * n_states], dtype=np.float64)
+    if vals.size != 2 * n_states:
+        raise RuntimeError("ERROR: statevector text is truncated\n")
+    amps = vals[0::2] + 1j * vals[1::2]
+    if endian == Endian.BIG:
+        amps = amps[bit_reversal(_qubits_of(n_states))]
+    return amps
+
+
+_FLOAT = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+_NUMPY_COMPLEX = re.compile(r"\(\s*(" + _FLOAT + r")\s*(" + _FLOAT + r")j\s*\)")
+
+
+def parse_numpy_statevector(text: str, input_endian: Endian = Endian.LITTLE) -> np.ndarray:
+    """Little-endian indexed amplitudes of a numpy text dump: n on the first line, then 2^n lines
+    ``(re+imj)``."""
+    head, _, body = text.partition("\n")
+    n = int(head.split()[0])
+    pairs = _NUMPY_COMPLEX.findall(body)
+    if len(pairs) < (1 << n):
+        raise RuntimeError("ERROR: numpy statevector text is truncated\n")
+    amps = np.array([complex(float(r), float(i)) for r, i in pairs[:1 << n]], dtype=np.complex128)
+    if input_endian == Endian.BIG:
+        amps = amps[bit_reversal(n)]
+    return amps
+
+
+def _read_text(source, what: str) -> str:
+    if isinstance(source, (str, os.PathLike)):
+        try:
+            with open(source, encoding="utf-8") as f:
+                return f.read()
+        except FileNotFoundError as e:
+            raise OSError(f"ERROR: unable to open file {what}: \n'{os.fspath(source)}'\n") from e
+    return source.read()
+
+
+def save_statevector(dest: str | os.PathLike | TextIO, state, endian: Endian = Endian.LITTLE) -> None:
+    """``state``: a device Statevector (downloaded first) or an array of amplitudes."""
+    from .api import Statevector
+
+    amps = state.download() if isinstance(state, Statevector) else state
+    text = statevector_text(amps, endian)
+    if isinstance(dest, (str, os.PathLike)):
+        try:
+            with open(dest, "w", encoding="utf-8") as f:
+                f.write(text)
+        except OSError as e:
+            raise OSError(f"ERROR: unable to open file to save statevector: \n'{os.fspath(dest)}'\n") from e
+    else:
+        dest.write(text)
+
+
+def load_statevector(source: str | os.PathLike | TextIO, precision: int | None = None):
+    """A device Statevector holding the snapshot's amplitudes."""
+    from .api import KQ_C128, Statevector
+
+    amps = parse_statevector_text(_read_text(source, "to load statevector"))
+    return Statevector(amps, KQ_C128 if precision is None else precision)
+
+
+def read_numpy_statevector(source: str | os.PathLike | TextIO, input_endian: Endian = Endian.LITTLE,
+                           precision: int | None = None):
+    from .api import KQ_C128, Statevector
+
+    amps = parse_numpy_statevector(_read_text(source, "for statevector"), input_endian)
+    return Statevector(amps, KQ_C128 if precision is None else precision)

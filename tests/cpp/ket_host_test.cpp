@@ -1,7 +1,8 @@
 // Host-only tests of the kettle-compatible C++ API (no device needed): the Hamiltonian reader,
 // Pauli-string bookkeeping, circuit builders, n_local and flattening.  They mirror the reference's
 // test/source/io/read_pauli_operator_test.cpp, operator/pauli/sparse_pauli_string_test.cpp,
-// circuit/circuit_test.cpp, optimize/n_local_test.cpp and io/{read,write}_tangelo_file_test.cpp.
+// circuit/circuit_test.cpp, optimize/n_local_test.cpp, io/{read,write}_tangelo_file_test.cpp,
+// io/statevector_test.cpp and io/numpy_statevector_test.cpp.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -377,6 +378,77 @@ int main()
         const auto back = ket::read_tangelo_circuit(3, text, 0);
         // matrix elements are cut (not rounded) to 13 decimals, so the read-back matrix is within 1e-12
         REQUIRE(same_circuit(original, back, 1.0e-12));
+    });
+
+    // numpy_statevector_test.cpp:12-49
+    run("read_numpy_statevector known answer", [] {
+        auto in = std::stringstream {
+            "3                                                    \n"
+            " (4.950737714883371443e-02+4.950737714883371443e-02j)\n"
+            " (9.901475429766742886e-02+9.901475429766742886e-02j)\n"
+            " (1.485221314465011433e-01+1.485221314465011433e-01j)\n"
+            " (1.980295085953348577e-01+1.980295085953348577e-01j)\n"
+            " (2.475368857441685444e-01+2.475368857441685444e-01j)\n"
+            " (2.970442628930022866e-01+2.970442628930022866e-01j)\n"
+            " (3.465516400418360288e-01+3.465516400418360288e-01j)\n"
+            " (3.960590171906697154e-01+3.960590171906697154e-01j)\n"};
+        const auto actual = ket::read_numpy_statevector(in);
+        std::vector<std::complex<double>> want;
+        double norm_sq = 0.0;
+        for (int i = 1; i < 9; ++i) {
+            want.emplace_back(1.1 * i, 1.1 * i);
+            norm_sq += std::norm(want.back());
+        }
+        for (auto& z : want) {
+            z /= std::sqrt(norm_sq);
+        }
+        REQUIRE(actual.n_qubits() == 3);
+        for (std::size_t i = 0; i < 8; ++i) {
+            REQUIRE(std::abs(actual[i] - want[i]) < 1.0e-15);
+        }
+        auto big = std::stringstream {"1\n (6.0e-01+0.0e+00j)\n (0.0e+00-8.0e-01j)\n"};
+        const auto b = ket::read_numpy_statevector(big, ket::Endian::BIG);
+        REQUIRE(std::abs(b[1] - std::complex<double> {0.0, -0.8}) < 1.0e-15);
+    });
+
+    // statevector_test.cpp:11-33 (the state comes from amplitudes here; the device suite saves a simulated one)
+    run("save_statevector / load_statevector", [] {
+        std::vector<std::complex<double>> amps;
+        double norm_sq = 0.0;
+        for (int i = 0; i < 8; ++i) {
+            amps.emplace_back(0.1 * (i + 1), i % 3 == 0 ? -0.05 * i : 0.07 * i);
+            norm_sq += std::norm(amps.back());
+        }
+        for (auto& z : amps) {
+            z /= std::sqrt(norm_sq);
+        }
+        const auto state = ket::Statevector {amps};
+        auto out = std::stringstream {};
+        ket::save_statevector(out, state);
+        const std::string text = out.str();
+        REQUIRE(text.rfind("ENDIANNESS: LITTLE\nNUMBER OF STATES: 8\n", 0) == 0);
+        char first[128];
+        std::snprintf(first, sizeof first, " %.14f    %.14f\n", amps[0].real(), amps[0].imag());
+        REQUIRE(text.find(first) != std::string::npos);
+        const auto back = ket::load_statevector(out);
+        REQUIRE(ket::almost_eq(state, back));
+
+        auto big = std::stringstream {};
+        ket::save_statevector(big, state, ket::Endian::BIG);
+        REQUIRE(big.str().rfind("ENDIANNESS: BIG\n", 0) == 0);
+        const auto back_big = ket::load_statevector(big);
+        for (std::size_t i = 0; i < 8; ++i) {
+            REQUIRE(std::abs(back_big[i] - state[i]) < 1.0e-13);
+        }
+        // line 1 of a BIG file is amplitude 0b100 (index 1 with its 3 bits reversed)
+        auto lines = std::stringstream {big.str()};
+        std::string line;
+        std::getline(lines, line);
+        std::getline(lines, line);
+        std::getline(lines, line);
+        std::getline(lines, line);
+        std::snprintf(first, sizeof first, " %.14f    %.14f", amps[4].real(), amps[4].imag());
+        REQUIRE(line == first);
     });
 
     std::printf("%d passed, %d failed\n", g_passed, g_failed);
