@@ -217,6 +217,11 @@ def debug_plan(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, is_dens
     return FusionPlan(order[:n_rec], pass_of[:n_rec], kind[:npass], cost[:npass])
 
 
+def set_fusion_low_qubits(low_qubits: int) -> None:
+    """Qubits 0..low_qubits-1 belong to every tile pass (default 5)."""
+    check(load().kq_set_fusion_low_qubits(int(low_qubits)))
+
+
 def set_jit(mode: int) -> None:
     """Run-time compiled tile passes: 0 off, 1 states of >= 18 qubits (default), 2 every tile pass."""
     check(load().kq_set_jit(int(mode)))

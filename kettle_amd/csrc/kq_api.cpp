@@ -184,7 +184,7 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         profiler_collect();
         return;
     }
-    const auto passes = plan_passes(recs, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
+    const auto passes = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
                                     fusion_max_stages());
     // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts][terms]
     // (interpreted) or [coefficients] (compiled for its structure)
@@ -653,7 +653,7 @@ void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<Ga
                          const std::vector<const GateRec*>& deriv, std::vector<double>& values)
 {
     const int max_k = fusion_max_k(s->precision, s->nvec);
-    const auto passes = plan_passes(inv, s->nvec, max_k, 5, fusion_max_gates(), register_qubits(s->precision),
+    const auto passes = plan_passes(inv, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
                                     fusion_max_stages(), false);
     const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
     std::vector<TileProgram> progs(passes.size());
@@ -1417,6 +1417,11 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
     });
 }
 
+int kq_set_fusion_low_qubits(int low_qubits)
+{
+    return guarded([&] { set_fusion_low_qubits(low_qubits); });
+}
+
 int kq_set_jit(int mode)
 {
     return guarded([&] { set_jit_mode(mode); });
@@ -1470,7 +1475,7 @@ std::vector<std::string> debug_jit_sources(uint32_t n_qubits, int precision, con
     if (max_k < register_qubits(precision)) {
         throw InvalidArg {"debug jit: state too small for a tile"};
     }
-    const auto passes = plan_passes(recs, n_qubits, max_k, 5, fusion_max_gates(), register_qubits(precision),
+    const auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
                                     fusion_max_stages());
     std::vector<std::string> srcs;
     for (const auto& p : passes) {
@@ -1626,7 +1631,7 @@ int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op*
             }
         }
         else {
-            const auto passes = plan_passes(recs, nvec, max_k, 5, fusion_max_gates(), register_qubits(precision),
+            const auto passes = plan_passes(recs, nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
                                             fusion_max_stages());
             for (const auto& p : passes) {
                 if (!p.sweep) {

@@ -167,8 +167,8 @@ void plan_instance(BatchInstance& in, int precision, int fwd_precision)
     if (in.k < register_qubits(precision)) {
         throw InvalidArg {"batch: the tile does not hold one register block"};
     }
-    in.fwd = plan_passes(recs, in.nv, in.k, 5, fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
-    in.bwd = plan_passes(inv, in.nv, in.k, 5, fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
+    in.fwd = plan_passes(recs, in.nv, in.k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
+    in.bwd = plan_passes(inv, in.nv, in.k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision), fusion_max_stages(), false);
     in.has_d = has_d;
     in.bwd_gate.resize(inv.size());
     for (size_t i = 0; i < inv.size(); ++i) {

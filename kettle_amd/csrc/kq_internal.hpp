@@ -262,6 +262,9 @@ bool is_diag(const GateRec& g);
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
                                   int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true);
 int fusion_max_stages();
+// qubits 0..c-1 always belong to a tile (contiguous 2^c-amplitude chunks per access)
+int fusion_low_qubits();
+void set_fusion_low_qubits(int c);
 int register_qubits(int precision);
 PlanStats& last_stats();
 int fusion_max_k(int precision, uint32_t nvec);

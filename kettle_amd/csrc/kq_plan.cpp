@@ -144,6 +144,21 @@ void set_fusion(int max_k, int max_gates, int max_stages, double max_cost)
 
 int fusion_setting_k() { return g_max_k; }
 int fusion_max_stages() { return g_max_stages; }
+
+namespace
+{
+int g_low_qubits = 5;
+}
+
+int fusion_low_qubits() { return g_low_qubits; }
+
+void set_fusion_low_qubits(int c)
+{
+    if (c < 0 || c > 12) {
+        throw InvalidArg {"low qubits per tile must be in [0, 12]"};
+    }
+    g_low_qubits = c;
+}
 int register_qubits(int) { return 4; }
 int fusion_max_gates() { return g_max_gates; }
 
