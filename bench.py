@@ -190,8 +190,9 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--jit", type=int, default=1,
                     help="run-time compiled tile passes: 0 off, 1 states of >= 18 qubits, 2 all")
-    ap.add_argument("--low-qubits", type=int, default=5,
-                    help="qubits 0..c-1 in every tile pass (contiguous chunk of 2^c amplitudes per access)")
+    ap.add_argument("--low-qubits", type=int, default=0,
+                    help="qubits 0..c-1 in every tile pass (contiguous chunk of 2^c amplitudes per access); "
+                         "0 keeps the library default")
     ap.add_argument("--jit-tuning", default="0,0,1",
                     help="min waves per SIMD, persistent blocks per CU, pivot normalisation (0/1) of the "
                          "run-time compiled passes")
@@ -238,7 +239,8 @@ def main() -> int:
         ka.apply_gates(sv, ops)
 
     ka.set_jit(args.jit)
-    ka.set_fusion_low_qubits(args.low_qubits)
+    if args.low_qubits > 0:
+        ka.set_fusion_low_qubits(args.low_qubits)
     waves, per_cu, normalise = (int(x) for x in args.jit_tuning.split(","))
     ka.set_jit_tuning(waves, per_cu, bool(normalise))
     for i in range(args.warmup):
@@ -306,7 +308,7 @@ def main() -> int:
             "qubits": n,
             "gates": n_gates,
             "passes_per_step": n_passes,
-            "low_qubits_per_tile": args.low_qubits,
+            "low_qubits_per_tile": args.low_qubits or "library default (4)",
             "tile_passes_per_step": n_tiles,
             "kernel_ms_per_step": round(kernel_ms, 3),
             "parallelism": "single GPU",
