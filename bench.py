@@ -186,6 +186,8 @@ def main() -> int:
     ap.add_argument("--vqe-qubits", type=int, default=24)
     ap.add_argument("--vqe-energies", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-stream-probe", action="store_true",
+                    help="skip the plain-stream HBM reference (read / write / in place / copy GB/s)")
     ap.add_argument("--batch-instances", type=int, default=4096)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--jit", type=int, default=1,
@@ -320,6 +322,15 @@ def main() -> int:
                 "compiles_inside_timed_region": jit_end["compiled"] - jit_warm["compiled"]},
     }
     del sv
+    if not args.no_stream_probe:
+        log("hbm stream probe")
+        gbps = (ctypes.c_double * 4)()
+        _lib.check(lib.kq_stream_probe(n, gbps))
+        probe = {"read": round(gbps[0], 1), "write": round(gbps[1], 1), "in_place": round(gbps[2], 1),
+                 "copy": round(gbps[3], 1), "unit": "GB/s", "elements": f"2^{n} x 16 B"}
+        out["hbm_stream_probe"] = probe
+        # the tile pass reads and writes every amplitude once, like the in-place stream
+        roofline["frac_of_in_place_stream"] = round(achieved / max(gbps[2], 1e-9), 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_gates)
     if not args.no_vqe:

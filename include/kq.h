@@ -169,6 +169,19 @@ int kq_probabilities(kq_state* s, const double* noise, double* out);
 /* perform_measurements_as_memory(state, shots, noise, seed): state indices per shot. */
 int kq_sample(kq_state* s, const double* noise, uint64_t n_shots, int has_seed, int32_t seed,
               uint64_t* out_indices);
+/* perform_measurements_as_counts_marginal(circuit, state, shots, ...) (measurements.cpp:178-208):
+ * every shot runs the circuit on a copy of `initial` (mid-circuit outcomes drawn afresh per shot,
+ * as simulate() without a seed does) and draws one index from its final probabilities (with noise)
+ * with a sampler seeded by `seed` (has_seed; then every shot of one outcome history gets the same
+ * index, as in the reference) or from std::random_device.  Shots with the same outcome history
+ * share one simulated branch, so the device work grows with the number of distinct histories, not
+ * with n_shots; branch states are cached up to max_cached_bytes (a branch that does not fit is
+ * rebuilt from its nearest cached ancestor when needed).  Loggers are ignored.  out_indices gets
+ * n_shots indices; *n_branches (optional) the number of branch nodes simulated. */
+int kq_sample_circuit_shots(const kq_state* initial, uint32_t circuit_n_qubits, const kq_op* ops, size_t n_ops,
+                            const int32_t* preds, size_t n_preds, size_t n_bits, const double* noise, uint64_t n_shots,
+                            int has_seed, int32_t seed, uint64_t max_cached_bytes, uint64_t* out_indices,
+                            uint64_t* n_branches);
 /* The same sampler over caller-provided raw probabilities (perform_measurements_as_memory(
  * probabilities_raw, ...)); probabilities are uploaded and scanned on the device. */
 int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_shots,
@@ -246,6 +259,10 @@ typedef struct kq_pass_stats {
     uint64_t gates;       /* gates applied by these launches */
 } kq_pass_stats;
 int kq_profile_begin(void);
+/* Device bandwidth reference: GB/s of plain streams of 16-byte elements over two 2^n_qubits
+ * buffers (16 <= n_qubits <= 32, 2 x 16 * 2^n bytes of device memory for the call): read only,
+ * write only, read-modify-write in place, copy.  No counterpart in the reference. */
+int kq_stream_probe(uint32_t n_qubits, double gbps[4]);
 int kq_profile_end(kq_pass_stats out[3]);
 
 /* ---- debug / parity probes ------------------------------------------------------------------

@@ -89,6 +89,9 @@ _SIGNATURES = {
     "kq_debug_pair_map_1q": (c_int, [c_uint32, c_uint32, c_void_p, c_void_p]),
     "kq_debug_pair_map_2q": (c_int, [c_uint32, c_uint32, c_uint32, c_int, c_void_p]),
     "kq_set_jit": (c_int, [c_int]),
+    "kq_stream_probe": (c_int, [c_uint32, c_void_p]),
+    "kq_sample_circuit_shots": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p,
+                                        c_uint64, c_int, c_int32, c_uint64, c_void_p, POINTER(c_uint64)]),
     "kq_set_fusion_low_qubits": (c_int, [c_int]),
     "kq_set_jit_tuning": (c_int, [c_int, c_int, c_int]),
     "kq_jit_info": (c_int, [POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double)]),

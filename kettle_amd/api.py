@@ -329,8 +329,44 @@ def perform_measurements_as_counts_raw(state_or_probs, n_shots: int, noise=None,
     return dict(sorted(Counter(int(i) for i in mem).items()))
 
 
-def perform_measurements_as_counts_marginal(state_or_probs, n_shots: int, marginal_qubits: Sequence[int] = (),
-                                            noise=None, seed=None) -> dict[str, int]:
+DEFAULT_SHOT_CACHE_BYTES = 8 << 30
+
+
+def perform_measurements_as_memory_circuit(circuit: QuantumCircuit, state: Statevector, n_shots: int,
+                                           noise: QuantumNoise | None = None, seed: int | None = None,
+                                           max_cached_bytes: int = DEFAULT_SHOT_CACHE_BYTES) -> tuple[np.ndarray, int]:
+    """Per-shot re-simulation of ``circuit`` from ``state`` (measurements.cpp:178-208): one index per
+    shot, and the number of branch states simulated (shots with the same mid-circuit outcome history
+    share one branch)."""
+    if circuit.n_qubits() != state.n_qubits():
+        raise RuntimeError("Invalid simulation; circuit and state have different number of qubits.")
+    ops, preds, _ = circuit.flatten()
+    out = np.empty(int(n_shots), dtype=np.uint64)
+    hs, sv = _seed(seed)
+    nz = None if noise is None else np.ascontiguousarray(noise.array())
+    nb = _lib.ctypes.c_uint64()
+    check(load().kq_sample_circuit_shots(state.handle, circuit.n_qubits(), ptr(ops), len(ops), ptr(preds), len(preds),
+                                         circuit.n_bits(), ptr(nz) if nz is not None else _p(0), int(n_shots), hs, sv,
+                                         int(max_cached_bytes), ptr(out), _lib.ctypes.byref(nb)))
+    return out, nb.value
+
+
+def perform_measurements_as_counts_marginal(state_or_probs, n_shots, marginal_qubits: Sequence[int] = (),
+                                            noise=None, seed=None, *args) -> dict[str, int]:
+    """kettle's overloads: (state | raw probabilities, shots, marginal, noise, seed), and
+    (circuit, state, shots, marginal, noise, seed), which re-simulates the circuit per shot."""
+    if isinstance(state_or_probs, QuantumCircuit):
+        circuit, state, shots = state_or_probs, n_shots, marginal_qubits
+        marginal = noise if noise is not None else ()
+        noise_, seed_ = (seed, args[0] if args else None)
+        n = circuit.n_qubits()
+        if any(q >= n for q in marginal):
+            raise RuntimeError("ERROR: marginal qubit index out of range.")
+        idx, _ = perform_measurements_as_memory_circuit(circuit, state, shots, noise_, seed_)
+        counts: Counter = Counter()
+        for i in idx.tolist():
+            counts[state_index_to_bitstring(i, n, marginal)] += 1
+        return dict(sorted(counts.items()))
     if isinstance(state_or_probs, _State):
         n = state_or_probs.n_qubits()
     else:

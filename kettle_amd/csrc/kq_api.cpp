@@ -369,29 +369,24 @@ std::vector<double> download_host(const kq_state* s)
 // simulate_loop_body_iterative_ (simulate.cpp:417-521): an explicit stack of op ranges.
 // Consecutive gates of one range are compiled, fused and launched together; the batch is
 // flushed at every measurement, control-flow element and logger.
-void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds, size_t n_preds,
-              bool has_seed, int32_t seed, int8_t* creg, size_t n_bits, kq_logger_fn logger, void* user)
+// Interpreter position in a flat program: a stack of [next, end) op ranges (bodies of taken
+// branches are pushed, kettle's simulate_loop_body_iterative_ at simulate.cpp:417-521).
+using Cursor = std::vector<std::pair<size_t, size_t>>;
+
+// Runs the program from `cur` on `s`, fusing gate runs, until the next measurement (returned, with
+// the cursor past it) or the end (nullptr).  Control flow reads `creg`; loggers are called when given.
+const kq_op* run_to_measurement(kq_state* s, Cursor& cur, const kq_op* ops, const int32_t* preds, size_t n_preds,
+                                const int8_t* creg, size_t n_bits, kq_logger_fn logger, void* user)
 {
-    if (circuit_n != s->n) {
-        throw std::runtime_error {"Invalid simulation; circuit and state have different number of qubits."};
-    }
-    if (n_bits > 0 && creg == nullptr) {
-        throw InvalidArg {"null classical register"};
-    }
-    for (size_t i = 0; i < n_bits; ++i) {
-        creg[i] = -1;
-    }
     std::vector<GateRec> batch;
     const auto flush = [&] {
         run_records(s, batch);
         batch.clear();
     };
-    std::vector<std::pair<size_t, size_t>> stack;
-    stack.emplace_back(0, n_ops);
-    while (!stack.empty()) {
-        auto& top = stack.back();
+    while (!cur.empty()) {
+        auto& top = cur.back();
         if (top.first >= top.second) {
-            stack.pop_back();
+            cur.pop_back();
             continue;
         }
         const size_t here = top.first;
@@ -407,7 +402,7 @@ void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, c
             top.first += 1 + len;
             flush();
             if (eval_predicate(preds, n_preds, op.q0, creg, n_bits)) {
-                stack.emplace_back(here + 1, here + 1 + len);
+                cur.emplace_back(here + 1, here + 1 + len);
             }
         }
         else if (op.gate == KQ_OP_IF_ELSE) {
@@ -416,10 +411,10 @@ void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, c
             top.first += 1 + len_if + len_else;
             flush();
             if (eval_predicate(preds, n_preds, op.q0, creg, n_bits)) {
-                stack.emplace_back(here + 1, here + 1 + len_if);
+                cur.emplace_back(here + 1, here + 1 + len_if);
             }
             else {
-                stack.emplace_back(here + 1 + len_if, here + 1 + len_if + len_else);
+                cur.emplace_back(here + 1 + len_if, here + 1 + len_if + len_else);
             }
         }
         else if (op.gate == KQ_OP_LOG_STATE || op.gate == KQ_OP_LOG_CREG) {
@@ -444,7 +439,7 @@ void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, c
                 throw InvalidArg {"measurement bit out of range"};
             }
             flush();
-            creg[op.q1] = static_cast<int8_t>(measure(s, static_cast<uint32_t>(op.q0), has_seed, seed));
+            return &op;
         }
         else {
             ++top.first;
@@ -455,6 +450,25 @@ void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, c
         }
     }
     flush();
+    return nullptr;
+}
+
+void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds, size_t n_preds,
+              bool has_seed, int32_t seed, int8_t* creg, size_t n_bits, kq_logger_fn logger, void* user)
+{
+    if (circuit_n != s->n) {
+        throw std::runtime_error {"Invalid simulation; circuit and state have different number of qubits."};
+    }
+    if (n_bits > 0 && creg == nullptr) {
+        throw InvalidArg {"null classical register"};
+    }
+    for (size_t i = 0; i < n_bits; ++i) {
+        creg[i] = -1;
+    }
+    Cursor cur {{0, n_ops}};
+    while (const kq_op* m = run_to_measurement(s, cur, ops, preds, n_preds, creg, n_bits, logger, user)) {
+        creg[m->q1] = static_cast<int8_t>(measure(s, static_cast<uint32_t>(m->q0), has_seed, seed));
+    }
 }
 
 // ---- sampling ---------------------------------------------------------------------------------------
@@ -509,6 +523,224 @@ void check_noise(const double* noise, uint32_t n)
         }
     }
 }
+
+// ---- per-shot re-simulation ----------------------------------------------------------------------
+
+// perform_measurements_as_counts_marginal(circuit, state, ...) (measurements.cpp:178-208) runs the
+// whole circuit once per shot.  Between measurements a run is deterministic given the classical
+// register, so shots with the same outcome history reach the same state: a branch node holds the
+// state at a measurement (or at the end) for one history, and every distinct history is simulated
+// once while its state fits the cache (an evicted branch is rebuilt from its nearest cached ancestor).
+struct ShotNode
+{
+    kq_state* state = nullptr;   // the state at this point (owned), or null once it did not fit the cache
+    Cursor cursor;               // program position right after the measurement op (or the end)
+    std::vector<int8_t> creg;
+    int32_t m_qubit = -1;        // the measurement reached here (-1: the program ended)
+    int32_t m_bit = -1;
+    double w[2] = {0.0, 0.0};    // its outcome weights
+    int64_t child[2] = {-1, -1};
+    int64_t parent = -1;
+    int outcome = -1;            // the parent's outcome that leads here
+    uint64_t shots = 0;          // shots that end here
+};
+
+struct ShotTree
+{
+    const kq_op* ops;
+    const int32_t* preds;
+    size_t n_preds;
+    size_t n_bits;
+    uint32_t n;
+    int precision;
+    uint64_t cache_left;
+    uint64_t state_bytes;
+    std::vector<ShotNode> nodes;
+
+    ~ShotTree()
+    {
+        for (auto& nd : nodes) {
+            if (nd.state) {
+                (void) kq_state_destroy(nd.state);
+            }
+        }
+    }
+
+    // runs from the node's cursor to the next measurement and records it (and its weights)
+    void settle(ShotNode& nd, kq_state* st)
+    {
+        const kq_op* m = run_to_measurement(st, nd.cursor, ops, preds, n_preds, nd.creg.data(), n_bits, nullptr, nullptr);
+        if (m) {
+            if (m->q0 < 0 || static_cast<uint32_t>(m->q0) >= n) {
+                throw InvalidArg {"measured qubit out of range"};
+            }
+            nd.m_qubit = m->q0;
+            nd.m_bit = m->q1;
+            launch_qubit_weights(st, static_cast<uint32_t>(m->q0), nd.w);
+        }
+    }
+
+    // keeps `st` as the node's state if the cache has room, else frees it
+    void keep_or_free(ShotNode& nd, kq_state* st)
+    {
+        if (cache_left >= state_bytes) {
+            cache_left -= state_bytes;
+            nd.state = st;
+        }
+        else {
+            (void) kq_state_destroy(st);
+        }
+    }
+
+    // a fresh state holding node i's state (the caller owns it)
+    kq_state* materialise(int64_t i)
+    {
+        kq_state* st = create_state(n, precision, false);
+        try {
+            const ShotNode& nd = nodes[static_cast<size_t>(i)];
+            if (nd.state) {
+                KQ_HIP_CHECK(hipStreamSynchronize(nd.state->stream));
+                launch_copy_kernel(st, st->amps, nd.state->amps, st->n_amps);
+                KQ_HIP_CHECK(hipStreamSynchronize(st->stream));
+                return st;
+            }
+            (void) kq_state_destroy(st);
+            st = materialise(nd.parent);
+            step(nodes[static_cast<size_t>(nd.parent)], nd.outcome, st, nullptr);
+            return st;
+        }
+        catch (...) {
+            (void) kq_state_destroy(st);
+            throw;
+        }
+    }
+
+    // collapses `st` (holding parent p's state) on outcome b and runs it to the next measurement;
+    // fills `out` with the child's bookkeeping when given
+    void step(const ShotNode& p, int b, kq_state* st, ShotNode* out)
+    {
+        const double norm = std::sqrt(1.0 / p.w[b]);
+        launch_collapse(st, static_cast<uint32_t>(p.m_qubit), b, norm);
+        ShotNode tmp;
+        ShotNode& nd = out ? *out : tmp;
+        nd.cursor = p.cursor;
+        nd.creg = p.creg;
+        nd.creg[static_cast<size_t>(p.m_bit)] = static_cast<int8_t>(b);
+        settle(nd, st);
+    }
+
+    int64_t child(int64_t i, int b)
+    {
+        if (nodes[static_cast<size_t>(i)].child[b] >= 0) {
+            return nodes[static_cast<size_t>(i)].child[b];
+        }
+        kq_state* st = materialise(i);
+        ShotNode nd;
+        try {
+            step(nodes[static_cast<size_t>(i)], b, st, &nd);
+        }
+        catch (...) {
+            (void) kq_state_destroy(st);
+            throw;
+        }
+        nd.parent = i;
+        nd.outcome = b;
+        keep_or_free(nd, st);
+        nodes.push_back(std::move(nd));
+        const auto id = static_cast<int64_t>(nodes.size() - 1);
+        nodes[static_cast<size_t>(i)].child[b] = id;
+        return id;
+    }
+};
+
+void sample_circuit_shots(const kq_state* init, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds,
+                          size_t n_preds, size_t n_bits, const double* noise, uint64_t n_shots, bool has_seed, int32_t seed,
+                          uint64_t max_cached_bytes, uint64_t* out, uint64_t* n_branches)
+{
+    if (circuit_n != init->n) {
+        throw std::runtime_error {"Invalid simulation; circuit and state have different number of qubits."};
+    }
+    if (init->is_dm) {
+        throw InvalidArg {"per-shot sampling takes a statevector"};
+    }
+    check_noise(noise, init->n);
+    ShotTree t {ops, preds, n_preds, n_bits, init->n, init->precision, 0,
+                init->n_amps * amp_bytes(init->precision), {}};
+    // the root state always stays
+    {
+        kq_state* st = create_state(init->n, init->precision, false);
+        t.nodes.push_back(ShotNode {});
+        t.nodes[0].state = st;
+        KQ_HIP_CHECK(hipStreamSynchronize(init->stream));
+        launch_copy_kernel(st, st->amps, init->amps, st->n_amps);
+        t.nodes[0].cursor = Cursor {{0, n_ops}};
+        t.nodes[0].creg.assign(n_bits, -1);
+        t.settle(t.nodes[0], st);
+    }
+    t.cache_left = max_cached_bytes;
+    // mid-circuit outcomes: independent draws for every shot, as kettle's simulate() without a seed
+    // draws each measurement from a std::random_device-seeded generator
+    std::mt19937_64 rng {std::random_device {}()};
+    std::vector<int64_t> leaf(n_shots);
+    for (uint64_t k = 0; k < n_shots; ++k) {
+        int64_t i = 0;
+        while (t.nodes[static_cast<size_t>(i)].m_qubit >= 0) {
+            const ShotNode& nd = t.nodes[static_cast<size_t>(i)];
+            auto flip = std::discrete_distribution<int> {{nd.w[0], nd.w[1]}};
+            i = t.child(i, flip(rng));
+        }
+        leaf[k] = i;
+        ++t.nodes[static_cast<size_t>(i)].shots;
+    }
+    // one sampler per shot over its final state (ProbabilitySampler_ seeded with `seed`, or from
+    // std::random_device): per branch, the shots that end there
+    const uint64_t N = uint64_t {1} << init->n;
+    void* d = device_alloc(N * sizeof(double));
+    try {
+        std::vector<uint64_t> idx;
+        std::vector<uint64_t> next(t.nodes.size(), 0);
+        std::vector<std::vector<uint64_t>> drawn(t.nodes.size());
+        for (size_t i = 0; i < t.nodes.size(); ++i) {
+            const uint64_t m = t.nodes[i].shots;
+            if (m == 0) {
+                continue;
+            }
+            kq_state* st = t.nodes[i].state ? nullptr : t.materialise(static_cast<int64_t>(i));
+            const kq_state* src = st ? st : t.nodes[i].state;
+            try {
+                launch_probabilities(const_cast<kq_state*>(src), static_cast<double*>(d), noise);
+                KQ_HIP_CHECK(hipStreamSynchronize(src->stream));
+                // a seeded sampler draws the same first number for every shot of a branch
+                const uint64_t draws = has_seed ? 1 : m;
+                idx.assign(draws, 0);
+                sample_from_device_probs(src->stream, static_cast<double*>(d), N, draws, has_seed, seed, idx.data());
+            }
+            catch (...) {
+                if (st) {
+                    (void) kq_state_destroy(st);
+                }
+                throw;
+            }
+            if (st) {
+                (void) kq_state_destroy(st);
+            }
+            drawn[i] = idx;
+        }
+        for (uint64_t k = 0; k < n_shots; ++k) {
+            const auto i = static_cast<size_t>(leaf[k]);
+            out[k] = has_seed ? drawn[i][0] : drawn[i][next[i]++];
+        }
+    }
+    catch (...) {
+        (void) hipFree(d);
+        throw;
+    }
+    KQ_HIP_CHECK(hipFree(d));
+    if (n_branches) {
+        *n_branches = t.nodes.size();
+    }
+}
+
 
 // ---- Pauli strings ----------------------------------------------------------------------------------
 
@@ -1225,6 +1457,22 @@ int kq_sample(kq_state* s, const double* noise, uint64_t n_shots, int has_seed, 
     });
 }
 
+int kq_sample_circuit_shots(const kq_state* initial, uint32_t circuit_n_qubits, const kq_op* ops, size_t n_ops,
+                            const int32_t* preds, size_t n_preds, size_t n_bits, const double* noise, uint64_t n_shots,
+                            int has_seed, int32_t seed, uint64_t max_cached_bytes, uint64_t* out_indices,
+                            uint64_t* n_branches)
+{
+    return guarded([&] {
+        check_state(initial);
+        if ((n_shots > 0 && !out_indices) || (!ops && n_ops) || (!preds && n_preds)) {
+            throw InvalidArg {"null argument"};
+        }
+        ensure_init();
+        sample_circuit_shots(initial, circuit_n_qubits, ops, n_ops, preds, n_preds, n_bits, noise, n_shots, has_seed != 0,
+                             seed, max_cached_bytes, out_indices, n_branches);
+    });
+}
+
 int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_shots, int has_seed, int32_t seed,
                             uint64_t* out_indices)
 {
@@ -1414,6 +1662,17 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
         KQ_HIP_CHECK(hipDeviceSynchronize());
         KQ_HIP_CHECK(hipMemcpy(out, d, words * sizeof(uint64_t), hipMemcpyDeviceToHost));
         KQ_HIP_CHECK(hipFree(d));
+    });
+}
+
+int kq_stream_probe(uint32_t n_qubits, double gbps[4])
+{
+    return guarded([&] {
+        if (n_qubits < 16 || n_qubits > 32 || !gbps) {
+            throw InvalidArg {"stream probe: 16 <= n_qubits <= 32 and a 4-value output"};
+        }
+        ensure_init();
+        stream_probe(n_qubits, gbps);
     });
 }
 

@@ -375,6 +375,8 @@ void launch_adjoint_gate(kq_state* s, void* psi, void* lam, const GateRec& inv, 
                          double* d_acc);
 void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
+// GB/s of plain 16-byte streams over 2^n_qubits elements: read, write, in place, copy
+void stream_probe(uint32_t n_qubits, double gbps[4]);
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
 

@@ -143,8 +143,7 @@ std::array<double, 8> record_values(const TileGate& g, bool c64)
 
 // Emits an uncontrolled one-qubit gate with its pivot entry divided out (orth / axis / dense1 / a
 // d0-normalised diagonal) and multiplies the pivot into the pass factor; false if the record has
-// another shape.  Which entry is the pivot (the larger in magnitude, for accuracy) is part of the
-// generated source; the normalised values are data.
+// another shape.  The normalised values are data; which entry is the pivot is structure.
 bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::vector<double>& coefs,
                      std::complex<double>& factor, bool& factor_complex)
 {
@@ -158,15 +157,22 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         coefs.push_back(x);
         return "static_cast<R>(cf[" + std::to_string(coefs.size() - 1) + "])";
     };
+    // The diagonal entry is the pivot unless it is below 1e-6 of the off-diagonal one: c * (a + t b)
+    // rounds like c a + s b for any finite t, so only a (nearly) vanishing pivot needs the other
+    // form, and the kernel's structure stays the same for almost every angle.
+    const auto diag_pivot_of = [](double diag, double off) { return std::abs(diag) >= 1.0e-6 * std::abs(off); };
+    // amplitudes grow by 1 / |factor| inside the pass: keep that far from overflow
+    const double floor = c64 ? 1.0e-20 : 1.0e-200;
+    const auto fits = [&](std::complex<double> pivot) { return std::abs(factor * pivot) >= floor; };
     if (kind == TK_REAL) {
         const bool rotation = m[6] == m[0] && m[4] == -m[2];
         const bool reflection = m[6] == -m[0] && m[4] == m[2];
         if (!rotation && !reflection) {
             return false;
         }
-        const bool diag_pivot = std::abs(m[0]) >= std::abs(m[2]);
+        const bool diag_pivot = diag_pivot_of(m[0], m[2]);
         const double pivot = diag_pivot ? m[0] : m[2];
-        if (pivot == 0.0) {
+        if (pivot == 0.0 || !fits(pivot)) {
             return false;
         }
         const int form = (diag_pivot ? 0 : 2) + (rotation ? 0 : 1);
@@ -178,9 +184,9 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         if (m[6] != m[0] || m[5] != m[3]) {
             return false;
         }
-        const bool diag_pivot = std::abs(m[0]) >= std::abs(m[3]);
+        const bool diag_pivot = diag_pivot_of(m[0], m[3]);
         const double pivot = diag_pivot ? m[0] : m[3];
-        if (pivot == 0.0) {
+        if (pivot == 0.0 || !fits(pivot)) {
             return false;
         }
         o << "  axis<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << val((diag_pivot ? m[3] : m[0]) / pivot)
@@ -190,7 +196,7 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
     }
     if (kind == TK_DIAG) {
         const std::complex<double> d0 {m[0], m[1]};
-        if (std::abs(d0) < 0.5) {
+        if (std::abs(d0) < 0.5 || !fits(d0)) {
             return false;
         }
         const std::complex<double> d = std::complex<double> {m[6], m[7]} / d0;
@@ -204,9 +210,9 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
     }
     if (kind == TK_DENSE) {
         const std::complex<double> m00 {m[0], m[1]}, m01 {m[2], m[3]}, m10 {m[4], m[5]}, m11 {m[6], m[7]};
-        const bool diag_pivot = std::abs(m00) >= std::abs(m01);
+        const bool diag_pivot = diag_pivot_of(std::abs(m00), std::abs(m01));
         const std::complex<double> pivot = diag_pivot ? m00 : m01;
-        if (std::abs(pivot) == 0.0) {
+        if (std::abs(pivot) == 0.0 || !fits(pivot)) {
             return false;
         }
         const std::complex<double> x = (diag_pivot ? m01 : m00) / pivot, y = m10 / pivot, z = m11 / pivot;

@@ -1368,6 +1368,100 @@ void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes)
     KQ_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s->stream));
 }
 
+// ---- HBM stream probe ---------------------------------------------------------------------------------
+// What this device's HBM delivers to plain streams of 16-byte elements (16 per thread in flight,
+// a few blocks per CU): read only, write only, read-modify-write in place, copy.  The reference
+// point for the tile passes, which read and write every amplitude once.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_stream_probe(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                        uint64_t n, double* __restrict__ sink)
+{
+    constexpr int U = 16;
+    const uint64_t chunk = uint64_t {BLOCK} * U;
+    double acc = 0.0;
+    for (uint64_t c = blockIdx.x * chunk; c + chunk <= n; c += uint64_t {gridDim.x} * chunk) {
+        double2 v[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            if constexpr (MODE == 1) {
+                v[j] = make_double2(static_cast<double>(j), 1.0);
+            }
+            else {
+                v[j] = src[c + uint64_t {BLOCK} * j + threadIdx.x];
+            }
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                acc += v[j].x + v[j].y;
+            }
+        }
+        else {
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                dst[c + uint64_t {BLOCK} * j + threadIdx.x] = v[j];
+            }
+        }
+    }
+    if (MODE == 0 && acc == -1.0) {
+        sink[0] = acc;   // keeps the reads; never true for the zero-filled probe buffers
+    }
+}
+
+void stream_probe(uint32_t n_qubits, double gbps[4])
+{
+    const uint64_t n = uint64_t {1} << n_qubits;
+    double2* a = nullptr;
+    double2* b = nullptr;
+    double* sink = nullptr;
+    KQ_HIP_CHECK(hipMalloc(&a, n * sizeof(double2)));
+    const auto release = [&] {
+        (void) hipFree(a);
+        (void) hipFree(b);
+        (void) hipFree(sink);
+    };
+    try {
+        KQ_HIP_CHECK(hipMalloc(&b, n * sizeof(double2)));
+        KQ_HIP_CHECK(hipMalloc(&sink, 64));
+        KQ_HIP_CHECK(hipMemset(a, 0, n * sizeof(double2)));
+        KQ_HIP_CHECK(hipMemset(b, 0, n * sizeof(double2)));
+        int dev = 0;
+        KQ_HIP_CHECK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        KQ_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        const unsigned grid = static_cast<unsigned>(prop.multiProcessorCount) * 2;
+        hipEvent_t e0, e1;
+        KQ_HIP_CHECK(hipEventCreate(&e0));
+        KQ_HIP_CHECK(hipEventCreate(&e1));
+        const double one = 16.0 * static_cast<double>(n);
+        const auto time = [&](auto launch, double bytes) {
+            constexpr int reps = 5;
+            launch();
+            KQ_HIP_CHECK(hipEventRecord(e0, nullptr));
+            for (int i = 0; i < reps; ++i) {
+                launch();
+            }
+            KQ_HIP_CHECK(hipEventRecord(e1, nullptr));
+            KQ_HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            KQ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return bytes / (static_cast<double>(ms) / reps * 1.0e-3) / 1.0e9;
+        };
+        gbps[0] = time([&] { k_stream_probe<0><<<grid, BLOCK>>>(a, b, n, sink); }, one);
+        gbps[1] = time([&] { k_stream_probe<1><<<grid, BLOCK>>>(a, b, n, sink); }, one);
+        gbps[2] = time([&] { k_stream_probe<2><<<grid, BLOCK>>>(a, a, n, sink); }, 2 * one);
+        gbps[3] = time([&] { k_stream_probe<3><<<grid, BLOCK>>>(a, b, n, sink); }, 2 * one);
+        KQ_HIP_CHECK(hipGetLastError());
+        (void) hipEventDestroy(e0);
+        (void) hipEventDestroy(e1);
+    }
+    catch (...) {
+        release();
+        throw;
+    }
+    release();
+}
+
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps)
 {
     const unsigned blocks = grid_for(n_amps, 4);

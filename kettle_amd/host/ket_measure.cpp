@@ -188,12 +188,21 @@ auto perform_measurements_as_counts_marginal(const QuantumCircuit& circuit, cons
             throw std::runtime_error {"ERROR: marginal qubit index out of range."};
         }
     }
+    if (circuit.n_qubits() != original_state.n_qubits()) {
+        throw std::runtime_error {"Invalid simulation; circuit and state have different number of qubits."};
+    }
+    // every shot re-simulates the circuit; shots with the same mid-circuit outcome history share one
+    // simulated branch on the device (kq_sample_circuit_shots)
+    const auto flat = circuit.flatten();
+    std::vector<uint64_t> idx(n_shots);
+    const auto [hs, sv] = seed_args(seed);
+    constexpr uint64_t cache_bytes = uint64_t {8} << 30;
+    ki::kq_check(kq_sample_circuit_shots(original_state.device_state(), static_cast<uint32_t>(n_qubits), flat.ops.data(),
+                                         flat.ops.size(), flat.predicates.data(), flat.predicates.size(), circuit.n_bits(),
+                                         noise_ptr(noise, n_qubits), n_shots, hs, sv, cache_bytes, idx.data(), nullptr));
     auto out = std::map<std::string, std::size_t> {};
-    for (std::size_t i = 0; i < n_shots; ++i) {
-        auto state = original_state;
-        simulate(circuit, state);
-        const auto idx = perform_measurements_as_memory(state, 1, noise, seed);
-        ++out[state_index_to_bitstring_marginal(idx[0], marginal_qubits, n_qubits, Endian::LITTLE)];
+    for (const uint64_t i : idx) {
+        ++out[state_index_to_bitstring_marginal(i, marginal_qubits, n_qubits, Endian::LITTLE)];
     }
     return out;
 }

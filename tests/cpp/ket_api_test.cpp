@@ -392,6 +392,31 @@ int main()
         }
     });
 
+    // measurements.cpp:178-208: per-shot re-simulation of a circuit with a mid-circuit measurement
+    run("marginal counts re-simulate the circuit per shot", [] {
+        auto c = ket::QuantumCircuit {2};
+        c.add_h_gate(0);
+        c.add_m_gate(0);
+        auto flip = ket::QuantumCircuit {2};
+        flip.add_x_gate(1);
+        c.add_if_statement(0, flip);
+        const auto state = ket::Statevector {2};
+        const auto counts = ket::perform_measurements_as_counts_marginal(c, state, 4000, {}, nullptr, std::nullopt);
+        // outcome 0 leaves |00>, outcome 1 gives |11>: only "00" and "11", about half each
+        std::size_t total = 0;
+        for (const auto& [bits, n] : counts) {
+            REQUIRE(bits == "00" || bits == "11");
+            total += n;
+        }
+        REQUIRE(total == 4000);
+        REQUIRE(counts.at("00") > 1700 && counts.at("11") > 1700);
+        const auto marginal = ket::perform_measurements_as_counts_marginal(c, state, 100, {1}, nullptr, 3);
+        for (const auto& [bits, n] : marginal) {
+            REQUIRE(bits == "0x" || bits == "1x");
+        }
+        REQUIRE(std::abs(state[0] - std::complex<double> {1.0, 0.0}) < 1e-15);
+    });
+
     std::printf("%d passed, %d failed\n", g_passed, g_failed);
     return g_failed == 0 ? 0 : 1;
 }

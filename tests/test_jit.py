@@ -124,13 +124,11 @@ def test_jit_reuses_kernels_for_new_angles():
             c.add_cp_gate(0, n - 1, float(theta[layer, 0]))
         return c
 
-    # |theta| < pi/2 keeps cos(theta/2) the larger entry of every RY, so the normalised kernels'
-    # structure (which entry is divided out) does not change with the angles
     rng = np.random.default_rng(11)
-    c1 = layer_circuit(rng.uniform(-1.5, 1.5, size=(3, n)))
+    c1 = layer_circuit(rng.uniform(0, 2 * math.pi, size=(3, n)))
     run_sv(c1)
     after_first = ka.jit_info()
-    c2 = layer_circuit(rng.uniform(-1.5, 1.5, size=(3, n)))
+    c2 = layer_circuit(rng.uniform(0, 2 * math.pi, size=(3, n)))
     out = run_sv(c2)
     after_second = ka.jit_info()
     assert after_second["compiled"] == after_first["compiled"]
