@@ -140,6 +140,10 @@ int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages
 /* Qubits 0..low_qubits-1 belong to every tile pass (default 4), so that each wavefront access
  * covers 2^low_qubits consecutive amplitudes; fewer leave more tile slots to other qubits. */
 int kq_set_fusion_low_qubits(int low_qubits);
+/* on != 0 (default): a statevector's gate passes may store qubits on other index bits of their tile,
+ * chosen so that the next pass takes the most gates; the state records where each qubit is, and
+ * every operation other than gates first stores qubit q back on bit q (gate-free tile passes). */
+int kq_set_fusion_relabel(int on);
 /* Tile passes compiled at run time for their exact gate structure (straight-line code, no per-gate
  * dispatch; coefficients stay data, so repeated structures with other angles reuse the kernel):
  * mode 0 = off (every pass interpreted), 1 = states of >= 18 qubits (default), 2 = every tile pass.

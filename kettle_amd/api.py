@@ -222,6 +222,11 @@ def set_fusion_low_qubits(low_qubits: int) -> None:
     check(load().kq_set_fusion_low_qubits(int(low_qubits)))
 
 
+def set_fusion_relabel(on: bool) -> None:
+    """Whether statevector gate passes may move qubits onto other index bits (default on)."""
+    check(load().kq_set_fusion_relabel(int(bool(on))))
+
+
 def set_jit(mode: int) -> None:
     """Run-time compiled tile passes: 0 off, 1 states of >= 18 qubits (default), 2 every tile pass."""
     check(load().kq_set_jit(int(mode)))

@@ -165,7 +165,24 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
     return n_amps >> (ctrl ? 1 : 0);
 }
 
-void run_records(kq_state* s, const std::vector<GateRec>& recs)
+void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
+
+// Stores every qubit q of the state back on index bit q (gate-free tile passes), if gate passes
+// left them permuted.
+void normalise_layout(kq_state* s)
+{
+    if (s->bit_of.empty()) {
+        return;
+    }
+    const int max_k = fusion_max_k(s->precision, s->nvec);
+    const auto passes = plan_restore(s->bit_of, s->nvec, max_k, fusion_low_qubits(), register_qubits(s->precision));
+    s->bit_of.clear();
+    execute_passes(s, passes);
+}
+
+void normalise_layout(const kq_state* s) { normalise_layout(const_cast<kq_state*>(s)); }
+
+void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relabel = true)
 {
     PlanStats& st = last_stats();
     st.n_gates += recs.size();
@@ -176,6 +193,7 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
     const int max_k = fusion_max_k(s->precision, s->nvec);
     // tiles smaller than one thread's register block (states of a few qubits) run gate by gate
     if (max_k < register_qubits(s->precision)) {
+        normalise_layout(s);
         for (const auto& g : recs) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(g, s->n_amps)), 1, [&] { launch_gate_sweep(s, g); });
             ++st.n_passes;
@@ -184,8 +202,30 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs)
         profiler_collect();
         return;
     }
+    // statevector passes may leave qubits on other index bits (the state's bit_of); everything
+    // else, and density matrices, run with qubit q on bit q
+    const bool relabel = allow_relabel && fusion_relabel() && !s->is_dm;
+    if (!relabel) {
+        normalise_layout(s);
+    }
+    std::vector<uint8_t> bit_of = s->bit_of;
     const auto passes = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
-                                    fusion_max_stages());
+                                    fusion_max_stages(), true, relabel ? &bit_of : nullptr);
+    execute_passes(s, passes);
+    if (relabel) {
+        bool identity = true;
+        for (size_t q = 0; q < bit_of.size(); ++q) {
+            identity = identity && bit_of[q] == q;
+        }
+        s->bit_of = identity ? std::vector<uint8_t> {} : bit_of;
+    }
+}
+
+// Lowers every tile pass, uploads all programs at once and launches the passes in order.
+void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
+{
+    PlanStats& st = last_stats();
+    const double ab = static_cast<double>(amp_bytes(s->precision));
     // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts][terms]
     // (interpreted) or [coefficients] (compiled for its structure)
     std::vector<TileProgram> progs(passes.size());
@@ -310,6 +350,7 @@ int measure(kq_state* s, uint32_t q, bool has_seed, int32_t seed)
     if (q >= s->n) {
         throw InvalidArg {"measured qubit out of range"};
     }
+    normalise_layout(s);
     double w[2];
     launch_qubit_weights(s, q, w);
     auto prng = get_prng(has_seed, seed);
@@ -425,6 +466,7 @@ const kq_op* run_to_measurement(kq_state* s, Cursor& cur, const kq_op* ops, cons
             flush();
             if (logger != nullptr) {
                 if (op.gate == KQ_OP_LOG_STATE) {
+                    normalise_layout(s);
                     const auto h = download_host(s);
                     logger(user, op.gate, h.data(), creg, n_bits);
                 }
@@ -570,6 +612,7 @@ struct ShotTree
     void settle(ShotNode& nd, kq_state* st)
     {
         const kq_op* m = run_to_measurement(st, nd.cursor, ops, preds, n_preds, nd.creg.data(), n_bits, nullptr, nullptr);
+        normalise_layout(st);   // branch states are kept and copied with qubit q on bit q
         if (m) {
             if (m->q0 < 0 || static_cast<uint32_t>(m->q0) >= n) {
                 throw InvalidArg {"measured qubit out of range"};
@@ -1033,7 +1076,9 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         }
         // forward sweep with checkpoints
         for (size_t j = 0; j + 1 < seg.size(); ++j) {
-            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(seg[j]), recs.begin() + static_cast<long>(seg[j + 1])));
+            // checkpoints are compared with the uncomputed state amplitude by amplitude: qubit q on bit q
+            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(seg[j]), recs.begin() + static_cast<long>(seg[j + 1])),
+                        false);
             if (j < bounds.size()) {
                 launch_copy(s, ckpt[j], s->amps, bytes);
             }
@@ -1246,6 +1291,8 @@ int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes)
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (ptr) { *ptr = s->amps; }
         if (bytes) { *bytes = s->n_amps * amp_bytes(s->precision); }
     });
@@ -1259,6 +1306,7 @@ int kq_state_upload(kq_state* s, const double* host)
             throw InvalidArg {"null host buffer"};
         }
         ensure_init();
+        s->bit_of.clear();   // overwritten: qubit q on bit q
         if (s->precision == KQ_C128) {
             KQ_HIP_CHECK(hipMemcpy(s->amps, host, s->n_amps * 16, hipMemcpyHostToDevice));
         }
@@ -1276,6 +1324,8 @@ int kq_state_download(const kq_state* s, double* host)
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (!host) {
             throw InvalidArg {"null host buffer"};
         }
@@ -1295,6 +1345,7 @@ int kq_state_set_basis(kq_state* s, uint64_t index)
             throw InvalidArg {"basis index out of range"};
         }
         ensure_init();
+        s->bit_of.clear();   // overwritten: qubit q on bit q
         launch_set_basis(s, s->is_dm ? index + index * N : index);
         KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
     });
@@ -1309,6 +1360,8 @@ int kq_state_copy(kq_state* dst, const kq_state* src)
             throw InvalidArg {"state copy: shapes differ"};
         }
         ensure_init();
+        normalise_layout(src);
+        dst->bit_of.clear();
         KQ_HIP_CHECK(hipStreamSynchronize(src->stream));
         launch_copy_kernel(dst, dst->amps, src->amps, dst->n_amps);
         KQ_HIP_CHECK(hipStreamSynchronize(dst->stream));
@@ -1377,6 +1430,8 @@ int kq_qubit_weights(kq_state* s, uint32_t qubit, double out[2])
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (qubit >= s->n) {
             throw InvalidArg {"qubit out of range"};
         }
@@ -1389,6 +1444,8 @@ int kq_collapse(kq_state* s, uint32_t qubit, int kept_bit, double scale)
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (qubit >= s->n || (kept_bit != 0 && kept_bit != 1)) {
             throw InvalidArg {"collapse: bad qubit or bit"};
         }
@@ -1403,6 +1460,8 @@ int kq_measure(kq_state* s, uint32_t qubit, int has_seed, int32_t seed, int* out
     return guarded([&] {
         check_state(s);
         ensure_init();
+        normalise_layout(s);
+        ensure_init();
         const int o = measure(s, qubit, has_seed != 0, seed);
         if (outcome) {
             *outcome = o;
@@ -1414,6 +1473,8 @@ int kq_probabilities(kq_state* s, const double* noise, double* out)
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (!out) {
             throw InvalidArg {"null output"};
         }
@@ -1438,6 +1499,8 @@ int kq_sample(kq_state* s, const double* noise, uint64_t n_shots, int has_seed, 
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (n_shots > 0 && !out_indices) {
             throw InvalidArg {"null output"};
         }
@@ -1468,6 +1531,7 @@ int kq_sample_circuit_shots(const kq_state* initial, uint32_t circuit_n_qubits, 
             throw InvalidArg {"null argument"};
         }
         ensure_init();
+        normalise_layout(initial);
         sample_circuit_shots(initial, circuit_n_qubits, ops, n_ops, preds, n_preds, n_bits, noise, n_shots, has_seed != 0,
                              seed, max_cached_bytes, out_indices, n_branches);
     });
@@ -1509,6 +1573,8 @@ int kq_pauli_expectation(kq_state* s, size_t n_strings, const uint32_t* term_off
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (s->is_dm) {
             throw InvalidArg {"Pauli expectation: statevector states only"};
         }
@@ -1529,6 +1595,8 @@ int kq_apply_pauli_string(kq_state* s, size_t n_terms, const uint32_t* term_qubi
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (s->is_dm) {
             throw InvalidArg {"Pauli string: statevector states only"};
         }
@@ -1544,6 +1612,8 @@ int kq_dm_trace(kq_state* s, double out[2])
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (!s->is_dm) {
             throw InvalidArg {"trace: density matrices only"};
         }
@@ -1556,6 +1626,8 @@ int kq_dm_partial_trace(kq_state* s, const uint32_t* qubits, size_t n, double* o
 {
     return guarded([&] {
         check_state(s);
+        ensure_init();
+        normalise_layout(s);
         if (!s->is_dm) {
             throw InvalidArg {"partial trace: density matrices only"};
         }
@@ -1583,6 +1655,8 @@ int kq_dm_from_statevector(kq_state* dm, const kq_state* sv)
             throw InvalidArg {"dm_from_statevector: shapes differ"};
         }
         ensure_init();
+        normalise_layout(sv);
+        dm->bit_of.clear();
         KQ_HIP_CHECK(hipStreamSynchronize(sv->stream));
         launch_dm_from_sv(dm, sv);
         KQ_HIP_CHECK(hipStreamSynchronize(dm->stream));
@@ -1600,6 +1674,7 @@ int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops, size_t n
             throw InvalidArg {"null output"};
         }
         ensure_init();
+        normalise_layout(state);
         last_stats() = PlanStats {};
         energy_gradient(state, ops, n_ops, n_strings, term_offsets, term_qubits, term_kinds, phases, coeffs, n_params,
                         energy, grad, checkpoint_every, drift_tol, n_restores);
@@ -1674,6 +1749,11 @@ int kq_stream_probe(uint32_t n_qubits, double gbps[4])
         ensure_init();
         stream_probe(n_qubits, gbps);
     });
+}
+
+int kq_set_fusion_relabel(int on)
+{
+    return guarded([&] { set_fusion_relabel(on != 0); });
 }
 
 int kq_set_fusion_low_qubits(int low_qubits)
@@ -1890,8 +1970,10 @@ int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op*
             }
         }
         else {
+            std::vector<uint8_t> bit_of;   // as run_records plans a statevector: qubits may move between bits
+            const bool relabel = fusion_relabel() && !is_density;
             const auto passes = plan_passes(recs, nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
-                                            fusion_max_stages());
+                                            fusion_max_stages(), true, relabel ? &bit_of : nullptr);
             for (const auto& p : passes) {
                 if (!p.sweep) {
                     TileProgram tp;

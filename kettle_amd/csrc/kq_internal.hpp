@@ -76,7 +76,9 @@ struct TilePass
     int k = 0;                   // tile qubits
     int c = 0;                   // low contiguous qubits 0..c-1 in the tile
     int rq = 0;                  // register bits per thread
-    std::vector<int> qubits;     // qubits[j] = global qubit of local bit j (ascending)
+    std::vector<int> qubits;     // qubits[j] = index bit of local bit j (ascending)
+    std::vector<int> out_qubits; // index bit local bit j is stored to (empty: qubits[j]); a permutation
+                                 // of `qubits` that moves qubits onto the low bits for the next pass
     std::vector<GateRec> gates;  // tile-encoded records
     std::vector<TileStage> stages;
     bool load_direct = false;    // load HBM straight into stage 0's layout
@@ -213,6 +215,9 @@ struct kq_state
     // per-state device buffer for adjoint partial sums
     void* aux = nullptr;
     size_t aux_bytes = 0;
+    // qubit q of the state is stored on index bit bit_of[q] (empty: bit q); only gate passes of a
+    // statevector leave it permuted, every other operation restores bit q first
+    std::vector<uint8_t> bit_of;
 };
 
 namespace kq
@@ -260,7 +265,13 @@ bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
 // allow_sweep: a pass holding a single diagonal gate may run as a plain sweep (off for adjoint plans)
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
-                                  int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true);
+                                  int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true,
+                                  std::vector<uint8_t>* bit_of = nullptr);
+// gate-free tile passes that store every qubit q back on index bit q
+std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, int max_k, int low_c, int rq);
+// whether statevector gate passes may move qubits onto other index bits (kq_set_fusion_relabel)
+bool fusion_relabel();
+void set_fusion_relabel(bool on);
 int fusion_max_stages();
 // qubits 0..c-1 always belong to a tile (contiguous 2^c-amplitude chunks per access)
 int fusion_low_qubits();

@@ -591,8 +591,14 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
     }
     if (tp.out_layout != cur) {
-        remap(tp.layouts[cur], tp.layouts[tp.out_layout]);
-        cur = tp.out_layout;
+        const TileLayout& A = tp.layouts[cur];
+        const TileLayout& B = tp.layouts[tp.out_layout];
+        const bool same_lds = std::equal(A.lds_slot, A.lds_slot + 16, B.lds_slot)
+                           && std::equal(A.lds_thr, A.lds_thr + 8, B.lds_thr);
+        if (!same_lds) {
+            remap(A, B);
+        }
+        cur = tp.out_layout;   // same registers, other HBM offsets: the store moves qubits to other bits
     }
     io(tp.layouts[cur], false);
     o << "  }\n}\n";

@@ -154,6 +154,14 @@ int g_low_qubits = 4;
 
 int fusion_low_qubits() { return g_low_qubits; }
 
+namespace
+{
+bool g_relabel = true;
+}
+
+bool fusion_relabel() { return g_relabel; }
+void set_fusion_relabel(bool on) { g_relabel = on; }
+
 void set_fusion_low_qubits(int c)
 {
     if (c < 0 || c > 12) {
@@ -282,76 +290,226 @@ uint32_t stage_regs(const TileStage& st, int rq)
 
 }  // namespace
 
-// Greedy pass builder.  A pass owns a tile qubit set Q (|Q| <= k, always containing the low
-// qubits 0..low_c-1).  Gates are taken in circuit order; a gate joins the current pass when
+namespace
+{
+
+// One greedy pass over `remaining` (records in index bits): which gates join, their stages, the
+// deferred rest and the tile's index bits.
+struct GreedyPass
+{
+    std::vector<size_t> taken;
+    std::vector<int> taken_stage;
+    std::vector<uint64_t> stage_targets {0};
+    std::vector<size_t> deferred;
+    uint64_t q_mask = 0;
+};
+
+GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size_t>& remaining, uint64_t low_mask, int k,
+                       int rq, int max_gates, int max_stages)
+{
+    GreedyPass gp;
+    gp.q_mask = low_mask;
+    int q_count = std::popcount(low_mask);
+    uint64_t d_any = 0, d_nd = 0;
+    double cost = 0.0;
+    const double max_cost = fusion_max_cost();
+    for (const size_t r : remaining) {
+        const GateRec& g = gates[r];
+        const uint64_t tbit = uint64_t {1} << g.target;
+        const uint64_t nd = is_diag(g) ? 0 : tbit;
+        const uint64_t all = tbit | (g.control >= 0 ? (uint64_t {1} << g.control) : 0);
+        bool ok = static_cast<int>(gp.taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd)
+               && (gp.taken.empty() || cost + gate_cost(g) <= max_cost);
+        uint64_t need = 0;
+        bool new_stage = false;
+        if (ok && nd) {
+            need = nd & ~gp.q_mask;
+            if (q_count + std::popcount(need) > k) {
+                ok = false;
+            }
+            else if (!(gp.stage_targets.back() & nd) && std::popcount(gp.stage_targets.back()) >= rq) {
+                if (static_cast<int>(gp.stage_targets.size()) >= max_stages) {
+                    ok = false;
+                }
+                else {
+                    new_stage = true;
+                }
+            }
+        }
+        if (ok) {
+            cost += gate_cost(g);
+            gp.q_mask |= need;
+            q_count += std::popcount(need);
+            if (new_stage) {
+                gp.stage_targets.push_back(0);
+            }
+            gp.stage_targets.back() |= nd;
+            gp.taken.push_back(r);
+            gp.taken_stage.push_back(static_cast<int>(gp.stage_targets.size()) - 1);
+        }
+        else {
+            gp.deferred.push_back(r);
+            d_any |= all;
+            d_nd |= nd;
+        }
+    }
+    return gp;
+}
+
+// the record with its qubits replaced by the index bits that hold them
+GateRec at_bits(const GateRec& g, const std::vector<uint8_t>& bit_of)
+{
+    GateRec t = g;
+    t.target = bit_of[g.target];
+    if (g.control >= 0) {
+        t.control = bit_of[static_cast<size_t>(g.control)];
+    }
+    return t;
+}
+
+// After a pass with tile index bits q_mask: which qubits of the tile should sit on index bits
+// 0..c-1 for the next pass.  Candidates are the tile's qubits in the order the deferred gates
+// first touch them; every c-subset of the first c + 3 is scored by how many gates a greedy next
+// pass would take, and the current placement wins ties.  Returns the new qubit -> bit map.
+std::vector<uint8_t> choose_low_qubits(const std::vector<GateRec>& gates, const std::vector<size_t>& deferred,
+                                       const std::vector<uint8_t>& bit_of, uint64_t q_mask, int c, int k, int rq,
+                                       int max_gates, int max_stages)
+{
+    const size_t nvec = bit_of.size();
+    std::vector<int> qubit_at(nvec);
+    for (size_t q = 0; q < nvec; ++q) {
+        qubit_at[bit_of[q]] = static_cast<int>(q);
+    }
+    const auto in_tile = [&](int q) { return ((q_mask >> bit_of[static_cast<size_t>(q)]) & 1ull) != 0; };
+    std::vector<int> order;
+    const auto add = [&](int q) {
+        if (q >= 0 && in_tile(q) && std::find(order.begin(), order.end(), q) == order.end()) {
+            order.push_back(q);
+        }
+    };
+    for (const size_t r : deferred) {
+        add(static_cast<int>(gates[r].target));
+        add(gates[r].control);
+        if (static_cast<int>(order.size()) >= c + 3) {
+            break;
+        }
+    }
+    for (uint32_t b = 0; b < nvec && static_cast<int>(order.size()) < c + 3; ++b) {
+        if ((q_mask >> b) & 1ull) {
+            add(qubit_at[b]);
+        }
+    }
+    const int m = static_cast<int>(order.size());
+    const uint64_t low_mask = (uint64_t {1} << c) - 1;
+    // a placement: the chosen qubits take bits 0..c-1; the qubits they displace take the bits
+    // the chosen ones leave (all within the tile)
+    const auto place = [&](const std::vector<int>& chosen) {
+        std::vector<uint8_t> nb = bit_of;
+        std::vector<int> incoming, outgoing;
+        for (const int q : chosen) {
+            if (bit_of[static_cast<size_t>(q)] >= c) {
+                incoming.push_back(q);
+            }
+        }
+        for (int b = 0; b < c; ++b) {
+            const int q = qubit_at[static_cast<size_t>(b)];
+            if (std::find(chosen.begin(), chosen.end(), q) == chosen.end()) {
+                outgoing.push_back(q);
+            }
+        }
+        for (size_t i = 0; i < incoming.size(); ++i) {
+            std::swap(nb[static_cast<size_t>(incoming[i])], nb[static_cast<size_t>(outgoing[i])]);
+        }
+        return nb;
+    };
+    std::vector<GateRec> trial(gates.size());
+    const auto score = [&](const std::vector<uint8_t>& nb) {
+        for (const size_t r : deferred) {
+            trial[r] = at_bits(gates[r], nb);
+        }
+        return greedy_pass(trial, deferred, low_mask, k, rq, max_gates, max_stages).taken.size();
+    };
+    std::vector<uint8_t> best = bit_of;
+    size_t best_score = score(bit_of);
+    if (m < c) {
+        return best;
+    }
+    std::vector<int> idx(static_cast<size_t>(c));
+    for (int i = 0; i < c; ++i) {
+        idx[static_cast<size_t>(i)] = i;
+    }
+    while (true) {
+        std::vector<int> chosen;
+        for (const int i : idx) {
+            chosen.push_back(order[static_cast<size_t>(i)]);
+        }
+        const auto nb = place(chosen);
+        const size_t sc = score(nb);
+        if (sc > best_score) {
+            best_score = sc;
+            best = nb;
+        }
+        int i = c - 1;
+        while (i >= 0 && idx[static_cast<size_t>(i)] == m - c + i) {
+            --i;
+        }
+        if (i < 0) {
+            break;
+        }
+        ++idx[static_cast<size_t>(i)];
+        for (int j = i + 1; j < c; ++j) {
+            idx[static_cast<size_t>(j)] = idx[static_cast<size_t>(j - 1)] + 1;
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+// Greedy pass builder.  A pass owns a tile set Q of index bits (|Q| <= k, always containing the
+// low bits 0..c-1).  Gates are taken in circuit order; a gate joins the current pass when
 //   - its non-diagonal target is in Q or Q still has room for it,
 //   - it commutes with every gate deferred so far (it shares no qubit with a deferred
 //     non-diagonal target, and its own non-diagonal target touches no deferred gate), and
 //   - its target fits the current register stage (at most rq distinct non-diagonal targets per
 //     stage) or a new stage may still be opened (at most max_stages per pass).
 // Diagonal gates and controls never need to be in Q: their effect depends only on index bits.
-std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k, int low_c, int max_gates,
-                                  int rq_max, int max_stages, bool allow_sweep)
+// With `bit_of` (qubit -> index bit holding it, updated on return) a pass may also move qubits of
+// its tile onto bits 0..c-1 as it stores the tile, chosen for the next pass (`out_qubits`); without
+// it qubit q is index bit q throughout.
+std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t nvec, int max_k, int low_c, int max_gates,
+                                  int rq_max, int max_stages, bool allow_sweep, std::vector<uint8_t>* bit_of_io)
 {
     std::vector<TilePass> passes;
     const int k = std::min<int>(max_k, static_cast<int>(nvec));
     const int rq = std::min(rq_max, k);
     // a tile covering every qubit needs no free slots; otherwise keep at least one slot free
     const int c = k == static_cast<int>(nvec) ? k : std::max(0, std::min(low_c, k - 1));
-    std::vector<size_t> remaining(gates.size());
-    for (size_t i = 0; i < gates.size(); ++i) {
+    std::vector<uint8_t> bit_of(nvec);
+    for (uint32_t q = 0; q < nvec; ++q) {
+        bit_of[q] = static_cast<uint8_t>(q);
+    }
+    if (bit_of_io && bit_of_io->size() == nvec) {
+        bit_of = *bit_of_io;
+    }
+    const bool relabel = bit_of_io != nullptr && k < static_cast<int>(nvec) && c > 0;
+    std::vector<GateRec> gates(gates_in.size());
+    std::vector<size_t> remaining(gates_in.size());
+    for (size_t i = 0; i < gates_in.size(); ++i) {
         remaining[i] = i;
     }
     while (!remaining.empty()) {
-        uint64_t q_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
-        int q_count = c;
-        std::vector<size_t> taken;
-        std::vector<int> taken_stage;
-        std::vector<uint64_t> stage_targets {0};
-        std::vector<size_t> deferred;
-        uint64_t d_any = 0, d_nd = 0;
-        double cost = 0.0;
-        const double max_cost = fusion_max_cost();
         for (const size_t r : remaining) {
-            const GateRec& g = gates[r];
-            const uint64_t tbit = uint64_t {1} << g.target;
-            const uint64_t nd = is_diag(g) ? 0 : tbit;
-            const uint64_t all = tbit | (g.control >= 0 ? (uint64_t {1} << g.control) : 0);
-            bool ok = static_cast<int>(taken.size()) < max_gates && !(nd & d_any) && !(all & d_nd)
-                   && (taken.empty() || cost + gate_cost(g) <= max_cost);
-            uint64_t need = 0;
-            bool new_stage = false;
-            if (ok && nd) {
-                need = nd & ~q_mask;
-                if (q_count + std::popcount(need) > k) {
-                    ok = false;
-                }
-                else if (!(stage_targets.back() & nd) && std::popcount(stage_targets.back()) >= rq) {
-                    if (static_cast<int>(stage_targets.size()) >= max_stages) {
-                        ok = false;
-                    }
-                    else {
-                        new_stage = true;
-                    }
-                }
-            }
-            if (ok) {
-                cost += gate_cost(g);
-                q_mask |= need;
-                q_count += std::popcount(need);
-                if (new_stage) {
-                    stage_targets.push_back(0);
-                }
-                stage_targets.back() |= nd;
-                taken.push_back(r);
-                taken_stage.push_back(static_cast<int>(stage_targets.size()) - 1);
-            }
-            else {
-                deferred.push_back(r);
-                d_any |= all;
-                d_nd |= nd;
-            }
+            gates[r] = at_bits(gates_in[r], bit_of);
         }
+        const uint64_t low_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
+        GreedyPass gp = greedy_pass(gates, remaining, low_mask, k, rq, max_gates, max_stages);
+        uint64_t q_mask = gp.q_mask;
+        int q_count = std::popcount(q_mask);
+        std::vector<size_t>& taken = gp.taken;
+        std::vector<int>& taken_stage = gp.taken_stage;
+        std::vector<uint64_t>& stage_targets = gp.stage_targets;
+        std::vector<size_t>& deferred = gp.deferred;
         if (taken.empty()) {
             throw InvalidArg {"fusion plan: no gate fits a tile"};
         }
@@ -446,8 +604,139 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nv
             }
             complete_stage(p.io_layout, top, 0, k, rq);
         }
+        if (relabel && !deferred.empty()) {
+            const auto nb = choose_low_qubits(gates_in, deferred, bit_of, q_mask, c, k, rq, max_gates, max_stages);
+            if (nb != bit_of) {
+                // local bit j holds the qubit on index bit p.qubits[j]; after the store it is on nb[qubit]
+                std::vector<int> qubit_at(nvec);
+                for (uint32_t q = 0; q < nvec; ++q) {
+                    qubit_at[bit_of[q]] = static_cast<int>(q);
+                }
+                p.out_qubits.resize(p.qubits.size());
+                for (size_t j = 0; j < p.qubits.size(); ++j) {
+                    p.out_qubits[j] = nb[static_cast<size_t>(qubit_at[static_cast<size_t>(p.qubits[j])])];
+                }
+                bit_of = nb;
+            }
+        }
         passes.push_back(std::move(p));
         remaining = std::move(deferred);
+    }
+    if (bit_of_io) {
+        *bit_of_io = bit_of;
+    }
+    return passes;
+}
+
+// Gate-free tile passes that bring every qubit q back to index bit q.  A pass may permute the
+// qubits held by its tile's bits freely: the tile takes the low bits (for coalescing) and follows the
+// displacement cycles from there, so each pass sends home every qubit whose bit and home bit it
+// holds; the rest of its qubits take its remaining bits.
+std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, int max_k, int low_c, int rq_max)
+{
+    std::vector<TilePass> passes;
+    const int k = std::min<int>(max_k, static_cast<int>(nvec));
+    const int rq = std::min(rq_max, k);
+    const int c = k == static_cast<int>(nvec) ? k : std::max(0, std::min(low_c, k - 1));
+    const auto displaced = [&] {
+        for (uint32_t q = 0; q < nvec; ++q) {
+            if (bit_of[q] != q) {
+                return true;
+            }
+        }
+        return false;
+    };
+    for (int guard = 0; displaced(); ++guard) {
+        if (guard > 4 * static_cast<int>(nvec)) {
+            throw InvalidArg {"restore plan: no progress"};
+        }
+        std::vector<int> qubit_at(nvec);
+        for (uint32_t q = 0; q < nvec; ++q) {
+            qubit_at[bit_of[q]] = static_cast<int>(q);
+        }
+        uint64_t t_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
+        const auto room = [&] { return std::popcount(t_mask) < k; };
+        // follow the cycles through the tile's bits: the home bit of the qubit a bit holds, and the
+        // bit holding the qubit whose home a bit is
+        for (bool grew = true; grew && room();) {
+            grew = false;
+            for (uint32_t b = 0; b < nvec && room(); ++b) {
+                if (!((t_mask >> b) & 1ull) || qubit_at[b] == static_cast<int>(b)) {
+                    continue;
+                }
+                const uint32_t home = static_cast<uint32_t>(qubit_at[b]);
+                if (!((t_mask >> home) & 1ull)) {
+                    t_mask |= uint64_t {1} << home;
+                    grew = true;
+                }
+                const uint32_t holder = bit_of[b];
+                if (room() && !((t_mask >> holder) & 1ull)) {
+                    t_mask |= uint64_t {1} << holder;
+                    grew = true;
+                }
+            }
+        }
+        for (uint32_t b = 0; b < nvec && room(); ++b) {
+            if (bit_of[b] != b && !((t_mask >> bit_of[b]) & 1ull)) {
+                t_mask |= uint64_t {1} << bit_of[b];   // another displaced qubit
+            }
+        }
+        for (uint32_t b = 0; b < nvec && room(); ++b) {
+            t_mask |= uint64_t {1} << b;
+        }
+        // the permutation within the tile
+        std::vector<uint8_t> nb = bit_of;
+        std::vector<int> stay_free;
+        uint64_t taken = 0;
+        for (uint32_t b = 0; b < nvec; ++b) {
+            if ((t_mask >> b) & 1ull) {
+                const int q = qubit_at[b];
+                if ((t_mask >> q) & 1ull) {
+                    nb[static_cast<size_t>(q)] = static_cast<uint8_t>(q);
+                    taken |= uint64_t {1} << q;
+                }
+            }
+        }
+        std::vector<uint32_t> free_bits;
+        for (uint32_t b = 0; b < nvec; ++b) {
+            if (((t_mask >> b) & 1ull) && !((taken >> b) & 1ull)) {
+                free_bits.push_back(b);
+            }
+        }
+        size_t fi = 0;
+        for (uint32_t b = 0; b < nvec; ++b) {
+            if ((t_mask >> b) & 1ull) {
+                const int q = qubit_at[b];
+                if (!((t_mask >> q) & 1ull)) {
+                    nb[static_cast<size_t>(q)] = static_cast<uint8_t>(free_bits[fi++]);
+                }
+            }
+        }
+        if (nb == bit_of) {
+            throw InvalidArg {"restore plan: no progress"};
+        }
+        TilePass p;
+        p.k = k;
+        p.rq = rq;
+        for (uint32_t b = 0; b < nvec; ++b) {
+            if ((t_mask >> b) & 1ull) {
+                p.qubits.push_back(static_cast<int>(b));
+                p.out_qubits.push_back(nb[static_cast<size_t>(qubit_at[b])]);
+            }
+        }
+        p.c = std::min(std::countr_one(t_mask), k);
+        if (p.k - p.c > 10) {
+            throw InvalidArg {"restore plan: tile has more than 10 non-contiguous qubits"};
+        }
+        p.stages.resize(1);
+        complete_stage(p.stages[0], 0, 0, k, rq);
+        p.stages[0].g_begin = 0;
+        p.stages[0].g_count = 0;
+        p.load_direct = true;
+        p.store_direct = true;
+        p.io_layout = p.stages[0];
+        passes.push_back(std::move(p));
+        bit_of = nb;
     }
     return passes;
 }
@@ -457,14 +746,15 @@ namespace
 
 uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
-TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab)
+TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab, const std::vector<int>* bits = nullptr)
 {
     TileLayout L {};
+    const std::vector<int>& q_of = bits ? *bits : p.qubits;
     const auto hbm = [&](uint32_t local) {
         uint64_t off = 0;
         for (int j = 0; j < p.k; ++j) {
             if ((local >> j) & 1u) {
-                off |= uint64_t {1} << p.qubits[static_cast<size_t>(j)];
+                off |= uint64_t {1} << q_of[static_cast<size_t>(j)];
             }
         }
         return off;
@@ -744,7 +1034,18 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
         step.layout = cur_idx;
         out.steps.push_back(step);
     }
-    if (p.store_direct || same_regs(p.io_layout, *cur, p.rq)) {
+    const bool store_cur = p.store_direct || same_regs(p.io_layout, *cur, p.rq);
+    if (!p.out_qubits.empty() && p.out_qubits != p.qubits) {
+        // the store moves qubits to other index bits of the tile: the store layout's HBM offsets
+        // follow out_qubits (its LDS offsets are the current layout's when no remap is needed)
+        if (p.out_qubits.size() != p.qubits.size()
+            || !std::is_permutation(p.out_qubits.begin(), p.out_qubits.end(), p.qubits.begin())) {
+            throw InvalidArg {"tile lowering: store bits are not a permutation of the tile bits"};
+        }
+        out.layouts.push_back(make_layout(store_cur ? *cur : p.io_layout, p, ab, &p.out_qubits));
+        out.out_layout = static_cast<uint32_t>(out.layouts.size() - 1);
+    }
+    else if (store_cur) {
         out.out_layout = cur_idx;
     }
     else {

@@ -358,6 +358,9 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
         }
         const TileGate* __restrict__ gp = gates + st.g_begin;
         const uint32_t g_count = st.g_count;
+        if (g_count == 0) {
+            continue;   // a gate-free step (a pass that only moves qubits to other bits)
+        }
         TileGate next = gp[0];
         for (uint32_t gi = 0; gi < g_count; ++gi) {
             // the record is copied into scalar registers once; the next one is fetched meanwhile
