@@ -285,10 +285,23 @@ int kq_debug_pair_map_2q(uint32_t n_qubits, uint32_t control, uint32_t target, i
  * units of kq_set_fusion's max_cost_per_pass.  order and pass_of hold one entry per record,
  * pass_kind and pass_cost at least as many; lds_bytes is the per-workgroup LDS the tile is
  * fitted to (0 = the value of the initialised device, 64 KiB before kq_init). */
+/* Host-only check of the planner's qubit moves: applies a straight-line gate list (<= 24 qubits) to
+ * `amps` (interleaved complex doubles, in place) pass by pass as planned with moves, each gate on the
+ * index bits holding its qubits at that point, moves the amplitudes as each pass's store would, then
+ * applies the restore passes; the result must equal the plain simulation. */
+int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, double* amps, size_t* n_passes,
+                           size_t* n_restore_passes);
 /* Host-only: generate and compile (without loading) the run-time pass kernels of a straight-line
  * gate list; fails with the compiler log if any does not compile.  No GPU needed. */
 int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
                          size_t* n_kernels, size_t* failures);
+/* Host-only: pass pass_index of a straight-line gate list as a statevector runs it (qubit moves and
+ * restore passes included).  info[0] = 1 for a sweep of one gate (coefs: its 8 matrix values,
+ * info[3] = target bit, info[4] = control bit + 1 or 0), else 0 for a tile pass with its generated
+ * kernel source, info[1] = workgroups, info[2] = threads per workgroup, coefs: its coefficient buffer. */
+int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes, size_t pass_index,
+                  char* src, size_t src_cap, size_t* src_len, double* coefs, size_t coef_cap, size_t* n_coefs,
+                  uint64_t info[5], size_t* n_passes);
 /* Host-only: the generated source of tile pass pass_index (NUL-terminated, truncated to cap - 1);
  * *len is the full length and *n_kernels the number of tile passes. */
 int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,

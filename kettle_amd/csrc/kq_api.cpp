@@ -1751,6 +1751,101 @@ int kq_stream_probe(uint32_t n_qubits, double gbps[4])
     });
 }
 
+int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, double* amps, size_t* n_passes,
+                           size_t* n_restore_passes)
+{
+    return guarded([&] {
+        if (n_qubits < 1 || n_qubits > 24 || (!ops && n_ops) || !amps) {
+            throw InvalidArg {"debug relabel: bad arguments"};
+        }
+        std::vector<GateRec> recs;
+        for (size_t i = 0; i < n_ops; ++i) {
+            if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+                throw InvalidArg {"debug relabel: op " + std::to_string(i) + " is not a unitary gate"};
+            }
+            compile_gate(ops[i], n_qubits, false, recs);
+        }
+        const int max_k = fusion_max_k_for(KQ_C128, n_qubits, device_lds_per_block());
+        std::vector<uint8_t> bit_of(n_qubits);
+        for (uint32_t q = 0; q < n_qubits; ++q) {
+            bit_of[q] = static_cast<uint8_t>(q);
+        }
+        std::vector<uint8_t> planned = bit_of;
+        const auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(KQ_C128),
+                                        fusion_max_stages(), true, &planned);
+        const size_t N = size_t {1} << n_qubits;
+        auto* z = reinterpret_cast<std::complex<double>*>(amps);
+        // the passes' gates in execution order on the host, each on the index bits holding its qubits
+        const auto apply = [&](const GateRec& g) {
+            const std::complex<double> m00 {g.m[0], g.m[1]}, m01 {g.m[2], g.m[3]}, m10 {g.m[4], g.m[5]}, m11 {g.m[6], g.m[7]};
+            const size_t tb = size_t {1} << g.target;
+            for (size_t i = 0; i < N; ++i) {
+                if ((i & tb) || (g.control >= 0 && !((i >> g.control) & 1u))) {
+                    continue;
+                }
+                const auto a = z[i], b = z[i | tb];
+                z[i] = m00 * a + m01 * b;
+                z[i | tb] = m10 * a + m11 * b;
+            }
+        };
+        const auto move = [&](const TilePass& p) {
+            if (p.out_qubits.empty()) {
+                return;
+            }
+            std::vector<std::complex<double>> out(N);
+            for (size_t i = 0; i < N; ++i) {
+                size_t j = i;
+                for (size_t l = 0; l < p.qubits.size(); ++l) {
+                    j &= ~(size_t {1} << p.out_qubits[l]);
+                }
+                for (size_t l = 0; l < p.qubits.size(); ++l) {
+                    j |= ((i >> p.qubits[l]) & 1u) << p.out_qubits[l];
+                }
+                out[j] = z[i];
+            }
+            std::copy(out.begin(), out.end(), z);
+            std::vector<uint8_t> nb = bit_of;
+            for (uint32_t q = 0; q < n_qubits; ++q) {
+                for (size_t l = 0; l < p.qubits.size(); ++l) {
+                    if (bit_of[q] == p.qubits[l]) {
+                        nb[q] = static_cast<uint8_t>(p.out_qubits[l]);
+                    }
+                }
+            }
+            bit_of = nb;
+        };
+        for (const auto& p : passes) {
+            for (const uint32_t r : p.src) {
+                GateRec g = recs[r];
+                g.target = bit_of[g.target];
+                if (g.control >= 0) {
+                    g.control = bit_of[static_cast<size_t>(g.control)];
+                }
+                apply(g);
+            }
+            move(p);
+        }
+        if (bit_of != planned) {
+            throw InvalidArg {"debug relabel: replayed qubit placement differs from the planner's"};
+        }
+        const auto restore = plan_restore(bit_of, n_qubits, max_k, fusion_low_qubits(), register_qubits(KQ_C128));
+        for (const auto& p : restore) {
+            move(p);
+        }
+        for (uint32_t q = 0; q < n_qubits; ++q) {
+            if (bit_of[q] != q) {
+                throw InvalidArg {"debug relabel: restore left a qubit off its bit"};
+            }
+        }
+        if (n_passes) {
+            *n_passes = passes.size();
+        }
+        if (n_restore_passes) {
+            *n_restore_passes = restore.size();
+        }
+    });
+}
+
 int kq_set_fusion_relabel(int on)
 {
     return guarded([&] { set_fusion_relabel(on != 0); });
@@ -1797,8 +1892,19 @@ namespace
 {
 
 // run-time pass kernel sources of a straight-line gate list (host only)
-std::vector<std::string> debug_jit_sources(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops,
-                                           size_t lds_bytes)
+// One pass of a straight-line gate list as run_records plans it for a statevector (qubit moves and
+// the final restore passes included): a generated tile kernel with its coefficients, or a sweep gate.
+struct DebugPass
+{
+    bool sweep = false;
+    std::string src;
+    std::vector<double> coefs;
+    uint32_t blocks = 0;
+    uint32_t threads = 0;
+    GateRec gate {};
+};
+
+std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes)
 {
     if (n_qubits < 4 || n_qubits > 40 || (precision != KQ_C128 && precision != KQ_C64) || (!ops && n_ops)) {
         throw InvalidArg {"debug jit: bad arguments"};
@@ -1814,17 +1920,45 @@ std::vector<std::string> debug_jit_sources(uint32_t n_qubits, int precision, con
     if (max_k < register_qubits(precision)) {
         throw InvalidArg {"debug jit: state too small for a tile"};
     }
-    const auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
-                                    fusion_max_stages());
-    std::vector<std::string> srcs;
-    for (const auto& p : passes) {
-        if (p.sweep) {
-            continue;
+    std::vector<uint8_t> bit_of;
+    auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
+                              fusion_max_stages(), true, fusion_relabel() ? &bit_of : nullptr);
+    bool moved = false;
+    for (size_t q = 0; q < bit_of.size(); ++q) {
+        moved = moved || bit_of[q] != q;
+    }
+    if (moved) {
+        for (auto& p : plan_restore(bit_of, n_qubits, max_k, fusion_low_qubits(), register_qubits(precision))) {
+            passes.push_back(std::move(p));
         }
-        TileProgram tp;
-        lower_tile_pass(p, precision, n_qubits, tp);
-        std::vector<double> coefs;
-        srcs.push_back(jit_source(tp, precision, coefs));
+    }
+    std::vector<DebugPass> out;
+    for (const auto& p : passes) {
+        DebugPass d;
+        if (p.sweep) {
+            d.sweep = true;
+            d.gate = p.single;
+        }
+        else {
+            TileProgram tp;
+            lower_tile_pass(p, precision, n_qubits, tp);
+            d.src = jit_source(tp, precision, d.coefs);
+            d.blocks = tile_pass_blocks(tp);
+            d.threads = 1u << tp.n_thr_bits;
+        }
+        out.push_back(std::move(d));
+    }
+    return out;
+}
+
+std::vector<std::string> debug_jit_sources(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops,
+                                           size_t lds_bytes)
+{
+    std::vector<std::string> srcs;
+    for (auto& d : debug_passes(n_qubits, precision, ops, n_ops, lds_bytes)) {
+        if (!d.sweep) {
+            srcs.push_back(std::move(d.src));
+        }
     }
     return srcs;
 }
@@ -1868,6 +2002,39 @@ int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size
             std::memcpy(buf, src.data(), n);
             buf[n] = '\0';
         }
+    });
+}
+
+int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes, size_t pass_index,
+                  char* src, size_t src_cap, size_t* src_len, double* coefs, size_t coef_cap, size_t* n_coefs,
+                  uint64_t info[5], size_t* n_passes)
+{
+    return guarded([&] {
+        if (!src_len || !n_coefs || !info || !n_passes) {
+            throw InvalidArg {"debug pass: null output"};
+        }
+        const auto passes = debug_passes(n_qubits, precision, ops, n_ops, lds_bytes);
+        *n_passes = passes.size();
+        if (pass_index >= passes.size()) {
+            throw InvalidArg {"debug pass: pass index out of range"};
+        }
+        const DebugPass& d = passes[pass_index];
+        *src_len = d.src.size();
+        *n_coefs = d.sweep ? 8 : d.coefs.size();
+        if (src && src_cap) {
+            const size_t n = std::min(src_cap - 1, d.src.size());
+            std::memcpy(src, d.src.data(), n);
+            src[n] = '\0';
+        }
+        if (coefs) {
+            const double* from = d.sweep ? d.gate.m : d.coefs.data();
+            std::memcpy(coefs, from, std::min(coef_cap, *n_coefs) * sizeof(double));
+        }
+        info[0] = d.sweep ? 1 : 0;
+        info[1] = d.blocks;
+        info[2] = d.threads;
+        info[3] = d.sweep ? d.gate.target : 0;
+        info[4] = d.sweep ? static_cast<uint64_t>(d.gate.control + 1) : 0;
     });
 }
 

@@ -598,7 +598,13 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         if (!same_lds) {
             remap(A, B);
         }
-        cur = tp.out_layout;   // same registers, other HBM offsets: the store moves qubits to other bits
+        else {
+            // Same registers, other HBM offsets: the store moves qubits to other bits of the tile, so
+            // a thread writes amplitudes other threads of the workgroup read.  Every load of the
+            // workgroup must have completed first: a full barrier (it waits for outstanding loads).
+            o << "  __syncthreads();\n";
+        }
+        cur = tp.out_layout;
     }
     io(tp.layouts[cur], false);
     o << "  }\n}\n";

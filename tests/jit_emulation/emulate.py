@@ -1,0 +1,178 @@
+"""Host execution of the run-time generated tile kernels (test infrastructure).
+
+libkq's `kq_debug_pass` returns, pass by pass, exactly what a statevector would run: the generated
+kernel source with its coefficient buffer, or a single sweep gate.  This module compiles those kernel
+sources with g++ against a small host runtime (one OS thread per GPU thread, a std::barrier for the
+workgroup barrier, the tile array shared by the workgroup's threads) and runs them on a host state, so
+the generated code itself is checked against the oracle without a GPU."""
+from __future__ import annotations
+
+import ctypes
+import hashlib
+import re
+import subprocess
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+import kettle_amd as ka
+from kettle_amd import _lib
+
+PRELUDE = r"""
+#include <cstdint>
+#include <cmath>
+#include <barrier>
+struct double2 { double x, y; };
+struct float2 { float x, y; };
+struct EmuDim { unsigned long long x; };
+extern thread_local EmuDim threadIdx;
+extern EmuDim blockIdx;
+extern EmuDim gridDim;
+extern std::barrier<>* emu_barrier;
+#define __device__
+#define __forceinline__ inline
+#define __global__
+#define __launch_bounds__(...)
+#define __shared__ static
+inline void __syncthreads() { emu_barrier->arrive_and_wait(); }
+"""
+
+RUNTIME = r"""
+#include <barrier>
+#include <cstdint>
+#include <thread>
+#include <vector>
+struct EmuDim { unsigned long long x; };
+thread_local EmuDim threadIdx;
+EmuDim blockIdx;
+EmuDim gridDim;
+std::barrier<>* emu_barrier = nullptr;
+extern "C" void emu_launch(void (*fn)(void*, const double*, uint32_t), void* a, const double* cf, uint32_t blocks,
+                           uint32_t threads)
+{
+    gridDim.x = blocks;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        blockIdx.x = b;
+        std::barrier<> bar(threads);
+        emu_barrier = &bar;
+        std::vector<std::thread> pool;
+        for (uint32_t t = 0; t < threads; ++t) {
+            pool.emplace_back([=] { threadIdx.x = t; fn(a, cf, blocks); });
+        }
+        for (auto& th : pool) {
+            th.join();
+        }
+    }
+}
+"""
+
+
+def host_source(src: str, name: str) -> str:
+    """The generated kernel as host C++: fixed-width typedefs from <cstdint>, the workgroup barrier
+    as the runtime's std::barrier, no GPU attributes."""
+    body = "\n".join(line for line in src.split("\n") if not line.startswith("typedef "))
+    body = re.sub(r"__device__ __forceinline__ void lds_barrier\(\)\s*\{[^}]*\}",
+                  "inline void lds_barrier() { emu_barrier->arrive_and_wait(); }", body)
+    body = body.replace("__attribute__((always_inline))", "")
+    body = re.sub(r"__attribute__\(\(amdgpu_waves_per_eu\([^)]*\)\)\)", "", body)
+    body = body.replace("void kq_pass(", f"void {name}(")
+    body = body.replace("#pragma once", "")
+    return PRELUDE + body
+
+
+def _passes(ops: np.ndarray, n: int, precision: int):
+    c = ctypes
+    lib = _lib.load()
+    out = []
+    i = 0
+    while True:
+        ln, nc, npas = c.c_size_t(), c.c_size_t(), c.c_size_t()
+        info = np.zeros(5, dtype=np.uint64)
+        args = (n, precision, _lib.ptr(ops), len(ops), 160 * 1024, i)
+        _lib.check(lib.kq_debug_pass(*args, None, 0, c.byref(ln), None, 0, c.byref(nc), _lib.ptr(info), c.byref(npas)))
+        buf = c.create_string_buffer(ln.value + 1)
+        coefs = np.zeros(max(nc.value, 1), dtype=np.float64)
+        _lib.check(lib.kq_debug_pass(*args, buf, ln.value + 1, c.byref(ln), _lib.ptr(coefs), coefs.size, c.byref(nc),
+                                     _lib.ptr(info), c.byref(npas)))
+        out.append({"sweep": bool(info[0]), "src": buf.value.decode(), "coefs": coefs[:nc.value].copy(),
+                    "blocks": int(info[1]), "threads": int(info[2]), "target": int(info[3]), "control": int(info[4]) - 1})
+        i += 1
+        if i >= npas.value:
+            return out
+
+
+_BUILD = Path(tempfile.gettempdir()) / "kq_jit_emulation"
+
+
+def _compile(sources: list[tuple[str, str]]) -> ctypes.CDLL:
+    """sources: (function name, host source); one shared library (cached by content)."""
+    _BUILD.mkdir(exist_ok=True)
+    key = hashlib.sha1("".join(s for _, s in sources).encode()).hexdigest()[:16]
+    lib_path = _BUILD / f"emu_{key}.so"
+    if not lib_path.exists():
+        files = []
+        rt = _BUILD / "runtime.cpp"
+        rt.write_text(RUNTIME)
+        files.append(rt)
+        for name, src in sources:
+            f = _BUILD / f"{key}_{name}.cpp"
+            f.write_text(src)
+            files.append(f)
+
+        def cc(f):
+            o = f.with_suffix(".o")
+            r = subprocess.run(["g++", "-std=c++20", "-O1", "-fPIC", "-w", "-c", str(f), "-o", str(o)],
+                               capture_output=True, text=True)
+            if r.returncode:
+                raise RuntimeError(f"host build of {f.name} failed:\n{r.stderr[:4000]}")
+            return o
+
+        with ThreadPoolExecutor(8) as ex:
+            objs = list(ex.map(cc, files))
+        r = subprocess.run(["g++", "-shared", "-o", str(lib_path), *map(str, objs), "-lpthread"], capture_output=True,
+                           text=True)
+        if r.returncode:
+            raise RuntimeError(r.stderr[:4000])
+    return ctypes.CDLL(str(lib_path))
+
+
+def _sweep(psi: np.ndarray, m: np.ndarray, target: int, control: int) -> None:
+    u = m[0::2] + 1j * m[1::2]
+    idx = np.arange(psi.size)
+    sel = ((idx >> target) & 1) == 0
+    if control >= 0:
+        sel &= ((idx >> control) & 1) == 1
+    i0 = idx[sel]
+    i1 = i0 | (1 << target)
+    a, b = psi[i0].copy(), psi[i1].copy()
+    psi[i0] = u[0] * a + u[1] * b
+    psi[i1] = u[2] * a + u[3] * b
+
+
+def run(ops: np.ndarray, n: int, initial: np.ndarray, precision: int = ka.KQ_C128) -> np.ndarray:
+    """The state after the gate list, as the generated kernels compute it (host emulation)."""
+    passes = _passes(np.ascontiguousarray(ops), n, precision)
+    tiles = [(f"kq_pass_{i}", host_source(p["src"], f"kq_pass_{i}")) for i, p in enumerate(passes) if not p["sweep"]]
+    lib = _compile(tiles)
+    fn_t = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32)
+    launch = lib.emu_launch
+    launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    c64 = precision == ka.KQ_C64
+    psi = np.ascontiguousarray(initial, dtype=np.complex64 if c64 else np.complex128).copy()
+    for i, p in enumerate(passes):
+        if p["sweep"]:
+            m = p["coefs"]
+            if c64:
+                # the sweep record keeps FP64 coefficients; apply in complex64 like the device
+                work = psi.astype(np.complex128)
+                _sweep(work, m, p["target"], p["control"])
+                psi = work.astype(np.complex64)
+            else:
+                _sweep(psi, m, p["target"], p["control"])
+            continue
+        fn = getattr(lib, f"kq_pass_{i}")
+        cf = np.ascontiguousarray(p["coefs"] if p["coefs"].size else np.zeros(1))
+        launch(ctypes.cast(fn, ctypes.c_void_p), psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"])
+    return psi.astype(np.complex128)
