@@ -629,87 +629,126 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
 }
 
 // Gate-free tile passes that bring every qubit q back to index bit q.  A pass may permute the
-// qubits held by its tile's bits freely: the tile takes the low bits (for coalescing) and follows the
-// displacement cycles from there, so each pass sends home every qubit whose bit and home bit it
-// holds; the rest of its qubits take its remaining bits.
+// qubits its tile's bits hold freely, and every tile holds the low bits 0..c-1, which act as a hub:
+// a pass takes in the home bits of the displaced qubits sitting on low bits and the bits holding the
+// low bits' own qubits (both then go home), follows the displacement cycles while it has room, and
+// sends home every qubit whose bit and home bit it holds.  When nothing can go home (every displaced
+// qubit sits on a high bit with a high home) it moves one displaced qubit onto a low bit, from where
+// the next pass sends it home.
 std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, int max_k, int low_c, int rq_max)
 {
     std::vector<TilePass> passes;
     const int k = std::min<int>(max_k, static_cast<int>(nvec));
     const int rq = std::min(rq_max, k);
-    const int c = k == static_cast<int>(nvec) ? k : std::max(0, std::min(low_c, k - 1));
-    const auto displaced = [&] {
+    const int c = k == static_cast<int>(nvec) ? k : std::max(1, std::min(low_c, k - 1));
+    const auto is_home = [&](uint32_t q) { return bit_of[q] == q; };
+    for (int guard = 0;; ++guard) {
+        bool displaced = false;
         for (uint32_t q = 0; q < nvec; ++q) {
-            if (bit_of[q] != q) {
-                return true;
-            }
+            displaced = displaced || !is_home(q);
         }
-        return false;
-    };
-    for (int guard = 0; displaced(); ++guard) {
-        if (guard > 4 * static_cast<int>(nvec)) {
+        if (!displaced) {
+            break;
+        }
+        if (guard > 8 * static_cast<int>(nvec)) {
             throw InvalidArg {"restore plan: no progress"};
         }
-        std::vector<int> qubit_at(nvec);
+        std::vector<uint32_t> qubit_at(nvec);
         for (uint32_t q = 0; q < nvec; ++q) {
-            qubit_at[bit_of[q]] = static_cast<int>(q);
+            qubit_at[bit_of[q]] = q;
         }
-        uint64_t t_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
-        const auto room = [&] { return std::popcount(t_mask) < k; };
-        // follow the cycles through the tile's bits: the home bit of the qubit a bit holds, and the
-        // bit holding the qubit whose home a bit is
-        for (bool grew = true; grew && room();) {
-            grew = false;
-            for (uint32_t b = 0; b < nvec && room(); ++b) {
-                if (!((t_mask >> b) & 1ull) || qubit_at[b] == static_cast<int>(b)) {
-                    continue;
+        const uint64_t low = (uint64_t {1} << c) - 1;
+        uint64_t t_mask = low;
+        int room = k - c;
+        const auto in_t = [&](uint32_t b) { return ((t_mask >> b) & 1ull) != 0; };
+        const auto add = [&](uint32_t b) {
+            if (!in_t(b) && room > 0) {
+                t_mask |= uint64_t {1} << b;
+                --room;
+            }
+        };
+        for (bool grew = true; grew && room > 0;) {
+            const uint64_t before = t_mask;
+            for (uint32_t b = 0; b < nvec; ++b) {
+                if (in_t(b) && qubit_at[b] != b) {
+                    add(qubit_at[b]);   // the home of the qubit on b
+                    add(bit_of[b]);     // the bit holding the qubit whose home is b
                 }
-                const uint32_t home = static_cast<uint32_t>(qubit_at[b]);
-                if (!((t_mask >> home) & 1ull)) {
-                    t_mask |= uint64_t {1} << home;
-                    grew = true;
-                }
-                const uint32_t holder = bit_of[b];
-                if (room() && !((t_mask >> holder) & 1ull)) {
-                    t_mask |= uint64_t {1} << holder;
-                    grew = true;
+            }
+            grew = t_mask != before;
+        }
+        // then whole displacement cycles away from the low bits, while they fit
+        for (uint32_t b0 = 0; b0 < nvec && room > 0; ++b0) {
+            if (in_t(b0) || qubit_at[b0] == b0) {
+                continue;
+            }
+            std::vector<uint32_t> cycle {b0};
+            for (uint32_t b = qubit_at[b0]; b != b0; b = qubit_at[b]) {
+                cycle.push_back(b);
+            }
+            int need = 0;
+            for (const uint32_t b : cycle) {
+                need += in_t(b) ? 0 : 1;
+            }
+            if (need <= room) {
+                for (const uint32_t b : cycle) {
+                    add(b);
                 }
             }
         }
-        for (uint32_t b = 0; b < nvec && room(); ++b) {
-            if (bit_of[b] != b && !((t_mask >> bit_of[b]) & 1ull)) {
-                t_mask |= uint64_t {1} << bit_of[b];   // another displaced qubit
+        bool fixable = false;
+        for (uint32_t b = 0; b < nvec; ++b) {
+            fixable = fixable || (in_t(b) && qubit_at[b] != b && in_t(qubit_at[b]));
+        }
+        uint32_t hub_qubit = nvec;   // a displaced qubit to move onto a low bit when none can go home
+        if (!fixable) {
+            for (uint32_t q = 0; q < nvec && hub_qubit == nvec; ++q) {
+                if (!is_home(q) && room > 0) {
+                    add(bit_of[q]);
+                    hub_qubit = q;
+                }
             }
         }
-        for (uint32_t b = 0; b < nvec && room(); ++b) {
-            t_mask |= uint64_t {1} << b;
+        for (uint32_t b = 0; b < nvec && room > 0; ++b) {
+            add(b);   // fill the tile (these qubits stay where they are)
         }
         // the permutation within the tile
         std::vector<uint8_t> nb = bit_of;
-        std::vector<int> stay_free;
+        if (hub_qubit != nvec) {
+            // nothing can go home: the hub qubit takes low bit 0, whose qubit takes the hub qubit's bit
+            const uint32_t ql = qubit_at[0];
+            nb[hub_qubit] = 0;
+            nb[ql] = bit_of[hub_qubit];
+        }
         uint64_t taken = 0;
-        for (uint32_t b = 0; b < nvec; ++b) {
-            if ((t_mask >> b) & 1ull) {
-                const int q = qubit_at[b];
-                if ((t_mask >> q) & 1ull) {
-                    nb[static_cast<size_t>(q)] = static_cast<uint8_t>(q);
-                    taken |= uint64_t {1} << q;
-                }
+        std::vector<uint32_t> rest;
+        for (uint32_t b = 0; b < nvec && hub_qubit == nvec; ++b) {
+            if (!in_t(b)) {
+                continue;
+            }
+            const uint32_t q = qubit_at[b];
+            if (in_t(q)) {
+                nb[q] = static_cast<uint8_t>(q);
+                taken |= uint64_t {1} << q;
+            }
+            else {
+                rest.push_back(q);
             }
         }
+        // the rest take the remaining bits in order
+        std::vector<uint32_t> moving = rest;
         std::vector<uint32_t> free_bits;
         for (uint32_t b = 0; b < nvec; ++b) {
-            if (((t_mask >> b) & 1ull) && !((taken >> b) & 1ull)) {
+            if (in_t(b) && !((taken >> b) & 1ull)) {
                 free_bits.push_back(b);
             }
         }
-        size_t fi = 0;
-        for (uint32_t b = 0; b < nvec; ++b) {
-            if ((t_mask >> b) & 1ull) {
-                const int q = qubit_at[b];
-                if (!((t_mask >> q) & 1ull)) {
-                    nb[static_cast<size_t>(q)] = static_cast<uint8_t>(free_bits[fi++]);
-                }
+        if (hub_qubit == nvec) {
+            if (free_bits.size() != moving.size()) {
+                throw InvalidArg {"restore plan: inconsistent tile"};
+            }
+            for (size_t i = 0; i < moving.size(); ++i) {
+                nb[moving[i]] = static_cast<uint8_t>(free_bits[i]);
             }
         }
         if (nb == bit_of) {
@@ -719,9 +758,9 @@ std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, i
         p.k = k;
         p.rq = rq;
         for (uint32_t b = 0; b < nvec; ++b) {
-            if ((t_mask >> b) & 1ull) {
+            if (in_t(b)) {
                 p.qubits.push_back(static_cast<int>(b));
-                p.out_qubits.push_back(nb[static_cast<size_t>(qubit_at[b])]);
+                p.out_qubits.push_back(nb[qubit_at[b]]);
             }
         }
         p.c = std::min(std::countr_one(t_mask), k);

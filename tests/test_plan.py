@@ -139,3 +139,39 @@ def test_debug_plan_rejects_non_gates():
     ops, _, _ = c.flatten()
     with pytest.raises(ka.KqError):
         ka.debug_plan(ops, 2)
+
+
+@pytest.mark.parametrize("k", [4, 6, 8, 12])
+@pytest.mark.parametrize("n", [5, 9, 13, 16])
+def test_qubit_moves_and_restore_on_the_host(k, n):
+    """Passes planned with qubit moves, replayed on the host on the index bits holding each qubit,
+    the moves applied as each store would, then the restore passes: the oracle's state."""
+    import ctypes
+
+    from kettle_amd import _lib
+    ka.set_fusion(k)
+    c = random_gate_circuit(n, 150, np.random.default_rng(31 * k + n))
+    ops, _, _ = c.flatten()
+    amps = np.zeros(1 << n, np.complex128)
+    amps[0] = 1
+    n_passes, n_restore = ctypes.c_size_t(), ctypes.c_size_t()
+    _lib.check(_lib.load().kq_debug_check_relabel(n, _lib.ptr(ops), len(ops), _lib.ptr(amps), ctypes.byref(n_passes),
+                                                   ctypes.byref(n_restore)))
+    start = np.zeros(1 << n, np.complex128)
+    start[0] = 1
+    ref, _ = orc.sv_simulate(start, n, ops)
+    assert np.max(np.abs(amps - ref)) <= 1e-12
+
+
+def test_qubit_moves_cut_the_30q_pass_count():
+    from kettle_amd import workloads
+    c = workloads.qft_plus_layers(n=30, layers=20)
+    ops, _, _ = c.flatten()
+    try:
+        ka.set_fusion_relabel(False)
+        fixed = len(ka.debug_plan(ops, 30).pass_cost)
+        ka.set_fusion_relabel(True)
+        moved = len(ka.debug_plan(ops, 30).pass_cost)
+    finally:
+        ka.set_fusion_relabel(True)
+    assert moved < fixed
