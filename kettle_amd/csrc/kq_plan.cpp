@@ -785,7 +785,8 @@ namespace
 
 uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
-TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab, const std::vector<int>* bits = nullptr)
+TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab, const std::vector<int>* bits = nullptr,
+                       const std::vector<int>* lane_order = nullptr)
 {
     TileLayout L {};
     const std::vector<int>& q_of = bits ? *bits : p.qubits;
@@ -808,15 +809,52 @@ TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab, const 
         L.lds_slot[r] = static_cast<uint32_t>(lds_swizzle(s_r) * ab);
         L.hbm_slot[r] = hbm(s_r);
     }
-    int j = 0;
-    for (int b = 0; b < p.k; ++b) {
-        if ((st.thr_mask >> b) & 1u) {
-            L.lds_thr[j] = static_cast<uint32_t>(lds_swizzle(1u << b) * ab);
-            L.hbm_thr[j] = hbm(1u << b);
-            ++j;
+    // thread bit j is the j-th local bit of thr_mask, or lane_order[j]
+    std::vector<int> order;
+    if (lane_order) {
+        order = *lane_order;
+    }
+    else {
+        for (int b = 0; b < p.k; ++b) {
+            if ((st.thr_mask >> b) & 1u) {
+                order.push_back(b);
+            }
         }
     }
+    for (size_t j = 0; j < order.size(); ++j) {
+        const uint32_t b = static_cast<uint32_t>(order[j]);
+        L.lds_thr[j] = static_cast<uint32_t>(lds_swizzle(1u << b) * ab);
+        L.hbm_thr[j] = hbm(1u << b);
+    }
     return L;
+}
+
+// A store layout for a pass that moves qubits: its lanes take the local bits in the order of the
+// index bits they are stored to, so each wavefront still writes contiguous runs from index bit 0
+// up, and the registers take the local bits stored highest.
+TileLayout moved_store_layout(const TilePass& p, size_t ab)
+{
+    std::vector<int> by_bit(static_cast<size_t>(p.k));
+    for (int j = 0; j < p.k; ++j) {
+        by_bit[static_cast<size_t>(j)] = j;
+    }
+    std::sort(by_bit.begin(), by_bit.end(), [&](int a, int b) {
+        return p.out_qubits[static_cast<size_t>(a)] < p.out_qubits[static_cast<size_t>(b)];
+    });
+    TileStage st {};
+    uint32_t regs = 0;
+    for (int i = 0; i < p.rq; ++i) {
+        regs |= 1u << by_bit[static_cast<size_t>(p.k - 1 - i)];
+    }
+    int r = 0;
+    for (int b = 0; b < p.k; ++b) {
+        if ((regs >> b) & 1u) {
+            st.reg[r++] = static_cast<uint8_t>(b);
+        }
+    }
+    st.thr_mask = ((1u << p.k) - 1) & ~regs;
+    std::vector<int> lanes(by_bit.begin(), by_bit.end() - p.rq);
+    return make_layout(st, p, ab, &p.out_qubits, &lanes);
 }
 
 bool same_regs(const TileStage& a, const TileStage& b, int rq)
@@ -1081,7 +1119,7 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
             || !std::is_permutation(p.out_qubits.begin(), p.out_qubits.end(), p.qubits.begin())) {
             throw InvalidArg {"tile lowering: store bits are not a permutation of the tile bits"};
         }
-        out.layouts.push_back(make_layout(store_cur ? *cur : p.io_layout, p, ab, &p.out_qubits));
+        out.layouts.push_back(moved_store_layout(p, ab));
         out.out_layout = static_cast<uint32_t>(out.layouts.size() - 1);
     }
     else if (store_cur) {
