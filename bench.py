@@ -317,7 +317,7 @@ def main() -> int:
             "qubits": n,
             "gates": n_gates,
             "passes_per_step": n_passes,
-            "low_qubits_per_tile": args.low_qubits or "library default (4)",
+            "low_qubits_per_tile": args.low_qubits or "library default (5)",
             "tile_passes_per_step": n_tiles,
             "kernel_ms_per_step": round(kernel_ms, 3),
             "ms_per_step_with_state_restored": round(ms_with_restore, 3),

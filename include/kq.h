@@ -137,7 +137,7 @@ int kq_get_fusion(int* max_tile_qubits, int* max_gates_per_pass, int* max_stages
 /* max_cost_per_pass bounds a pass's arithmetic in units of one general complex 2x2 gate (real or
  * axis-rotation matrices count 1/2, one-sided diagonals 1/4, swaps 1/10): above roughly the HBM
  * time of one sweep a pass stops being memory-bound. */
-/* Qubits 0..low_qubits-1 belong to every tile pass (default 4), so that each wavefront access
+/* Qubits 0..low_qubits-1 belong to every tile pass (default 5), so that each wavefront access
  * covers 2^low_qubits consecutive amplitudes; fewer leave more tile slots to other qubits. */
 int kq_set_fusion_low_qubits(int low_qubits);
 /* on != 0 (default): a statevector's gate passes may store qubits on other index bits of their tile,

@@ -218,7 +218,7 @@ def debug_plan(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, is_dens
 
 
 def set_fusion_low_qubits(low_qubits: int) -> None:
-    """Qubits 0..low_qubits-1 belong to every tile pass (default 4)."""
+    """Qubits 0..low_qubits-1 belong to every tile pass (default 5)."""
     check(load().kq_set_fusion_low_qubits(int(low_qubits)))
 
 

@@ -147,9 +147,10 @@ int fusion_max_stages() { return g_max_stages; }
 
 namespace
 {
-// 4: 256-byte contiguous chunks per access; measured on the 30-qubit QFT + layers circuit against
-// 3 (more passes' worth of slots, slower access) and 5 (faster access, 11% more passes)
-int g_low_qubits = 4;
+// 5: 512-byte contiguous chunks per access.  With qubits moving onto the low bits between passes
+// the 30-qubit QFT + layers circuit needs 28 passes at 5 and 29 at 4 (35 and 39 without moves), and
+// 5 streams faster.
+int g_low_qubits = 5;
 }
 
 int fusion_low_qubits() { return g_low_qubits; }

@@ -227,11 +227,11 @@ def test_jit_normalised_forms_at_extreme_angles():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("low", [2, 3, 5])
+@pytest.mark.parametrize("low", [2, 3, 4])
 @pytest.mark.parametrize("jit", [0, 2], ids=["interpreted", "compiled"])
 def test_low_qubit_counts_match_the_oracle(low, jit):
-    """Tiles holding other counts of the lowest qubits than the default four (shorter or longer
-    contiguous chunks), through both pass kernels, against the oracle."""
+    """Tiles holding fewer of the lowest qubits than the default five (shorter contiguous chunks),
+    through both pass kernels, against the oracle."""
     from kettle_amd import workloads
     n = 19
     c = workloads.qft_plus_layers(n=n, layers=4)
@@ -240,6 +240,6 @@ def test_low_qubit_counts_match_the_oracle(low, jit):
     try:
         out = run_sv(c)
     finally:
-        ka.set_fusion_low_qubits(4)
+        ka.set_fusion_low_qubits(5)
     ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
     assert np.max(np.abs(out - ref)) <= TOL128

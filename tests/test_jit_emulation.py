@@ -19,7 +19,7 @@ from jit_emulation import emulate
 def _restore():
     yield
     ka.set_fusion_relabel(True)
-    ka.set_fusion_low_qubits(4)
+    ka.set_fusion_low_qubits(5)
 
 
 def check(circuit, n, precision, tol):
@@ -40,6 +40,6 @@ def test_random_gates_complex128(n):
     check(random_gate_circuit(n, 250, np.random.default_rng(9100 + n)), n, ka.KQ_C128, 1e-12)
 
 
-def test_random_gates_complex64_low_five():
-    ka.set_fusion_low_qubits(5)
+def test_random_gates_complex64_low_four():
+    ka.set_fusion_low_qubits(4)
     check(random_gate_circuit(14, 200, np.random.default_rng(9200)), 14, ka.KQ_C64, 2e-5)
