@@ -192,6 +192,8 @@ def main() -> int:
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--jit", type=int, default=1,
                     help="run-time compiled tile passes: 0 off, 1 states of >= 18 qubits, 2 all")
+    ap.add_argument("--max-stages", type=int, default=0,
+                    help="register stages (LDS remaps + 1) per tile pass; 0 keeps the library default")
     ap.add_argument("--low-qubits", type=int, default=0,
                     help="qubits 0..c-1 in every tile pass (contiguous chunk of 2^c amplitudes per access); "
                          "0 keeps the library default")
@@ -243,6 +245,8 @@ def main() -> int:
     ka.set_jit(args.jit)
     if args.low_qubits > 0:
         ka.set_fusion_low_qubits(args.low_qubits)
+    if args.max_stages > 0:
+        ka.set_fusion(-1, 0, args.max_stages, 0.0)
     waves, per_cu, normalise = (int(x) for x in args.jit_tuning.split(","))
     ka.set_jit_tuning(waves, per_cu, bool(normalise))
     for i in range(args.warmup):
@@ -263,8 +267,12 @@ def main() -> int:
 
     ms_per_step = elapsed * 1e3 / args.steps
     gates_per_s = n_gates / (elapsed / args.steps)
+    jit_end = ka.jit_info()
     # a step leaves qubits on the index bits the last pass chose; reading the state first puts every
-    # qubit back (restore passes): one more step timed with that read-out included
+    # qubit back (restore passes): one more step (after one that compiles the restore passes) timed
+    # with that read-out included
+    step()
+    sv.device_ptr()
     t1 = time.perf_counter()
     step()
     sv.device_ptr()
@@ -286,7 +294,6 @@ def main() -> int:
     tile["launches"] = stats[0]["launches"] + stats[1]["launches"]
     kernel_ms = float(stats["ms"].sum()) / args.steps
     achieved = float(tile["bytes"]) / (float(tile["ms"]) * 1e-3) / 1e9 if tile["ms"] else 0.0
-    jit_end = ka.jit_info()
     jit_used = jit_end["cache_hits"] > jit_warm["cache_hits"]
     roofline = {
         "bound": "hbm",
@@ -318,6 +325,7 @@ def main() -> int:
             "gates": n_gates,
             "passes_per_step": n_passes,
             "low_qubits_per_tile": args.low_qubits or "library default (5)",
+            "max_stages_per_pass": args.max_stages or "library default",
             "tile_passes_per_step": n_tiles,
             "kernel_ms_per_step": round(kernel_ms, 3),
             "ms_per_step_with_state_restored": round(ms_with_restore, 3),

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <complex>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -55,16 +56,16 @@ JitCache& cache()
 
 std::string hex64(uint64_t x)
 {
-    std::ostringstream o;
-    o << "0x" << std::hex << x << "ull";
-    return o.str();
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%llxull", static_cast<unsigned long long>(x));
+    return b;
 }
 
 std::string hex32(uint32_t x)
 {
-    std::ostringstream o;
-    o << "0x" << std::hex << x << "u";
-    return o.str();
+    char b[24];
+    std::snprintf(b, sizeof b, "0x%xu", x);
+    return b;
 }
 
 // block index deposited into the free qubits, as runs of consecutive bits

@@ -1,3 +1,4 @@
+#include <cstdint>
 // Gate compilation (kettle gate -> device record) and the fusion planner.
 //
 // Coefficients are evaluated here, on the host, with kettle's own expressions: cos/sin of
@@ -306,7 +307,7 @@ struct GreedyPass
 };
 
 GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size_t>& remaining, uint64_t low_mask, int k,
-                       int rq, int max_gates, int max_stages)
+                       int rq, int max_gates, int max_stages, size_t max_scan = SIZE_MAX)
 {
     GreedyPass gp;
     gp.q_mask = low_mask;
@@ -314,7 +315,11 @@ GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size
     uint64_t d_any = 0, d_nd = 0;
     double cost = 0.0;
     const double max_cost = fusion_max_cost();
+    size_t scanned = 0;
     for (const size_t r : remaining) {
+        if (++scanned > max_scan) {
+            break;   // scoring only: the rest would be deferred or join rarely
+        }
         const GateRec& g = gates[r];
         const uint64_t tbit = uint64_t {1} << g.target;
         const uint64_t nd = is_diag(g) ? 0 : tbit;
@@ -423,12 +428,18 @@ std::vector<uint8_t> choose_low_qubits(const std::vector<GateRec>& gates, const 
         }
         return nb;
     };
+    // a candidate is scored on the first deferred gates only (a pass rarely reaches further)
+    constexpr size_t SCAN = 400;
     std::vector<GateRec> trial(gates.size());
     const auto score = [&](const std::vector<uint8_t>& nb) {
+        size_t i = 0;
         for (const size_t r : deferred) {
+            if (++i > SCAN) {
+                break;
+            }
             trial[r] = at_bits(gates[r], nb);
         }
-        return greedy_pass(trial, deferred, low_mask, k, rq, max_gates, max_stages).taken.size();
+        return greedy_pass(trial, deferred, low_mask, k, rq, max_gates, max_stages, SCAN).taken.size();
     };
     std::vector<uint8_t> best = bit_of;
     size_t best_score = score(bit_of);
