@@ -166,6 +166,8 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 }
 
 void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
+PreparedRun prepare_passes(kq_state* s, std::vector<TilePass> passes);
+void launch_prepared(kq_state* s, const PreparedRun& r);
 
 // Stores every qubit q of the state back on index bit q (gate-free tile passes), if gate passes
 // left them permuted.
@@ -181,6 +183,43 @@ void normalise_layout(kq_state* s)
 }
 
 void normalise_layout(const kq_state* s) { normalise_layout(const_cast<kq_state*>(s)); }
+
+// kettle's SWAP gate is three controlled X gates, CX(a, b) CX(b, a) CX(a, b) (circuit.cpp:801-810).
+// Where qubits may sit on any index bit, the exchange moves no amplitude: the triple is dropped and
+// every later gate on a acts on b and vice versa.  Returns the other gates, relabelled; sigma[q] is
+// the label that qubit q's amplitudes carry at the end (a permutation).
+std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uint32_t nvec, std::vector<uint8_t>& sigma)
+{
+    sigma.resize(nvec);
+    for (uint32_t q = 0; q < nvec; ++q) {
+        sigma[q] = static_cast<uint8_t>(q);
+    }
+    const auto cx = [](const GateRec& g, int32_t c, uint32_t t) {
+        return g.kind == GK_SWAP && g.control == c && g.target == t;
+    };
+    std::vector<GateRec> out;
+    out.reserve(recs.size());
+    std::vector<uint8_t> label = sigma;   // label[q]: the qubit that true qubit q's gates act on now
+    for (size_t i = 0; i < recs.size(); ++i) {
+        const GateRec& g = recs[i];
+        if (g.kind == GK_SWAP && g.control >= 0 && i + 2 < recs.size()) {
+            const uint32_t a = static_cast<uint32_t>(g.control), b = g.target;
+            if (cx(recs[i + 1], static_cast<int32_t>(b), a) && cx(recs[i + 2], static_cast<int32_t>(a), b)) {
+                std::swap(label[a], label[b]);
+                i += 2;
+                continue;
+            }
+        }
+        GateRec r = g;
+        r.target = label[g.target];
+        if (g.control >= 0) {
+            r.control = label[static_cast<uint32_t>(g.control)];
+        }
+        out.push_back(r);
+    }
+    sigma = label;
+    return out;
+}
 
 void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relabel = true)
 {
@@ -209,6 +248,31 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
         normalise_layout(s);
     }
     std::vector<uint8_t> bit_of = s->bit_of;
+    if (relabel) {
+        std::vector<uint8_t> sigma;
+        const std::vector<GateRec> rest = exchanges_as_relabels(recs, s->nvec, sigma);
+        if (rest.size() != recs.size()) {
+            // the planner works on the relabelled gates; qubit q ends on the bit of qubit sigma[q]
+            if (bit_of.empty()) {
+                bit_of.resize(s->nvec);
+                for (uint32_t q = 0; q < s->nvec; ++q) {
+                    bit_of[q] = static_cast<uint8_t>(q);
+                }
+            }
+            if (!rest.empty()) {
+                execute_passes(s, plan_passes(rest, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(),
+                                              register_qubits(s->precision), fusion_max_stages(), true, &bit_of));
+            }
+            std::vector<uint8_t> out(s->nvec);
+            bool identity = true;
+            for (uint32_t q = 0; q < s->nvec; ++q) {
+                out[q] = bit_of[sigma[q]];
+                identity = identity && out[q] == q;
+            }
+            s->bit_of = identity ? std::vector<uint8_t> {} : out;
+            return;
+        }
+    }
     const auto passes = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
                                     fusion_max_stages(), true, relabel ? &bit_of : nullptr);
     execute_passes(s, passes);
@@ -221,91 +285,101 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
     }
 }
 
-// Lowers every tile pass, uploads all programs at once and launches the passes in order.
-void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
+size_t align256(size_t x) { return (x + 255) & ~size_t {255}; }
+
+// Lowers every tile pass (and compiles its kernel when passes run compiled) and lays all programs out
+// in one host blob: per pass [gates][steps][layouts][terms] (interpreted) or [coefficients] (compiled).
+PreparedRun prepare_passes(kq_state* s, std::vector<TilePass> passes)
 {
-    PlanStats& st = last_stats();
-    const double ab = static_cast<double>(amp_bytes(s->precision));
-    // lower every tile pass and upload all programs at once: per pass [gates][steps][layouts][terms]
-    // (interpreted) or [coefficients] (compiled for its structure)
-    std::vector<TileProgram> progs(passes.size());
-    size_t total = 0;
-    const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
-    std::vector<size_t> offs(passes.size());
+    PreparedRun r;
+    r.passes = std::move(passes);
+    const size_t n = r.passes.size();
+    r.progs.resize(n);
+    r.offs.assign(n, 0);
+    r.jfn.assign(n, nullptr);
     const bool use_jit = jit_wanted(s) && jit_available();
-    std::vector<std::vector<double>> jcoef(passes.size());
+    std::vector<std::vector<double>> jcoef(n);
     std::vector<std::string> jsrc;
     std::vector<size_t> jpass;
-    for (size_t i = 0; i < passes.size(); ++i) {
-        if (passes[i].sweep) {
+    for (size_t i = 0; i < n; ++i) {
+        if (r.passes[i].sweep) {
             continue;
         }
-        lower_tile_pass(passes[i], s->precision, s->nvec, progs[i]);
+        lower_tile_pass(r.passes[i], s->precision, s->nvec, r.progs[i]);
         if (use_jit) {
-            jsrc.push_back(jit_source(progs[i], s->precision, jcoef[i]));
+            jsrc.push_back(jit_source(r.progs[i], s->precision, jcoef[i]));
             jpass.push_back(i);
         }
     }
-    std::vector<hipFunction_t> jfn(passes.size(), nullptr);
     if (use_jit) {
         const auto fns = jit_kernels(jsrc);
         for (size_t j = 0; j < jpass.size(); ++j) {
-            jfn[jpass[j]] = fns[j];
+            r.jfn[jpass[j]] = fns[j];
         }
     }
-    for (size_t i = 0; i < passes.size(); ++i) {
-        if (passes[i].sweep) {
+    size_t total = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (r.passes[i].sweep) {
             continue;
         }
-        offs[i] = total;
-        const TileProgram& tp = progs[i];
-        if (jfn[i]) {
-            total += align(std::max<size_t>(jcoef[i].size(), 1) * sizeof(double));
+        r.offs[i] = total;
+        const TileProgram& tp = r.progs[i];
+        if (r.jfn[i]) {
+            total += align256(std::max<size_t>(jcoef[i].size(), 1) * sizeof(double));
         }
         else {
-            total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
-                   + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
+            total += align256(tp.gates.size() * sizeof(TileGate)) + align256(tp.steps.size() * sizeof(TileStep))
+                   + align256(tp.layouts.size() * sizeof(TileLayout))
+                   + align256(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
         }
     }
-    std::vector<unsigned char> blob(std::max<size_t>(total, 256), 0);
-    for (size_t i = 0; i < passes.size(); ++i) {
-        if (passes[i].sweep) {
+    r.blob.assign(std::max<size_t>(total, 256), 0);
+    for (size_t i = 0; i < n; ++i) {
+        if (r.passes[i].sweep) {
             continue;
         }
-        const TileProgram& tp = progs[i];
-        size_t o = offs[i];
-        if (jfn[i]) {
-            std::memcpy(blob.data() + o, jcoef[i].data(), jcoef[i].size() * sizeof(double));
+        const TileProgram& tp = r.progs[i];
+        size_t o = r.offs[i];
+        if (r.jfn[i]) {
+            std::memcpy(r.blob.data() + o, jcoef[i].data(), jcoef[i].size() * sizeof(double));
             continue;
         }
-        std::memcpy(blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
-        o += align(tp.gates.size() * sizeof(TileGate));
-        std::memcpy(blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
-        o += align(tp.steps.size() * sizeof(TileStep));
-        std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
-        o += align(tp.layouts.size() * sizeof(TileLayout));
-        std::memcpy(blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
+        std::memcpy(r.blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+        o += align256(tp.gates.size() * sizeof(TileGate));
+        std::memcpy(r.blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+        o += align256(tp.steps.size() * sizeof(TileStep));
+        std::memcpy(r.blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+        o += align256(tp.layouts.size() * sizeof(TileLayout));
+        std::memcpy(r.blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
     }
-    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
-    KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
-    for (size_t i = 0; i < passes.size(); ++i) {
-        const TilePass& p = passes[i];
+    return r;
+}
+
+// Uploads a prepared run's programs and launches its passes in order.
+void launch_prepared(kq_state* s, const PreparedRun& r)
+{
+    PlanStats& st = last_stats();
+    const double ab = static_cast<double>(amp_bytes(s->precision));
+    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, r.blob.size()));
+    KQ_HIP_CHECK(hipMemcpyAsync(d, r.blob.data(), r.blob.size(), hipMemcpyHostToDevice, s->stream));
+    for (size_t i = 0; i < r.passes.size(); ++i) {
+        const TilePass& p = r.passes[i];
         if (p.sweep) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s->n_amps)), 1,
                      [&] { launch_gate_sweep(s, p.single); });
         }
         else {
-            const TileProgram& tp = progs[i];
-            const unsigned char* g = d + offs[i];
+            const TileProgram& tp = r.progs[i];
+            const unsigned char* g = d + r.offs[i];
             const int cls = p.c >= p.k ? 0 : 1;
-            if (jfn[i]) {
+            if (r.jfn[i]) {
                 profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                         [&] { launch_jit_pass(s, jfn[i], tp, reinterpret_cast<const double*>(g)); });
+                         [&] { launch_jit_pass(s, r.jfn[i], tp, reinterpret_cast<const double*>(g)); });
             }
             else {
-                const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
-                const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
-                const unsigned char* tm = ly + align(tp.layouts.size() * sizeof(TileLayout));
+                const unsigned char* sp = g + align256(tp.gates.size() * sizeof(TileGate));
+                const unsigned char* ly = sp + align256(tp.steps.size() * sizeof(TileStep));
+                const unsigned char* tm = ly + align256(tp.layouts.size() * sizeof(TileLayout));
                 profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
                          [&] { launch_tile_pass(s, tp, g, sp, ly, tm); });
             }
@@ -316,6 +390,8 @@ void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
     KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
     profiler_collect();
 }
+
+void execute_passes(kq_state* s, const std::vector<TilePass>& passes) { launch_prepared(s, prepare_passes(s, passes)); }
 
 void run_ops(kq_state* s, const kq_op* ops, size_t n)
 {
@@ -1771,6 +1847,8 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
             bit_of[q] = static_cast<uint8_t>(q);
         }
         std::vector<uint8_t> planned = bit_of;
+        std::vector<uint8_t> sigma;
+        recs = exchanges_as_relabels(recs, n_qubits, sigma);
         const auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(KQ_C128),
                                         fusion_max_stages(), true, &planned);
         const size_t N = size_t {1} << n_qubits;
@@ -1827,6 +1905,10 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
         }
         if (bit_of != planned) {
             throw InvalidArg {"debug relabel: replayed qubit placement differs from the planner's"};
+        }
+        // exchanged qubits: qubit q's amplitudes are those of the relabelled qubit sigma[q]
+        for (uint32_t q = 0; q < n_qubits; ++q) {
+            bit_of[q] = planned[sigma[q]];
         }
         const auto restore = plan_restore(bit_of, n_qubits, max_k, fusion_low_qubits(), register_qubits(KQ_C128));
         for (const auto& p : restore) {
@@ -1921,8 +2003,18 @@ std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_o
         throw InvalidArg {"debug jit: state too small for a tile"};
     }
     std::vector<uint8_t> bit_of;
+    std::vector<uint8_t> sigma;
+    if (fusion_relabel()) {
+        recs = exchanges_as_relabels(recs, n_qubits, sigma);
+    }
     auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
                               fusion_max_stages(), true, fusion_relabel() ? &bit_of : nullptr);
+    if (!sigma.empty() && bit_of.size() == n_qubits) {
+        const std::vector<uint8_t> planned = bit_of;
+        for (uint32_t q = 0; q < n_qubits; ++q) {
+            bit_of[q] = planned[sigma[q]];
+        }
+    }
     bool moved = false;
     for (size_t q = 0; q < bit_of.size(); ++q) {
         moved = moved || bit_of[q] != q;

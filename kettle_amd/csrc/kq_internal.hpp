@@ -184,6 +184,19 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
 // merge runs of commuting diagonal gates of one step (appends to out / terms)
 void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, std::vector<TileGate>& out,
                          std::vector<TileTerm>& terms);
+// A lowered, ready-to-launch sequence of passes (programs laid out in one host blob).
+struct PreparedRun
+{
+    std::vector<TilePass> passes;
+    std::vector<TileProgram> progs;
+    std::vector<hipFunction_t> jfn;   // compiled kernel per pass (null: interpreted or sweep)
+    std::vector<size_t> offs;         // program offsets in the blob
+    std::vector<unsigned char> blob;
+};
+
+// Default LDS placement of a tile: entry of local index l (its own inverse; linear over GF(2)).
+inline uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
+
 uint32_t tile_pass_blocks(const TileProgram& prog);
 uint32_t tile_pass_waves(const TileProgram& prog);
 

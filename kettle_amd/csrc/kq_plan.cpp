@@ -106,7 +106,7 @@ GateRec make_rec(GateKind kind, uint32_t target, int32_t control, const M& m)
 PlanStats g_stats;
 int g_max_k = -1;          // -1: automatic
 int g_max_gates = 4096;
-int g_max_stages = 6;
+int g_max_stages = 12;
 double g_max_cost = 1.0e9;
 
 }  // namespace
@@ -794,8 +794,6 @@ std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, i
 
 namespace
 {
-
-uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
 TileLayout make_layout(const TileStage& st, const TilePass& p, size_t ab, const std::vector<int>* bits = nullptr,
                        const std::vector<int>* lane_order = nullptr)

@@ -24,9 +24,9 @@ TOL64 = 2e-5
 
 @pytest.fixture(autouse=True)
 def _fusion_default():
-    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_fusion(-1, 4096, 12, 1.0e9)
     yield
-    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_fusion(-1, 4096, 12, 1.0e9)
 
 
 def gpu_run(circuit, initial, precision=ka.KQ_C128, seed=None):

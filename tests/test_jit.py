@@ -21,9 +21,9 @@ TOL64 = 2e-5
 
 @pytest.fixture(autouse=True)
 def _restore():
-    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_fusion(-1, 4096, 12, 1.0e9)
     yield
-    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_fusion(-1, 4096, 12, 1.0e9)
     ka.set_jit(1)
 
 

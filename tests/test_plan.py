@@ -21,7 +21,7 @@ CONTROLLED = {g for g in Gate if Gate.CH <= g <= Gate.CP} | {Gate.CU}
 @pytest.fixture(autouse=True)
 def _fusion_default():
     yield
-    ka.set_fusion(-1, 4096, 6, 1.0e9)
+    ka.set_fusion(-1, 4096, 12, 1.0e9)
 
 
 def op_matrix(op) -> np.ndarray:
@@ -152,6 +152,52 @@ def test_qubit_moves_and_restore_on_the_host(k, n):
     ka.set_fusion(k)
     c = random_gate_circuit(n, 150, np.random.default_rng(31 * k + n))
     ops, _, _ = c.flatten()
+    amps = np.zeros(1 << n, np.complex128)
+    amps[0] = 1
+    n_passes, n_restore = ctypes.c_size_t(), ctypes.c_size_t()
+    _lib.check(_lib.load().kq_debug_check_relabel(n, _lib.ptr(ops), len(ops), _lib.ptr(amps), ctypes.byref(n_passes),
+                                                   ctypes.byref(n_restore)))
+    start = np.zeros(1 << n, np.complex128)
+    start[0] = 1
+    ref, _ = orc.sv_simulate(start, n, ops)
+    assert np.max(np.abs(amps - ref)) <= 1e-12
+
+
+def exchange_circuit(n, rng):
+    """Random layers with SWAP gates (three CX each) between them, near misses that are not
+    exchanges (CX(a, b) CX(b, a) CX(b, a); an exchange split by another gate), then a QFT (whose
+    swap network is SWAP gates)."""
+    from kettle_amd.workloads import add_brick_ladder, add_random_1q_layer
+    c = ka.QuantumCircuit(n)
+    for layer in range(10):
+        add_random_1q_layer(c, rng)
+        add_brick_ladder(c, rng, layer % 2)
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.add_swap_gate(a, b)
+        c.add_rz_gate(a, float(rng.uniform(0, 6)))
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.add_cx_gate(a, b)
+        c.add_cx_gate(b, a)
+        c.add_cx_gate(b, a)
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.add_cx_gate(a, b)
+        c.add_cx_gate(b, a)
+        c.add_ry_gate(a, float(rng.uniform(0, 6)))
+        c.add_cx_gate(a, b)
+    c.add_qft_gate(list(range(n)))
+    return c
+
+
+@pytest.mark.parametrize("k", [6, 12])
+@pytest.mark.parametrize("n", [5, 9, 14])
+def test_exchanges_as_relabels_on_the_host(k, n):
+    """SWAP gates dropped and the gates after them relabelled, the rest planned with qubit moves and
+    replayed on the host, then the restore passes: the oracle's state (which runs every CX)."""
+    import ctypes
+
+    from kettle_amd import _lib
+    ka.set_fusion(k)
+    ops, _, _ = exchange_circuit(n, np.random.default_rng(17 * k + n)).flatten()
     amps = np.zeros(1 << n, np.complex128)
     amps[0] = 1
     n_passes, n_restore = ctypes.c_size_t(), ctypes.c_size_t()
