@@ -2,6 +2,7 @@
 // simulate() interpreter, measurement draws, sampling, Pauli canonicalisation and the adjoint
 // energy gradient.  Device work goes through the launchers in kq_kernels.hip.
 #include <algorithm>
+#include <functional>
 #include <bit>
 #include <cmath>
 #include <complex>
@@ -166,8 +167,7 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 }
 
 void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
-PreparedRun prepare_passes(kq_state* s, std::vector<TilePass> passes);
-void launch_prepared(kq_state* s, const PreparedRun& r);
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of);
 
 // Stores every qubit q of the state back on index bit q (gate-free tile passes), if gate passes
 // left them permuted.
@@ -260,8 +260,7 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
                 }
             }
             if (!rest.empty()) {
-                execute_passes(s, plan_passes(rest, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(),
-                                              register_qubits(s->precision), fusion_max_stages(), true, &bit_of));
+                plan_and_run(s, rest, max_k, &bit_of);
             }
             std::vector<uint8_t> out(s->nvec);
             bool identity = true;
@@ -273,9 +272,7 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
             return;
         }
     }
-    const auto passes = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
-                                    fusion_max_stages(), true, relabel ? &bit_of : nullptr);
-    execute_passes(s, passes);
+    plan_and_run(s, recs, max_k, relabel ? &bit_of : nullptr);
     if (relabel) {
         bool identity = true;
         for (size_t q = 0; q < bit_of.size(); ++q) {
@@ -287,111 +284,160 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
 
 size_t align256(size_t x) { return (x + 255) & ~size_t {255}; }
 
-// Lowers every tile pass (and compiles its kernel when passes run compiled) and lays all programs out
-// in one host blob: per pass [gates][steps][layouts][terms] (interpreted) or [coefficients] (compiled).
-PreparedRun prepare_passes(kq_state* s, std::vector<TilePass> passes)
+// Streams passes to the device as the planner produces them: each pass is lowered, its compiled
+// kernel looked up, its program (coefficients, or the interpreted kernel's tables) staged in pinned
+// host memory, copied and the pass launched, while the GPU runs the passes before it and the host
+// plans the next one.  A pass whose kernel is not compiled yet is held back, with every later pass
+// (launch order is execution order), and the held kernels compile together, in parallel, once the
+// plan is complete.  Programs take consecutive regions of the state's device and pinned buffers; a
+// region is reused only after the stream has drained.
+class PassStream
 {
-    PreparedRun r;
-    r.passes = std::move(passes);
-    const size_t n = r.passes.size();
-    r.progs.resize(n);
-    r.offs.assign(n, 0);
-    r.jfn.assign(n, nullptr);
-    const bool use_jit = jit_wanted(s) && jit_available();
-    std::vector<std::vector<double>> jcoef(n);
-    std::vector<std::string> jsrc;
-    std::vector<size_t> jpass;
-    for (size_t i = 0; i < n; ++i) {
-        if (r.passes[i].sweep) {
-            continue;
-        }
-        lower_tile_pass(r.passes[i], s->precision, s->nvec, r.progs[i]);
-        if (use_jit) {
-            jsrc.push_back(jit_source(r.progs[i], s->precision, jcoef[i]));
-            jpass.push_back(i);
-        }
-    }
-    if (use_jit) {
-        const auto fns = jit_kernels(jsrc);
-        for (size_t j = 0; j < jpass.size(); ++j) {
-            r.jfn[jpass[j]] = fns[j];
-        }
-    }
-    size_t total = 0;
-    for (size_t i = 0; i < n; ++i) {
-        if (r.passes[i].sweep) {
-            continue;
-        }
-        r.offs[i] = total;
-        const TileProgram& tp = r.progs[i];
-        if (r.jfn[i]) {
-            total += align256(std::max<size_t>(jcoef[i].size(), 1) * sizeof(double));
-        }
-        else {
-            total += align256(tp.gates.size() * sizeof(TileGate)) + align256(tp.steps.size() * sizeof(TileStep))
-                   + align256(tp.layouts.size() * sizeof(TileLayout))
-                   + align256(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
-        }
-    }
-    r.blob.assign(std::max<size_t>(total, 256), 0);
-    for (size_t i = 0; i < n; ++i) {
-        if (r.passes[i].sweep) {
-            continue;
-        }
-        const TileProgram& tp = r.progs[i];
-        size_t o = r.offs[i];
-        if (r.jfn[i]) {
-            std::memcpy(r.blob.data() + o, jcoef[i].data(), jcoef[i].size() * sizeof(double));
-            continue;
-        }
-        std::memcpy(r.blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
-        o += align256(tp.gates.size() * sizeof(TileGate));
-        std::memcpy(r.blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
-        o += align256(tp.steps.size() * sizeof(TileStep));
-        std::memcpy(r.blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
-        o += align256(tp.layouts.size() * sizeof(TileLayout));
-        std::memcpy(r.blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
-    }
-    return r;
-}
+public:
+    explicit PassStream(kq_state* s) : s_(s), use_jit_(jit_wanted(s) && jit_available()) {}
 
-// Uploads a prepared run's programs and launches its passes in order.
-void launch_prepared(kq_state* s, const PreparedRun& r)
-{
-    PlanStats& st = last_stats();
-    const double ab = static_cast<double>(amp_bytes(s->precision));
-    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, r.blob.size()));
-    KQ_HIP_CHECK(hipMemcpyAsync(d, r.blob.data(), r.blob.size(), hipMemcpyHostToDevice, s->stream));
-    for (size_t i = 0; i < r.passes.size(); ++i) {
-        const TilePass& p = r.passes[i];
-        if (p.sweep) {
-            profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s->n_amps)), 1,
-                     [&] { launch_gate_sweep(s, p.single); });
+    void push(TilePass&& p)
+    {
+        Item it;
+        it.pass = std::move(p);
+        if (!it.pass.sweep) {
+            lower_tile_pass(it.pass, s_->precision, s_->nvec, it.prog);
+            if (use_jit_) {
+                it.src = jit_source(it.prog, s_->precision, it.coefs);
+                it.fn = jit_lookup(it.src);
+            }
+        }
+        if (held_.empty() && (it.pass.sweep || !use_jit_ || it.fn)) {
+            launch(it);
         }
         else {
-            const TileProgram& tp = r.progs[i];
-            const unsigned char* g = d + r.offs[i];
-            const int cls = p.c >= p.k ? 0 : 1;
-            if (r.jfn[i]) {
-                profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                         [&] { launch_jit_pass(s, r.jfn[i], tp, reinterpret_cast<const double*>(g)); });
-            }
-            else {
-                const unsigned char* sp = g + align256(tp.gates.size() * sizeof(TileGate));
-                const unsigned char* ly = sp + align256(tp.steps.size() * sizeof(TileStep));
-                const unsigned char* tm = ly + align256(tp.layouts.size() * sizeof(TileLayout));
-                profiled(s, cls, 2.0 * ab * static_cast<double>(s->n_amps), p.gates.size(),
-                         [&] { launch_tile_pass(s, tp, g, sp, ly, tm); });
-            }
-            ++st.n_tile_passes;
+            held_.push_back(std::move(it));
         }
+    }
+
+    void finish()
+    {
+        if (!held_.empty()) {
+            std::vector<std::string> srcs;
+            std::vector<size_t> idx;
+            for (size_t i = 0; i < held_.size(); ++i) {
+                if (!held_[i].pass.sweep && !held_[i].fn) {
+                    srcs.push_back(held_[i].src);
+                    idx.push_back(i);
+                }
+            }
+            const auto fns = jit_kernels(srcs);
+            for (size_t j = 0; j < idx.size(); ++j) {
+                held_[idx[j]].fn = fns[j];   // null (compile failure): the interpreted tile kernel runs it
+            }
+            for (Item& it : held_) {
+                launch(it);
+            }
+            held_.clear();
+        }
+        KQ_HIP_CHECK(hipStreamSynchronize(s_->stream));
+        profiler_collect();
+    }
+
+private:
+    struct Item
+    {
+        TilePass pass;
+        TileProgram prog;
+        std::string src;
+        std::vector<double> coefs;
+        hipFunction_t fn = nullptr;
+    };
+
+    // device and pinned staging regions of `bytes` for one program
+    std::pair<unsigned char*, unsigned char*> reserve(size_t bytes)
+    {
+        bytes = align256(std::max<size_t>(bytes, 1));
+        if (used_ + bytes > s_->prog_bytes || used_ + bytes > s_->stage_bytes) {
+            if (used_ > 0) {
+                KQ_HIP_CHECK(hipStreamSynchronize(s_->stream));   // every earlier program has been read
+                used_ = 0;
+            }
+            const size_t want = std::max<size_t>({bytes, size_t {1} << 20, 2 * s_->prog_bytes});
+            if (bytes > s_->prog_bytes) {
+                grow(s_->prog, s_->prog_bytes, want);
+            }
+            if (bytes > s_->stage_bytes) {
+                if (s_->stage) {
+                    KQ_HIP_CHECK(hipHostFree(s_->stage));
+                    s_->stage = nullptr;
+                    s_->stage_bytes = 0;
+                }
+                KQ_HIP_CHECK(hipHostMalloc(&s_->stage, want, hipHostMallocDefault));
+                s_->stage_bytes = want;
+            }
+        }
+        auto* d = static_cast<unsigned char*>(s_->prog) + used_;
+        auto* h = static_cast<unsigned char*>(s_->stage) + used_;
+        used_ += bytes;
+        return {d, h};
+    }
+
+    void launch(Item& it)
+    {
+        PlanStats& st = last_stats();
+        const double ab = static_cast<double>(amp_bytes(s_->precision));
+        const TilePass& p = it.pass;
         ++st.n_passes;
+        if (p.sweep) {
+            profiled(s_, 2, 2.0 * ab * static_cast<double>(sweep_amps(p.single, s_->n_amps)), 1,
+                     [&] { launch_gate_sweep(s_, p.single); });
+            return;
+        }
+        ++st.n_tile_passes;
+        const TileProgram& tp = it.prog;
+        const int cls = p.c >= p.k ? 0 : 1;
+        if (it.fn) {
+            const size_t bytes = std::max<size_t>(it.coefs.size(), 1) * sizeof(double);
+            auto [d, h] = reserve(bytes);
+            std::memcpy(h, it.coefs.data(), it.coefs.size() * sizeof(double));
+            KQ_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_->stream));
+            profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
+                     [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d)); });
+            return;
+        }
+        const size_t g_b = align256(tp.gates.size() * sizeof(TileGate)), s_b = align256(tp.steps.size() * sizeof(TileStep)),
+                     l_b = align256(tp.layouts.size() * sizeof(TileLayout));
+        const size_t bytes = g_b + s_b + l_b + std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm);
+        auto [d, h] = reserve(bytes);
+        std::memcpy(h, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+        std::memcpy(h + g_b, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+        std::memcpy(h + g_b + s_b, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+        std::memcpy(h + g_b + s_b + l_b, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
+        KQ_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_->stream));
+        profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
+                 [&] { launch_tile_pass(s_, tp, d, d + g_b, d + g_b + s_b, d + g_b + s_b + l_b); });
     }
-    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
-    profiler_collect();
+
+    kq_state* s_;
+    bool use_jit_;
+    std::vector<Item> held_;
+    size_t used_ = 0;
+};
+
+void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
+{
+    PassStream stream(s);
+    for (const TilePass& p : passes) {
+        stream.push(TilePass {p});
+    }
+    stream.finish();
 }
 
-void execute_passes(kq_state* s, const std::vector<TilePass>& passes) { launch_prepared(s, prepare_passes(s, passes)); }
+// Plans a gate list and runs its passes as they are planned.
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of)
+{
+    PassStream stream(s);
+    const std::function<void(TilePass&&)> emit = [&](TilePass&& p) { stream.push(std::move(p)); };
+    plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision), fusion_max_stages(),
+                true, bit_of, &emit);
+    stream.finish();
+}
 
 void run_ops(kq_state* s, const kq_op* ops, size_t n)
 {
@@ -1345,6 +1391,9 @@ int kq_state_destroy(kq_state* s)
         }
         if (s->prog) {
             (void) hipFree(s->prog);
+        }
+        if (s->stage) {
+            (void) hipHostFree(s->stage);
         }
         if (s->aux) {
             (void) hipFree(s->aux);

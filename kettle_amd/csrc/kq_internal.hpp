@@ -5,6 +5,7 @@
 
 #include <complex>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -184,16 +185,6 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
 // merge runs of commuting diagonal gates of one step (appends to out / terms)
 void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, std::vector<TileGate>& out,
                          std::vector<TileTerm>& terms);
-// A lowered, ready-to-launch sequence of passes (programs laid out in one host blob).
-struct PreparedRun
-{
-    std::vector<TilePass> passes;
-    std::vector<TileProgram> progs;
-    std::vector<hipFunction_t> jfn;   // compiled kernel per pass (null: interpreted or sweep)
-    std::vector<size_t> offs;         // program offsets in the blob
-    std::vector<unsigned char> blob;
-};
-
 // Default LDS placement of a tile: entry of local index l (its own inverse; linear over GF(2)).
 inline uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
@@ -222,9 +213,11 @@ struct kq_state
     // per-state device scratch (reductions, CDFs), grown on demand
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
-    // per-state device buffer for uploaded gate programs
+    // per-state device buffer for uploaded gate programs, and its pinned host staging buffer
     void* prog = nullptr;
     size_t prog_bytes = 0;
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
     // per-state device buffer for adjoint partial sums
     void* aux = nullptr;
     size_t aux_bytes = 0;
@@ -277,9 +270,11 @@ void compile_gate_stable(const kq_op& op, uint32_t n, std::vector<GateRec>& out)
 bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
 // allow_sweep: a pass holding a single diagonal gate may run as a plain sweep (off for adjoint plans)
+// With `emit`, each pass is handed over as soon as it is planned and the returned list is empty.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
                                   int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true,
-                                  std::vector<uint8_t>* bit_of = nullptr);
+                                  std::vector<uint8_t>* bit_of = nullptr,
+                                  const std::function<void(TilePass&&)>* emit = nullptr);
 // gate-free tile passes that store every qubit q back on index bit q
 std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, int max_k, int low_c, int rq);
 // whether statevector gate passes may move qubits onto other index bits (kq_set_fusion_relabel)
@@ -434,6 +429,8 @@ bool jit_available();
 std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
 // compiled kernels for the sources (null where compilation failed); compiles missing ones in parallel
 std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
+// the compiled kernel of a source if it is cached (a cache hit), else null; never compiles
+hipFunction_t jit_lookup(const std::string& source);
 // compile without loading (no device needed): number of failures, last log in *log
 size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
 void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);

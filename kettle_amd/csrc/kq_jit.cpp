@@ -874,6 +874,18 @@ std::vector<Compiled> compile_all(const std::vector<std::string>& sources, const
 }  // namespace
 
 // Kernels for the given sources (compiling the missing ones in parallel); null where compilation failed.
+hipFunction_t jit_lookup(const std::string& source)
+{
+    JitCache& c = cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.kernels.find(source);
+    if (it == c.kernels.end()) {
+        return nullptr;
+    }
+    ++c.stats.hits;
+    return it->second;
+}
+
 std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources)
 {
     JitCache& c = cache();

@@ -490,9 +490,18 @@ std::vector<uint8_t> choose_low_qubits(const std::vector<GateRec>& gates, const 
 // its tile onto bits 0..c-1 as it stores the tile, chosen for the next pass (`out_qubits`); without
 // it qubit q is index bit q throughout.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t nvec, int max_k, int low_c, int max_gates,
-                                  int rq_max, int max_stages, bool allow_sweep, std::vector<uint8_t>* bit_of_io)
+                                  int rq_max, int max_stages, bool allow_sweep, std::vector<uint8_t>* bit_of_io,
+                                  const std::function<void(TilePass&&)>* emit)
 {
     std::vector<TilePass> passes;
+    const auto hand_over = [&](TilePass&& p) {
+        if (emit) {
+            (*emit)(std::move(p));
+        }
+        else {
+            passes.push_back(std::move(p));
+        }
+    };
     const int k = std::min<int>(max_k, static_cast<int>(nvec));
     const int rq = std::min(rq_max, k);
     // a tile covering every qubit needs no free slots; otherwise keep at least one slot free
@@ -531,7 +540,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
             p.sweep = true;
             p.single = gates[taken[0]];
             p.src.push_back(static_cast<uint32_t>(taken[0]));
-            passes.push_back(std::move(p));
+            hand_over(std::move(p));
             remaining = std::move(deferred);
             continue;
         }
@@ -631,7 +640,7 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
                 bit_of = nb;
             }
         }
-        passes.push_back(std::move(p));
+        hand_over(std::move(p));
         remaining = std::move(deferred);
     }
     if (bit_of_io) {
