@@ -105,7 +105,9 @@ int kq_sv_create(kq_state** out, uint32_t n_qubits, int precision);
 int kq_dm_create(kq_state** out, uint32_t n_qubits, int precision);
 int kq_state_destroy(kq_state* s);
 int kq_state_info(const kq_state* s, uint32_t* n_qubits, int* precision, int* is_density);
-/* Raw device pointer and byte size of the amplitude buffer (for zero-copy interop). */
+/* Raw device pointer and byte size of the amplitude buffer (for zero-copy interop), qubit q on index
+ * bit q.  Valid until the next call that applies gates to the state: a gate run may leave qubits on
+ * other index bits, and putting them back may move the amplitudes to a new buffer. */
 int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes);
 /* Host <-> device copies of the whole state, interleaved complex doubles (2*N or 2*N*N). */
 int kq_state_upload(kq_state* s, const double* host);

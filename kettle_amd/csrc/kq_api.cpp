@@ -169,11 +169,111 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
 void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of);
 
-// Stores every qubit q of the state back on index bit q (gate-free tile passes), if gate passes
-// left them permuted.
+// Tables of the one-pass permutation that moves the qubit on index bit bit_of[q] to bit q.
+PermuteTables permute_tables(const std::vector<uint8_t>& bit_of)
+{
+    const uint32_t nv = static_cast<uint32_t>(bit_of.size());
+    if (nv < 12 || nv > 63) {
+        throw InvalidArg {"index-bit permutation: 12 to 63 index bits"};
+    }
+    std::vector<uint8_t> qubit_at(nv);
+    for (uint32_t q = 0; q < nv; ++q) {
+        qubit_at[bit_of[q]] = static_cast<uint8_t>(q);
+    }
+    // tile: the source's six lowest bits and the six bits that land on the destination's lowest
+    std::vector<int> tile;
+    const auto add = [&](int b) {
+        if (static_cast<int>(tile.size()) < 12 && std::find(tile.begin(), tile.end(), b) == tile.end()) {
+            tile.push_back(b);
+        }
+    };
+    for (int b = 0; b < 6; ++b) {
+        add(b);
+    }
+    for (int d = 0; d < 6; ++d) {
+        add(bit_of[static_cast<size_t>(d)]);
+    }
+    for (int b = 0; b < static_cast<int>(nv); ++b) {
+        add(b);
+    }
+    PermuteTables t {};
+    std::vector<int> local_of(nv, -1);
+    for (int j = 0; j < 12; ++j) {
+        local_of[static_cast<size_t>(tile[static_cast<size_t>(j)])] = j;
+        t.src_of[j] = uint64_t {1} << tile[static_cast<size_t>(j)];
+        t.dst_of[j] = uint64_t {1} << qubit_at[static_cast<size_t>(tile[static_cast<size_t>(j)])];
+    }
+    for (int b = 0; b < static_cast<int>(nv); ++b) {
+        if (local_of[static_cast<size_t>(b)] < 0) {
+            t.free_src[t.n_free] = static_cast<uint8_t>(b);
+            t.free_dst[t.n_free] = qubit_at[static_cast<size_t>(b)];
+            ++t.n_free;
+        }
+    }
+    // layouts: six lanes on the given local bits, the other six local bits as two wave bits and four
+    // registers
+    const auto layout = [&](const int* lanes, uint8_t* thr, uint8_t* reg) {
+        uint32_t used = 0;
+        for (int j = 0; j < 6; ++j) {
+            thr[j] = static_cast<uint8_t>(lanes[j]);
+            used |= 1u << lanes[j];
+        }
+        int nt = 6, nr = 0;
+        for (int l = 0; l < 12; ++l) {
+            if (!((used >> l) & 1u)) {
+                if (nt < 8) {
+                    thr[nt++] = static_cast<uint8_t>(l);
+                }
+                else {
+                    reg[nr++] = static_cast<uint8_t>(l);
+                }
+            }
+        }
+    };
+    int ld_lanes[6], st_lanes[6];
+    for (int j = 0; j < 6; ++j) {
+        ld_lanes[j] = local_of[static_cast<size_t>(j)];
+        st_lanes[j] = local_of[bit_of[static_cast<size_t>(j)]];
+    }
+    layout(ld_lanes, t.ld_thr, t.ld_reg);
+    layout(st_lanes, t.st_thr, t.st_reg);
+    return t;
+}
+
+// Moves every qubit q of the state back on index bit q in one out-of-place pass (k_permute_bits) into
+// a new buffer that replaces the state's.  False (nothing done) if the state has fewer than 12 index
+// bits or the second buffer cannot be allocated.
+bool permute_out_of_place(kq_state* s)
+{
+    if (s->nvec < 12 || s->nvec > 63) {
+        return false;
+    }
+    const PermuteTables t = permute_tables(s->bit_of);
+    void* dst = nullptr;
+    try {
+        dst = device_alloc(s->n_amps * amp_bytes(s->precision));
+    }
+    catch (const AllocError&) {
+        return false;
+    }
+    launch_permute_bits(s, s->amps, dst, t);
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    KQ_HIP_CHECK(hipFree(s->amps));
+    s->amps = dst;
+    return true;
+}
+
+// Stores every qubit q of the state back on index bit q, if gate passes left them permuted: one
+// out-of-place pass when a second buffer fits, else gate-free in-place tile passes.
 void normalise_layout(kq_state* s)
 {
     if (s->bit_of.empty()) {
+        return;
+    }
+    if (permute_out_of_place(s)) {
+        s->bit_of.clear();
+        PlanStats& st = last_stats();
+        ++st.n_passes;
         return;
     }
     const int max_k = fusion_max_k(s->precision, s->nvec);
@@ -1959,6 +2059,49 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
         for (uint32_t q = 0; q < n_qubits; ++q) {
             bit_of[q] = planned[sigma[q]];
         }
+        // the one-pass out-of-place restore, its index arithmetic replayed on the host, must give what
+        // the in-place restore passes give, bit for bit
+        std::vector<std::complex<double>> one_pass;
+        if (n_qubits >= 12) {
+            const PermuteTables t = permute_tables(bit_of);
+            one_pass.assign(N, {});
+            std::vector<int> seen(N, 0);
+            for (uint64_t blk = 0; blk < (N >> 12); ++blk) {
+                uint64_t sb = 0, db = 0;
+                for (uint32_t i = 0; i < t.n_free; ++i) {
+                    if ((blk >> i) & 1u) {
+                        sb |= uint64_t {1} << t.free_src[i];
+                        db |= uint64_t {1} << t.free_dst[i];
+                    }
+                }
+                for (uint32_t l = 0; l < 4096; ++l) {
+                    uint64_t so = sb, dof = db;
+                    for (int j = 0; j < 12; ++j) {
+                        if ((l >> j) & 1u) {
+                            so |= t.src_of[j];
+                            dof |= t.dst_of[j];
+                        }
+                    }
+                    if (so >= N || dof >= N || seen[dof]++) {
+                        throw InvalidArg {"debug relabel: one-pass restore is not a permutation"};
+                    }
+                    one_pass[dof] = z[so];
+                }
+            }
+            for (const uint8_t* lay : {t.ld_thr, t.st_thr}) {
+                uint32_t m = 0;
+                for (int j = 0; j < 8; ++j) {
+                    m |= 1u << lay[j];
+                }
+                const uint8_t* reg = lay == t.ld_thr ? t.ld_reg : t.st_reg;
+                for (int q = 0; q < 4; ++q) {
+                    m |= 1u << reg[q];
+                }
+                if (m != 4095u) {
+                    throw InvalidArg {"debug relabel: one-pass restore layout does not cover the tile"};
+                }
+            }
+        }
         const auto restore = plan_restore(bit_of, n_qubits, max_k, fusion_low_qubits(), register_qubits(KQ_C128));
         for (const auto& p : restore) {
             move(p);
@@ -1967,6 +2110,9 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
             if (bit_of[q] != q) {
                 throw InvalidArg {"debug relabel: restore left a qubit off its bit"};
             }
+        }
+        if (!one_pass.empty() && !std::equal(one_pass.begin(), one_pass.end(), z)) {
+            throw InvalidArg {"debug relabel: one-pass restore differs from the restore passes"};
         }
         if (n_passes) {
             *n_passes = passes.size();

@@ -396,6 +396,25 @@ void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
 // GB/s of plain 16-byte streams over 2^n_qubits elements: read, write, in place, copy
 void stream_probe(uint32_t n_qubits, double gbps[4]);
+
+// Tables of the one-pass index-bit permutation (k_permute_bits): tile local bit j is source index bit
+// src_of[j] (a single-bit mask) and destination bit dst_of[j]; block bit i is source bit free_src[i]
+// and destination bit free_dst[i]; the load and store layouts give the local bit of each thread bit
+// and register bit.
+struct PermuteTables
+{
+    uint64_t src_of[12];
+    uint64_t dst_of[12];
+    uint8_t free_src[52];
+    uint8_t free_dst[52];
+    uint32_t n_free;
+    uint8_t ld_thr[8];
+    uint8_t ld_reg[4];
+    uint8_t st_thr[8];
+    uint8_t st_reg[4];
+};
+// dst[pi(i)] = src[i], pi the bit permutation the tables describe (s->n_amps amplitudes, nvec >= 12)
+void launch_permute_bits(kq_state* s, const void* src, void* dst, const PermuteTables& t);
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
 

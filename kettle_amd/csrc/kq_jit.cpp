@@ -401,8 +401,12 @@ std::array<std::complex<double>, 2> term_values(const TileTerm& t, bool c64)
 // A merged diagonal run whose terms are all pure phases (|d0| = |d1| = 1, as for CP, RZ, T, S and
 // P) as a sum of FP64 angles selected by each term's condition and one sincos per factor, in place
 // of a complex product per term.  Entries exactly 1 are skipped (structure, like the conditions).
-bool emit_phase_run(std::ostringstream& o, const TileGate& g, const TileProgram& tp, bool c64,
-                    std::vector<double>& coefs)
+// The conditions read only the thread index and the tile's outside bits, so the angle sums and the
+// sincos go to `pre`, which the kernel runs before it waits for the tile's amplitudes (the arithmetic
+// overlaps the loads); the sums run in four accumulators (short dependency chains).  `o` gets the
+// multiplication of the amplitudes.
+bool emit_phase_run(std::ostringstream& o, std::ostringstream& pre, int id, const TileGate& g, const TileProgram& tp,
+                    bool c64, std::vector<double>& coefs)
 {
     // one FP64 sincos costs about as much as the complex products of eight terms
     constexpr uint64_t MIN_TERMS = 8;
@@ -420,11 +424,14 @@ bool emit_phase_run(std::ostringstream& o, const TileGate& g, const TileProgram&
         any0 = any0 || d[0] != std::complex<double> {1.0, 0.0};
     }
     const bool need0 = any0 && !uniform;
-    o << "  {\n    double a0 = 0.0, a1 = 0.0;\n";
+    const std::string v = "ph" + std::to_string(id) + "_";
+    pre << "  double " << v << "s1, " << v << "c1" << (need0 ? ", " + v + "s0, " + v + "c0" : std::string {}) << ";\n"
+        << "  {\n    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};\n";
     for (uint64_t j = 0; j < g.glb_sel; ++j) {
         const TileTerm& t = tp.terms[g.lane_sel + j];
         const auto d = term_values(t, c64);
         const std::string ok = cond_expr(t.glb_ctrl, t.lane_ctrl);
+        const std::string acc = "[" + std::to_string(j % 4) + "]";
         const auto angle = [&](std::complex<double> z) {
             coefs.push_back(std::arg(z));
             return "cf[" + std::to_string(coefs.size() - 1) + "]";
@@ -434,30 +441,33 @@ bool emit_phase_run(std::ostringstream& o, const TileGate& g, const TileProgram&
             const bool one0 = d[0] == std::complex<double> {1.0, 0.0};
             const std::string p1 = angle(d[1]);
             const std::string p0 = one0 ? std::string {"0.0"} : angle(d[0]);
-            o << "    a1 += (" << ok << ") ? (" << sel << " ? " << p1 << " : " << p0 << ") : 0.0;\n";
+            pre << "    a1" << acc << " += (" << ok << ") ? (" << sel << " ? " << p1 << " : " << p0 << ") : 0.0;\n";
         }
         else {
-            o << "    if (" << ok << ") {";
+            pre << "    if (" << ok << ") {";
             if (need0 && d[0] != std::complex<double> {1.0, 0.0}) {
-                o << " a0 += " << angle(d[0]) << ";";
+                pre << " a0" << acc << " += " << angle(d[0]) << ";";
             }
-            o << " a1 += " << angle(d[1]) << "; }\n";
+            pre << " a1" << acc << " += " << angle(d[1]) << "; }\n";
         }
     }
-    o << "    double s1, c1;\n    sincos(a1, &s1, &c1);\n";
+    pre << "    sincos((a1[0] + a1[1]) + (a1[2] + a1[3]), &" << v << "s1, &" << v << "c1);\n";
+    if (need0) {
+        pre << "    sincos((a0[0] + a0[1]) + (a0[2] + a0[3]), &" << v << "s0, &" << v << "c0);\n";
+    }
+    pre << "  }\n";
+    const std::string c1 = "static_cast<R>(" + v + "c1)", s1 = "static_cast<R>(" + v + "s1)";
     if (uniform) {
-        o << "    scale<R, " << (key - 20) << ">(v, static_cast<R>(c1), static_cast<R>(s1));\n  }\n";
+        o << "  scale<R, " << (key - 20) << ">(v, " << c1 << ", " << s1 << ");\n";
         return true;
     }
     if (need0) {
-        o << "    double s0, c0;\n    sincos(a0, &s0, &c0);\n"
-          << "    const Coef<R> m {{static_cast<R>(c0), static_cast<R>(s0), 0, 0, 0, 0, static_cast<R>(c1), "
-             "static_cast<R>(s1)}};\n    apply<R, K_DIAG, "
-          << key / 5 << ", " << key % 5 << ">(v, m);\n  }\n";
+        o << "  { const Coef<R> m {{static_cast<R>(" << v << "c0), static_cast<R>(" << v << "s0), 0, 0, 0, 0, " << c1 << ", " << s1
+          << "}}; apply<R, K_DIAG, " << key / 5 << ", " << key % 5 << ">(v, m); }\n";
     }
     else {
-        o << "    const Coef<R> m {{1, 0, 0, 0, 0, 0, static_cast<R>(c1), static_cast<R>(s1)}};\n    apply<R, K_DIAG1, "
-          << key / 5 << ", " << key % 5 << ">(v, m);\n  }\n";
+        o << "  { const Coef<R> m {{1, 0, 0, 0, 0, 0, " << c1 << ", " << s1 << "}}; apply<R, K_DIAG1, " << key / 5 << ", "
+          << key % 5 << ">(v, m); }\n";
     }
     return true;
 }
@@ -708,6 +718,12 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
             }
         }
     };
+    // what precedes the gates (declarations, the loads); phase sums emitted into `pre` go after it
+    const std::string head = o.str();
+    o.str("");
+    o.clear();
+    std::ostringstream pre;
+    int n_runs = 0;
     uint32_t cur = 0;
     for (const TileStep& st : tp.steps) {
         if (st.layout != cur) {
@@ -722,7 +738,8 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
                 pending.push_back(PendingX {static_cast<int>((g.code / 5) % 4), cond_expr(g.glb_ctrl, g.lane_ctrl), g.glb_ctrl, g.lane_ctrl});
                 continue;
             }
-            if (g.code >= TV_MERGE_BASE && emit_phase_run(o, g, tp, c64, coefs)) {
+            if (g.code >= TV_MERGE_BASE && emit_phase_run(o, pre, n_runs, g, tp, c64, coefs)) {
+                ++n_runs;
                 continue;
             }
             if (g.code >= TV_MERGE_BASE) {
@@ -797,7 +814,7 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     }
     io(tp.layouts[cur], false);
     o << "  }\n}\n";
-    return o.str();
+    return head + pre.str() + o.str();
 }
 
 bool jit_available()

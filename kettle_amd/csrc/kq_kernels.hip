@@ -1470,6 +1470,85 @@ void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps
                 (k_copy<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(dst), static_cast<const float2*>(src), n_amps)));
 }
 
+// ---- index-bit permutation ---------------------------------------------------------------------------
+// dst[pi(i)] = src[i] for every index i, pi a permutation of the index bits, in one pass.  A workgroup
+// moves the 2^12 amplitudes that share the values of the bits outside a 12-bit tile set T: it loads
+// them with lanes on the source's lowest bits (contiguous reads), exchanges them through LDS and
+// stores them with lanes on the bits that land on the destination's lowest bits (contiguous writes).
+template <typename A>
+__global__ __launch_bounds__(256) void k_permute_bits(const A* __restrict__ src, A* __restrict__ dst, PermuteTables t)
+{
+    __shared__ A tile[4096];
+    const uint32_t tid = threadIdx.x;
+    uint64_t sb = 0, db = 0;
+    const uint64_t blk = blockIdx.x;
+    for (uint32_t i = 0; i < t.n_free; ++i) {
+        if ((blk >> i) & 1u) {
+            sb |= uint64_t {1} << t.free_src[i];
+            db |= uint64_t {1} << t.free_dst[i];
+        }
+    }
+    const auto lds = [](uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); };
+    uint32_t lt = 0;
+    uint64_t so = sb;
+    for (int j = 0; j < 8; ++j) {
+        if ((tid >> j) & 1u) {
+            lt |= 1u << t.ld_thr[j];
+            so |= t.src_of[t.ld_thr[j]];
+        }
+    }
+    A v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint64_t o = so;
+        for (int q = 0; q < 4; ++q) {
+            if ((r >> q) & 1) {
+                o |= t.src_of[t.ld_reg[q]];
+            }
+        }
+        v[r] = src[o];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint32_t l = lt;
+        for (int q = 0; q < 4; ++q) {
+            if ((r >> q) & 1) {
+                l |= 1u << t.ld_reg[q];
+            }
+        }
+        tile[lds(l)] = v[r];
+    }
+    __syncthreads();
+    lt = 0;
+    uint64_t d0 = db;
+    for (int j = 0; j < 8; ++j) {
+        if ((tid >> j) & 1u) {
+            lt |= 1u << t.st_thr[j];
+            d0 |= t.dst_of[t.st_thr[j]];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        uint32_t l = lt;
+        uint64_t o = d0;
+        for (int q = 0; q < 4; ++q) {
+            if ((r >> q) & 1) {
+                l |= 1u << t.st_reg[q];
+                o |= t.dst_of[t.st_reg[q]];
+            }
+        }
+        dst[o] = tile[lds(l)];
+    }
+}
+
+void launch_permute_bits(kq_state* s, const void* src, void* dst, const PermuteTables& t)
+{
+    const unsigned blocks = static_cast<unsigned>(s->n_amps >> 12);
+    KQ_DISPATCH(s,
+                (k_permute_bits<double2><<<blocks, 256, 0, s->stream>>>(static_cast<const double2*>(src), static_cast<double2*>(dst), t)),
+                (k_permute_bits<float2><<<blocks, 256, 0, s->stream>>>(static_cast<const float2*>(src), static_cast<float2*>(dst), t)));
+}
+
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b)
 {
     const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
