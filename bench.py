@@ -197,7 +197,7 @@ def main() -> int:
     ap.add_argument("--low-qubits", type=int, default=0,
                     help="qubits 0..c-1 in every tile pass (contiguous chunk of 2^c amplitudes per access); "
                          "0 keeps the library default")
-    ap.add_argument("--jit-tuning", default="0,0,1",
+    ap.add_argument("--jit-tuning", default="3,0,1",
                     help="min waves per SIMD, persistent blocks per CU, pivot normalisation (0/1) of the "
                          "run-time compiled passes")
     ap.add_argument("--calib-copy", type=int, default=0,

@@ -152,7 +152,7 @@ int kq_set_fusion_relabel(int on);
  * A pass whose kernel fails to compile runs interpreted; kq_jit_info counts it. */
 int kq_set_jit(int mode);
 /* Code shape of the run-time pass kernels: min_waves_per_simd > 0 caps their registers so that at
- * least that many waves fit per SIMD; blocks_per_cu > 0 launches that many blocks per CU, each walking
+ * least that many waves fit per SIMD (default 3; 0: the compiler's choice); blocks_per_cu > 0 launches that many blocks per CU, each walking
  * several tiles and loading the next tile while the current one computes (0: one block per tile);
  * normalise != 0 (default) divides each uncontrolled one-qubit gate by its larger first-row entry
  * and multiplies the product of those pivots back in once per pass. */

@@ -232,8 +232,8 @@ def set_jit(mode: int) -> None:
     check(load().kq_set_jit(int(mode)))
 
 
-def set_jit_tuning(min_waves_per_simd: int = 0, blocks_per_cu: int = 0, normalise: bool = True) -> None:
-    """Register cap (min waves per SIMD, 0 = compiler's choice), persistent blocks per CU (0 = one
+def set_jit_tuning(min_waves_per_simd: int = 3, blocks_per_cu: int = 0, normalise: bool = True) -> None:
+    """Register cap (min waves per SIMD, default 3, 0 = compiler's choice), persistent blocks per CU (0 = one
     block per tile) and pivot normalisation of one-qubit gates in the run-time compiled pass kernels."""
     check(load().kq_set_jit_tuning(int(min_waves_per_simd), int(blocks_per_cu), int(bool(normalise))))
 

@@ -434,7 +434,7 @@ int jit_mode();
 // blocks_per_cu > 0 launches that many blocks per CU, each walking several tiles (0: one block per tile)
 struct JitTuning
 {
-    int min_waves_per_simd = 0;
+    int min_waves_per_simd = 3;   // register cap (<= 170 VGPRs): at least 3 waves per SIMD
     int blocks_per_cu = 0;
     bool normalise = true;   // uncontrolled one-qubit gates with their pivot divided out
 };

@@ -398,6 +398,172 @@ std::array<std::complex<double>, 2> term_values(const TileTerm& t, bool c64)
     return {std::complex<double> {f[0], f[1]}, std::complex<double> {f[2], f[3]}};
 }
 
+// The factor of a merged run of pure-phase terms with d0 = 1 (or, for a run on a thread or outside
+// bit, a factor selected by that bit) from tables built on the host: the product over the terms
+// conditioned on thread bits only is one entry of a table indexed by the thread index; the terms
+// conditioned on outside bits only are grouped by the bits they read (at most 6 per group) into
+// tables indexed by those bits of the tile's position; terms reading both kinds are summed as
+// angles with one sincos.  Each table entry is the cosine and sine of the entry's angle sum (host
+// FP64), so the device does a few loads and complex products in place of a sum per term and a
+// sincos.  The factor goes to `pre` (before the amplitudes are waited for), its use to `o`.
+void emit_phase_tables(std::ostringstream& o, std::ostringstream& pre, const std::string& v, uint32_t key,
+                       const TileGate& g, const TileProgram& tp, bool c64, std::vector<double>& coefs)
+{
+    const bool uniform = key >= 20;
+    struct PT
+    {
+        uint32_t lane_ctrl, lane_sel;
+        uint64_t glb_ctrl, glb_sel;
+        double th1, th0;
+    };
+    std::vector<PT> lane, glb, mixed;
+    for (uint64_t j = 0; j < g.glb_sel; ++j) {
+        const TileTerm& t = tp.terms[g.lane_sel + j];
+        const auto d = term_values(t, c64);
+        PT x {t.lane_ctrl, uniform ? t.lane_sel : 0u, t.glb_ctrl, uniform && t.lane_sel == 0 ? t.glb_sel : 0ull,
+              std::arg(d[1]), d[0] == std::complex<double> {1.0, 0.0} ? 0.0 : std::arg(d[0])};
+        if (!uniform) {
+            x.th0 = 0.0;   // a run on a register bit: the factor multiplies the slots whose bit is 1
+        }
+        const bool lane_only = x.glb_ctrl == 0 && x.glb_sel == 0;
+        const bool glb_only = x.lane_ctrl == 0 && x.lane_sel == 0;
+        if (lane_only) {
+            lane.push_back(x);
+        }
+        else if (glb_only) {
+            glb.push_back(x);
+        }
+        else {
+            mixed.push_back(x);
+        }
+    }
+    // angle of a term at given thread / outside bits
+    const auto angle = [&](const PT& x, uint32_t tid, uint64_t base) {
+        const bool ok = (tid & x.lane_ctrl) == x.lane_ctrl && (base & x.glb_ctrl) == x.glb_ctrl;
+        if (!ok) {
+            return 0.0;
+        }
+        if (!uniform) {
+            return x.th1;
+        }
+        const bool sel = x.lane_sel ? (tid & x.lane_sel) != 0 : (base & x.glb_sel) != 0;
+        return sel ? x.th1 : x.th0;
+    };
+    const auto push_entry = [&](double a) {
+        coefs.push_back(std::cos(a));
+        coefs.push_back(std::sin(a));
+    };
+    pre << "  double " << v << "s1, " << v << "c1;\n  {\n    double fr = 1.0, fi = 0.0;\n";
+    const auto mul = [&](const std::string& xr, const std::string& xi) {
+        pre << "    { const double xr = " << xr << ", xi = " << xi << "; const double nr = fr * xr - fi * xi; fi = fr * xi + fi * xr; fr = nr; }\n";
+    };
+    if (!lane.empty()) {
+        const uint32_t nt = 1u << tp.n_thr_bits;
+        const size_t off = coefs.size();
+        for (uint32_t tid = 0; tid < nt; ++tid) {
+            double a = 0.0;
+            for (const PT& x : lane) {
+                a += angle(x, tid, 0);
+            }
+            push_entry(a);
+        }
+        pre << "    fr = cf[" << off << " + 2 * tid]; fi = cf[" << off + 1 << " + 2 * tid];\n";
+    }
+    // outside-bit terms: connected groups of the bits they read, packed into tables of <= 6 bits
+    std::vector<std::pair<uint64_t, std::vector<PT>>> comps;
+    for (const PT& x : glb) {
+        const uint64_t m = x.glb_ctrl | x.glb_sel;
+        std::pair<uint64_t, std::vector<PT>> merged {m, {x}};
+        for (size_t i = 0; i < comps.size();) {
+            if (comps[i].first & merged.first) {
+                merged.first |= comps[i].first;
+                merged.second.insert(merged.second.end(), comps[i].second.begin(), comps[i].second.end());
+                comps.erase(comps.begin() + static_cast<std::ptrdiff_t>(i));
+            }
+            else {
+                ++i;
+            }
+        }
+        comps.push_back(std::move(merged));
+    }
+    std::vector<std::pair<uint64_t, std::vector<PT>>> groups;
+    for (auto& c : comps) {
+        if (std::popcount(c.first) > 6) {
+            mixed.insert(mixed.end(), c.second.begin(), c.second.end());
+            continue;
+        }
+        bool placed = false;
+        for (auto& gr : groups) {
+            if (std::popcount(gr.first | c.first) <= 6) {
+                gr.first |= c.first;
+                gr.second.insert(gr.second.end(), c.second.begin(), c.second.end());
+                placed = true;
+                break;
+            }
+        }
+        if (!placed) {
+            groups.push_back(std::move(c));
+        }
+    }
+    for (const auto& gr : groups) {
+        std::vector<int> bits;
+        for (int b = 0; b < 64; ++b) {
+            if ((gr.first >> b) & 1ull) {
+                bits.push_back(b);
+            }
+        }
+        const size_t off = coefs.size();
+        for (uint32_t idx = 0; idx < (1u << bits.size()); ++idx) {
+            uint64_t base = 0;
+            for (size_t i = 0; i < bits.size(); ++i) {
+                if ((idx >> i) & 1u) {
+                    base |= uint64_t {1} << bits[i];
+                }
+            }
+            double a = 0.0;
+            for (const PT& x : gr.second) {
+                a += angle(x, 0, base);
+            }
+            push_entry(a);
+        }
+        std::string ix;
+        for (size_t i = 0; i < bits.size(); ++i) {
+            ix += (i ? " | " : "") + std::string {"(((base >> "} + std::to_string(bits[i]) + ") & 1ull) << " + std::to_string(i) + ")";
+        }
+        pre << "    { const uint32_t gi = static_cast<uint32_t>(" << ix << ");\n  ";
+        mul("cf[" + std::to_string(off) + " + 2 * gi]", "cf[" + std::to_string(off + 1) + " + 2 * gi]");
+        pre << "    }\n";
+    }
+    if (!mixed.empty()) {
+        pre << "    { double a = 0.0;\n";
+        for (const PT& x : mixed) {
+            coefs.push_back(x.th1);
+            const size_t i1 = coefs.size() - 1;
+            const std::string ok = cond_expr(x.glb_ctrl, x.lane_ctrl);
+            if (uniform) {
+                coefs.push_back(x.th0);
+                const std::string sel = sel_expr(x.lane_sel, x.glb_sel);
+                pre << "      a += (" << ok << ") ? (" << sel << " ? cf[" << i1 << "] : cf[" << i1 + 1 << "]) : 0.0;\n";
+            }
+            else {
+                pre << "      a += (" << ok << ") ? cf[" << i1 << "] : 0.0;\n";
+            }
+        }
+        pre << "      double s, c; sincos(a, &s, &c);\n  ";
+        mul("c", "s");
+        pre << "    }\n";
+    }
+    pre << "    " << v << "c1 = fr; " << v << "s1 = fi;\n  }\n";
+    const std::string c1 = "static_cast<R>(" + v + "c1)", s1 = "static_cast<R>(" + v + "s1)";
+    if (uniform) {
+        o << "  scale<R, " << (key - 20) << ">(v, " << c1 << ", " << s1 << ");\n";
+    }
+    else {
+        o << "  { const Coef<R> m {{1, 0, 0, 0, 0, 0, " << c1 << ", " << s1 << "}}; apply<R, K_DIAG1, " << key / 5 << ", "
+          << key % 5 << ">(v, m); }\n";
+    }
+}
+
 // A merged diagonal run whose terms are all pure phases (|d0| = |d1| = 1, as for CP, RZ, T, S and
 // P) as a sum of FP64 angles selected by each term's condition and one sincos per factor, in place
 // of a complex product per term.  Entries exactly 1 are skipped (structure, like the conditions).
@@ -425,6 +591,10 @@ bool emit_phase_run(std::ostringstream& o, std::ostringstream& pre, int id, cons
     }
     const bool need0 = any0 && !uniform;
     const std::string v = "ph" + std::to_string(id) + "_";
+    if (!need0) {
+        emit_phase_tables(o, pre, v, key, g, tp, c64, coefs);
+        return true;
+    }
     pre << "  double " << v << "s1, " << v << "c1" << (need0 ? ", " + v + "s0, " + v + "c0" : std::string {}) << ";\n"
         << "  {\n    double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};\n";
     for (uint64_t j = 0; j < g.glb_sel; ++j) {

@@ -326,3 +326,62 @@ def test_dm_from_statevector():
     dm2.set_basis(5)
     m = dm2.matrix()
     assert m[5, 5] == 1.0 and np.count_nonzero(m) == 1
+
+
+# ---- qubit placement: SWAP gates as relabels, then every read path -------------------------------------------
+
+@pytest.mark.parametrize("n", [9, 14, 17])
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_swaps_and_moves_then_reads(n, precision):
+    """Random layers with SWAP gates (run as relabels where qubits may sit on any index bit), near
+    misses that are not SWAPs, and a QFT (its swap network is SWAP gates); then the amplitudes, the
+    device pointer's bytes, probabilities, a Pauli expectation and a second circuit on the same
+    state (it starts from the placement the first one left), each against the oracle."""
+    from kettle_amd.workloads import add_brick_ladder, add_random_1q_layer
+    rng = np.random.default_rng(900 + n + precision)
+    c = ka.QuantumCircuit(n)
+    for layer in range(6):
+        add_random_1q_layer(c, rng)
+        add_brick_ladder(c, rng, layer % 2)
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.add_swap_gate(a, b)
+        a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+        c.add_cx_gate(a, b)
+        c.add_cx_gate(b, a)
+        c.add_cx_gate(b, a)
+    c.add_qft_gate(list(range(n)))
+    ref, _ = oracle_run(c, basis("0" * n))
+    tol = TOL128 if precision == ka.KQ_C128 else TOL64
+    # the device pointer's bytes (the state is put back in qubit order first), then a download
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    sv = ka.Statevector(n, precision)
+    ka.simulate(c, sv)
+    ptr, nbytes = sv.device_ptr()
+    assert nbytes == (1 << n) * (16 if precision == ka.KQ_C128 else 8)
+    host = np.empty(1 << n, np.complex128 if precision == ka.KQ_C128 else np.complex64)
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(ctypes.c_void_p(host.ctypes.data), ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2) == 0
+    assert np.max(np.abs(host.astype(np.complex128) - ref)) <= tol
+    assert np.max(np.abs(sv.download() - ref)) <= tol
+    # probabilities after a fresh run (the state left permuted again)
+    sv2 = ka.Statevector(n, precision)
+    ka.simulate(c, sv2)
+    p = ka.calculate_probabilities_raw(sv2)
+    assert np.max(np.abs(p - np.abs(ref) ** 2)) <= (1e-12 if precision == ka.KQ_C128 else 1e-6)
+    # a Pauli expectation after a fresh run
+    sv3 = ka.Statevector(n, precision)
+    ka.simulate(c, sv3)
+    op = ka.PauliOperator(n_qubits=n)
+    for _ in range(12):
+        terms = [(int(rng.integers(n)), int(rng.integers(0, 4))) for _ in range(int(rng.integers(1, n + 1)))]
+        op.add(complex(rng.normal(), 0.0), ka.SparsePauliString(terms, n))
+    e = ka.expectation_value(op, sv3)
+    e_ref = orc.pauli_expectation(ref, n, op.packed())
+    assert abs(e - e_ref) <= (TOL128 if precision == ka.KQ_C128 else 1e-5) * max(1.0, abs(e_ref))
+    # a second circuit on a state still permuted by the first
+    sv4 = ka.Statevector(n, precision)
+    ka.simulate(c, sv4)
+    ka.simulate(c, sv4)
+    ref2, _ = oracle_run(c, ref)
+    assert np.max(np.abs(sv4.download() - ref2)) <= 2 * tol
