@@ -18,6 +18,7 @@
 #include <complex>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <functional>
@@ -1016,8 +1017,9 @@ Compiled compile_one(const std::string& src)
         out.log = "hiprtcCreateProgram failed";
         return out;
     }
+    // the resource-usage remarks in the log tell whether a register cap made the kernel spill
     const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++20", "-Xclang", "-target-feature", "-Xclang",
-                          "-packed-fp32-ops"};
+                          "-packed-fp32-ops", "-Rpass-analysis=kernel-resource-usage"};
     const hiprtcResult r = hiprtcCompileProgram(prog, static_cast<int>(sizeof opts / sizeof opts[0]), opts);
     size_t n = 0;
     if (hiprtcGetProgramLogSize(prog, &n) == HIPRTC_SUCCESS && n > 1) {
@@ -1037,6 +1039,14 @@ Compiled compile_one(const std::string& src)
 namespace
 {
 
+// ScratchSize [bytes/lane] of the kernel-resource-usage remark in a compile log (0 if absent)
+size_t scratch_bytes(const std::string& log)
+{
+    const std::string key = "ScratchSize [bytes/lane]: ";
+    const size_t at = log.find(key);
+    return at == std::string::npos ? 0 : static_cast<size_t>(std::strtoull(log.c_str() + at + key.size(), nullptr, 10));
+}
+
 // compile sources[idx[j]] for every j on up to 16 host threads
 std::vector<Compiled> compile_all(const std::vector<std::string>& sources, const std::vector<size_t>& idx)
 {
@@ -1049,6 +1059,17 @@ std::vector<Compiled> compile_all(const std::vector<std::string>& sources, const
         pool.emplace_back([&] {
             for (size_t j = next++; j < idx.size(); j = next++) {
                 res[j] = compile_one(sources[idx[j]]);
+                // a kernel that spills under the register cap is compiled again without the cap: private
+                // (scratch) memory traffic costs more than the occupancy the cap buys
+                const std::string& src = sources[idx[j]];
+                const size_t cap = src.find(" __attribute__((amdgpu_waves_per_eu(");
+                if (res[j].ok && cap != std::string::npos && scratch_bytes(res[j].log) > 0) {
+                    const size_t end = src.find("))", cap) + 2;
+                    Compiled uncapped = compile_one(src.substr(0, cap) + src.substr(end));
+                    if (uncapped.ok) {
+                        res[j] = std::move(uncapped);
+                    }
+                }
             }
         });
     }
