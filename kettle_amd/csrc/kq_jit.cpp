@@ -719,7 +719,18 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         o << "    if (b + gridDim.x < n_blocks) {\n      fetch(b + gridDim.x);\n    }\n";
     }
     else {
-        o << "  {\n    const uint64_t blk = blockIdx.x;\n    const uint64_t base = " << deposit_expr(tp.freepos) << ";\n";
+        // Workgroups are dealt to the 8 XCDs round robin; workgroup i takes tile (i >> 3) | (i & 7) << (NB - 3),
+        // so each XCD walks a contiguous eighth of the tiles and the outside bits that vary fastest on
+        // one XCD are the low ones (every HBM channel busy), not the ones above the XCD index.
+        const size_t nb = tp.freepos.size();
+        o << "  {\n    const uint64_t blk = ";
+        if (nb >= 3) {
+            o << "(uint64_t {blockIdx.x} >> 3) | (uint64_t {blockIdx.x & 7u} << " << nb - 3 << ")";
+        }
+        else {
+            o << "blockIdx.x";
+        }
+        o << ";\n    const uint64_t base = " << deposit_expr(tp.freepos) << ";\n";
     }
     // Controlled X gates whose control is a thread or block bit (a register swap under a lane- or
     // block-dependent condition) are not executed in registers: each stays pending until the next

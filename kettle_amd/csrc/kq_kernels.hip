@@ -1481,7 +1481,9 @@ __global__ __launch_bounds__(256) void k_permute_bits(const A* __restrict__ src,
     __shared__ A tile[4096];
     const uint32_t tid = threadIdx.x;
     uint64_t sb = 0, db = 0;
-    const uint64_t blk = blockIdx.x;
+    // each XCD (workgroups dealt round robin) takes a contiguous eighth of the tiles
+    const uint64_t blk = t.n_free >= 3 ? (uint64_t {blockIdx.x} >> 3) | (uint64_t {blockIdx.x & 7u} << (t.n_free - 3))
+                                       : uint64_t {blockIdx.x};
     for (uint32_t i = 0; i < t.n_free; ++i) {
         if ((blk >> i) & 1u) {
             sb |= uint64_t {1} << t.free_src[i];
