@@ -253,6 +253,7 @@ def main() -> int:
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
     jit_warm = ka.jit_info()
+    tuner = ka.fusion_autotune_info()
     _, n_passes, n_tiles = ka.plan_stats()
     _lib.check(lib.kq_synchronize())
     stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
@@ -324,8 +325,10 @@ def main() -> int:
             "qubits": n,
             "gates": n_gates,
             "passes_per_step": n_passes,
-            "low_qubits_per_tile": args.low_qubits or "library default (5)",
-            "max_stages_per_pass": args.max_stages or "library default",
+            "low_qubits_per_tile": tuner["low_qubits"] if tuner["on"] else (args.low_qubits or 5),
+            "max_stages_per_pass": tuner["max_stages"] if tuner["on"] else (args.max_stages or 12),
+            "tile_settings": ("plan tuner (cost model, host-only, chosen in the warm-up call; "
+                              f"{tuner['plans_tried']} candidate plans)") if tuner["on"] else "fixed by flags",
             "tile_passes_per_step": n_tiles,
             "kernel_ms_per_step": round(kernel_ms, 3),
             "ms_per_step_with_state_restored": round(ms_with_restore, 3),

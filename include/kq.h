@@ -146,6 +146,15 @@ int kq_set_fusion_low_qubits(int low_qubits);
  * chosen so that the next pass takes the most gates; the state records where each qubit is, and
  * every operation other than gates first stores qubit q back on bit q (gate-free tile passes). */
 int kq_set_fusion_relabel(int on);
+/* on != 0 (default): for statevector gate runs through the run-time compiled passes, the planner
+ * tries tiles with 4-7 low qubits and 12-28 register stages per pass on the host, keeps the plan a
+ * cost model prices lowest (passes, plus LDS remaps above six per pass) and caches that choice per
+ * gate structure (not the angles).  kq_set_fusion with max_stages_per_pass > 0 and
+ * kq_set_fusion_low_qubits switch it off (explicit settings win); this call switches it back on.
+ * No counterpart in the reference (kettle has no fusion). */
+int kq_set_fusion_autotune(int on);
+/* The tuner's state and its last choice; plans_tried counts candidate plans built so far. */
+int kq_fusion_autotune_info(int* on, int* low_qubits, int* max_stages, size_t* plans_tried, size_t* cached);
 /* Tile passes compiled at run time for their exact gate structure (straight-line code, no per-gate
  * dispatch; coefficients stay data, so repeated structures with other angles reuse the kernel):
  * mode 0 = off (every pass interpreted), 1 = states of >= 18 qubits (default), 2 = every tile pass.

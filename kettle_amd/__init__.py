@@ -12,7 +12,7 @@ from .api import (DensityMatrix, EnergyGradient, FusionPlan, PackedPauliOperator
                   SimulationResult, SparsePauliString, Statevector, apply_gates, calculate_probabilities_raw,
                   energy_and_gradient, expectation_value, partial_trace, perform_measurements_as_counts,
                   perform_measurements_as_counts_marginal, perform_measurements_as_counts_raw,
-                  perform_measurements_as_memory, perform_measurements_as_memory_circuit, plan_stats, set_fusion, set_fusion_low_qubits, set_fusion_relabel, simulate, simulate_pauli_string,
+                  perform_measurements_as_memory, perform_measurements_as_memory_circuit, plan_stats, set_fusion, set_fusion_autotune, fusion_autotune_info, set_fusion_low_qubits, set_fusion_relabel, simulate, simulate_pauli_string,
                   state_index_to_bitstring, statevector_to_density_matrix)
 from .io import (Endian, load_statevector, read_numpy_statevector, read_pauli_operator, read_tangelo_circuit,
                  save_statevector, write_tangelo_circuit)

@@ -94,6 +94,8 @@ _SIGNATURES = {
                                         c_uint64, c_int, c_int32, c_uint64, c_void_p, POINTER(c_uint64)]),
     "kq_set_fusion_low_qubits": (c_int, [c_int]),
     "kq_set_fusion_relabel": (c_int, [c_int]),
+    "kq_set_fusion_autotune": (c_int, [c_int]),
+    "kq_fusion_autotune_info": (c_int, [POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_debug_pass": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, c_size_t, c_char_p, c_size_t,
                               POINTER(c_size_t), c_void_p, c_size_t, POINTER(c_size_t), c_void_p, POINTER(c_size_t)]),
     "kq_debug_check_relabel": (c_int, [c_uint32, c_void_p, c_size_t, c_void_p, POINTER(c_size_t), POINTER(c_size_t)]),

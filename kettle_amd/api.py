@@ -227,6 +227,20 @@ def set_fusion_relabel(on: bool) -> None:
     check(load().kq_set_fusion_relabel(int(bool(on))))
 
 
+def set_fusion_autotune(on: bool) -> None:
+    """Plan tuner (default on; explicit stage / low-qubit settings switch it off): per gate structure,
+    the tile settings whose plan a cost model prices lowest."""
+    check(load().kq_set_fusion_autotune(int(bool(on))))
+
+
+def fusion_autotune_info() -> dict:
+    c = _lib.ctypes
+    on, low, st, tried, cached = c.c_int(), c.c_int(), c.c_int(), c.c_size_t(), c.c_size_t()
+    check(load().kq_fusion_autotune_info(c.byref(on), c.byref(low), c.byref(st), c.byref(tried), c.byref(cached)))
+    return {"on": bool(on.value), "low_qubits": low.value, "max_stages": st.value, "plans_tried": tried.value,
+            "cached_choices": cached.value}
+
+
 def set_jit(mode: int) -> None:
     """Run-time compiled tile passes: 0 off, 1 states of >= 18 qubits (default), 2 every tile pass."""
     check(load().kq_set_jit(int(mode)))

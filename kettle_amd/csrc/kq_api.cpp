@@ -2,6 +2,7 @@
 // simulate() interpreter, measurement draws, sampling, Pauli canonicalisation and the adjoint
 // energy gradient.  Device work goes through the launchers in kq_kernels.hip.
 #include <algorithm>
+#include <unordered_map>
 #include <functional>
 #include <bit>
 #include <cmath>
@@ -168,6 +169,16 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 
 void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
 void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of);
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of, int low_c,
+                  int max_stages);
+// tile settings of one gate run, chosen by the plan tuner
+struct FusionChoice
+{
+    int low_c = 5;
+    int max_stages = 12;
+};
+bool tuned(const kq_state* s, int max_k);
+FusionChoice choose_fusion(const kq_state* s, const std::vector<GateRec>& recs, int max_k, const std::vector<uint8_t>& bit_of);
 
 // Tables of the one-pass permutation that moves the qubit on index bit bit_of[q] to bit q.
 PermuteTables permute_tables(const std::vector<uint8_t>& bit_of)
@@ -360,7 +371,9 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
                 }
             }
             if (!rest.empty()) {
-                plan_and_run(s, rest, max_k, &bit_of);
+                const FusionChoice fc = tuned(s, max_k) ? choose_fusion(s, rest, max_k, bit_of)
+                                                        : FusionChoice {fusion_low_qubits(), fusion_max_stages()};
+                plan_and_run(s, rest, max_k, &bit_of, fc.low_c, fc.max_stages);
             }
             std::vector<uint8_t> out(s->nvec);
             bool identity = true;
@@ -372,7 +385,19 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
             return;
         }
     }
-    plan_and_run(s, recs, max_k, relabel ? &bit_of : nullptr);
+    if (relabel && tuned(s, max_k)) {
+        if (bit_of.empty()) {
+            bit_of.resize(s->nvec);
+            for (uint32_t q = 0; q < s->nvec; ++q) {
+                bit_of[q] = static_cast<uint8_t>(q);
+            }
+        }
+        const FusionChoice fc = choose_fusion(s, recs, max_k, bit_of);
+        plan_and_run(s, recs, max_k, &bit_of, fc.low_c, fc.max_stages);
+    }
+    else {
+        plan_and_run(s, recs, max_k, relabel ? &bit_of : nullptr);
+    }
     if (relabel) {
         bool identity = true;
         for (size_t q = 0; q < bit_of.size(); ++q) {
@@ -530,13 +555,115 @@ void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
 }
 
 // Plans a gate list and runs its passes as they are planned.
-void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of)
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of, int low_c,
+                  int max_stages)
 {
     PassStream stream(s);
     const std::function<void(TilePass&&)> emit = [&](TilePass&& p) { stream.push(std::move(p)); };
-    plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision), fusion_max_stages(),
-                true, bit_of, &emit);
+    plan_passes(recs, s->nvec, max_k, low_c, fusion_max_gates(), register_qubits(s->precision), max_stages, true, bit_of, &emit);
     stream.finish();
+}
+
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of)
+{
+    plan_and_run(s, recs, max_k, bit_of, fusion_low_qubits(), fusion_max_stages());
+}
+
+// ---- plan tuner ------------------------------------------------------------------------------
+// The greedy planner's pass count moves a lot with the stage cap and the number of low qubits per
+// tile (30-qubit QFT + layers: 13 to 23 passes over the grid below) while the total number of LDS
+// remaps stays near 150-175, so the tuner plans a statevector gate list of a compiled-pass state
+// once per candidate (host only) and keeps the cheapest by a model fitted on MI355X: a pass costs
+// its HBM time, plus 0.126 of it for each LDS remap above six (the remaps and the arithmetic beyond
+// that no longer overlap the HBM traffic; measured within 2% over six plans).  The choice is cached
+// per gate structure (kinds, bits, controls; not the angles), starting placement and tile size.
+struct TunerState
+{
+    std::mutex mu;
+    std::unordered_map<uint64_t, FusionChoice> cache;
+    FusionChoice last {};
+    size_t plans_tried = 0;
+};
+
+TunerState& tuner()
+{
+    static TunerState t;
+    return t;
+}
+
+double plan_cost(const std::vector<TilePass>& passes)
+{
+    double c = 0.0;
+    for (const TilePass& p : passes) {
+        c += p.sweep ? 0.5 : 1.0 + 0.126 * std::max(0, pass_remaps(p) - 6);
+    }
+    return c;
+}
+
+FusionChoice choose_fusion(const kq_state* s, const std::vector<GateRec>& recs, int max_k, const std::vector<uint8_t>& bit_of)
+{
+    uint64_t h = 1469598103934665603ull;
+    const auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    mix(s->nvec);
+    mix(static_cast<uint64_t>(s->precision));
+    mix(static_cast<uint64_t>(max_k));
+    mix(static_cast<uint64_t>(fusion_max_gates()));
+    for (const uint8_t b : bit_of) {
+        mix(b);
+    }
+    for (const GateRec& g : recs) {
+        mix((uint64_t {g.kind} << 40) ^ (uint64_t {g.target} << 20) ^ static_cast<uint64_t>(static_cast<int64_t>(g.control) & 0xfffff));
+    }
+    TunerState& t = tuner();
+    {
+        std::lock_guard<std::mutex> lk(t.mu);
+        auto it = t.cache.find(h);
+        if (it != t.cache.end()) {
+            t.last = it->second;
+            return it->second;
+        }
+    }
+    FusionChoice best {fusion_low_qubits(), fusion_max_stages()};
+    double best_cost = 1.0e300;
+    size_t tried = 0;
+    for (const int low : {4, 5, 6, 7}) {
+        if (low >= max_k || max_k - low > 10) {
+            continue;
+        }
+        for (const int st : {12, 14, 16, 18, 20, 24, 28}) {
+            std::vector<uint8_t> b = bit_of;
+            try {
+                const double c = plan_cost(plan_passes(recs, s->nvec, max_k, low, fusion_max_gates(), register_qubits(s->precision),
+                                                       st, true, &b));
+                ++tried;
+                if (c < best_cost - 1e-9) {
+                    best_cost = c;
+                    best = {low, st};
+                }
+            }
+            catch (const InvalidArg&) {
+                // a candidate the planner cannot build (tile shape) is skipped
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lk(t.mu);
+    if (t.cache.size() >= 256) {
+        t.cache.clear();
+    }
+    t.cache.emplace(h, best);
+    t.last = best;
+    t.plans_tried += tried;
+    return best;
+}
+
+// the tuner applies to statevector runs with qubit moves through the compiled pass kernels
+bool tuned(const kq_state* s, int max_k)
+{
+    return fusion_autotune() && !s->is_dm && jit_wanted(s) && jit_available() && max_k >= 8
+        && max_k < static_cast<int>(s->nvec);
 }
 
 void run_ops(kq_state* s, const kq_op* ops, size_t n)
@@ -2126,6 +2253,34 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
 int kq_set_fusion_relabel(int on)
 {
     return guarded([&] { set_fusion_relabel(on != 0); });
+}
+
+int kq_set_fusion_autotune(int on)
+{
+    return guarded([&] { set_fusion_autotune(on != 0); });
+}
+
+int kq_fusion_autotune_info(int* on, int* low_qubits, int* max_stages, size_t* plans_tried, size_t* cached)
+{
+    return guarded([&] {
+        TunerState& t = tuner();
+        std::lock_guard<std::mutex> lk(t.mu);
+        if (on) {
+            *on = fusion_autotune() ? 1 : 0;
+        }
+        if (low_qubits) {
+            *low_qubits = t.last.low_c;
+        }
+        if (max_stages) {
+            *max_stages = t.last.max_stages;
+        }
+        if (plans_tried) {
+            *plans_tried = t.plans_tried;
+        }
+        if (cached) {
+            *cached = t.cache.size();
+        }
+    });
 }
 
 int kq_set_fusion_low_qubits(int low_qubits)

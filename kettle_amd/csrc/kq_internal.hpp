@@ -280,6 +280,9 @@ std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, i
 // whether statevector gate passes may move qubits onto other index bits (kq_set_fusion_relabel)
 bool fusion_relabel();
 void set_fusion_relabel(bool on);
+bool fusion_autotune();
+void set_fusion_autotune(bool on);
+int pass_remaps(const TilePass& p);
 int fusion_max_stages();
 // qubits 0..c-1 always belong to a tile (contiguous 2^c-amplitude chunks per access)
 int fusion_low_qubits();

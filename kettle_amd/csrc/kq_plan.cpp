@@ -107,6 +107,7 @@ PlanStats g_stats;
 int g_max_k = -1;          // -1: automatic
 int g_max_gates = 4096;
 int g_max_stages = 12;
+bool g_autotune = true;
 double g_max_cost = 1.0e9;
 
 }  // namespace
@@ -140,6 +141,7 @@ void set_fusion(int max_k, int max_gates, int max_stages, double max_cost)
     }
     if (max_stages > 0) {
         g_max_stages = max_stages;
+        g_autotune = false;   // an explicit setting wins over the tuner
     }
 }
 
@@ -164,12 +166,45 @@ bool g_relabel = true;
 bool fusion_relabel() { return g_relabel; }
 void set_fusion_relabel(bool on) { g_relabel = on; }
 
+bool fusion_autotune() { return g_autotune; }
+void set_fusion_autotune(bool on) { g_autotune = on; }
+
 void set_fusion_low_qubits(int c)
 {
     if (c < 0 || c > 12) {
         throw InvalidArg {"low qubits per tile must be in [0, 12]"};
     }
     g_low_qubits = c;
+    g_autotune = false;   // an explicit setting wins over the tuner
+}
+
+// LDS remaps a tile pass runs: one per change of register bits between its layouts, counting the
+// load layout (when the first stage does not load directly) and the store layout.
+int pass_remaps(const TilePass& p)
+{
+    if (p.sweep || p.stages.empty()) {
+        return 0;
+    }
+    const auto regs = [&](const TileStage& s) {
+        uint32_t m = 0;
+        for (int q = 0; q < p.rq; ++q) {
+            m |= 1u << s.reg[q];
+        }
+        return m;
+    };
+    int n = 0;
+    uint32_t cur = regs(p.load_direct ? p.stages.front() : p.io_layout);
+    n += p.load_direct ? 0 : 1;
+    for (const TileStage& s : p.stages) {
+        if (regs(s) != cur) {
+            ++n;
+            cur = regs(s);
+        }
+    }
+    if (!p.out_qubits.empty() || !(p.store_direct || regs(p.io_layout) == cur)) {
+        ++n;
+    }
+    return n;
 }
 int register_qubits(int) { return 4; }
 int fusion_max_gates() { return g_max_gates; }

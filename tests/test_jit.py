@@ -243,3 +243,42 @@ def test_low_qubit_counts_match_the_oracle(low, jit):
         ka.set_fusion_low_qubits(5)
     ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
     assert np.max(np.abs(out - ref)) <= TOL128
+
+
+@pytest.mark.gpu
+def test_plan_tuner_matches_the_oracle_and_caches_its_choice():
+    """The plan tuner (switched off by the fixtures' explicit settings, on here) picks tile settings for a
+    20-qubit circuit with a QFT and rotation layers; the state matches the oracle, and the same gate
+    structure with other angles reuses the cached choice (no new candidate plans)."""
+    n = 20
+
+    def circuit(theta):
+        c = ka.QuantumCircuit(n)
+        c.add_qft_gate(list(range(n)))
+        for layer in range(4):
+            for q in range(n):
+                c.add_ry_gate(q, float(theta[layer, q]))
+                c.add_rz_gate(q, float(theta[layer, q] * 0.7))
+            for q in range(layer % 2, n - 1, 2):
+                c.add_cx_gate(q, q + 1)
+        return c
+
+    rng = np.random.default_rng(123)
+    ka.set_jit(1)
+    ka.set_fusion_autotune(True)
+    try:
+        c1 = circuit(rng.uniform(0, 2 * math.pi, size=(4, n)))
+        out1 = run_sv(c1)
+        first = ka.fusion_autotune_info()
+        assert first["on"] and first["plans_tried"] > 0
+        assert 4 <= first["low_qubits"] <= 7 and 12 <= first["max_stages"] <= 28
+        ref1, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c1))
+        assert np.max(np.abs(out1 - ref1)) <= TOL128
+        c2 = circuit(rng.uniform(0, 2 * math.pi, size=(4, n)))
+        out2 = run_sv(c2)
+        again = ka.fusion_autotune_info()
+        assert again["plans_tried"] == first["plans_tried"]
+        ref2, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c2))
+        assert np.max(np.abs(out2 - ref2)) <= TOL128
+    finally:
+        ka.set_fusion_autotune(False)
