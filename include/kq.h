@@ -161,12 +161,17 @@ int kq_fusion_autotune_info(int* on, int* low_qubits, int* max_stages, size_t* p
  * A pass whose kernel fails to compile runs interpreted; kq_jit_info counts it. */
 int kq_set_jit(int mode);
 /* Code shape of the run-time pass kernels: min_waves_per_simd > 0 caps their registers so that at
- * least that many waves fit per SIMD (default 3; 0: the compiler's choice); blocks_per_cu > 0 launches that many blocks per CU, each walking
- * several tiles and loading the next tile while the current one computes (0: one block per tile);
+ * least that many waves fit per SIMD (default 3; 0: the compiler's choice; a kernel that would spill
+ * under the cap is compiled again without it); blocks_per_cu > 0 launches that many blocks per CU,
+ * each walking several tiles and loading the next tile while the current one computes (0: one block
+ * per tile);
  * normalise != 0 (default) divides each uncontrolled one-qubit gate by its larger first-row entry
  * and multiplies the product of those pivots back in once per pass. */
 int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise);
-int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds);
+/* Counters of the run-time compiled kernels; uncapped counts kernels compiled again without the
+ * register cap because they spilled under it. */
+int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds,
+                size_t* uncapped);
 /* Fusion plan statistics of the last kq_apply_gates / kq_simulate call. */
 int kq_last_plan_stats(size_t* n_gates, size_t* n_passes, size_t* n_tile_passes);
 

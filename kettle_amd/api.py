@@ -254,10 +254,10 @@ def set_jit_tuning(min_waves_per_simd: int = 3, blocks_per_cu: int = 0, normalis
 
 def jit_info() -> dict:
     c = _lib.ctypes
-    mode, comp, hits, fails, secs = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double()
-    check(load().kq_jit_info(c.byref(mode), c.byref(comp), c.byref(hits), c.byref(fails), c.byref(secs)))
+    mode, comp, hits, fails, secs, unc = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double(), c.c_size_t()
+    check(load().kq_jit_info(c.byref(mode), c.byref(comp), c.byref(hits), c.byref(fails), c.byref(secs), c.byref(unc)))
     return {"mode": mode.value, "compiled": comp.value, "cache_hits": hits.value, "failures": fails.value,
-            "compile_seconds": secs.value}
+            "compile_seconds": secs.value, "uncapped": unc.value}
 
 
 def debug_jit_compile(ops: np.ndarray, n_qubits: int, precision: int = KQ_C128, lds_bytes: int = 160 * 1024) -> int:

@@ -2298,7 +2298,8 @@ int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise)
     return guarded([&] { set_jit_tuning(JitTuning {min_waves_per_simd, blocks_per_cu, normalise != 0}); });
 }
 
-int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds)
+int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds,
+                size_t* uncapped)
 {
     return guarded([&] {
         const JitStats& st = jit_stats();
@@ -2316,6 +2317,9 @@ int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failure
         }
         if (compile_seconds) {
             *compile_seconds = st.compile_seconds;
+        }
+        if (uncapped) {
+            *uncapped = st.uncapped;
         }
     });
 }

@@ -428,6 +428,7 @@ struct JitStats
     size_t hits = 0;           // passes served from the cache
     size_t failures = 0;       // compilations that failed (the pass then runs interpreted)
     double compile_seconds = 0.0;
+    size_t uncapped = 0;       // kernels that spilled under the register cap, compiled again without it
     std::string last_log;
 };
 JitStats& jit_stats();

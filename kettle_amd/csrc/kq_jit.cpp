@@ -1018,6 +1018,7 @@ struct Compiled
     std::vector<char> code;
     std::string log;
     bool ok = false;
+    bool uncapped = false;   // compiled again without the register cap (the capped kernel spilled)
 };
 
 Compiled compile_one(const std::string& src)
@@ -1075,10 +1076,11 @@ std::vector<Compiled> compile_all(const std::vector<std::string>& sources, const
                 const std::string& src = sources[idx[j]];
                 const size_t cap = src.find(" __attribute__((amdgpu_waves_per_eu(");
                 if (res[j].ok && cap != std::string::npos && scratch_bytes(res[j].log) > 0) {
-                    const size_t end = src.find("))", cap) + 2;
+                    const size_t end = src.find(")))", cap) + 3;   // __attribute__((amdgpu_waves_per_eu(N)))
                     Compiled uncapped = compile_one(src.substr(0, cap) + src.substr(end));
                     if (uncapped.ok) {
                         res[j] = std::move(uncapped);
+                        res[j].uncapped = true;
                     }
                 }
             }
@@ -1146,6 +1148,7 @@ std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources)
             c.modules.push_back(mod);
             c.kernels.emplace(sources[missing[j]], fn);
             ++c.stats.compiled;
+            c.stats.uncapped += res[j].uncapped ? 1 : 0;
         }
         c.stats.compile_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         for (size_t i = 0; i < sources.size(); ++i) {
@@ -1164,8 +1167,9 @@ size_t jit_compile_only(const std::vector<std::string>& sources, std::string* lo
     for (size_t i = 0; i < idx.size(); ++i) {
         idx[i] = i;
     }
-    size_t failures = 0;
+    size_t failures = 0, uncapped = 0;
     for (const Compiled& c : compile_all(sources, idx)) {
+        uncapped += c.uncapped ? 1 : 0;
         if (!c.ok) {
             ++failures;
             if (log) {
@@ -1173,6 +1177,8 @@ size_t jit_compile_only(const std::vector<std::string>& sources, std::string* lo
             }
         }
     }
+    std::lock_guard<std::mutex> lk(cache().mu);
+    cache().stats.uncapped += uncapped;
     return failures;
 }
 

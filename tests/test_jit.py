@@ -282,3 +282,19 @@ def test_plan_tuner_matches_the_oracle_and_caches_its_choice():
         assert np.max(np.abs(out2 - ref2)) <= TOL128
     finally:
         ka.set_fusion_autotune(False)
+
+
+def test_kernels_that_spill_under_the_register_cap_compile_without_it():
+    """Host only: the 28-qubit QFT + layers circuit at 6 low qubits and up to 28 stages per pass has
+    passes whose kernel spills under the default cap of three waves per SIMD (private memory traffic
+    on every tile); those compile again without the cap, and every kernel still compiles."""
+    from kettle_amd import workloads
+    ops = ops_of(workloads.qft_plus_layers(n=28, layers=6))
+    before = ka.jit_info()["uncapped"]
+    ka.set_fusion(-1, 4096, 28, 1.0e9)
+    ka.set_fusion_low_qubits(6)
+    try:
+        assert ka.debug_jit_compile(ops, 28, ka.KQ_C128) >= 1
+    finally:
+        ka.set_fusion_low_qubits(5)
+    assert ka.jit_info()["uncapped"] > before
