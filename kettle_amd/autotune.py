@@ -5,7 +5,9 @@
 For each candidate setting it times simulate() of the benchmark circuit (QFT + random layers with
 CX brick ladders) on the device and keeps the fastest.  Results are stored per (precision, qubit
 count) in kettle_amd/tuning.json; ``apply_tuning(precision, n)`` sets libkq's fusion parameters
-from that table (nearest tuned qubit count), and kettle_amd.simulate() calls it automatically.
+from that table (nearest tuned qubit count).  It is an offline, measured sweep; libkq's own plan
+tuner (kq_set_fusion_autotune, on by default) chooses tile settings per gate structure from a cost
+model without running anything, and an explicit setting from this table switches that tuner off.
 """
 from __future__ import annotations
 
