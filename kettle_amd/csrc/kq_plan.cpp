@@ -337,6 +337,7 @@ struct GreedyPass
     std::vector<size_t> taken;
     std::vector<int> taken_stage;
     std::vector<uint64_t> stage_targets {0};
+    std::vector<uint64_t> stage_all {0};   // every bit a gate of the stage touches (targets, controls)
     std::vector<size_t> deferred;
     uint64_t q_mask = 0;
 };
@@ -363,17 +364,44 @@ GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size
                && (gp.taken.empty() || cost + gate_cost(g) <= max_cost);
         uint64_t need = 0;
         bool new_stage = false;
+        int place = -1;
         if (ok && nd) {
             need = nd & ~gp.q_mask;
             if (q_count + std::popcount(need) > k) {
                 ok = false;
             }
-            else if (!(gp.stage_targets.back() & nd) && std::popcount(gp.stage_targets.back()) >= rq) {
-                if (static_cast<int>(gp.stage_targets.size()) >= max_stages) {
-                    ok = false;
+            else {
+                // Backfill: a non-diagonal gate may join an earlier stage that holds its target in
+                // registers (or has a free register) when it commutes with every gate of the later
+                // stages (bit-mask test, as for deferral); it runs after that stage's gates.  A stage
+                // holding the target wins over one with room; among those the latest, which leaves
+                // earlier stages open to later gates.
+                uint64_t later_nd = 0, later_all = 0;
+                int room = -1;
+                for (int st = static_cast<int>(gp.stage_targets.size()) - 1; st >= 0; --st) {
+                    if ((nd & later_all) || (all & later_nd)) {
+                        break;
+                    }
+                    if (gp.stage_targets[static_cast<size_t>(st)] & nd) {
+                        place = st;
+                        break;
+                    }
+                    if (room < 0 && std::popcount(gp.stage_targets[static_cast<size_t>(st)]) < rq) {
+                        room = st;
+                    }
+                    later_nd |= gp.stage_targets[static_cast<size_t>(st)];
+                    later_all |= gp.stage_all[static_cast<size_t>(st)];
                 }
-                else {
-                    new_stage = true;
+                if (place < 0) {
+                    place = room;
+                }
+                if (place < 0) {
+                    if (static_cast<int>(gp.stage_targets.size()) >= max_stages) {
+                        ok = false;
+                    }
+                    else {
+                        new_stage = true;
+                    }
                 }
             }
         }
@@ -383,10 +411,15 @@ GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size
             q_count += std::popcount(need);
             if (new_stage) {
                 gp.stage_targets.push_back(0);
+                gp.stage_all.push_back(0);
             }
-            gp.stage_targets.back() |= nd;
+            if (place < 0) {
+                place = static_cast<int>(gp.stage_targets.size()) - 1;
+            }
+            gp.stage_targets[static_cast<size_t>(place)] |= nd;
+            gp.stage_all[static_cast<size_t>(place)] |= all;
             gp.taken.push_back(r);
-            gp.taken_stage.push_back(static_cast<int>(gp.stage_targets.size()) - 1);
+            gp.taken_stage.push_back(place);
         }
         else {
             gp.deferred.push_back(r);
@@ -560,6 +593,22 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
         }
         const uint64_t low_mask = (c == 64) ? ~0ull : ((uint64_t {1} << c) - 1);
         GreedyPass gp = greedy_pass(gates, remaining, low_mask, k, rq, max_gates, max_stages);
+        {
+            // backfilled gates run with their stage: group by stage, circuit order within a stage
+            std::vector<size_t> ord(gp.taken.size());
+            for (size_t i = 0; i < ord.size(); ++i) {
+                ord[i] = i;
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return gp.taken_stage[a] < gp.taken_stage[b]; });
+            std::vector<size_t> t2(ord.size());
+            std::vector<int> s2(ord.size());
+            for (size_t i = 0; i < ord.size(); ++i) {
+                t2[i] = gp.taken[ord[i]];
+                s2[i] = gp.taken_stage[ord[i]];
+            }
+            gp.taken = std::move(t2);
+            gp.taken_stage = std::move(s2);
+        }
         uint64_t q_mask = gp.q_mask;
         int q_count = std::popcount(q_mask);
         std::vector<size_t>& taken = gp.taken;
