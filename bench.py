@@ -36,10 +36,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per tile-pass launch over algorithmic bytes (one read + one write of the state), from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the 28-qubit version of this workload (kq_pass, 82
-# dispatches), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (the factors a plain copy kernel of known
-# size reads): (2 * 2.097278e6 + 4.194306e6) KiB / (2 * 16 * 2^28 B) per dispatch
-TRAFFIC_PER_ALGORITHMIC_BYTE = 1.00003
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the 28-qubit version of this workload (kq_pass),
+# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (the factors a plain copy kernel of known size
+# reads): measured HBM bytes per dispatch / (2 * 16 * 2^28 B), the excess being coefficient tables
+TRAFFIC_PER_ALGORITHMIC_BYTE = 1.0004
 
 
 def log(msg: str) -> None:
