@@ -305,8 +305,12 @@ std::array<double, 8> record_values(const TileGate& g, bool c64)
 // Emits an uncontrolled one-qubit gate with its pivot entry divided out (orth / axis / dense1 / a
 // d0-normalised diagonal) and multiplies the pivot into the pass factor; false if the record has
 // another shape.  The normalised values are data; which entry is the pivot is structure.
+// `flip`: a condition (thread / outside bits) under which controlled X gates on the gate's register
+// bit are still pending; the gate is then emitted conjugated by X under it (X G X), which leaves the
+// X gates pending.  Only forms closed under that conjugation take it: an axis rotation is its own
+// conjugate, a rotation with the diagonal pivot [[1, t], [-t, 1]] becomes [[1, -t], [t, 1]].
 bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::vector<double>& coefs,
-                     std::complex<double>& factor, bool& factor_complex)
+                     std::complex<double>& factor, bool& factor_complex, const std::string& flip = {})
 {
     if (g.code >= TV_UNIFORM_BASE || g.lane_ctrl != 0 || g.glb_ctrl != 0 || g.code % 5 != 0) {
         return false;
@@ -337,7 +341,12 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
             return false;
         }
         const int form = (diag_pivot ? 0 : 2) + (rotation ? 0 : 1);
-        o << "  orth<R, " << P << ", " << form << ">(v, " << val((diag_pivot ? m[2] : m[0]) / pivot) << ");\n";
+        if (!flip.empty() && form != 0) {
+            return false;
+        }
+        const std::string t = val((diag_pivot ? m[2] : m[0]) / pivot);
+        o << "  orth<R, " << P << ", " << form << ">(v, " << (flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")")
+          << ");\n";
         factor *= pivot;
         return true;
     }
@@ -354,6 +363,9 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
           << ");\n";
         factor *= pivot;
         return true;
+    }
+    if (!flip.empty() && kind != TK_AXIS) {
+        return false;
     }
     if (kind == TK_DIAG) {
         const std::complex<double> d0 {m[0], m[1]};
@@ -914,7 +926,31 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
         for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
             const TileGate& g = tp.gates[gi];
-            materialise(touched(g));
+            uint32_t bits = touched(g);
+            if (g.code < TV_UNIFORM_BASE) {
+                const uint32_t kind = g.code / 20;
+                const int Pg = static_cast<int>((g.code / 5) % 4);
+                const uint32_t pbit = 1u << Pg;
+                std::string flip;
+                for (const PendingX& x : pending) {
+                    if (x.P == Pg) {
+                        flip += (flip.empty() ? "(" : " != (") + x.cond + ")";
+                    }
+                }
+                if (kind == TK_SWAP) {
+                    // an X on the same register bit (whatever controls it) commutes with the pending ones
+                    bits &= ~pbit;
+                }
+                else if (!flip.empty() && normalise && g.code % 5 == 0 && g.lane_ctrl == 0 && g.glb_ctrl == 0
+                         && (kind == TK_REAL || kind == TK_AXIS)) {
+                    materialise(bits & ~pbit);
+                    if (emit_normalised(o, g, c64, coefs, factor, factor_complex, flip)) {
+                        normalised = true;
+                        continue;
+                    }
+                }
+            }
+            materialise(bits);
             if (all_linear && g.code < TV_UNIFORM_BASE && g.code / 20 == TK_SWAP && g.code % 5 == 0
                 && (g.lane_ctrl != 0 || g.glb_ctrl != 0)) {
                 pending.push_back(PendingX {static_cast<int>((g.code / 5) % 4), cond_expr(g.glb_ctrl, g.lane_ctrl), g.glb_ctrl, g.lane_ctrl});

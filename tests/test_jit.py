@@ -285,16 +285,18 @@ def test_plan_tuner_matches_the_oracle_and_caches_its_choice():
 
 
 def test_kernels_that_spill_under_the_register_cap_compile_without_it():
-    """Host only: the 28-qubit QFT + layers circuit at 6 low qubits and up to 28 stages per pass has
-    passes whose kernel spills under the default cap of three waves per SIMD (private memory traffic
-    on every tile); those compile again without the cap, and every kernel still compiles."""
+    """Host only: under a cap of four waves per SIMD (128 VGPRs) the passes of the 28-qubit QFT + layers
+    circuit with 6 low qubits and up to 28 stages spill (private memory traffic on every tile); those
+    compile again without the cap, and every kernel still compiles."""
     from kettle_amd import workloads
-    ops = ops_of(workloads.qft_plus_layers(n=28, layers=6))
+    ops = ops_of(workloads.qft_plus_layers(n=28, layers=20))
     before = ka.jit_info()["uncapped"]
     ka.set_fusion(-1, 4096, 28, 1.0e9)
     ka.set_fusion_low_qubits(6)
+    ka.set_jit_tuning(4, 0, True)
     try:
         assert ka.debug_jit_compile(ops, 28, ka.KQ_C128) >= 1
     finally:
         ka.set_fusion_low_qubits(5)
+        ka.set_jit_tuning()
     assert ka.jit_info()["uncapped"] > before
