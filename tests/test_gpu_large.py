@@ -71,3 +71,47 @@ def test_qft_of_zero_state_is_uniform(n, precision, tol):
     ka.simulate(c, sv)
     assert abs(mean_pauli(sv, n, ka.PauliTerm.X) - 1.0) <= tol
     assert abs(mean_pauli(sv, n, ka.PauliTerm.Z)) <= tol
+
+
+def reduced_spectrum(psi: np.ndarray, n: int, keep: list[int]) -> np.ndarray:
+    """Eigenvalues of the reduced density matrix of |psi> on `keep` (independent of index order)."""
+    idx = np.arange(1 << n)
+    a = np.zeros(1 << n, dtype=np.int64)
+    for j, q in enumerate(keep):
+        a |= ((idx >> q) & 1) << j
+    rest_bits = [q for q in range(n) if q not in keep]
+    r = np.zeros(1 << n, dtype=np.int64)
+    for j, q in enumerate(rest_bits):
+        r |= ((idx >> q) & 1) << j
+    m = np.zeros((1 << len(keep), 1 << len(rest_bits)), dtype=np.complex128)
+    m[a, r] = psi
+    return np.sort(np.linalg.eigvalsh(m @ m.conj().T))
+
+
+@pytest.mark.parametrize("n", [12, 14])
+def test_density_matrix_full_size(n):
+    """Density matrices of 12 and 14 qubits (vec(rho) of 2^24 / 2^28 amplitudes): the depth-30 random
+    circuit on |0..0><0..0| against the statevector path on the same circuit (itself checked against the
+    oracle): trace 1, diagonal = |psi|^2, and the spectrum of the 2-qubit partial trace equals that of
+    |psi>'s reduced state (1e-12); with mid-circuit measurement and if-else every 10 layers, trace 1 and a
+    Hermitian, positive 2-qubit partial trace."""
+    c = workloads.random_circuit(n, 30, seed=n)
+    dm = ka.DensityMatrix(n)
+    ka.simulate(c, dm)
+    sv = ka.Statevector(n)
+    ka.simulate(c, sv)
+    psi = sv.download()
+    assert abs(dm.trace() - 1.0) <= 1e-12
+    p = ka.calculate_probabilities_raw(dm)
+    assert np.max(np.abs(p - np.abs(psi) ** 2)) <= 1e-12
+    keep = [0, n - 1]
+    pt = ka.partial_trace(dm, [q for q in range(n) if q not in keep])
+    assert np.max(np.abs(np.sort(np.linalg.eigvalsh(pt)) - reduced_spectrum(psi, n, keep))) <= 1e-12
+    cm = workloads.density_matrix_circuit(n=n, depth=30, seed=n, measure_every=10)
+    dm2 = ka.DensityMatrix(n)
+    res = ka.simulate(cm, dm2, 5)
+    assert res.classical_register[0] in (0, 1)
+    assert abs(dm2.trace() - 1.0) <= 1e-12
+    pt2 = ka.partial_trace(dm2, [q for q in range(n) if q not in keep])
+    assert np.max(np.abs(pt2 - pt2.conj().T)) <= 1e-12
+    assert np.min(np.linalg.eigvalsh(pt2)) >= -1e-12
