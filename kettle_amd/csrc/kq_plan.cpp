@@ -405,6 +405,19 @@ GreedyPass greedy_pass(const std::vector<GateRec>& gates, const std::vector<size
                 }
             }
         }
+        if (ok && !nd) {
+            // a diagonal gate needs no register: it joins the earliest stage after which every gate
+            // commutes with it, so it does not stand in the way of later gates joining earlier stages
+            uint64_t later_nd = 0;
+            place = static_cast<int>(gp.stage_targets.size()) - 1;
+            for (int st = place; st > 0; --st) {
+                later_nd |= gp.stage_targets[static_cast<size_t>(st)];
+                if (all & later_nd) {
+                    break;
+                }
+                place = st - 1;
+            }
+        }
         if (ok) {
             cost += gate_cost(g);
             gp.q_mask |= need;
