@@ -47,6 +47,37 @@ def log(msg: str) -> None:
     sys.stderr.flush()
 
 
+def c64_section(n: int, layers: int, steps: int) -> dict:
+    """The same circuit family at n qubits in complex64 (the large-statevector workload's second size:
+    33 qubits = 64 GiB), timed like the headline: basis reset + gates per step, one warm-up step."""
+    import kettle_amd as ka
+    from kettle_amd import _lib, workloads
+
+    ops, _, _ = workloads.qft_plus_layers(n=n, layers=layers).flatten()
+    sv = ka.Statevector(n, ka.KQ_C64)
+    lib = _lib.load()
+
+    def step():
+        sv.set_basis(0)
+        ka.apply_gates(sv, ops)
+
+    step()
+    _lib.check(lib.kq_synchronize())
+    _, n_passes, _ = ka.plan_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    _lib.check(lib.kq_synchronize())
+    dt = (time.perf_counter() - t0) / steps
+    tuner = ka.fusion_autotune_info()
+    del sv
+    return {"workload": f"{n}q complex64 QFT + {layers} random 1q layers with CX brick ladders",
+            "gates": len(ops), "value": round(len(ops) / dt, 1), "unit": "gates/s", "ms_per_step": round(dt * 1e3, 2),
+            "steps": steps, "passes_per_step": n_passes, "low_qubits_per_tile": tuner["low_qubits"],
+            "max_stages_per_pass": tuner["max_stages"],
+            "GB_moved_per_pass": round(2 * 8 * (1 << n) / 1e9, 1)}
+
+
 def cpu_baseline(ops, n: int, n_sample: int) -> dict:
     from oracle import oracle as orc
 
@@ -186,6 +217,9 @@ def main() -> int:
     ap.add_argument("--vqe-qubits", type=int, default=24)
     ap.add_argument("--vqe-energies", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--c64-qubits", type=int, default=33,
+                    help="also time the circuit family in complex64 at this many qubits (0: skip)")
+    ap.add_argument("--c64-steps", type=int, default=3)
     ap.add_argument("--no-stream-probe", action="store_true",
                     help="skip the plain-stream HBM reference (read / write / in place / copy GB/s)")
     ap.add_argument("--batch-instances", type=int, default=4096)
@@ -350,6 +384,9 @@ def main() -> int:
         out["hbm_stream_probe"] = probe
         # the tile pass reads and writes every amplitude once, like the in-place stream
         roofline["frac_of_in_place_stream"] = round(achieved / max(gbps[2], 1e-9), 4)
+    if args.c64_qubits > 0:
+        log(f"complex64 section: {args.c64_qubits} qubits")
+        out["complex64_large"] = c64_section(args.c64_qubits, args.layers, args.c64_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ops, n, args.cpu_gates)
     if not args.no_vqe:
