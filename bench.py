@@ -234,6 +234,9 @@ def main() -> int:
     ap.add_argument("--jit-tuning", default="3,0,1",
                     help="min waves per SIMD, persistent blocks per CU, pivot normalisation (0/1) of the "
                          "run-time compiled passes")
+    ap.add_argument("--jit-priority", type=int, default=0,
+                    help="wave priority around a tile's HBM traffic in the run-time compiled passes: 0 off, "
+                         "1 loads, 2 loads and stores")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
@@ -283,6 +286,7 @@ def main() -> int:
         ka.set_fusion(-1, 0, args.max_stages, 0.0)
     waves, per_cu, normalise = (int(x) for x in args.jit_tuning.split(","))
     ka.set_jit_tuning(waves, per_cu, bool(normalise))
+    ka.set_jit_priority(args.jit_priority)
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")

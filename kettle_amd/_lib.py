@@ -100,6 +100,7 @@ _SIGNATURES = {
                               POINTER(c_size_t), c_void_p, c_size_t, POINTER(c_size_t), c_void_p, POINTER(c_size_t)]),
     "kq_debug_check_relabel": (c_int, [c_uint32, c_void_p, c_size_t, c_void_p, POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_set_jit_tuning": (c_int, [c_int, c_int, c_int]),
+    "kq_set_jit_priority": (c_int, [c_int]),
     "kq_jit_info": (c_int, [POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double),
                             POINTER(c_size_t)]),
     "kq_debug_jit_compile": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, POINTER(c_size_t), POINTER(c_size_t)]),

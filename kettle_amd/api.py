@@ -252,6 +252,12 @@ def set_jit_tuning(min_waves_per_simd: int = 3, blocks_per_cu: int = 0, normalis
     check(load().kq_set_jit_tuning(int(min_waves_per_simd), int(blocks_per_cu), int(bool(normalise))))
 
 
+def set_jit_priority(mode: int) -> None:
+    """Wave priority in the run-time pass kernels: 0 off, 1 raised while a wave issues its tile's loads,
+    2 also while it issues its stores."""
+    check(load().kq_set_jit_priority(int(mode)))
+
+
 def jit_info() -> dict:
     c = _lib.ctypes
     mode, comp, hits, fails, secs, unc = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double(), c.c_size_t()

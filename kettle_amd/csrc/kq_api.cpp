@@ -2295,7 +2295,22 @@ int kq_set_jit(int mode)
 
 int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise)
 {
-    return guarded([&] { set_jit_tuning(JitTuning {min_waves_per_simd, blocks_per_cu, normalise != 0}); });
+    return guarded([&] {
+        JitTuning t = jit_tuning();
+        t.min_waves_per_simd = min_waves_per_simd;
+        t.blocks_per_cu = blocks_per_cu;
+        t.normalise = normalise != 0;
+        set_jit_tuning(t);
+    });
+}
+
+int kq_set_jit_priority(int mode)
+{
+    return guarded([&] {
+        JitTuning t = jit_tuning();
+        t.priority = mode;
+        set_jit_tuning(t);
+    });
 }
 
 int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds,

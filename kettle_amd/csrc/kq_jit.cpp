@@ -389,6 +389,29 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
             return false;
         }
         const std::complex<double> x = (diag_pivot ? m01 : m00) / pivot, y = m10 / pivot, z = m11 / pivot;
+        // A complex matrix whose normalised form is an axis rotation [[1, i t], [i t, 1]] (SX, SXDAG) or
+        // a real rotation / reflection takes the cheaper form: 4 FP64 operations per pair instead of 12.
+        // Entries within a few ulps of the form's zeros count as zero (a change below 1e-15 per gate).
+        constexpr double ULPS = 4.0e-16;
+        const auto near = [](double a, double b) { return std::abs(a - b) <= ULPS * std::max(1.0, std::abs(b)); };
+        if (diag_pivot && near(z.imag(), 0.0)) {
+            if (near(z.real(), 1.0) && near(x.real(), 0.0) && near(y.real(), 0.0) && near(y.imag(), x.imag())) {
+                o << "  axis<R, " << P << ", 0>(v, " << val(x.imag()) << ");\n";
+                factor *= pivot;
+                factor_complex = true;
+                return true;
+            }
+            if (near(x.imag(), 0.0) && near(y.imag(), 0.0)) {
+                const bool rotation = near(z.real(), 1.0) && near(y.real(), -x.real());
+                const bool reflection = near(z.real(), -1.0) && near(y.real(), x.real());
+                if (rotation || reflection) {
+                    o << "  orth<R, " << P << ", " << (rotation ? 0 : 1) << ">(v, " << val(x.real()) << ");\n";
+                    factor *= pivot;
+                    factor_complex = true;
+                    return true;
+                }
+            }
+        }
         const auto xr = val(x.real()), xi = val(x.imag()), yr = val(y.real()), yi = val(y.imag());
         const auto zr = val(z.real()), zi = val(z.imag());
         o << "  dense1<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << xr << ", " << xi << ", " << yr << ", "
@@ -672,8 +695,10 @@ int jit_mode() { return cache().mode; }
 
 void set_jit_tuning(const JitTuning& t)
 {
-    if (t.min_waves_per_simd < 0 || t.min_waves_per_simd > 8 || t.blocks_per_cu < 0 || t.blocks_per_cu > 64) {
-        throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8] and blocks per CU in [0, 64]"};
+    if (t.min_waves_per_simd < 0 || t.min_waves_per_simd > 8 || t.blocks_per_cu < 0 || t.blocks_per_cu > 64
+        || t.priority < 0 || t.priority > 2) {
+        throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8], blocks per CU in [0, 64] and priority "
+                          "in [0, 2]"};
     }
     std::lock_guard<std::mutex> lk(cache().mu);
     cache().tuning = t;
@@ -860,7 +885,13 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         return off;
     };
     if (!persistent) {
+        if (tune.priority > 0) {
+            o << "  wave_priority<3>();\n";
+        }
         io(tp.layouts[0], true);
+        if (tune.priority > 0) {
+            o << "  wave_priority<0>();\n";
+        }
     }
     // pivots divided out of normalised gates, multiplied back in once before the store
     const bool normalise = tune.normalise;
@@ -1029,6 +1060,9 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
             o << "  __syncthreads();\n";
         }
         cur = tp.out_layout;
+    }
+    if (tune.priority > 1) {
+        o << "  wave_priority<2>();\n";
     }
     io(tp.layouts[cur], false);
     o << "  }\n}\n";

@@ -45,6 +45,14 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Wave priority (0..3) for the instructions that follow: a wave that is about to issue its tile's HBM
+// loads or stores goes ahead of the waves that are computing, so the memory pipeline stays fed.
+template <int PRIO>
+__device__ __forceinline__ void wave_priority()
+{
+    __builtin_amdgcn_s_setprio(PRIO);
+}
+
 // the eight coefficients of a gate record, already in registers
 template <typename R>
 struct Coef

@@ -75,6 +75,7 @@ def host_source(src: str, name: str) -> str:
     body = "\n".join(line for line in src.split("\n") if not line.startswith("typedef "))
     body = re.sub(r"__device__ __forceinline__ void lds_barrier\(\)\s*\{[^}]*\}",
                   "inline void lds_barrier() { emu_barrier->arrive_and_wait(); }", body)
+    body = re.sub(r"__builtin_amdgcn_s_setprio\([^)]*\)", "(void) 0", body)
     body = body.replace("__attribute__((always_inline))", "")
     body = re.sub(r"__attribute__\(\(amdgpu_waves_per_eu\([^)]*\)\)\)", "", body)
     body = body.replace("void kq_pass(", f"void {name}(")
