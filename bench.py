@@ -234,7 +234,7 @@ def main() -> int:
     ap.add_argument("--jit-tuning", default="3,0,1",
                     help="min waves per SIMD, persistent blocks per CU, pivot normalisation (0/1) of the "
                          "run-time compiled passes")
-    ap.add_argument("--jit-priority", type=int, default=0,
+    ap.add_argument("--jit-priority", type=int, default=1,
                     help="wave priority around a tile's HBM traffic in the run-time compiled passes: 0 off, "
                          "1 loads, 2 loads and stores")
     ap.add_argument("--calib-copy", type=int, default=0,

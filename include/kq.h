@@ -168,7 +168,7 @@ int kq_set_jit(int mode);
  * normalise != 0 (default) divides each uncontrolled one-qubit gate by its larger first-row entry
  * and multiplies the product of those pivots back in once per pass. */
 int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise);
-/* Wave priority in the run-time pass kernels: 0 (default) leaves it alone, 1 raises it while a wave
+/* Wave priority in the run-time pass kernels: 0 leaves it alone, 1 (default) raises it while a wave
  * issues its tile's HBM loads, 2 also while it issues its stores, so that waves starting or finishing
  * a tile go ahead of the waves computing one.  No counterpart in the reference. */
 int kq_set_jit_priority(int mode);

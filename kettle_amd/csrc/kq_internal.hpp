@@ -441,7 +441,7 @@ struct JitTuning
     int min_waves_per_simd = 3;   // register cap (<= 170 VGPRs): at least 3 waves per SIMD
     int blocks_per_cu = 0;
     bool normalise = true;   // uncontrolled one-qubit gates with their pivot divided out
-    int priority = 0;        // wave priority around HBM traffic: 0 off, 1 loads, 2 loads and stores
+    int priority = 1;        // wave priority around HBM traffic: 0 off, 1 loads, 2 loads and stores
 };
 void set_jit_tuning(const JitTuning& t);
 JitTuning jit_tuning();

@@ -364,7 +364,7 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         factor *= pivot;
         return true;
     }
-    if (!flip.empty() && kind != TK_AXIS) {
+    if (!flip.empty() && kind != TK_DENSE) {
         return false;
     }
     if (kind == TK_DIAG) {
@@ -404,13 +404,18 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
             if (near(x.imag(), 0.0) && near(y.imag(), 0.0)) {
                 const bool rotation = near(z.real(), 1.0) && near(y.real(), -x.real());
                 const bool reflection = near(z.real(), -1.0) && near(y.real(), x.real());
-                if (rotation || reflection) {
-                    o << "  orth<R, " << P << ", " << (rotation ? 0 : 1) << ">(v, " << val(x.real()) << ");\n";
+                if ((rotation || reflection) && (flip.empty() || rotation)) {
+                    const std::string t = val(x.real());
+                    o << "  orth<R, " << P << ", " << (rotation ? 0 : 1) << ">(v, "
+                      << (flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")") << ");\n";
                     factor *= pivot;
                     factor_complex = true;
                     return true;
                 }
             }
+        }
+        if (!flip.empty()) {
+            return false;
         }
         const auto xr = val(x.real()), xi = val(x.imag()), yr = val(y.real()), yi = val(y.imag());
         const auto zr = val(z.real()), zi = val(z.imag());
@@ -434,6 +439,23 @@ std::array<std::complex<double>, 2> term_values(const TileTerm& t, bool c64)
     return {std::complex<double> {f[0], f[1]}, std::complex<double> {f[2], f[3]}};
 }
 
+// diag(d0, d1) on register bit key / 5 with register control key % 5 (d0_one: d0 is exactly 1).  Under
+// `flip` (controlled X gates still pending on that bit, key % 5 == 0) the lanes where it holds apply
+// diag(d1, d0), and the X gates stay pending.
+void emit_diag_apply(std::ostringstream& o, uint32_t key, const std::string& d0r, const std::string& d0i,
+                     const std::string& d1r, const std::string& d1i, bool d0_one, const std::string& flip)
+{
+    const std::string m = "{{" + d0r + ", " + d0i + ", 0, 0, 0, 0, " + d1r + ", " + d1i + "}}";
+    if (flip.empty()) {
+        o << "  { const Coef<R> m " << m << "; apply<R, " << (d0_one ? "K_DIAG1" : "K_DIAG") << ", " << key / 5 << ", "
+          << key % 5 << ">(v, m); }\n";
+    }
+    else {
+        o << "  { const Coef<R> m = flip_coef<R>(Coef<R> " << m << ", " << flip << "); apply<R, K_DIAG, " << key / 5
+          << ", 0>(v, m); }\n";
+    }
+}
+
 // The factor of a merged run of pure-phase terms with d0 = 1 (or, for a run on a thread or outside
 // bit, a factor selected by that bit) from tables built on the host: the product over the terms
 // conditioned on thread bits only is one entry of a table indexed by the thread index; the terms
@@ -443,7 +465,8 @@ std::array<std::complex<double>, 2> term_values(const TileTerm& t, bool c64)
 // FP64), so the device does a few loads and complex products in place of a sum per term and a
 // sincos.  The factor goes to `pre` (before the amplitudes are waited for), its use to `o`.
 void emit_phase_tables(std::ostringstream& o, std::ostringstream& pre, const std::string& v, uint32_t key,
-                       const TileGate& g, const TileProgram& tp, bool c64, std::vector<double>& coefs)
+                       const TileGate& g, const TileProgram& tp, bool c64, std::vector<double>& coefs,
+                       const std::string& flip)
 {
     const bool uniform = key >= 20;
     struct PT
@@ -595,8 +618,7 @@ void emit_phase_tables(std::ostringstream& o, std::ostringstream& pre, const std
         o << "  scale<R, " << (key - 20) << ">(v, " << c1 << ", " << s1 << ");\n";
     }
     else {
-        o << "  { const Coef<R> m {{1, 0, 0, 0, 0, 0, " << c1 << ", " << s1 << "}}; apply<R, K_DIAG1, " << key / 5 << ", "
-          << key % 5 << ">(v, m); }\n";
+        emit_diag_apply(o, key, "1", "0", c1, s1, true, flip);
     }
 }
 
@@ -608,7 +630,7 @@ void emit_phase_tables(std::ostringstream& o, std::ostringstream& pre, const std
 // overlaps the loads); the sums run in four accumulators (short dependency chains).  `o` gets the
 // multiplication of the amplitudes.
 bool emit_phase_run(std::ostringstream& o, std::ostringstream& pre, int id, const TileGate& g, const TileProgram& tp,
-                    bool c64, std::vector<double>& coefs)
+                    bool c64, std::vector<double>& coefs, const std::string& flip)
 {
     // one FP64 sincos costs about as much as the complex products of eight terms
     constexpr uint64_t MIN_TERMS = 8;
@@ -628,7 +650,7 @@ bool emit_phase_run(std::ostringstream& o, std::ostringstream& pre, int id, cons
     const bool need0 = any0 && !uniform;
     const std::string v = "ph" + std::to_string(id) + "_";
     if (!need0) {
-        emit_phase_tables(o, pre, v, key, g, tp, c64, coefs);
+        emit_phase_tables(o, pre, v, key, g, tp, c64, coefs, flip);
         return true;
     }
     pre << "  double " << v << "s1, " << v << "c1" << (need0 ? ", " + v + "s0, " + v + "c0" : std::string {}) << ";\n"
@@ -668,12 +690,10 @@ bool emit_phase_run(std::ostringstream& o, std::ostringstream& pre, int id, cons
         return true;
     }
     if (need0) {
-        o << "  { const Coef<R> m {{static_cast<R>(" << v << "c0), static_cast<R>(" << v << "s0), 0, 0, 0, 0, " << c1 << ", " << s1
-          << "}}; apply<R, K_DIAG, " << key / 5 << ", " << key % 5 << ">(v, m); }\n";
+        emit_diag_apply(o, key, "static_cast<R>(" + v + "c0)", "static_cast<R>(" + v + "s0)", c1, s1, false, flip);
     }
     else {
-        o << "  { const Coef<R> m {{1, 0, 0, 0, 0, 0, " << c1 << ", " << s1 << "}}; apply<R, K_DIAG1, " << key / 5 << ", "
-          << key % 5 << ">(v, m); }\n";
+        emit_diag_apply(o, key, "1", "0", c1, s1, true, flip);
     }
     return true;
 }
@@ -972,13 +992,31 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
                     // an X on the same register bit (whatever controls it) commutes with the pending ones
                     bits &= ~pbit;
                 }
-                else if (!flip.empty() && normalise && g.code % 5 == 0 && g.lane_ctrl == 0 && g.glb_ctrl == 0
-                         && (kind == TK_REAL || kind == TK_AXIS)) {
+                else if (!flip.empty() && g.code % 5 == 0 && g.lane_ctrl == 0 && g.glb_ctrl == 0) {
+                    // the pending X gates stay pending: where they apply, the lanes run X G X instead of G
                     materialise(bits & ~pbit);
-                    if (emit_normalised(o, g, c64, coefs, factor, factor_complex, flip)) {
+                    if (normalise && (kind == TK_REAL || kind == TK_AXIS || kind == TK_DENSE)
+                        && emit_normalised(o, g, c64, coefs, factor, factor_complex, flip)) {
                         normalised = true;
                         continue;
                     }
+                    const size_t off = push_gate_coefs(g);
+                    o << "  { const Coef<R> m = flip_coef<R>(coef_at<R>(cf + " << off << "), " << flip << "); apply<R, "
+                      << (kind == TK_DIAG1 ? static_cast<uint32_t>(TK_DIAG) : kind) << ", " << Pg << ", 0>(v, m); }\n";
+                    continue;
+                }
+            }
+            // a merged diagonal run on a register bit with controlled X gates pending on it runs flipped
+            std::string run_flip;
+            if (g.code >= TV_MERGE_BASE && g.code - TV_MERGE_BASE < 20 && (g.code - TV_MERGE_BASE) % 5 == 0) {
+                const int Pm = static_cast<int>((g.code - TV_MERGE_BASE) / 5);
+                for (const PendingX& x : pending) {
+                    if (x.P == Pm) {
+                        run_flip += (run_flip.empty() ? "(" : " != (") + x.cond + ")";
+                    }
+                }
+                if (!run_flip.empty()) {
+                    bits &= ~(1u << Pm);
                 }
             }
             materialise(bits);
@@ -987,7 +1025,7 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
                 pending.push_back(PendingX {static_cast<int>((g.code / 5) % 4), cond_expr(g.glb_ctrl, g.lane_ctrl), g.glb_ctrl, g.lane_ctrl});
                 continue;
             }
-            if (g.code >= TV_MERGE_BASE && emit_phase_run(o, pre, n_runs, g, tp, c64, coefs)) {
+            if (g.code >= TV_MERGE_BASE && emit_phase_run(o, pre, n_runs, g, tp, c64, coefs, run_flip)) {
                 ++n_runs;
                 continue;
             }
@@ -1012,8 +1050,8 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
                     o << "    scale<R, " << (key - 20) << ">(v, f1r, f1i);\n  }\n";
                 }
                 else {
-                    o << "    const Coef<R> m {{f0r, f0i, 0, 0, 0, 0, f1r, f1i}};\n    apply<R, K_DIAG, " << key / 5 << ", "
-                      << key % 5 << ">(v, m);\n  }\n";
+                    emit_diag_apply(o, key, "f0r", "f0i", "f1r", "f1i", false, run_flip);
+                    o << "  }\n";
                 }
                 continue;
             }

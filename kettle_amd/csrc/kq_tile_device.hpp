@@ -60,6 +60,15 @@ struct Coef
     R m[8];
 };
 
+// The coefficients of X G X (G with its rows and its columns exchanged) where `fl` holds, else G's own:
+// a gate that meets controlled X gates still pending on its bit runs in this form, and they stay pending.
+template <typename R>
+__device__ __forceinline__ Coef<R> flip_coef(const Coef<R>& c, bool fl)
+{
+    return {{fl ? c.m[6] : c.m[0], fl ? c.m[7] : c.m[1], fl ? c.m[4] : c.m[2], fl ? c.m[5] : c.m[3],
+             fl ? c.m[2] : c.m[4], fl ? c.m[3] : c.m[5], fl ? c.m[0] : c.m[6], fl ? c.m[1] : c.m[7]}};
+}
+
 // compile-time loop over register slots
 template <int I, int N>
 struct Unroll

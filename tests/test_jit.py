@@ -179,6 +179,32 @@ def test_jit_tunings_match_the_oracle(tuning):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_jit_flipped_forms_20q(precision):
+    """Six low qubits per tile and up to 16 stages on the layered circuit: one-qubit gates (dense, real,
+    diagonal) and merged diagonal runs that meet controlled X gates still pending on their register bit
+    run as X G X under the X gates' condition, against the oracle."""
+    from kettle_amd import workloads
+    n = 20
+    c = workloads.qft_plus_layers(n=n, layers=12)
+    ops = ops_of(c)
+    ka.set_fusion_low_qubits(6)
+    ka.set_fusion(-1, 4096, 16, 1.0e9)
+    try:
+        src = "".join(ka.debug_jit_source(ops, n, i)[0] for i in range(ka.debug_jit_source(ops, n, 0)[1]))
+        assert "flip_coef<R>(coef_at" in src and "flip_coef<R>(Coef<R>" in src
+        ka.set_jit(2)
+        sv = ka.Statevector(n, precision)
+        ka.apply_gates(sv, ops)
+        out = sv.download()
+    finally:
+        ka.set_fusion_low_qubits(5)
+        ka.set_fusion_autotune(True)
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops)
+    assert np.max(np.abs(out - ref)) <= (TOL128 if precision == ka.KQ_C128 else TOL64)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("normalise", [True, False], ids=["normalised", "plain"])
 def test_jit_layered_workload_20q(normalise):
     """QFT + random one-qubit layers with CX brick ladders at 20 qubits: controlled X gates with

@@ -35,6 +35,18 @@ def test_layered_circuit_16q(relabel):
     check(workloads.qft_plus_layers(n=16, layers=8), 16, ka.KQ_C128, 1e-12)
 
 
+def test_layered_circuit_flipped_runs_16q():
+    """Tiles of six low qubits and up to 16 stages: merged diagonal runs and one-qubit gates meet controlled
+    X gates still pending on their register bit and run in the flipped (X G X) form."""
+    ka.set_fusion_low_qubits(6)
+    ka.set_fusion(-1, 0, 16, 0.0)
+    try:
+        check(workloads.qft_plus_layers(n=16, layers=12), 16, ka.KQ_C128, 1e-12)
+    finally:
+        ka.set_fusion(-1, 0, 12, 0.0)
+        ka.set_fusion_autotune(True)
+
+
 @pytest.mark.parametrize("n", [13, 15])
 def test_random_gates_complex128(n):
     check(random_gate_circuit(n, 250, np.random.default_rng(9100 + n)), n, ka.KQ_C128, 1e-12)
