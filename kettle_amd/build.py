@@ -22,7 +22,8 @@ BUILD_DIR = ROOT / "kettle_amd" / "_build"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_batch.cpp", "kq_jit.cpp"]
+KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_run.cpp", "kq_measure.cpp", "kq_pauli.cpp",
+              "kq_adjoint.cpp", "kq_debug.cpp", "kq_batch.cpp", "kq_jit.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -60,7 +61,7 @@ def embed_device_header() -> Path:
     return out
 
 
-def build_kq(jobs: int = 4) -> Path:
+def build_kq(jobs: int = 8) -> Path:
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     LIB_DIR.mkdir(parents=True, exist_ok=True)
     embed_device_header()

@@ -1,0 +1,87 @@
+// Host-side helpers shared by the C-ABI translation units of libkq (kq_api.cpp, kq_run.cpp,
+// kq_measure.cpp, kq_pauli.cpp, kq_adjoint.cpp, kq_debug.cpp).
+#pragma once
+
+#include <cstdint>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kq_internal.hpp"
+
+namespace kq
+{
+
+// Runs the body of a C entry point: maps the library's exception types to kq.h status codes and
+// keeps the message for kq_last_error().
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return KQ_OK;
+    }
+    catch (const InvalidArg& e) {
+        set_error(e.what());
+        return KQ_ERR_INVALID;
+    }
+    catch (const HipError& e) {
+        set_error(e.what());
+        return KQ_ERR_HIP;
+    }
+    catch (const AllocError& e) {
+        set_error(e.what());
+        return KQ_ERR_ALLOC;
+    }
+    catch (const std::out_of_range& e) {
+        set_error(e.what());
+        return KQ_ERR_OUT_OF_RANGE;
+    }
+    catch (const std::exception& e) {
+        set_error(e.what());
+        return KQ_ERR_RUNTIME;
+    }
+    catch (...) {
+        set_error("unknown exception");
+        return KQ_ERR_RUNTIME;
+    }
+}
+
+// ---- library and states (kq_api.cpp) ----------------------------------------------------------------
+void ensure_init();                 // selects the library's device (initialises it once)
+void check_state(const kq_state* s);
+void* device_alloc(size_t bytes);   // hipMalloc or AllocError
+void* grow(void*& buf, size_t& have, size_t need);   // device buffer of at least `need` bytes
+kq_state* create_state(uint32_t n, int precision, bool dm);
+
+// ---- gate runs (kq_run.cpp) ---------------------------------------------------------------------------
+// device records of a straight-line list through the fused passes (qubit moves where allowed)
+void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relabel = true);
+void run_ops(kq_state* s, const kq_op* ops, size_t n);
+// puts qubit q back on index bit q (after gate passes that moved qubits)
+void normalise_layout(kq_state* s);
+void normalise_layout(const kq_state* s);
+// SWAP triples dropped as relabels: sigma receives the qubit permutation they leave
+std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uint32_t nvec, std::vector<uint8_t>& sigma);
+PermuteTables permute_tables(const std::vector<uint8_t>& bit_of);
+
+// ---- measurement, interpreter, sampling (kq_measure.cpp) ---------------------------------------------
+std::mt19937 get_prng(bool has_seed, int32_t seed);
+int measure(kq_state* s, uint32_t q, bool has_seed, int32_t seed);
+std::vector<double> download_host(const kq_state* s);
+void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds, size_t n_preds,
+              bool has_seed, int32_t seed, int8_t* creg, size_t n_bits, kq_logger_fn logger, void* logger_user);
+void sample_from_device_probs(hipStream_t st, double* d_p, uint64_t n, uint64_t shots, bool has_seed, int32_t seed,
+                              uint64_t* out);
+void check_noise(const double* noise, uint32_t n);
+void sample_circuit_shots(const kq_state* init, uint32_t circuit_n, const kq_op* ops, size_t n_ops, const int32_t* preds,
+                          size_t n_preds, size_t n_bits, const double* noise, uint64_t n_shots, bool has_seed, int32_t seed,
+                          uint64_t max_cached_bytes, uint64_t* out, uint64_t* n_branches);
+
+// ---- adjoint gradient (kq_adjoint.cpp) -----------------------------------------------------------------
+void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strings, const uint32_t* off,
+                     const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs, size_t n_params,
+                     double* energy, double* grad, uint32_t checkpoint_every, double drift_tol, size_t* n_restores);
+
+}  // namespace kq

@@ -113,7 +113,6 @@ struct BankRule
     int read_lanes;
     bool read_hash34;
 };
-constexpr BankRule BANKS_16B {3, 4, true};
 constexpr BankRule BANKS_8B {4, 5, false};
 
 // The LDS entries of one remap.  The tile is stored at entry F(l) of local index l, F(l) = l ^ G(l)
