@@ -326,9 +326,11 @@ void launch_noise(hipStream_t st, double* d_probs, uint32_t n, uint32_t q, doubl
 // inclusive scan of d_probs in place; returns last value and the largest value < last
 void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, size_t scratch_bytes,
                 double* last, double* second_last);
-void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots,
+// per-shot index; cdf_scratch: the scratch launch_cdf used (its positive-probability bits)
+void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots, void* cdf_scratch,
                         uint64_t* d_out);
 size_t cdf_scratch_bytes(uint64_t n);
+
 
 struct PauliGroup
 {

@@ -324,14 +324,23 @@ __device__ double block_exclusive_scan(double v, double* total)
     return wave_off + incl - v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_block_sums(const double* p, uint64_t n, double* sums)
+// Block sums of the scan, and one bit per entry of whether its probability is positive (8 entries per
+// thread, one byte of `positive` each).
+__global__ __launch_bounds__(BLOCK) void k_scan_block_sums(const double* p, uint64_t n, double* sums, uint8_t* positive)
 {
+    static_assert(SCAN_PER_THREAD == 8, "one byte of positive-probability bits per thread");
     const uint64_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * uint64_t {SCAN_PER_THREAD};
     double s = 0.0;
+    uint32_t bits = 0;
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
         if (base + j < n) {
-            s += p[base + j];
+            const double x = p[base + j];
+            s += x;
+            bits |= (x > 0.0 ? 1u : 0u) << j;
         }
+    }
+    if (base < n) {
+        positive[base / 8] = static_cast<uint8_t>(bits);
     }
     double total;
     block_exclusive_scan(s, &total);
@@ -405,8 +414,28 @@ __global__ __launch_bounds__(BLOCK) void k_last_below(const double* cdf, uint64_
     }
 }
 
-// std::ranges::lower_bound per shot (measurements.cpp:297-312)
-__global__ void k_lower_bound(const double* cdf, uint64_t n, const double* u, uint64_t shots, uint64_t* out)
+// std::ranges::lower_bound per shot (measurements.cpp:297-312).  The device CDF sums in a tree order, so
+// at a thread or block boundary it may differ from a sequential partial sum by an ulp, in either
+// direction; a draw in that sliver may land on an entry of probability zero, which a sequential CDF
+// never returns.  Such a draw belongs to the next entry of positive probability (the previous one when
+// only zeros follow): the search steps there using the positive-probability bits of the scan.
+__device__ uint64_t next_positive(const uint8_t* positive, uint64_t n, uint64_t i)
+{
+    for (uint64_t j = i; j < n; ++j) {
+        if ((positive[j >> 3] >> (j & 7)) & 1u) {
+            return j;
+        }
+    }
+    for (uint64_t j = i; j-- > 0;) {
+        if ((positive[j >> 3] >> (j & 7)) & 1u) {
+            return j;
+        }
+    }
+    return i;
+}
+
+__global__ void k_lower_bound(const double* cdf, uint64_t n, const double* u, uint64_t shots, const uint8_t* positive,
+                              uint64_t* out)
 {
     const uint64_t s = blockIdx.x * uint64_t {BLOCK} + threadIdx.x;
     if (s >= shots) {
@@ -423,6 +452,9 @@ __global__ void k_lower_bound(const double* cdf, uint64_t n, const double* u, ui
         else {
             len = half;
         }
+    }
+    if (lo < n && !((positive[lo >> 3] >> (lo & 7)) & 1u)) {
+        lo = next_positive(positive, n, lo);
     }
     out[s] = lo;
 }
@@ -1080,11 +1112,21 @@ void launch_noise(hipStream_t st, double* d_probs, uint32_t n, uint32_t q, doubl
     KQ_HIP_CHECK(hipGetLastError());
 }
 
+// scratch of the CDF: block sums, the index of the last entry below the total, positive-probability bits
 size_t cdf_scratch_bytes(uint64_t n)
 {
     const uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    return nb * sizeof(double) + 64;
+    return nb * sizeof(double) + 64 + (n + 7) / 8;
 }
+
+namespace
+{
+uint8_t* cdf_positive_bits(void* scratch_ptr, uint64_t n)
+{
+    const uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    return static_cast<uint8_t*>(scratch_ptr) + nb * sizeof(double) + 64;
+}
+}  // namespace
 
 void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, size_t scratch_bytes, double* last,
                 double* second_last)
@@ -1095,7 +1137,7 @@ void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, si
     }
     auto* sums = static_cast<double*>(scratch_ptr);
     auto* best = reinterpret_cast<long long*>(static_cast<char*>(scratch_ptr) + nb * sizeof(double));
-    k_scan_block_sums<<<static_cast<unsigned>(nb), BLOCK, 0, st>>>(d_cdf, n, sums);
+    k_scan_block_sums<<<static_cast<unsigned>(nb), BLOCK, 0, st>>>(d_cdf, n, sums, cdf_positive_bits(scratch_ptr, n));
     KQ_HIP_CHECK(hipGetLastError());
     k_scan_sums<<<1, BLOCK, 0, st>>>(sums, nb);
     KQ_HIP_CHECK(hipGetLastError());
@@ -1117,12 +1159,14 @@ void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, si
     }
 }
 
-void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots, uint64_t* d_out)
+void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots, void* cdf_scratch,
+                        uint64_t* d_out)
 {
     if (shots == 0) {
         return;
     }
-    k_lower_bound<<<static_cast<unsigned>((shots + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(d_cdf, n, d_u, shots, d_out);
+    k_lower_bound<<<static_cast<unsigned>((shots + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(d_cdf, n, d_u, shots,
+                                                                                       cdf_positive_bits(cdf_scratch, n), d_out);
     KQ_HIP_CHECK(hipGetLastError());
 }
 

@@ -211,34 +211,47 @@ void simulate(kq_state* s, uint32_t circuit_n, const kq_op* ops, size_t n_ops, c
 // ProbabilitySampler_ (measurements.cpp:288-312) with the CDF on the device.  The uniform draws
 // use kettle's std::uniform_real_distribution(0, last - offset) on std::mt19937 (libstdc++), so
 // the draw sequence is kettle's; lower_bound runs per shot on the device.
+// The sampler over device probabilities d_p (scanned into their CDF in place) with a caller-provided
+// device workspace `sc` of sc_bytes (CDF partials, the uniform draws, the drawn indices).  The host
+// draws are staged through `out` itself, which the indices then overwrite (same stream, in order).
+void sample_with_workspace(hipStream_t st, double* d_p, uint64_t n, uint64_t shots, bool has_seed, int32_t seed,
+                           uint64_t* out, void* sc, size_t sc_bytes)
+{
+    const size_t sb = (cdf_scratch_bytes(n) + 255) / 256 * 256;
+    if (sc_bytes < sb + shots * (sizeof(double) + sizeof(uint64_t))) {
+        throw InvalidArg {"sampler: workspace too small"};
+    }
+    double last = 0.0, second_last = 0.0;
+    launch_cdf(st, d_p, n, sc, sb, &last, &second_last);
+    const double offset = (last - second_last) * 1.0e-4;
+    auto prng = get_prng(has_seed, seed);
+    auto dist = std::uniform_real_distribution<double> {0.0, last - offset};
+    for (uint64_t i = 0; i < shots; ++i) {
+        const double u = dist(prng);
+        std::memcpy(out + i, &u, sizeof u);
+    }
+    auto* d_u = reinterpret_cast<double*>(static_cast<char*>(sc) + sb);
+    auto* d_out = reinterpret_cast<uint64_t*>(d_u + shots);
+    if (shots) {
+        KQ_HIP_CHECK(hipMemcpyAsync(d_u, out, shots * sizeof(double), hipMemcpyHostToDevice, st));
+        launch_lower_bound(st, d_p, n, d_u, shots, sc, d_out);
+        KQ_HIP_CHECK(hipMemcpyAsync(out, d_out, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    }
+    KQ_HIP_CHECK(hipStreamSynchronize(st));
+    for (uint64_t i = 0; i < shots; ++i) {
+        if (out[i] >= n) {
+            throw std::runtime_error {"LOGIC BUG: Ended up with measurement of state past end of cumulative\nprobability distribution, which shouldn't happen?"};
+        }
+    }
+}
+
 void sample_from_device_probs(hipStream_t st, double* d_p, uint64_t n, uint64_t shots, bool has_seed, int32_t seed,
                               uint64_t* out)
 {
-    const size_t sb = cdf_scratch_bytes(n);
-    void* sc = device_alloc(sb + shots * (sizeof(double) + sizeof(uint64_t)) + 256);
+    const size_t bytes = (cdf_scratch_bytes(n) + 255) / 256 * 256 + shots * (sizeof(double) + sizeof(uint64_t));
+    void* sc = device_alloc(bytes);
     try {
-        double last = 0.0, second_last = 0.0;
-        launch_cdf(st, d_p, n, sc, sb, &last, &second_last);
-        const double offset = (last - second_last) * 1.0e-4;
-        auto prng = get_prng(has_seed, seed);
-        auto dist = std::uniform_real_distribution<double> {0.0, last - offset};
-        std::vector<double> u(shots);
-        for (auto& x : u) {
-            x = dist(prng);
-        }
-        auto* d_u = reinterpret_cast<double*>(static_cast<char*>(sc) + ((sb + 127) / 128) * 128);
-        auto* d_out = reinterpret_cast<uint64_t*>(d_u + shots);
-        if (shots) {
-            KQ_HIP_CHECK(hipMemcpyAsync(d_u, u.data(), shots * sizeof(double), hipMemcpyHostToDevice, st));
-            launch_lower_bound(st, d_p, n, d_u, shots, d_out);
-            KQ_HIP_CHECK(hipMemcpyAsync(out, d_out, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        }
-        KQ_HIP_CHECK(hipStreamSynchronize(st));
-        for (uint64_t i = 0; i < shots; ++i) {
-            if (out[i] >= n) {
-                throw std::runtime_error {"LOGIC BUG: Ended up with measurement of state past end of cumulative\nprobability distribution, which shouldn't happen?"};
-            }
-        }
+        sample_with_workspace(st, d_p, n, shots, has_seed, seed, out, sc, bytes);
     }
     catch (...) {
         (void) hipFree(sc);
