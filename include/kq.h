@@ -211,6 +211,16 @@ int kq_sample_circuit_shots(const kq_state* initial, uint32_t circuit_n_qubits, 
 int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_shots,
                             int has_seed, int32_t seed, uint64_t* out_indices);
 
+/* kq_sample_probabilities without any device allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The probabilities, their CDF, the draws and the indices live in a
+ * fixed workspace of the library, 24.2 MiB per device, held from the first use on a device until the
+ * process ends; the first call on a device also creates the stream these calls run on.  Limits:
+ * n_probs <= 2^20 and n_shots <= 2^20, else KQ_ERR_INVALID.  Every call on one device shares that
+ * device's workspace: calls that overlap on one GPU (from several host threads) race. */
+int kq_sample_probabilities_noalloc(const double* probs, uint64_t n_probs, uint64_t n_shots,
+                                    int has_seed, int32_t seed, uint64_t* out_indices);
+
 /* ---- Pauli strings (operator/pauli/pauli_operator.cpp:76-105, simulation/simulate_pauli.cpp)
  * A Pauli operator of S strings is given as:
  *   term_offsets[S+1]: string s owns terms [term_offsets[s], term_offsets[s+1])

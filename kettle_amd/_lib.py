@@ -74,6 +74,7 @@ _SIGNATURES = {
     "kq_probabilities": (c_int, [c_void_p, c_void_p, c_void_p]),
     "kq_sample": (c_int, [c_void_p, c_void_p, c_uint64, c_int, c_int32, c_void_p]),
     "kq_sample_probabilities": (c_int, [c_void_p, c_uint64, c_uint64, c_int, c_int32, c_void_p]),
+    "kq_sample_probabilities_noalloc": (c_int, [c_void_p, c_uint64, c_uint64, c_int, c_int32, c_void_p]),
     "kq_pauli_expectation": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "kq_apply_pauli_string": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p]),
     "kq_dm_trace": (c_int, [c_void_p, c_void_p]),

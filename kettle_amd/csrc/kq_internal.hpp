@@ -331,6 +331,21 @@ void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const d
                         uint64_t* d_out);
 size_t cdf_scratch_bytes(uint64_t n);
 
+// Fixed device workspace of the allocation-free sampler (kq_sample_probabilities_noalloc): up to
+// NOALLOC_MAX_PROBS probabilities (scanned into their CDF in place), the scan partials, and up to
+// NOALLOC_MAX_SHOTS uniform draws and drawn indices.  One copy per device, held until the process ends.
+constexpr uint64_t NOALLOC_MAX_PROBS = uint64_t {1} << 20;
+constexpr uint64_t NOALLOC_MAX_SHOTS = uint64_t {1} << 20;
+constexpr size_t NOALLOC_CDF_BYTES = (NOALLOC_MAX_PROBS / 8) + 8192;
+constexpr size_t NOALLOC_SCRATCH_BYTES = NOALLOC_MAX_PROBS * 8 + NOALLOC_CDF_BYTES + NOALLOC_MAX_SHOTS * 16;
+struct NoallocWorkspace
+{
+    void* ptr;               // device address of the workspace on the current device
+    hipStream_t stream;      // the stream its calls run on (one per device)
+};
+// the current device's workspace: its address (hipGetSymbolAddress) and stream, looked up and created
+// at the first call on that device
+NoallocWorkspace noalloc_workspace();
 
 struct PauliGroup
 {

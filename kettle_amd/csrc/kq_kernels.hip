@@ -12,9 +12,16 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "kq_internal.hpp"
+
+// The allocation-free sampler's workspace (kq::NOALLOC_SCRATCH_BYTES per device).  In host code this
+// name is only a stand-in for the symbol: kernels and copies use the device address that
+// kq::noalloc_workspace() obtains with hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kq::NOALLOC_SCRATCH_BYTES];
 
 namespace kq
 {
@@ -1157,6 +1164,31 @@ void launch_cdf(hipStream_t st, double* d_cdf, uint64_t n, void* scratch_ptr, si
         KQ_HIP_CHECK(hipMemcpyAsync(second_last, d_cdf + j, sizeof(double), hipMemcpyDeviceToHost, st));
         KQ_HIP_CHECK(hipStreamSynchronize(st));
     }
+}
+
+static_assert((NOALLOC_MAX_PROBS + SCAN_CHUNK - 1) / SCAN_CHUNK * sizeof(double) + 64 + NOALLOC_MAX_PROBS / 8
+                  <= NOALLOC_CDF_BYTES,
+              "the allocation-free sampler's scan partials must fit their share of the workspace");
+
+NoallocWorkspace noalloc_workspace()
+{
+    int dev = 0;
+    KQ_HIP_CHECK(hipGetDevice(&dev));
+    static std::mutex mu;
+    static std::map<int, NoallocWorkspace> per_device;
+    std::lock_guard<std::mutex> lock {mu};
+    const auto it = per_device.find(dev);
+    if (it != per_device.end()) {
+        return it->second;
+    }
+    NoallocWorkspace ws {nullptr, nullptr};
+    KQ_HIP_CHECK(hipGetSymbolAddress(&ws.ptr, HIP_SYMBOL(::scratch)));
+    if (ws.ptr == nullptr) {
+        throw HipError {"hipGetSymbolAddress returned no address for the sampler workspace"};
+    }
+    KQ_HIP_CHECK(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
+    per_device.emplace(dev, ws);
+    return ws;
 }
 
 void launch_lower_bound(hipStream_t st, const double* d_cdf, uint64_t n, const double* d_u, uint64_t shots, void* cdf_scratch,

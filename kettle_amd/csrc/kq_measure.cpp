@@ -655,4 +655,29 @@ int kq_sample_probabilities(const double* probs, uint64_t n_probs, uint64_t n_sh
     });
 }
 
+int kq_sample_probabilities_noalloc(const double* probs, uint64_t n_probs, uint64_t n_shots, int has_seed, int32_t seed,
+                                    uint64_t* out_indices)
+{
+    return guarded([&] {
+        if (!probs || n_probs == 0) {
+            throw InvalidArg {"empty probability array"};
+        }
+        if (n_shots > 0 && !out_indices) {
+            throw InvalidArg {"null output"};
+        }
+        if (n_probs > NOALLOC_MAX_PROBS || n_shots > NOALLOC_MAX_SHOTS) {
+            throw InvalidArg {"allocation-free sampling takes at most " + std::to_string(NOALLOC_MAX_PROBS)
+                              + " probabilities and " + std::to_string(NOALLOC_MAX_SHOTS) + " shots (got "
+                              + std::to_string(n_probs) + " and " + std::to_string(n_shots) + ")"};
+        }
+        ensure_init();
+        const NoallocWorkspace ws = noalloc_workspace();
+        auto* base = static_cast<char*>(ws.ptr);
+        auto* d_p = reinterpret_cast<double*>(base);
+        KQ_HIP_CHECK(hipMemcpyAsync(d_p, probs, n_probs * sizeof(double), hipMemcpyHostToDevice, ws.stream));
+        sample_with_workspace(ws.stream, d_p, n_probs, n_shots, has_seed != 0, seed, out_indices,
+                              base + NOALLOC_MAX_PROBS * sizeof(double), NOALLOC_SCRATCH_BYTES - NOALLOC_MAX_PROBS * sizeof(double));
+    });
+}
+
 }  // extern "C"
