@@ -65,6 +65,10 @@ public:
     [[nodiscard]] auto n_qubits() const noexcept -> std::size_t { return n_qubits_; }
     [[nodiscard]] auto precision() const noexcept -> Precision { return precision_; }
     [[nodiscard]] auto trace() const -> std::complex<double>;
+    // |Trace(matrix^2) - 1| within tolerance, as std::norm (density_matrix.hpp:86-91)
+    [[nodiscard]] auto is_pure(double tolerance = DENSITY_MATRIX_TRACE_TOLERANCE) const -> bool;
+    // the trace of matrix * matrix (density_matrix.hpp:100-104)
+    [[nodiscard]] auto trace_of_square() const -> std::complex<double>;
 
     [[nodiscard]] auto device_state() const -> kq_state*;
     void mark_device_modified() const;

@@ -236,6 +236,10 @@ int kq_apply_pauli_string(kq_state* s, size_t n_terms, const uint32_t* term_qubi
 
 /* ---- density-matrix read-out (state/density_matrix.cpp:150-278) ------------------------- */
 int kq_dm_trace(kq_state* s, double out[2]);
+/* DensityMatrix::trace_of_square (density_matrix.hpp:100-104, density_matrix.cpp:164-168): the trace
+ * of rho * rho, sum_{r,c} rho(r,c) rho(c,r), accumulated in FP64; kettle's is_pure(tol)
+ * (density_matrix.cpp:151-155) is std::norm(trace_of_square - 1) <= tol on top of it. */
+int kq_dm_trace_of_square(kq_state* s, double out[2]);
 /* partial_trace(rho, qubits): traces out `qubits`; out: (2^(n-k))^2 complex, column-major. */
 int kq_dm_partial_trace(kq_state* s, const uint32_t* qubits, size_t n, double* out);
 /* |psi><psi| of a statevector (statevector_to_density_matrix, density_matrix.cpp:170-185). */

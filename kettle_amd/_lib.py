@@ -78,6 +78,7 @@ _SIGNATURES = {
     "kq_pauli_expectation": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "kq_apply_pauli_string": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p]),
     "kq_dm_trace": (c_int, [c_void_p, c_void_p]),
+    "kq_dm_trace_of_square": (c_int, [c_void_p, c_void_p]),
     "kq_dm_partial_trace": (c_int, [c_void_p, c_void_p, c_size_t, c_void_p]),
     "kq_dm_from_statevector": (c_int, [c_void_p, c_void_p]),
     "kq_energy_gradient": (

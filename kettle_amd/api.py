@@ -129,6 +129,17 @@ class DensityMatrix(_State):
         check(load().kq_dm_trace(self._h, ptr(out)))
         return complex(out[0], out[1])
 
+    def trace_of_square(self) -> complex:
+        """Trace of rho * rho (density_matrix.cpp:164-168), FP64 on the device."""
+        out = np.zeros(2)
+        check(load().kq_dm_trace_of_square(self._h, ptr(out)))
+        return complex(out[0], out[1])
+
+    def is_pure(self, tolerance: float = 1.0e-8) -> bool:
+        """kettle's |Trace(rho^2) - 1|^2 <= tolerance (std::norm, density_matrix.cpp:151-155)."""
+        d = self.trace_of_square() - 1.0
+        return d.real * d.real + d.imag * d.imag <= tolerance
+
     @staticmethod
     def from_statevector(sv: Statevector) -> "DensityMatrix":
         dm = DensityMatrix(sv.n_qubits(), sv.precision)

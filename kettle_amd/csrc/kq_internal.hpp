@@ -401,6 +401,8 @@ void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const st
 void launch_pauli_apply_one(kq_state* s, uint64_t x, uint64_t z, const double w[2]);
 
 void launch_dm_trace(kq_state* s, double out[2]);
+// tr(rho^2) = sum_{r,c} rho(r,c) rho(c,r), FP64
+void launch_dm_trace_of_square(kq_state* s, double out[2]);
 void launch_dm_partial_trace(kq_state* s, const std::vector<uint32_t>& traced, double* out_host);
 void launch_dm_from_sv(kq_state* dm, const kq_state* sv);
 

@@ -584,6 +584,46 @@ __global__ __launch_bounds__(BLOCK) void k_dm_trace(const Amp<R>* a, uint64_t N,
     }
 }
 
+// trace_of_square (density_matrix.cpp:164-168): tr(rho^2) = sum_{r,c} rho(r,c) rho(c,r) in FP64, over pairs
+// (I <= J) of T x T blocks of rho; a pair and its mirror (J, I) give the same products, so an
+// off-diagonal pair counts twice.  A block's rho(c, r) tile is staged through LDS (transposed) so both
+// tiles are read in coalesced columns.  Each workgroup walks the pairs with a grid stride in a fixed
+// order and writes one partial; the partials are reduced in a fixed order (bit-reproducible).
+constexpr uint32_t TSQ_TILE = 32;
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_dm_trace_sq(const Amp<R>* a, uint64_t N, uint32_t T, double2* part)
+{
+    __shared__ double2 bt[TSQ_TILE * (TSQ_TILE + 1)];
+    const uint64_t nt = N / T;
+    double sr = 0.0, si = 0.0;
+    for (uint64_t q = blockIdx.x; q < nt * nt; q += gridDim.x) {
+        const uint64_t I = q % nt, J = q / nt;
+        if (I > J) {
+            continue;   // uniform over the workgroup
+        }
+        for (uint32_t e = threadIdx.x; e < T * T; e += BLOCK) {
+            const uint32_t cl = e % T, rl = e / T;
+            const Amp<R> v = a[(J * T + cl) + (I * T + rl) * N];   // rho(c, r), c fastest
+            bt[rl * (TSQ_TILE + 1) + cl] = make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+        }
+        __syncthreads();
+        const double w = I == J ? 1.0 : 2.0;
+        for (uint32_t e = threadIdx.x; e < T * T; e += BLOCK) {
+            const uint32_t rl = e % T, cl = e / T;
+            const Amp<R> x = a[(I * T + rl) + (J * T + cl) * N];   // rho(r, c), r fastest
+            const double2 y = bt[rl * (TSQ_TILE + 1) + cl];
+            const double xr = static_cast<double>(x.x), xi = static_cast<double>(x.y);
+            sr += w * (xr * y.x - xi * y.y);
+            si += w * (xr * y.y + xi * y.x);
+        }
+        __syncthreads();
+    }
+    const double2 r = block_sum2(sr, si);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = r;
+    }
+}
+
 // partial_trace (density_matrix.cpp:231-278): out(ir, ic) = sum_e rho(ir (+) e, ic (+) e) over
 // assignments e of the traced qubits.  The sum runs as a binary counter over e with bit 0 of e on
 // the highest traced qubit, which is the association kettle's one-qubit-at-a-time loop produces.
@@ -1276,6 +1316,19 @@ void launch_dm_trace(kq_state* s, double out[2])
         KQ_DISPATCH(s,
                     (k_dm_trace<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), N, part)),
                     (k_dm_trace<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), N, part)));
+    }, out);
+}
+
+void launch_dm_trace_of_square(kq_state* s, double out[2])
+{
+    const uint64_t N = uint64_t {1} << s->n;
+    const uint32_t T = static_cast<uint32_t>(std::min<uint64_t>(N, TSQ_TILE));
+    const uint64_t pairs = (N / T) * (N / T);
+    const unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(pairs, RED_BLOCKS));
+    reduce2(s, blocks, [&](double2* part) {
+        KQ_DISPATCH(s,
+                    (k_dm_trace_sq<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), N, T, part)),
+                    (k_dm_trace_sq<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), N, T, part)));
     }, out);
 }
 

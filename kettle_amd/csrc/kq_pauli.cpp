@@ -161,6 +161,21 @@ int kq_dm_trace(kq_state* s, double out[2])
     });
 }
 
+int kq_dm_trace_of_square(kq_state* s, double out[2])
+{
+    return guarded([&] {
+        check_state(s);
+        if (!s->is_dm) {
+            throw InvalidArg {"trace of square: density matrices only"};
+        }
+        if (!out) {
+            throw InvalidArg {"null output"};
+        }
+        ensure_init();
+        launch_dm_trace_of_square(s, out);
+    });
+}
+
 int kq_dm_partial_trace(kq_state* s, const uint32_t* qubits, size_t n, double* out)
 {
     return guarded([&] {

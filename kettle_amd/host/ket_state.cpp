@@ -390,6 +390,18 @@ auto DensityMatrix::trace() const -> std::complex<double>
     return {out[0], out[1]};
 }
 
+auto DensityMatrix::trace_of_square() const -> std::complex<double>
+{
+    double out[2];
+    ki::kq_check(kq_dm_trace_of_square(device_state(), out));
+    return {out[0], out[1]};
+}
+
+auto DensityMatrix::is_pure(double tolerance) const -> bool
+{
+    return std::norm(trace_of_square() - std::complex<double> {1.0, 0.0}) <= tolerance;
+}
+
 auto DensityMatrix::device_state() const -> kq_state*
 {
     if (!device_) {

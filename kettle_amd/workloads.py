@@ -114,22 +114,41 @@ def qft_plus_layers(n: int = 30, layers: int = 20, seed: int = 0) -> QuantumCirc
     return c
 
 
-def density_matrix_circuit(n: int = 12, depth: int = 30, seed: int = 0, measure_every: int = 10) -> QuantumCircuit:
+def density_matrix_circuit(n: int = 12, depth: int = 30, seed: int = 0, measure_every: int = 10,
+                           distinct_bits: bool = False) -> QuantumCircuit:
     """Random circuit as the plain workload with, after every ``measure_every`` layers, M on qubit 0
-    into bit 0 and an if-else on that bit: X(0) when 1, RZ(0, pi/4) when 0."""
+    into bit 0 and an if-else on that bit: X(0) when 1, RZ(0, pi/4) when 0.  With ``distinct_bits`` the
+    k-th measurement writes (and its if-else reads) bit k, so the register keeps the whole outcome
+    history; the gates are the same."""
     rng = np.random.default_rng(seed)
     c = QuantumCircuit(n)
+    k = 0
     for layer in range(depth):
         add_random_1q_layer(c, rng)
         add_brick_ladder(c, rng, layer % 2)
         if (layer + 1) % measure_every == 0:
-            c.add_m_gate(0, 0)
+            bit = k if distinct_bits else 0
+            c.add_m_gate(0, bit)
             if_sub = QuantumCircuit(n)
             if_sub.add_x_gate(0)
             else_sub = QuantumCircuit(n)
             else_sub.add_rz_gate(0, math.pi / 4)
-            c.add_if_else_statement(0, if_sub, else_sub)
+            c.add_if_else_statement(bit, if_sub, else_sub)
+            k += 1
     return c
+
+
+def density_matrix_segments(n: int = 12, depth: int = 30, seed: int = 0, measure_every: int = 10) -> list[QuantumCircuit]:
+    """The gate runs of ``density_matrix_circuit`` between its measurements (same gates, same seed), for
+    replaying a recorded outcome history on a statevector."""
+    rng = np.random.default_rng(seed)
+    segs = [QuantumCircuit(n)]
+    for layer in range(depth):
+        add_random_1q_layer(segs[-1], rng)
+        add_brick_ladder(segs[-1], rng, layer % 2)
+        if (layer + 1) % measure_every == 0:
+            segs.append(QuantumCircuit(n))
+    return segs
 
 
 class NLocalEntanglement(enum.IntEnum):

@@ -967,6 +967,26 @@ int oracle_dm_trace(const double* rho_cm, uint32_t n, double* out)
     });
 }
 
+/* DensityMatrix::trace_of_square (density_matrix.cpp:164-168): the trace of rho * rho, i.e.
+   sum_i sum_k rho(i, k) rho(k, i), with the inner sum over k for each diagonal entry i */
+int oracle_dm_trace_of_square(const double* rho_cm, uint32_t n, double* out)
+{
+    return guarded([&] {
+        const std::size_t N = std::size_t {1} << n;
+        DM rho {n, N, load(rho_cm, N * N)};
+        cd t {};
+        for (std::size_t i = 0; i < N; ++i) {
+            cd d {};
+            for (std::size_t k = 0; k < N; ++k) {
+                d += rho(i, k) * rho(k, i);
+            }
+            t += d;
+        }
+        out[0] = t.real();
+        out[1] = t.imag();
+    });
+}
+
 /* partial_trace (density_matrix.cpp:71-112, 231-278) */
 int oracle_dm_partial_trace(const double* rho_cm, uint32_t n, const uint32_t* qubits, size_t nq, double* out)
 {

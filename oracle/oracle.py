@@ -34,6 +34,7 @@ _SIG = {
     "oracle_apply_pauli_string": (c_int, [c_void_p, c_uint32, c_size_t, c_void_p, c_void_p]),
     "oracle_sv_to_dm": (c_int, [c_void_p, c_uint32, c_void_p]),
     "oracle_dm_trace": (c_int, [c_void_p, c_uint32, c_void_p]),
+    "oracle_dm_trace_of_square": (c_int, [c_void_p, c_uint32, c_void_p]),
     "oracle_dm_partial_trace": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p]),
 }
 
@@ -151,6 +152,13 @@ def dm_trace(rho_vec, n):
     r = np.ascontiguousarray(rho_vec, np.complex128)
     out = np.zeros(2)
     _check(load().oracle_dm_trace(_ptr(r), n, _ptr(out)))
+    return complex(out[0], out[1])
+
+
+def dm_trace_of_square(rho_vec, n):
+    r = np.ascontiguousarray(rho_vec, np.complex128)
+    out = np.zeros(2)
+    _check(load().oracle_dm_trace_of_square(_ptr(r), n, _ptr(out)))
     return complex(out[0], out[1])
 
 

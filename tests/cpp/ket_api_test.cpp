@@ -321,6 +321,17 @@ int main()
         }
         auto from_sv = ket::statevector_to_density_matrix(s);
         REQUIRE(ket::almost_eq(from_sv, dm));
+        REQUIRE(std::abs(dm.trace_of_square() - 1.0) < 1e-13);
+        REQUIRE(dm.is_pure());
+        // the 2-qubit reduced state of an entangled 3-qubit state is mixed: tr(rho^2) = sum |rho(r,k)|^2 < 1
+        auto purity = std::complex<double> {};
+        for (std::size_t r = 0; r < 4; ++r) {
+            for (std::size_t k = 0; k < 4; ++k) {
+                purity += reduced.matrix()(r, k) * reduced.matrix()(k, r);
+            }
+        }
+        REQUIRE(std::abs(reduced.trace_of_square() - purity) < 1e-13);
+        REQUIRE(reduced.is_pure() == (std::norm(purity - 1.0) <= ket::DENSITY_MATRIX_TRACE_TOLERANCE));
     });
     run("density matrix measurement and loggers", [] {
         auto c = ket::QuantumCircuit {2};

@@ -10,6 +10,8 @@ Expectation values are FP64 reductions on the device.  Tolerances: 1e-9 (complex
 and 2e-3 (complex64: FP32 rounding over ~2800 gates)."""
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import pytest
 
@@ -102,6 +104,7 @@ def test_density_matrix_full_size(n):
     ka.simulate(c, sv)
     psi = sv.download()
     assert abs(dm.trace() - 1.0) <= 1e-12
+    assert abs(dm.trace_of_square() - 1.0) <= 1e-12 and dm.is_pure()
     p = ka.calculate_probabilities_raw(dm)
     assert np.max(np.abs(p - np.abs(psi) ** 2)) <= 1e-12
     keep = [0, n - 1]
@@ -115,3 +118,44 @@ def test_density_matrix_full_size(n):
     pt2 = ka.partial_trace(dm2, [q for q in range(n) if q not in keep])
     assert np.max(np.abs(pt2 - pt2.conj().T)) <= 1e-12
     assert np.min(np.linalg.eigvalsh(pt2)) >= -1e-12
+
+
+@pytest.mark.parametrize("n", [12, 14])
+@pytest.mark.parametrize("seed", [5, 6])
+def test_density_matrix_measurements_replayed_on_statevector(n, seed):
+    """The depth-30 density-matrix circuit with M on qubit 0 and an if-else every 10 layers, each outcome
+    kept in its own bit: the statevector path replays the recorded outcome history (gates, collapse onto
+    the recorded outcome renormalised by 1 / sqrt(p), the branch the outcome selects), and rho must be
+    |psi><psi| of the replayed state: its diagonal is |psi|^2, it is pure, and its 2-qubit partial trace
+    has |psi>'s reduced spectrum (1e-12).  A wrong branch or collapse scale fails these."""
+    from kettle_amd import _lib
+    cm = workloads.density_matrix_circuit(n=n, depth=30, seed=n, measure_every=10, distinct_bits=True)
+    dm = ka.DensityMatrix(n)
+    res = ka.simulate(cm, dm, seed)
+    outcomes = res.classical_register[:3]
+    assert all(b in (0, 1) for b in outcomes)
+    lib = _lib.load()
+    sv = ka.Statevector(n)
+    segs = workloads.density_matrix_segments(n=n, depth=30, seed=n, measure_every=10)
+    for k, seg in enumerate(segs):
+        ka.simulate(seg, sv)
+        if k < len(outcomes):
+            b = outcomes[k]
+            w = np.zeros(2)
+            _lib.check(lib.kq_qubit_weights(sv.handle, 0, _lib.ptr(w)))
+            _lib.check(lib.kq_collapse(sv.handle, 0, int(b), float(1.0 / np.sqrt(w[b]))))
+            branch = ka.QuantumCircuit(n)
+            if b == 1:
+                branch.add_x_gate(0)
+            else:
+                branch.add_rz_gate(0, math.pi / 4)
+            ka.simulate(branch, sv)
+    psi = sv.download()
+    assert abs(np.linalg.norm(psi) - 1.0) <= 1e-12
+    assert abs(dm.trace() - 1.0) <= 1e-12
+    assert abs(dm.trace_of_square() - 1.0) <= 1e-12
+    p = ka.calculate_probabilities_raw(dm)
+    assert np.max(np.abs(p - np.abs(psi) ** 2)) <= 1e-12
+    keep = [0, n - 1]
+    pt = ka.partial_trace(dm, [q for q in range(n) if q not in keep])
+    assert np.max(np.abs(np.sort(np.linalg.eigvalsh(pt)) - reduced_spectrum(psi, n, keep))) <= 1e-12

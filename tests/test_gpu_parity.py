@@ -315,6 +315,33 @@ def test_density_matrix_measurement_and_if_else(seed):
     np.testing.assert_allclose(p, np.real(ref.reshape(32, 32).T.diagonal()), atol=TOL128)
 
 
+@pytest.mark.parametrize("n", [3, 5, 8])
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_dm_trace_of_square_and_is_pure(n, precision):
+    """trace_of_square on the device (FP64 accumulation) against the oracle: a simulated pure state, an
+    uploaded random mixed state, the maximally mixed state (2^-n) and a non-Hermitian matrix; is_pure
+    follows kettle's std::norm(tr - 1) <= tol."""
+    tol = TOL128 if precision == ka.KQ_C128 else 1e-6
+    rng = np.random.default_rng(300 + n)
+    c = random_gate_circuit(n, 40, rng)
+    dm = ka.DensityMatrix(n, precision)
+    ka.simulate(c, dm)
+    ref, _ = orc.dm_simulate(orc.sv_to_dm(basis("0" * n), n), n, c.flatten()[0])
+    assert abs(dm.trace_of_square() - orc.dm_trace_of_square(ref, n)) <= tol
+    assert abs(dm.trace_of_square() - 1.0) <= tol and dm.is_pure()
+    N = 1 << n
+    a = rng.normal(size=(N, N)) + 1j * rng.normal(size=(N, N))
+    mixed = a @ a.conj().T
+    mixed /= np.trace(mixed)
+    for m in (mixed, np.eye(N) / N, a / N):
+        dm.upload(m.T.reshape(-1))
+        want = orc.dm_trace_of_square(m.T.reshape(-1), n)
+        assert abs(dm.trace_of_square() - want) <= tol * max(1.0, abs(want))
+    dm.upload((np.eye(N) / N).T.reshape(-1))
+    assert abs(dm.trace_of_square() - 2.0 ** -n) <= tol
+    assert not dm.is_pure() and dm.is_pure(tolerance=1.0)
+
+
 def test_dm_from_statevector():
     rng = np.random.default_rng(11)
     a = rng.normal(size=16) + 1j * rng.normal(size=16)

@@ -379,6 +379,33 @@ def test_dm_measurement_weights_are_block_frobenius_norms():
     assert abs(ones / trials - expected) < 4 * math.sqrt(expected * (1 - expected) / trials)
 
 
+def test_dm_trace_of_square_closed_forms():
+    """trace_of_square (density_matrix.cpp:164-168): 1 for a pure state, 2^-n for the maximally mixed
+    state, the product of the factors' values for rho_A (x) rho_B, and numpy's trace(M @ M) for a random
+    matrix that is neither Hermitian nor of trace 1 (so a dropped conjugate or a transposed operand shows)."""
+    rng = np.random.default_rng(17)
+
+    def rand_rho(k):
+        a = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        m = a @ a.conj().T
+        return m / np.trace(m)
+
+    for n in (1, 3, 5):
+        psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+        psi /= np.linalg.norm(psi)
+        assert abs(orc.dm_trace_of_square(orc.sv_to_dm(psi, n), n) - 1.0) < 1e-13
+        mixed = np.eye(1 << n) / (1 << n)
+        assert abs(orc.dm_trace_of_square(mixed.T.reshape(-1), n) - 2.0 ** -n) < 1e-15
+    ra, rb = rand_rho(2), rand_rho(2)
+    full = np.kron(rb, ra)
+    want = orc.dm_trace_of_square(ra.T.reshape(-1), 2) * orc.dm_trace_of_square(rb.T.reshape(-1), 2)
+    assert abs(orc.dm_trace_of_square(full.T.reshape(-1), 4) - want) < 1e-14
+    assert abs(want - np.trace(ra @ ra) * np.trace(rb @ rb)) < 1e-14
+    m = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+    assert abs(orc.dm_trace_of_square(m.T.reshape(-1), 4) - np.trace(m @ m)) < 1e-11
+    assert abs(np.trace(m @ m) - np.sum(np.abs(m) ** 2)) > 1.0   # the non-Hermitian case is not sum |m|^2
+
+
 def test_dm_partial_trace_of_tensor_product():
     """density_matrix_test.cpp:253-473: tracing out one factor of rho_A (x) rho_B leaves the other."""
     rng = np.random.default_rng(5)
