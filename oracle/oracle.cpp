@@ -869,6 +869,24 @@ int oracle_time_gates(uint32_t n, const kq_op* ops, size_t n_ops, double* second
     });
 }
 
+/* CPU baseline timing: each op applied `reps` times to one state of n qubits (|0...0> with a Hadamard
+   on every qubit, so no amplitude is zero); seconds[i * reps + r] is the r-th run of op i */
+int oracle_time_gate_reps(uint32_t n, const kq_op* ops, size_t n_ops, uint32_t reps, double* seconds)
+{
+    return guarded([&] {
+        const std::size_t N = std::size_t {1} << n;
+        State s(N, cd {1.0 / std::sqrt(static_cast<double>(N)), 0.0});
+        for (size_t i = 0; i < n_ops; ++i) {
+            for (uint32_t r = 0; r < reps; ++r) {
+                const auto t0 = std::chrono::steady_clock::now();
+                sv_gate(s, n, ops[i]);
+                const auto t1 = std::chrono::steady_clock::now();
+                seconds[i * reps + r] = std::chrono::duration<double>(t1 - t0).count();
+            }
+        }
+    });
+}
+
 /* calculate_probabilities_raw (probabilities.cpp:49-71) */
 int oracle_probabilities(const double* amps, uint32_t n, const double* noise, double* out)
 {

@@ -198,6 +198,16 @@ def pair_at_2q(n, c, t, k):
     return a.value, b.value
 
 
+def time_gate_reps(n, ops, reps):
+    """Seconds of each of `reps` runs of every op on one n-qubit state: array (len(ops), reps)."""
+    load().oracle_time_gate_reps.restype = c_int
+    load().oracle_time_gate_reps.argtypes = [c_uint32, c_void_p, c_size_t, c_uint32, c_void_p]
+    ops = np.ascontiguousarray(ops)
+    out = np.zeros((len(ops), reps))
+    _check(load().oracle_time_gate_reps(n, _ptr(ops), len(ops), reps, _ptr(out)))
+    return out
+
+
 def time_gates(n, ops):
     """Seconds the oracle spends applying ``ops`` to |0...0> of n qubits (gate loop only)."""
     load().oracle_time_gates.restype = c_int
