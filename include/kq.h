@@ -250,9 +250,12 @@ int kq_dm_from_statevector(kq_state* dm, const kq_state* sv);
  * Pauli operator H (same packing as kq_pauli_expectation, real weights), computes
  * E = <psi|H|psi> and dE/dtheta_p for p < n_params by one forward and one reverse sweep.
  * `state` is the work state (its precision is the compute precision); it ends holding psi(theta).
- * Checkpoints every `checkpoint_every` gates (0 = sqrt(G)) bound inverse-gate drift: at each
- * checkpoint the uncomputed state is compared with the stored one and restored when the
- * max-abs deviation exceeds drift_tol.  n_restores (nullable) reports how often that happened. */
+ * The forward sweep is one plan of fused passes; checkpoints of psi sit at pass boundaries, every
+ * ceil(sqrt(P)) of its P passes (checkpoint_every = 0) or at the first boundary past every
+ * checkpoint_every gates, at most 32 GiB of them.  They bound inverse-gate drift: at each checkpoint
+ * the uncomputed state is compared with the stored one on the device and replaced by it when the
+ * max-abs deviation exceeds drift_tol.  n_restores (nullable) reports how often that happened.
+ * Buffers (lambda, checkpoints, partials) stay in the state's workspace between calls. */
 int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops,
                        size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
                        const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs,

@@ -80,95 +80,93 @@ bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d)
 namespace
 {
 
-// One segment of the reverse sweep through fused adjoint tile passes.  inv[i] is the inverse of the
-// i-th gate in reverse order and deriv[i] its derivative (or null); values[i] receives
-// Re <lambda| dG |psi> for each i with a derivative.
-void run_adjoint_segment(kq_state* s, void* psi, void* lam, const std::vector<GateRec>& inv,
-                         const std::vector<const GateRec*>& deriv, std::vector<double>& values)
+size_t align256b(size_t x) { return (x + 255) & ~size_t {255}; }
+
+// Bytes of one lowered adjoint pass in the program blob: gates, steps, layouts, derivative records,
+// merged-run terms, each 256-byte aligned.
+size_t program_bytes(const TileProgram& tp)
 {
-    const int max_k = fusion_max_k(s->precision, s->nvec);
-    const auto passes = plan_passes(inv, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(s->precision),
-                                    fusion_max_stages(), false);
-    const auto align = [](size_t x) { return (x + 255) & ~size_t {255}; };
-    std::vector<TileProgram> progs(passes.size());
-    std::vector<size_t> offs(passes.size());
-    size_t total = 0;
-    uint64_t n_rows_total = 0;
-    std::vector<uint64_t> row_off;
-    std::vector<uint32_t> row_cnt;
-    std::vector<size_t> part_off(passes.size());
-    uint64_t part_total = 0;
-    for (size_t i = 0; i < passes.size(); ++i) {
-        lower_tile_pass(passes[i], s->precision, s->nvec, progs[i], &deriv);
-        const TileProgram& tp = progs[i];
-        offs[i] = total;
-        total += align(tp.gates.size() * sizeof(TileGate)) + align(tp.steps.size() * sizeof(TileStep))
-               + align(tp.layouts.size() * sizeof(TileLayout)) + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate))
-               + align(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
-        const uint64_t n_part = uint64_t {tile_pass_blocks(tp)} * tile_pass_waves(tp);
-        part_off[i] = part_total;
-        for (size_t k = 0; k < tp.derivs.size(); ++k) {
-            row_off.push_back(part_total + k * n_part);
-            row_cnt.push_back(static_cast<uint32_t>(n_part));
-        }
-        part_total += tp.derivs.size() * n_part;
-        n_rows_total += tp.derivs.size();
+    return align256b(tp.gates.size() * sizeof(TileGate)) + align256b(tp.steps.size() * sizeof(TileStep))
+         + align256b(tp.layouts.size() * sizeof(TileLayout)) + align256b(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate))
+         + align256b(std::max<size_t>(tp.terms.size(), 1) * sizeof(TileTerm));
+}
+
+void write_program(unsigned char* dst, const TileProgram& tp)
+{
+    size_t o = 0;
+    std::memcpy(dst + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
+    o += align256b(tp.gates.size() * sizeof(TileGate));
+    std::memcpy(dst + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
+    o += align256b(tp.steps.size() * sizeof(TileStep));
+    std::memcpy(dst + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
+    o += align256b(tp.layouts.size() * sizeof(TileLayout));
+    std::memcpy(dst + o, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
+    o += align256b(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+    std::memcpy(dst + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
+}
+
+// one planned pass of the reverse sweep
+struct ReversePass
+{
+    TileProgram tp;
+    size_t blob_off = 0;      // its program in the blob
+    uint64_t part_off = 0;    // its first derivative partial
+};
+
+// Gate-by-gate adjoint for states too small for a tile (fewer qubits than one thread's register block).
+void energy_gradient_unfused(kq_state* s, const kq_op* ops, const std::vector<GateRec>& recs,
+                             const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings, size_t n_params,
+                             double* energy, double* grad)
+{
+    const size_t G = recs.size();
+    const size_t bytes = s->n_amps * amp_bytes(s->precision);
+    const size_t acc_bytes = align256b(std::max<size_t>(G, 1) * 2 * sizeof(double));
+    auto* ws = static_cast<unsigned char*>(grow(s->adj, s->adj_bytes, 2 * align256b(bytes) + acc_bytes));
+    void* lam = ws;
+    void* psi = ws + align256b(bytes);
+    auto* accs = reinterpret_cast<double*>(ws + 2 * align256b(bytes));
+    run_records(s, recs, false);
+    launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
+    double e[2];
+    launch_dot(s, s->amps, lam, e);
+    *energy = e[0];
+    launch_copy(s, psi, s->amps, bytes);
+    std::fill(grad, grad + n_params, 0.0);
+    std::vector<char> has_d(G, 0);
+    std::vector<GateRec> derivs(G);
+    for (size_t gi = G; gi-- > 0;) {
+        has_d[gi] = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]) ? 1 : 0;
+        launch_adjoint_gate(s, psi, lam, inverse_of(recs[gi]), has_d[gi] ? &derivs[gi] : nullptr, has_d[gi] ? accs + 2 * gi : nullptr);
     }
-    const size_t tables = align(row_off.size() * sizeof(uint64_t)) + align(row_cnt.size() * sizeof(uint32_t));
-    std::vector<unsigned char> blob(std::max<size_t>(total + tables, 256), 0);
-    for (size_t i = 0; i < passes.size(); ++i) {
-        const TileProgram& tp = progs[i];
-        size_t o = offs[i];
-        std::memcpy(blob.data() + o, tp.gates.data(), tp.gates.size() * sizeof(TileGate));
-        o += align(tp.gates.size() * sizeof(TileGate));
-        std::memcpy(blob.data() + o, tp.steps.data(), tp.steps.size() * sizeof(TileStep));
-        o += align(tp.steps.size() * sizeof(TileStep));
-        std::memcpy(blob.data() + o, tp.layouts.data(), tp.layouts.size() * sizeof(TileLayout));
-        o += align(tp.layouts.size() * sizeof(TileLayout));
-        std::memcpy(blob.data() + o, tp.derivs.data(), tp.derivs.size() * sizeof(TileGate));
-        o += align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
-        std::memcpy(blob.data() + o, tp.terms.data(), tp.terms.size() * sizeof(TileTerm));
-    }
-    std::memcpy(blob.data() + total, row_off.data(), row_off.size() * sizeof(uint64_t));
-    std::memcpy(blob.data() + total + align(row_off.size() * sizeof(uint64_t)), row_cnt.data(),
-                row_cnt.size() * sizeof(uint32_t));
-    auto* d = static_cast<unsigned char*>(grow(s->prog, s->prog_bytes, blob.size()));
-    KQ_HIP_CHECK(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s->stream));
-    // partials, then one sum per derivative slot
-    const size_t part_bytes = align(std::max<uint64_t>(part_total, 1) * sizeof(double));
-    auto* aux = static_cast<unsigned char*>(grow(s->aux, s->aux_bytes, part_bytes + std::max<uint64_t>(n_rows_total, 1) * sizeof(double)));
-    auto* d_part = reinterpret_cast<double*>(aux);
-    auto* d_sums = reinterpret_cast<double*>(aux + part_bytes);
-    const double ab = static_cast<double>(amp_bytes(s->precision));
-    for (size_t i = 0; i < passes.size(); ++i) {
-        const TileProgram& tp = progs[i];
-        const unsigned char* g = d + offs[i];
-        const unsigned char* sp = g + align(tp.gates.size() * sizeof(TileGate));
-        const unsigned char* ly = sp + align(tp.steps.size() * sizeof(TileStep));
-        const unsigned char* dv = ly + align(tp.layouts.size() * sizeof(TileLayout));
-        const unsigned char* tm = dv + align(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
-        profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
-                 [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, tm, d_part + part_off[i]); });
-    }
-    std::vector<double> sums(n_rows_total);
-    if (n_rows_total) {
-        launch_sum_rows(s->stream, d_part, reinterpret_cast<const uint64_t*>(d + total),
-                        reinterpret_cast<const uint32_t*>(d + total + align(row_off.size() * sizeof(uint64_t))),
-                        static_cast<uint32_t>(n_rows_total), d_sums);
-        KQ_HIP_CHECK(hipMemcpyAsync(sums.data(), d_sums, n_rows_total * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    std::vector<double> h(2 * G);
+    if (G) {
+        KQ_HIP_CHECK(hipMemcpyAsync(h.data(), accs, h.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     }
     KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
-    profiler_collect();
-    size_t row = 0;
-    for (const TileProgram& tp : progs) {
-        for (const uint32_t src : tp.deriv_src) {
-            values[src] = sums[row++];
+    for (size_t gi = 0; gi < G; ++gi) {
+        if (has_d[gi]) {
+            grad[ops[gi].aux] += 2.0 * h[2 * gi];
         }
     }
 }
 
 }  // namespace
 
+// Adjoint energy gradient through fused passes (Jones & Gacon, arXiv:2009.02823).
+//   - The forward sweep is one whole-circuit plan of tile passes; checkpoints of psi sit at pass
+//     boundaries, every ceil(sqrt(P)) passes of the P (or, with checkpoint_every > 0, at the first pass
+//     boundary past every checkpoint_every gates).
+//   - lambda = H psi and E = <psi|lambda> (FP64).
+//   - The reverse sweep runs segment by segment between checkpoints: each segment's gates, inverted and
+//     in reverse execution order, are planned as adjoint tile passes that update psi and lambda in one
+//     read and write of each and reduce Re <lambda| dG |psi> for every parameterised gate into per-wave
+//     FP64 partials.  At each checkpoint the uncomputed psi is compared with the stored one on the
+//     device and replaced by it when the max-abs drift exceeds drift_tol.
+//   - Every buffer (lambda, the sweep's psi, checkpoints, partials, programs) lives in the state's
+//     adjoint workspace, grown when a call needs more and kept; all reverse programs are planned on
+//     the host and uploaded in one copy; the derivative rows are summed on the device in a fixed order
+//     and read back once, so a call waits for the device three times (after the forward sweep, for the
+//     energy, at the end).
 void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strings, const uint32_t* off,
                      const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs, size_t n_params,
                      double* energy, double* grad, uint32_t every, double drift_tol, size_t* n_restores)
@@ -190,133 +188,194 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     std::vector<PauliGroup> groups;
     std::vector<PauliZ> strings;
     build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
-
+    if (n_restores) {
+        *n_restores = 0;
+    }
     const size_t G = recs.size();
+    const int max_k = fusion_max_k(s->precision, s->nvec);
+    const int rq = register_qubits(s->precision);
+    if (max_k < rq) {
+        energy_gradient_unfused(s, ops, recs, groups, strings, n_params, energy, grad);
+        return;
+    }
     const size_t bytes = s->n_amps * amp_bytes(s->precision);
-    if (every == 0) {
-        every = static_cast<uint32_t>(std::max<double>(1.0, std::ceil(std::sqrt(static_cast<double>(G)))));
-    }
-    // checkpoints at gate boundaries b = every, 2 every, ... (< G), capped at 32 GiB of memory; the
-    // forward and the reverse sweep are planned segment by segment between them, so the state at
-    // each boundary is exactly the checkpointed one
+    const size_t vec_bytes = align256b(bytes);
+
+    // ---- forward plan and checkpoint boundaries (after pass index b) -----------------------------------
+    const std::vector<TilePass> fwd = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), rq,
+                                                  fusion_max_stages(), true, nullptr);
+    const size_t P = fwd.size();
     std::vector<size_t> bounds;
-    for (size_t b = every; b < G; b += every) {
-        bounds.push_back(b);
+    if (every == 0) {
+        const size_t stride = std::max<size_t>(1, static_cast<size_t>(std::ceil(std::sqrt(static_cast<double>(P)))));
+        for (size_t b = stride; b < P; b += stride) {
+            bounds.push_back(b - 1);
+        }
     }
-    const size_t cap = std::max<size_t>(1, (size_t {32} << 30) / std::max<size_t>(bytes, 1));
+    else {
+        size_t done = 0, next = every;
+        for (size_t p = 0; p + 1 < P; ++p) {
+            done += fwd[p].src.size();
+            if (done >= next) {
+                bounds.push_back(p);
+                while (next <= done) {
+                    next += every;
+                }
+            }
+        }
+    }
+    const size_t cap = std::max<size_t>(1, (size_t {32} << 30) / std::max<size_t>(vec_bytes, 1));
     if (bounds.size() > cap) {
-        bounds.clear();
-        const size_t step = (G + cap) / (cap + 1);
-        for (size_t b = step; b < G && bounds.size() < cap; b += step) {
-            bounds.push_back(b);
+        std::vector<size_t> thin;
+        for (size_t i = 0; i < cap; ++i) {
+            thin.push_back(bounds[(i * bounds.size()) / cap]);
+        }
+        bounds = thin;
+    }
+    const size_t C = bounds.size();
+    std::vector<size_t> seg_first {0};   // first pass of each segment
+    for (const size_t b : bounds) {
+        seg_first.push_back(b + 1);
+    }
+    seg_first.push_back(P);
+
+    // ---- reverse plan: per segment, the inverted gates in reverse execution order ----------------------
+    std::vector<GateRec> derivs(G);
+    std::vector<char> has_d(G, 0);
+    for (size_t gi = 0; gi < G; ++gi) {
+        has_d[gi] = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]) ? 1 : 0;
+    }
+    std::vector<std::vector<ReversePass>> rev(C + 1);
+    std::vector<uint32_t> row_gate;   // record index of each derivative row
+    std::vector<uint64_t> row_off;
+    std::vector<uint32_t> row_cnt;
+    size_t blob_bytes = 0;
+    uint64_t part_total = 0;
+    for (size_t j = 0; j <= C; ++j) {
+        std::vector<uint32_t> order;
+        for (size_t p = seg_first[j]; p < seg_first[j + 1]; ++p) {
+            order.insert(order.end(), fwd[p].src.begin(), fwd[p].src.end());
+        }
+        std::reverse(order.begin(), order.end());
+        std::vector<GateRec> inv;
+        std::vector<const GateRec*> dptr;
+        inv.reserve(order.size());
+        for (const uint32_t r : order) {
+            inv.push_back(inverse_of(recs[r]));
+            dptr.push_back(has_d[r] ? &derivs[r] : nullptr);
+        }
+        const auto passes = plan_passes(inv, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), rq, fusion_max_stages(), false);
+        for (const TilePass& tp_in : passes) {
+            ReversePass rp;
+            lower_tile_pass(tp_in, s->precision, s->nvec, rp.tp, &dptr);
+            rp.blob_off = blob_bytes;
+            blob_bytes += program_bytes(rp.tp);
+            rp.part_off = part_total;
+            const uint64_t n_part = uint64_t {tile_pass_blocks(rp.tp)} * tile_pass_waves(rp.tp);
+            for (size_t d = 0; d < rp.tp.derivs.size(); ++d) {
+                row_off.push_back(part_total + d * n_part);
+                row_cnt.push_back(static_cast<uint32_t>(n_part));
+                row_gate.push_back(order[rp.tp.deriv_src[d]]);
+            }
+            part_total += rp.tp.derivs.size() * n_part;
+            rev[j].push_back(std::move(rp));
         }
     }
-    std::vector<size_t> seg {0};
-    seg.insert(seg.end(), bounds.begin(), bounds.end());
-    seg.push_back(G);
-    std::vector<void*> ckpt(bounds.size(), nullptr);
-    void* lam = nullptr;
-    void* accs = nullptr;
-    void* psi = nullptr;
-    const auto cleanup = [&] {
-        for (auto* p : ckpt) {
-            if (p) {
-                (void) hipFree(p);
-            }
-        }
-        for (void* p : {lam, accs, psi}) {
-            if (p) {
-                (void) hipFree(p);
-            }
-        }
-    };
-    try {
-        lam = device_alloc(bytes);
-        for (auto& p : ckpt) {
-            p = device_alloc(bytes);
-        }
-        // forward sweep with checkpoints
-        for (size_t j = 0; j + 1 < seg.size(); ++j) {
-            // checkpoints are compared with the uncomputed state amplitude by amplitude: qubit q on bit q
-            run_records(s, std::vector<GateRec>(recs.begin() + static_cast<long>(seg[j]), recs.begin() + static_cast<long>(seg[j + 1])),
-                        false);
-            if (j < bounds.size()) {
-                launch_copy(s, ckpt[j], s->amps, bytes);
-            }
-        }
-        // lambda = H psi, E = <psi | lambda>
-        launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
-        double e[2];
-        launch_dot(s, s->amps, lam, e);
-        *energy = e[0];
-        // reverse sweep on a copy of psi (the state keeps psi(theta) for the caller)
-        psi = device_alloc(bytes);
-        launch_copy(s, psi, s->amps, bytes);
-        std::vector<GateRec> derivs(G);
-        std::vector<char> has_d(G, 0);
-        for (size_t gi = 0; gi < G; ++gi) {
-            has_d[gi] = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]) ? 1 : 0;
-        }
-        const bool fused = fusion_max_k(s->precision, s->nvec) >= register_qubits(s->precision);
-        std::fill(grad, grad + n_params, 0.0);
-        std::vector<double> h;
-        if (!fused) {
-            accs = device_alloc(std::max<size_t>(G, 1) * 2 * sizeof(double));
-        }
-        size_t restores = 0;
-        for (size_t j = seg.size() - 1; j-- > 0;) {
-            const size_t b0 = seg[j], b1 = seg[j + 1];
-            if (fused) {
-                std::vector<GateRec> inv;
-                std::vector<const GateRec*> dptr;
-                inv.reserve(b1 - b0);
-                for (size_t gi = b1; gi-- > b0;) {
-                    inv.push_back(inverse_of(recs[gi]));
-                    dptr.push_back(has_d[gi] ? &derivs[gi] : nullptr);
-                }
-                std::vector<double> values(inv.size(), 0.0);
-                run_adjoint_segment(s, psi, lam, inv, dptr, values);
-                for (size_t i = 0; i < inv.size(); ++i) {
-                    const size_t gi = b1 - 1 - i;
-                    if (has_d[gi]) {
-                        grad[ops[gi].aux] += 2.0 * values[i];
-                    }
-                }
-            }
-            else {
-                for (size_t gi = b1; gi-- > b0;) {
-                    launch_adjoint_gate(s, psi, lam, inverse_of(recs[gi]), has_d[gi] ? &derivs[gi] : nullptr,
-                                        has_d[gi] ? static_cast<double*>(accs) + 2 * gi : nullptr);
-                }
-                h.resize(2 * (b1 - b0));
-                KQ_HIP_CHECK(hipMemcpyAsync(h.data(), static_cast<double*>(accs) + 2 * b0, h.size() * sizeof(double),
-                                            hipMemcpyDeviceToHost, s->stream));
-                KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
-                for (size_t gi = b0; gi < b1; ++gi) {
-                    if (has_d[gi]) {
-                        grad[ops[gi].aux] += 2.0 * h[2 * (gi - b0)];
-                    }
-                }
-            }
-            // psi should now be the forward state after gates [0, b0): compare with the checkpoint
-            if (j >= 1) {
-                const double drift = launch_max_abs_diff(s, psi, ckpt[j - 1]);
-                if (drift > drift_tol) {
-                    launch_copy(s, psi, ckpt[j - 1], bytes);
-                    ++restores;
-                }
-            }
-        }
-        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
-        if (n_restores) {
-            *n_restores = restores;
+    const size_t R = row_gate.size();
+
+    // ---- workspace layout -----------------------------------------------------------------------------
+    // [lambda][psi][checkpoints][partials][row sums][drift partials][drift values][uploaded: row tables,
+    //  restore counter, programs]
+    size_t o = 0;
+    const size_t o_lam = o;
+    o += vec_bytes;
+    const size_t o_psi = o;
+    o += vec_bytes;
+    const size_t o_ckpt = o;
+    o += C * vec_bytes;
+    const size_t o_part = o;
+    o += align256b(std::max<uint64_t>(part_total, 1) * sizeof(double));
+    const size_t o_sums = o;
+    o += align256b(std::max<size_t>(R, 1) * sizeof(double));
+    const size_t o_red = o;
+    o += align256b(REDUCTION_BLOCKS * sizeof(double2));
+    const size_t o_drift = o;
+    o += align256b(std::max<size_t>(C, 1) * sizeof(double));
+    const size_t o_up = o;   // everything from here is one host-to-device copy
+    const size_t u_off = 0;
+    const size_t u_cnt = u_off + align256b(std::max<size_t>(R, 1) * sizeof(uint64_t));
+    const size_t u_count = u_cnt + align256b(std::max<size_t>(R, 1) * sizeof(uint32_t));
+    const size_t u_blob = u_count + 256;
+    const size_t up_bytes = u_blob + std::max<size_t>(blob_bytes, 256);
+    o += up_bytes;
+    auto* ws = static_cast<unsigned char*>(grow(s->adj, s->adj_bytes, o));
+    std::vector<unsigned char> up(up_bytes, 0);   // restore counter starts at 0
+    std::memcpy(up.data() + u_off, row_off.data(), R * sizeof(uint64_t));
+    std::memcpy(up.data() + u_cnt, row_cnt.data(), R * sizeof(uint32_t));
+    for (const auto& seg : rev) {
+        for (const ReversePass& rp : seg) {
+            write_program(up.data() + u_blob + rp.blob_off, rp.tp);
         }
     }
-    catch (...) {
-        cleanup();
-        throw;
+    void* lam = ws + o_lam;
+    void* psi = ws + o_psi;
+    const auto ckpt = [&](size_t i) { return static_cast<void*>(ws + o_ckpt + i * vec_bytes); };
+    auto* d_part = reinterpret_cast<double*>(ws + o_part);
+    auto* d_sums = reinterpret_cast<double*>(ws + o_sums);
+    auto* d_red = reinterpret_cast<double2*>(ws + o_red);
+    auto* d_drift = reinterpret_cast<double*>(ws + o_drift);
+    unsigned char* d_up = ws + o_up;
+    auto* d_count = reinterpret_cast<uint32_t*>(d_up + u_count);
+    KQ_HIP_CHECK(hipMemcpyAsync(d_up, up.data(), up_bytes, hipMemcpyHostToDevice, s->stream));
+
+    // ---- forward sweep with checkpoint copies, lambda, energy -------------------------------------------
+    std::vector<std::pair<size_t, std::function<void()>>> after;
+    for (size_t i = 0; i < C; ++i) {
+        after.emplace_back(bounds[i], [&, i] { launch_copy(s, ckpt(i), s->amps, bytes); });
     }
-    cleanup();
+    execute_passes_with(s, fwd, after);
+    last_stats().n_gates += G;
+    launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
+    double e[2];
+    launch_dot(s, s->amps, lam, e);
+    *energy = e[0];
+
+    // ---- reverse sweep on a copy of psi (the state keeps psi(theta)) ------------------------------------
+    launch_copy(s, psi, s->amps, bytes);
+    const double ab = static_cast<double>(amp_bytes(s->precision));
+    for (size_t j = C + 1; j-- > 0;) {
+        for (const ReversePass& rp : rev[j]) {
+            const TileProgram& tp = rp.tp;
+            const unsigned char* g = d_up + u_blob + rp.blob_off;
+            const unsigned char* sp = g + align256b(tp.gates.size() * sizeof(TileGate));
+            const unsigned char* ly = sp + align256b(tp.steps.size() * sizeof(TileStep));
+            const unsigned char* dv = ly + align256b(tp.layouts.size() * sizeof(TileLayout));
+            const unsigned char* tm = dv + align256b(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
+            profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
+                     [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, tm, d_part + rp.part_off); });
+        }
+        // psi is now the forward state at the segment's start: the checkpoint taken there
+        if (j >= 1) {
+            launch_drift_restore(s, psi, ckpt(j - 1), drift_tol, d_red, d_drift + (j - 1), d_count);
+        }
+    }
+    std::vector<double> sums(R);
+    uint32_t count = 0;
+    if (R) {
+        launch_sum_rows(s->stream, d_part, reinterpret_cast<const uint64_t*>(d_up + u_off),
+                        reinterpret_cast<const uint32_t*>(d_up + u_cnt), static_cast<uint32_t>(R), d_sums);
+        KQ_HIP_CHECK(hipMemcpyAsync(sums.data(), d_sums, R * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    }
+    KQ_HIP_CHECK(hipMemcpyAsync(&count, d_count, sizeof count, hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+    profiler_collect();
+    std::fill(grad, grad + n_params, 0.0);
+    for (size_t r = 0; r < R; ++r) {
+        grad[ops[row_gate[r]].aux] += 2.0 * sums[r];
+    }
+    if (n_restores) {
+        *n_restores = count;
+    }
 }
 
 }  // namespace kq

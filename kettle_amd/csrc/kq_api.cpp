@@ -244,6 +244,9 @@ int kq_state_destroy(kq_state* s)
         if (s->aux) {
             (void) hipFree(s->aux);
         }
+        if (s->adj) {
+            (void) hipFree(s->adj);
+        }
         delete s;
     });
 }

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -65,6 +66,10 @@ void normalise_layout(const kq_state* s);
 // SWAP triples dropped as relabels: sigma receives the qubit permutation they leave
 std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uint32_t nvec, std::vector<uint8_t>& sigma);
 PermuteTables permute_tables(const std::vector<uint8_t>& bit_of);
+// planned passes in order on the state's stream, with after[j].second enqueued right after pass
+// after[j].first (after sorted by pass); returns once the stream has drained
+void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
+                         const std::vector<std::pair<size_t, std::function<void()>>>& after);
 
 // ---- measurement, interpreter, sampling (kq_measure.cpp) ---------------------------------------------
 std::mt19937 get_prng(bool has_seed, int32_t seed);

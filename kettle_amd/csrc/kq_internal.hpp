@@ -221,6 +221,10 @@ struct kq_state
     // per-state device buffer for adjoint partial sums
     void* aux = nullptr;
     size_t aux_bytes = 0;
+    // per-state workspace of the adjoint gradient (lambda, the reverse-sweep psi, checkpoints, derivative
+    // partials, reverse-pass programs), grown when a call needs more and kept for the next call
+    void* adj = nullptr;
+    size_t adj_bytes = 0;
     // qubit q of the state is stored on index bit bit_of[q] (empty: bit q); only gate passes of a
     // statevector leave it permuted, every other operation restores bit q first
     std::vector<uint8_t> bit_of;
@@ -438,6 +442,11 @@ struct PermuteTables
 // dst[pi(i)] = src[i], pi the bit permutation the tables describe (s->n_amps amplitudes, nvec >= 12)
 void launch_permute_bits(kq_state* s, const void* src, void* dst, const PermuteTables& t);
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b);
+// Device-side drift check of the adjoint sweep: *d_drift = max |psi_i - ckpt_i| (block partials in
+// d_part, RED_BLOCKS entries), then psi <- ckpt when it exceeds tol, counted in *d_count; no host sync.
+constexpr int REDUCTION_BLOCKS = 2048;
+void launch_drift_restore(kq_state* s, void* psi, const void* ckpt, double tol, double2* d_part, double* d_drift,
+                          uint32_t* d_count);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
 
 // ---- tile passes compiled at run time (kq_jit.cpp) ---------------------------------------------------

@@ -29,7 +29,7 @@ namespace
 {
 
 constexpr int BLOCK = 256;
-constexpr int RED_BLOCKS = 2048;   // upper bound on partial-sum blocks of a reduction
+constexpr int RED_BLOCKS = REDUCTION_BLOCKS;   // upper bound on partial-sum blocks of a reduction
 
 template <typename R> struct AmpT;
 template <> struct AmpT<double> { using T = double2; };
@@ -1678,6 +1678,61 @@ void launch_permute_bits(kq_state* s, const void* src, void* dst, const PermuteT
     KQ_DISPATCH(s,
                 (k_permute_bits<double2><<<blocks, 256, 0, s->stream>>>(static_cast<const double2*>(src), static_cast<double2*>(dst), t)),
                 (k_permute_bits<float2><<<blocks, 256, 0, s->stream>>>(static_cast<const float2*>(src), static_cast<float2*>(dst), t)));
+}
+
+// max over the block partials of k_max_abs_diff, into out[0] (one block, fixed order)
+__global__ __launch_bounds__(BLOCK) void k_max_partials(const double2* part, int n, double* out)
+{
+    __shared__ double red[BLOCK];
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        mx = fmax(mx, part[i].x);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = red[0];
+    }
+}
+
+// dst <- src when *drift > tol (every workgroup reads the same flag), counting the restores
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_restore_if(Amp<R>* dst, const Amp<R>* src, uint64_t n, const double* drift,
+                                                      double tol, uint32_t* count)
+{
+    if (!(*drift > tol)) {
+        return;
+    }
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
+        dst[i] = src[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(count, 1u);
+    }
+}
+
+void launch_drift_restore(kq_state* s, void* psi, const void* ckpt, double tol, double2* d_part, double* d_drift,
+                          uint32_t* d_count)
+{
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    KQ_DISPATCH(s,
+                (k_max_abs_diff<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(psi), static_cast<const double2*>(ckpt), s->n_amps, d_part)),
+                (k_max_abs_diff<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(psi), static_cast<const float2*>(ckpt), s->n_amps, d_part)));
+    KQ_HIP_CHECK(hipGetLastError());
+    k_max_partials<<<1, BLOCK, 0, s->stream>>>(d_part, static_cast<int>(blocks), d_drift);
+    KQ_HIP_CHECK(hipGetLastError());
+    const unsigned cblocks = grid_for(s->n_amps, 4);
+    KQ_DISPATCH(s,
+                (k_restore_if<double><<<cblocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(psi), static_cast<const double2*>(ckpt), s->n_amps, d_drift, tol, d_count)),
+                (k_restore_if<float><<<cblocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(psi), static_cast<const float2*>(ckpt), s->n_amps, d_drift, tol, d_count)));
+    KQ_HIP_CHECK(hipGetLastError());
 }
 
 double launch_max_abs_diff(kq_state* s, const void* a, const void* b)

@@ -308,13 +308,27 @@ public:
         }
     }
 
+    // device work (copies, small kernels) that must run after every pass pushed so far and before the
+    // passes pushed later, whether or not those passes wait for their kernels to compile
+    void push_action(std::function<void()> action)
+    {
+        Item it;
+        it.action = std::move(action);
+        if (held_.empty()) {
+            it.action();
+        }
+        else {
+            held_.push_back(std::move(it));
+        }
+    }
+
     void finish()
     {
         if (!held_.empty()) {
             std::vector<std::string> srcs;
             std::vector<size_t> idx;
             for (size_t i = 0; i < held_.size(); ++i) {
-                if (!held_[i].pass.sweep && !held_[i].fn) {
+                if (!held_[i].action && !held_[i].pass.sweep && !held_[i].fn) {
                     srcs.push_back(held_[i].src);
                     idx.push_back(i);
                 }
@@ -340,6 +354,7 @@ private:
         std::string src;
         std::vector<double> coefs;
         hipFunction_t fn = nullptr;
+        std::function<void()> action;   // set: not a pass, run this instead
     };
 
     // device and pinned staging regions of `bytes` for one program
@@ -373,6 +388,10 @@ private:
 
     void launch(Item& it)
     {
+        if (it.action) {
+            it.action();
+            return;
+        }
         PlanStats& st = last_stats();
         const double ab = static_cast<double>(amp_bytes(s_->precision));
         const TilePass& p = it.pass;
@@ -418,6 +437,20 @@ void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
     PassStream stream(s);
     for (const TilePass& p : passes) {
         stream.push(TilePass {p});
+    }
+    stream.finish();
+}
+
+void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
+                         const std::vector<std::pair<size_t, std::function<void()>>>& after)
+{
+    PassStream stream(s);
+    size_t a = 0;
+    for (size_t i = 0; i < passes.size(); ++i) {
+        stream.push(TilePass {passes[i]});
+        for (; a < after.size() && after[a].first == i; ++a) {
+            stream.push_action(after[a].second);
+        }
     }
     stream.finish();
 }
