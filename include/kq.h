@@ -339,6 +339,11 @@ int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, siz
 int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes, size_t pass_index,
                   char* src, size_t src_cap, size_t* src_len, double* coefs, size_t coef_cap, size_t* n_coefs,
                   uint64_t info[5], size_t* n_passes);
+/* Host-only: generate and compile (without loading) the run-time adjoint reverse-pass kernels of a
+ * straight-line parameterised gate list (ops with aux >= 0 carry derivatives), the whole list as one
+ * reverse segment; fails with the compiler log if any does not compile.  No GPU needed. */
+int kq_debug_adjoint_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                                 size_t* n_kernels, size_t* failures);
 /* Host-only: the generated source of tile pass pass_index (NUL-terminated, truncated to cap - 1);
  * *len is the full length and *n_kernels the number of tile passes. */
 int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,

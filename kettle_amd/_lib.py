@@ -106,6 +106,8 @@ _SIGNATURES = {
     "kq_jit_info": (c_int, [POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double),
                             POINTER(c_size_t)]),
     "kq_debug_jit_compile": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, POINTER(c_size_t), POINTER(c_size_t)]),
+    "kq_debug_adjoint_jit_compile": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, POINTER(c_size_t),
+                                             POINTER(c_size_t)]),
     "kq_debug_jit_source": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, c_size_t, c_char_p, c_size_t,
                                     POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_batch_create": (c_int, [POINTER(c_void_p), c_int, c_int]),

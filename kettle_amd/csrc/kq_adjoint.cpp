@@ -109,8 +109,10 @@ void write_program(unsigned char* dst, const TileProgram& tp)
 struct ReversePass
 {
     TileProgram tp;
-    size_t blob_off = 0;      // its program in the blob
+    size_t blob_off = 0;      // its program (interpreted) or coefficients (compiled) in the blob
     uint64_t part_off = 0;    // its first derivative partial
+    hipFunction_t fn = nullptr;   // the compiled pass kernel, or null: the interpreted tile kernel runs it
+    std::vector<double> coefs;
 };
 
 // Gate-by-gate adjoint for states too small for a tile (fewer qubits than one thread's register block).
@@ -268,8 +270,6 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         for (const TilePass& tp_in : passes) {
             ReversePass rp;
             lower_tile_pass(tp_in, s->precision, s->nvec, rp.tp, &dptr);
-            rp.blob_off = blob_bytes;
-            blob_bytes += program_bytes(rp.tp);
             rp.part_off = part_total;
             const uint64_t n_part = uint64_t {tile_pass_blocks(rp.tp)} * tile_pass_waves(rp.tp);
             for (size_t d = 0; d < rp.tp.derivs.size(); ++d) {
@@ -282,6 +282,28 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         }
     }
     const size_t R = row_gate.size();
+    // reverse passes as run-time compiled kernels where the state is large enough for them; the
+    // programs of the others (and of any that fail to compile) go to the interpreted tile kernel
+    if (jit_wanted(s) && jit_available()) {
+        std::vector<std::string> srcs;
+        std::vector<ReversePass*> who;
+        for (auto& seg : rev) {
+            for (ReversePass& rp : seg) {
+                srcs.push_back(jit_adjoint_source(rp.tp, s->precision, rp.coefs));
+                who.push_back(&rp);
+            }
+        }
+        const auto fns = jit_kernels(srcs);
+        for (size_t i = 0; i < who.size(); ++i) {
+            who[i]->fn = fns[i];
+        }
+    }
+    for (auto& seg : rev) {
+        for (ReversePass& rp : seg) {
+            rp.blob_off = blob_bytes;
+            blob_bytes += rp.fn ? align256b(std::max<size_t>(rp.coefs.size(), 1) * sizeof(double)) : program_bytes(rp.tp);
+        }
+    }
 
     // ---- workspace layout -----------------------------------------------------------------------------
     // [lambda][psi][checkpoints][partials][row sums][drift partials][drift values][uploaded: row tables,
@@ -314,7 +336,12 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     std::memcpy(up.data() + u_cnt, row_cnt.data(), R * sizeof(uint32_t));
     for (const auto& seg : rev) {
         for (const ReversePass& rp : seg) {
-            write_program(up.data() + u_blob + rp.blob_off, rp.tp);
+            if (rp.fn) {
+                std::memcpy(up.data() + u_blob + rp.blob_off, rp.coefs.data(), rp.coefs.size() * sizeof(double));
+            }
+            else {
+                write_program(up.data() + u_blob + rp.blob_off, rp.tp);
+            }
         }
     }
     void* lam = ws + o_lam;
@@ -346,6 +373,13 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     for (size_t j = C + 1; j-- > 0;) {
         for (const ReversePass& rp : rev[j]) {
             const TileProgram& tp = rp.tp;
+            if (rp.fn) {
+                profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(), [&] {
+                    launch_jit_adjoint_pass(s, rp.fn, tp, psi, lam, reinterpret_cast<const double*>(d_up + u_blob + rp.blob_off),
+                                            d_part + rp.part_off);
+                });
+                continue;
+            }
             const unsigned char* g = d_up + u_blob + rp.blob_off;
             const unsigned char* sp = g + align256b(tp.gates.size() * sizeof(TileGate));
             const unsigned char* ly = sp + align256b(tp.steps.size() * sizeof(TileStep));

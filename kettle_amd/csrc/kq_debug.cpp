@@ -315,6 +315,51 @@ int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, siz
     });
 }
 
+int kq_debug_adjoint_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                                 size_t* n_kernels, size_t* failures)
+{
+    return guarded([&] {
+        if (!n_kernels || !failures || n_qubits < 4 || n_qubits > 40 || (precision != KQ_C128 && precision != KQ_C64)
+            || (!ops && n_ops)) {
+            throw InvalidArg {"debug adjoint jit: bad arguments"};
+        }
+        std::vector<GateRec> recs;
+        for (size_t i = 0; i < n_ops; ++i) {
+            if (ops[i].gate < 0 || ops[i].gate >= KQ_M) {
+                throw InvalidArg {"debug adjoint jit: op " + std::to_string(i) + " is not a unitary gate"};
+            }
+            compile_gate(ops[i], n_qubits, false, recs);
+        }
+        std::vector<GateRec> derivs(recs.size()), inv;
+        std::vector<const GateRec*> dptr;
+        for (size_t gi = recs.size(); gi-- > 0;) {
+            inv.push_back(inverse_of(recs[gi]));
+            const bool d = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]);
+            dptr.push_back(d ? &derivs[gi] : nullptr);
+        }
+        const int max_k = fusion_max_k_for(precision, n_qubits, lds_bytes ? lds_bytes : device_lds_per_block());
+        if (max_k < register_qubits(precision)) {
+            throw InvalidArg {"debug adjoint jit: state too small for a tile"};
+        }
+        const auto passes = plan_passes(inv, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(),
+                                        register_qubits(precision), fusion_max_stages(), false);
+        std::vector<std::string> srcs;
+        for (const auto& p : passes) {
+            TileProgram tp;
+            lower_tile_pass(p, precision, n_qubits, tp, &dptr);
+            std::vector<double> coefs;
+            srcs.push_back(jit_adjoint_source(tp, precision, coefs));
+        }
+        std::string log;
+        *failures = jit_compile_only(srcs, &log);
+        *n_kernels = srcs.size();
+        if (*failures) {
+            throw InvalidArg {"debug adjoint jit: " + std::to_string(*failures) + " of " + std::to_string(srcs.size())
+                              + " adjoint pass kernels failed to compile:\n" + log.substr(0, 4000)};
+        }
+    });
+}
+
 int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
                         size_t pass_index, char* buf, size_t cap, size_t* len, size_t* n_kernels)
 {

@@ -182,6 +182,11 @@ struct TileProgram
 // gets TileGate::pad = derivative index + 1 and its derivative record the next partial slot.
 void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgram& out,
                      const std::vector<const GateRec*>* deriv_of_src = nullptr);
+// The adjoint reverse of a forward tile pass on the same tile: loads with its store layout, runs its
+// stages backwards with inverted gates (and derivative records from deriv_of_rec, indexed by the
+// pass's src), stores with its load layout.
+void lower_adjoint_mirror(const TilePass& p, int precision, uint32_t nvec, const std::vector<const GateRec*>& deriv_of_rec,
+                          TileProgram& out);
 // merge runs of commuting diagonal gates of one step (appends to out / terms)
 void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, std::vector<TileGate>& out,
                          std::vector<TileTerm>& terms);
@@ -486,6 +491,12 @@ hipFunction_t jit_lookup(const std::string& source);
 // compile without loading (no device needed): number of failures, last log in *log
 size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
 void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);
+// kernel source of one adjoint reverse pass over (psi, lambda) with derivative reductions (the records of
+// an adjoint-lowered program); coefs receives its coefficient buffer
+std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
+// partials: [slot][blocks * waves] doubles, as the interpreted adjoint pass writes them
+void launch_jit_adjoint_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, void* psi, void* lam,
+                             const double* d_coefs, double* d_partials);
 
 // ---- batched energy + adjoint gradient (kq_batch.cpp) ------------------------------------------------
 // precision: the reverse sweep's (and lambda's); forward_precision: the forward sweep's and the

@@ -1106,6 +1106,207 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     return head + pre.str() + o.str();
 }
 
+// Generate the kernel source of one adjoint reverse pass (two vectors: psi `a`, lambda `b`) and the
+// layout of its coefficient buffer.  The same records as the interpreted adjoint tile kernel, as
+// straight-line code: for every record, psi <- G psi; if it has a derivative record, Re <lambda| dG |psi>
+// over the tile, reduced per wavefront into partials[slot * n_part + column]; then lambda <- G lambda.
+// Remaps move psi, then lambda, through one LDS tile of whole amplitudes.
+std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
+{
+    coefs.clear();
+    const bool c64 = precision == KQ_C64;
+    const int nt = 1 << tp.n_thr_bits;
+    const int waves = (nt + 63) / 64;
+    const int width = nt < 64 ? nt : 64;
+    std::ostringstream o;
+    o << "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\ntypedef int int32_t;\n"
+      << DEVICE_HEADER << "\nusing R = " << (c64 ? "float" : "double") << ";\n"
+      << "__device__ __forceinline__ double wsum(double x)\n{\n"
+      << "  for (int off = 1; off < " << width << "; off <<= 1) {\n    x += __shfl_xor(x, off, 64);\n  }\n  return x;\n}\n"
+      << "extern \"C\" __global__ __launch_bounds__(" << nt << ") void kq_adjoint_pass(kq::dev::Amp<R>* __restrict__ a, "
+      << "kq::dev::Amp<R>* __restrict__ b, const double* __restrict__ cf, double* __restrict__ partials, uint32_t n_part)\n{\n"
+      << "  using namespace kq::dev;\n";
+    bool remaps = tp.out_layout != 0;
+    for (const TileStep& st : tp.steps) {
+        remaps = remaps || st.layout != 0;
+    }
+    if (remaps) {
+        o << "  __shared__ Amp<R> tile[" << (1 << tp.k) << "];\n  char* const lds = reinterpret_cast<char*>(tile);\n";
+    }
+    const size_t nb = tp.freepos.size();
+    o << "  const uint32_t tid = threadIdx.x;\n  Cx<R> v[16];\n  Cx<R> w[16];\n"
+      << "  const uint64_t blk = ";
+    if (nb >= 3) {
+        o << "(uint64_t {blockIdx.x} >> 3) | (uint64_t {blockIdx.x & 7u} << " << nb - 3 << ")";
+    }
+    else {
+        o << "blockIdx.x";
+    }
+    o << ";\n  const uint64_t base = " << deposit_expr(tp.freepos) << ";\n"
+      << "  const uint64_t col = uint64_t {blockIdx.x} * " << waves << "u + (tid >> 6);\n";
+    const auto io = [&](const TileLayout& L, bool load) {
+        o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
+        for (int r = 0; r < 16; ++r) {
+            if (load) {
+                o << "    { const Amp<R> x = a[off ^ " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n"
+                  << "    { const Amp<R> y = b[off ^ " << hex64(L.hbm_slot[r]) << "]; w[" << r << "] = {y.x, y.y}; }\n";
+            }
+            else {
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off ^ " << hex64(L.hbm_slot[r]) << "] = x; }\n"
+                  << "    { Amp<R> y; y.x = w[" << r << "].re; y.y = w[" << r << "].im; b[off ^ " << hex64(L.hbm_slot[r]) << "] = y; }\n";
+            }
+        }
+        o << "  }\n";
+    };
+    const auto remap = [&](const TileLayout& A, const TileLayout& B) {
+        o << "  {\n    const uint32_t xa = " << lds_thread_expr(A, tp.n_thr_bits) << ";\n    const uint32_t xb = "
+          << lds_thread_expr(B, tp.n_thr_bits) << ";\n";
+        for (const char* u : {"v", "w"}) {
+            o << "    lds_barrier();\n";
+            for (int r = 0; r < 16; ++r) {
+                o << "    { Amp<R> x; x.x = " << u << "[" << r << "].re; x.y = " << u << "[" << r << "].im; "
+                  << "*reinterpret_cast<Amp<R>*>(lds + (xa ^ " << hex32(A.lds_slot[r]) << ")) = x; }\n";
+            }
+            o << "    lds_barrier();\n";
+            for (int r = 0; r < 16; ++r) {
+                o << "    { const Amp<R> x = *reinterpret_cast<const Amp<R>*>(lds + (xb ^ " << hex32(B.lds_slot[r]) << ")); "
+                  << u << "[" << r << "] = {x.x, x.y}; }\n";
+            }
+        }
+        o << "  }\n";
+    };
+    const auto push8 = [&](const double* m) {
+        const size_t off = coefs.size();
+        if (c64) {
+            for (int i = 0; i < 4; ++i) {
+                coefs.push_back(m[i]);   // 8 packed floats
+            }
+        }
+        else {
+            coefs.insert(coefs.end(), m, m + 8);
+        }
+        return off;
+    };
+    const auto push_term = [&](const TileTerm& t) {
+        const size_t off = coefs.size();
+        if (c64) {
+            coefs.push_back(t.d[0]);
+            coefs.push_back(t.d[1]);
+        }
+        else {
+            coefs.insert(coefs.end(), t.d, t.d + 4);
+        }
+        return off;
+    };
+    // the statement applying record g to vector u (coefficients / factors already in scope)
+    const auto apply_to = [&](const TileGate& g, const char* u) {
+        std::ostringstream a;
+        if (g.code >= TV_MERGE_BASE) {
+            const uint32_t key = g.code - TV_MERGE_BASE;
+            if (key >= 20) {
+                a << "scale<R, " << (key - 20) << ">(" << u << ", f1r, f1i);";
+            }
+            else {
+                a << "apply<R, K_DIAG, " << key / 5 << ", " << key % 5 << ">(" << u << ", fm);";
+            }
+        }
+        else if (g.code >= TV_UNIFORM_BASE) {
+            a << "scale<R, " << (g.code - TV_UNIFORM_BASE) << ">(" << u << ", fr, fi);";
+        }
+        else {
+            a << "apply<R, " << g.code / 20 << ", " << (g.code / 5) % 4 << ", " << g.code % 5 << ">(" << u << ", m);";
+        }
+        return a.str();
+    };
+    if (tp.layouts.empty()) {
+        throw InvalidArg {"jit adjoint: program without layouts"};
+    }
+    io(tp.layouts[0], true);
+    uint32_t cur = 0;
+    for (const TileStep& st : tp.steps) {
+        if (st.layout != cur) {
+            remap(tp.layouts[cur], tp.layouts[st.layout]);
+            cur = st.layout;
+        }
+        for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
+            const TileGate& g = tp.gates[gi];
+            o << "  {\n    const bool ok = " << cond_expr(g.glb_ctrl, g.lane_ctrl) << ";\n";
+            if (g.code >= TV_MERGE_BASE) {
+                const uint32_t key = g.code - TV_MERGE_BASE;
+                const bool uniform = key >= 20;
+                o << "    R f0r = 1, f0i = 0, f1r = 1, f1i = 0;\n";
+                for (uint64_t j = 0; j < g.glb_sel; ++j) {
+                    const TileTerm& t = tp.terms[g.lane_sel + j];
+                    const size_t off = push_term(t);
+                    o << "    { R d0r, d0i, d1r, d1i; term_at<R>(cf + " << off << ", d0r, d0i, d1r, d1i); const bool tk = "
+                      << cond_expr(t.glb_ctrl, t.lane_ctrl) << ";";
+                    if (uniform) {
+                        o << " const bool sel = " << sel_expr(t.lane_sel, t.glb_sel)
+                          << "; mul_if<R>(tk, f1r, f1i, sel ? d1r : d0r, sel ? d1i : d0i); }\n";
+                    }
+                    else {
+                        o << " mul_if<R>(tk, f0r, f0i, d0r, d0i); mul_if<R>(tk, f1r, f1i, d1r, d1i); }\n";
+                    }
+                }
+                if (!uniform) {
+                    o << "    const Coef<R> fm {{f0r, f0i, 0, 0, 0, 0, f1r, f1i}};\n";
+                }
+            }
+            else if (g.code >= TV_UNIFORM_BASE) {
+                double m[8];
+                const auto mv = record_values(g, c64);
+                std::copy(mv.begin(), mv.end(), m);
+                const size_t off = push8(c64 ? reinterpret_cast<const double*>(g.m) : m);
+                o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n    const bool sel = " << sel_expr(g.lane_sel, g.glb_sel)
+                  << ";\n    const R fr = sel ? m.m[6] : m.m[0], fi = sel ? m.m[7] : m.m[1];\n";
+            }
+            else {
+                const size_t off = push8(g.m);
+                o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n";
+            }
+            o << "    if (ok) { " << apply_to(g, "v") << " }\n";
+            if (g.pad != 0) {
+                const TileGate& dg = tp.derivs[g.pad - 1];
+                const size_t doff = push8(dg.m);
+                o << "    {\n      const Coef<R> dm = coef_at<R>(cf + " << doff << ");\n      double acc = 0.0;\n"
+                  << "      if (" << cond_expr(dg.glb_ctrl, dg.lane_ctrl) << ") {\n";
+                if (dg.code >= TV_UNIFORM_BASE) {
+                    o << "        const bool dsel = " << sel_expr(dg.lane_sel, dg.glb_sel) << ";\n"
+                      << "        acc = ddot_uniform<R, " << (dg.code - TV_UNIFORM_BASE)
+                      << ">(v, w, dsel ? dm.m[6] : dm.m[0], dsel ? dm.m[7] : dm.m[1]);\n";
+                }
+                else {
+                    o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
+                }
+                o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
+                  << "u} * n_part + col] = acc;\n      }\n    }\n";
+            }
+            o << "    if (ok) { " << apply_to(g, "w") << " }\n  }\n";
+        }
+    }
+    if (tp.out_layout != cur) {
+        remap(tp.layouts[cur], tp.layouts[tp.out_layout]);
+        cur = tp.out_layout;
+    }
+    io(tp.layouts[cur], false);
+    o << "}\n";
+    return o.str();
+}
+
+void launch_jit_adjoint_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, void* psi, void* lam,
+                             const double* d_coefs, double* d_partials)
+{
+    if (tp.freepos.size() > 31 || tp.freepos.size() + tp.k != s->nvec) {
+        throw InvalidArg {"jit adjoint pass: bad program shape"};
+    }
+    const double* cf = d_coefs;
+    double* part = d_partials;
+    const uint32_t n_blocks = tile_pass_blocks(tp);
+    uint32_t n_part = n_blocks * tile_pass_waves(tp);
+    void* args[] = {&psi, &lam, &cf, &part, &n_part};
+    KQ_HIP_CHECK(hipModuleLaunchKernel(fn, n_blocks, 1, 1, 1u << tp.n_thr_bits, 1, 1, 0, s->stream, args, nullptr));
+}
+
 bool jit_available()
 {
     JitCache& c = cache();
@@ -1246,7 +1447,8 @@ std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources)
             hipModule_t mod = nullptr;
             hipFunction_t fn = nullptr;
             if (hipModuleLoadData(&mod, res[j].code.data()) != hipSuccess
-                || hipModuleGetFunction(&fn, mod, "kq_pass") != hipSuccess) {
+                || hipModuleGetFunction(&fn, mod, sources[missing[j]].find(" kq_adjoint_pass(") != std::string::npos
+                                                      ? "kq_adjoint_pass" : "kq_pass") != hipSuccess) {
                 (void) hipGetLastError();
                 ++c.stats.failures;
                 c.stats.last_log = "module load failed";

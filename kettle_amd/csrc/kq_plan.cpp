@@ -1307,4 +1307,66 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
     }
 }
 
+// The adjoint mirror of a forward tile pass: the same tile, registers and layouts walked backwards.
+// It loads with the forward pass's store layout (so it reads the amplitudes where the forward pass left
+// them, qubit moves included), runs the forward stages in reverse order, each with its gates inverted
+// and in reverse order, and stores with the forward pass's load layout, so it leaves every qubit on the
+// index bit it had before the forward pass.  Gate i of the pass gets its derivative record from
+// deriv_of_rec[p.src[i]] (null: none), as lower_tile_pass does.
+void lower_adjoint_mirror(const TilePass& p, int precision, uint32_t nvec, const std::vector<const GateRec*>& deriv_of_rec,
+                          TileProgram& out)
+{
+    if (p.sweep) {
+        throw InvalidArg {"adjoint mirror: sweep passes have no tile"};
+    }
+    TileProgram fwd;
+    lower_tile_pass(p, precision, nvec, fwd);
+    out = TileProgram {};
+    out.k = fwd.k;
+    out.n_thr_bits = fwd.n_thr_bits;
+    out.freepos = fwd.freepos;
+    // layouts: the forward store layout first (the mirror's load), the rest in their order
+    const size_t nl = fwd.layouts.size();
+    std::vector<uint32_t> idx(nl);
+    out.layouts.push_back(fwd.layouts[fwd.out_layout]);
+    idx[fwd.out_layout] = 0;
+    for (size_t i = 0; i < nl; ++i) {
+        if (i != fwd.out_layout) {
+            idx[i] = static_cast<uint32_t>(out.layouts.size());
+            out.layouts.push_back(fwd.layouts[i]);
+        }
+    }
+    out.out_layout = idx[0];
+    out.steps.resize(p.stages.size());
+    for (size_t k = 0; k < p.stages.size(); ++k) {
+        const size_t si = p.stages.size() - 1 - k;
+        const TileStage& st = p.stages[si];
+        std::vector<TileGate> lowered;
+        lowered.reserve(st.g_count);
+        for (uint32_t j = st.g_count; j-- > 0;) {
+            const size_t i = st.g_begin + j;
+            TileGate t = lower_gate(inverse_of(p.gates[i]), precision, false);
+            const GateRec* dr = p.src[i] < deriv_of_rec.size() ? deriv_of_rec[p.src[i]] : nullptr;
+            if (dr) {
+                GateRec dg = *dr;
+                dg.target = p.gates[i].target;
+                dg.control = p.gates[i].control;
+                const bool diag = dg.m[2] == 0.0 && dg.m[3] == 0.0 && dg.m[4] == 0.0 && dg.m[5] == 0.0;
+                dg.kind = diag ? GK_DIAG : GK_DENSE;
+                TileGate td = lower_gate(dg, precision, true);
+                td.pad = static_cast<uint32_t>(out.derivs.size());
+                out.derivs.push_back(td);
+                out.deriv_src.push_back(p.src[i]);
+                t.pad = static_cast<uint32_t>(out.derivs.size());
+            }
+            lowered.push_back(t);
+        }
+        const uint32_t begin = static_cast<uint32_t>(out.gates.size());
+        merge_diagonal_runs(lowered.data(), static_cast<uint32_t>(lowered.size()), precision, out.gates, out.terms);
+        out.steps[k].g_begin = begin;
+        out.steps[k].g_count = static_cast<uint32_t>(out.gates.size()) - begin;
+        out.steps[k].layout = idx[fwd.steps[si].layout];
+    }
+}
+
 }  // namespace kq

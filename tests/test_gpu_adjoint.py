@@ -98,6 +98,34 @@ def test_adjoint_complex128_vs_oracle(controlled):
         assert np.max(np.abs(r.gradient - shift)) <= 1e-10
 
 
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_adjoint_compiled_reverse_passes(precision):
+    """Reverse passes as run-time compiled kernels (every tile pass compiled) against the oracle's energy and
+    parameter shift and against the interpreted reverse passes: every gate class, parameterised RX / RY /
+    RZ / P and controlled rotations, a shared parameter, checkpoints with restores forced on."""
+    n = 8
+    c, ids = mixed_circuit(n, 12, True)
+    h = workloads.random_pauli_operator(n, 30, np.random.default_rng(13))
+    ops, _, _ = c.flatten(ids)
+    tol_e, tol_g = (1e-12, 1e-9) if precision == ka.KQ_C128 else (1e-6, 1e-4)
+    ka.set_fusion(-1, 12, 0, 0.0)
+    try:
+        ka.set_jit(2)
+        compiled = ka.energy_and_gradient(c, h, ids, precision=precision, checkpoint_every=10, drift_tol=0.0)
+        ka.set_jit(0)
+        interpreted = ka.energy_and_gradient(c, h, ids, precision=precision, checkpoint_every=10, drift_tol=0.0)
+    finally:
+        ka.set_jit(1)
+        ka.set_fusion(-1, 4096, 12, 1.0e9)
+        ka.set_fusion_autotune(True)
+    e_ref = oracle_energy(ops, n, _packed(h))
+    fd = oracle_fd(ops, n, _packed(h), len(ids))
+    assert abs(compiled.energy - e_ref) <= tol_e
+    assert np.max(np.abs(compiled.gradient - fd)) <= tol_g
+    assert np.max(np.abs(compiled.gradient - interpreted.gradient)) <= (1e-12 if precision == ka.KQ_C128 else 1e-5)
+    assert compiled.restores > 0
+
+
 def test_adjoint_complex64_error_bounds():
     n = 8
     c, ids = mixed_circuit(n, 5, True)

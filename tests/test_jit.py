@@ -42,6 +42,22 @@ def test_generated_sources_compile(precision):
     assert ka.debug_jit_compile(ops, 12, precision) >= 1
 
 
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_generated_adjoint_sources_compile(precision):
+    """The run-time adjoint reverse-pass kernels (two vectors, derivative reductions) of a parameterised
+    circuit with every gate class, controls on register, thread and block bits and merged diagonal runs."""
+    import ctypes
+    from kettle_amd import _lib, workloads
+    rng = np.random.default_rng(40 + precision)
+    c = random_gate_circuit(14, 120, rng)
+    w = workloads.vqe(n=14, layers=3, n_random=10, seed=2)
+    for ops in (ops_of(c), w.circuit.flatten(w.params)[0]):
+        nk, nf = ctypes.c_size_t(), ctypes.c_size_t()
+        _lib.check(_lib.load().kq_debug_adjoint_jit_compile(14, precision, _lib.ptr(ops), len(ops), 160 * 1024,
+                                                            ctypes.byref(nk), ctypes.byref(nf)))
+        assert nk.value >= 1 and nf.value == 0
+
+
 def test_generated_source_is_straight_line():
     rng = np.random.default_rng(3)
     ops = ops_of(random_gate_circuit(12, 80, rng))
