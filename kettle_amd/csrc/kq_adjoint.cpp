@@ -187,9 +187,6 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         }
         compile_gate(ops[i], s->n, false, recs);
     }
-    std::vector<PauliGroup> groups;
-    std::vector<PauliZ> strings;
-    build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
     if (n_restores) {
         *n_restores = 0;
     }
@@ -197,16 +194,36 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     const int max_k = fusion_max_k(s->precision, s->nvec);
     const int rq = register_qubits(s->precision);
     if (max_k < rq) {
+        std::vector<PauliGroup> groups;
+        std::vector<PauliZ> strings;
+        build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
         energy_gradient_unfused(s, ops, recs, groups, strings, n_params, energy, grad);
         return;
     }
     const size_t bytes = s->n_amps * amp_bytes(s->precision);
     const size_t vec_bytes = align256b(bytes);
 
-    // ---- forward plan and checkpoint boundaries (after pass index b) -----------------------------------
-    const std::vector<TilePass> fwd = plan_passes(recs, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), rq,
-                                                  fusion_max_stages(), true, nullptr);
+    // ---- forward plan (tile passes only: each has a mirror), checkpoint boundaries (after pass b) -------
+    std::vector<uint8_t> bit_of;
+    const std::vector<TilePass> fwd = plan_statevector_passes(s, recs, false, bit_of);
     const size_t P = fwd.size();
+    // the operator acts on the state as the forward passes leave it: qubit q on index bit bit_of[q]
+    if (n_strings > 0 && off == nullptr) {
+        throw InvalidArg {"null Pauli operator arrays"};
+    }
+    std::vector<uint32_t> q_at(n_strings ? off[n_strings] : 0);
+    if (!q_at.empty() && q == nullptr) {
+        throw InvalidArg {"null Pauli operator arrays"};
+    }
+    for (size_t t = 0; t < q_at.size(); ++t) {
+        if (q[t] >= s->n) {
+            throw InvalidArg {"energy gradient: Pauli term qubit out of range"};
+        }
+        q_at[t] = bit_of[q[t]];
+    }
+    std::vector<PauliGroup> groups;
+    std::vector<PauliZ> strings;
+    build_pauli(s->n, n_strings, off, q_at.data(), k, phases, coeffs, groups, strings);
     std::vector<size_t> bounds;
     if (every == 0) {
         const size_t stride = std::max<size_t>(1, static_cast<size_t>(std::ceil(std::sqrt(static_cast<double>(P)))));
@@ -241,11 +258,54 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     }
     seg_first.push_back(P);
 
-    // ---- reverse plan: per segment, the inverted gates in reverse execution order ----------------------
+    // ---- workspace of the vectors: [lambda][psi][checkpoints][drift partials][drift][energy] ---------
+    size_t o = 0;
+    const size_t o_lam = o;
+    o += vec_bytes;
+    const size_t o_psi = o;
+    o += vec_bytes;
+    const size_t o_ckpt = o;
+    o += C * vec_bytes;
+    const size_t o_red = o;
+    o += align256b(REDUCTION_BLOCKS * sizeof(double2));
+    const size_t o_drift = o;
+    o += align256b(std::max<size_t>(C, 1) * sizeof(double));
+    const size_t o_energy = o;
+    o += 256;
+    auto* ws = static_cast<unsigned char*>(grow(s->adj, s->adj_bytes, o));
+    void* lam = ws + o_lam;
+    void* psi = ws + o_psi;
+    const auto ckpt = [&](size_t i) { return static_cast<void*>(ws + o_ckpt + i * vec_bytes); };
+    auto* d_red = reinterpret_cast<double2*>(ws + o_red);
+    auto* d_drift = reinterpret_cast<double*>(ws + o_drift);
+    auto* d_energy = reinterpret_cast<double2*>(ws + o_energy);
+
+    // ---- forward sweep with checkpoint copies, lambda = H psi, E = <psi|lambda>: enqueued only, so the
+    //      host plans the reverse sweep while the device runs them -------------------------------------
+    std::vector<std::pair<size_t, std::function<void()>>> after;
+    for (size_t i = 0; i < C; ++i) {
+        after.emplace_back(bounds[i], [&, i] { launch_copy(s, ckpt(i), s->amps, bytes); });
+    }
+    execute_passes_with(s, fwd, after, false);
+    last_stats().n_gates += G;
+    {
+        bool identity = true;
+        for (size_t b = 0; b < bit_of.size(); ++b) {
+            identity = identity && bit_of[b] == b;
+        }
+        s->bit_of = identity ? std::vector<uint8_t> {} : bit_of;
+    }
+    launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
+    launch_dot_to_device(s, s->amps, lam, d_red, d_energy);
+    launch_copy(s, psi, s->amps, bytes);   // the reverse sweep runs on a copy: the state keeps psi(theta)
+
+    // ---- reverse plan: the mirror of every forward pass, segment by segment ---------------------------
     std::vector<GateRec> derivs(G);
-    std::vector<char> has_d(G, 0);
+    std::vector<const GateRec*> deriv_of_rec(G, nullptr);
     for (size_t gi = 0; gi < G; ++gi) {
-        has_d[gi] = ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi]) ? 1 : 0;
+        if (ops[gi].aux >= 0 && derivative_of(ops[gi], recs[gi], derivs[gi])) {
+            deriv_of_rec[gi] = &derivs[gi];
+        }
     }
     std::vector<std::vector<ReversePass>> rev(C + 1);
     std::vector<uint32_t> row_gate;   // record index of each derivative row
@@ -254,28 +314,15 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     size_t blob_bytes = 0;
     uint64_t part_total = 0;
     for (size_t j = 0; j <= C; ++j) {
-        std::vector<uint32_t> order;
-        for (size_t p = seg_first[j]; p < seg_first[j + 1]; ++p) {
-            order.insert(order.end(), fwd[p].src.begin(), fwd[p].src.end());
-        }
-        std::reverse(order.begin(), order.end());
-        std::vector<GateRec> inv;
-        std::vector<const GateRec*> dptr;
-        inv.reserve(order.size());
-        for (const uint32_t r : order) {
-            inv.push_back(inverse_of(recs[r]));
-            dptr.push_back(has_d[r] ? &derivs[r] : nullptr);
-        }
-        const auto passes = plan_passes(inv, s->nvec, max_k, fusion_low_qubits(), fusion_max_gates(), rq, fusion_max_stages(), false);
-        for (const TilePass& tp_in : passes) {
+        for (size_t p = seg_first[j + 1]; p-- > seg_first[j];) {
             ReversePass rp;
-            lower_tile_pass(tp_in, s->precision, s->nvec, rp.tp, &dptr);
+            lower_adjoint_mirror(fwd[p], s->precision, s->nvec, deriv_of_rec, rp.tp);
             rp.part_off = part_total;
             const uint64_t n_part = uint64_t {tile_pass_blocks(rp.tp)} * tile_pass_waves(rp.tp);
             for (size_t d = 0; d < rp.tp.derivs.size(); ++d) {
                 row_off.push_back(part_total + d * n_part);
                 row_cnt.push_back(static_cast<uint32_t>(n_part));
-                row_gate.push_back(order[rp.tp.deriv_src[d]]);
+                row_gate.push_back(rp.tp.deriv_src[d]);
             }
             part_total += rp.tp.derivs.size() * n_part;
             rev[j].push_back(std::move(rp));
@@ -305,32 +352,21 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         }
     }
 
-    // ---- workspace layout -----------------------------------------------------------------------------
-    // [lambda][psi][checkpoints][partials][row sums][drift partials][drift values][uploaded: row tables,
-    //  restore counter, programs]
-    size_t o = 0;
-    const size_t o_lam = o;
-    o += vec_bytes;
-    const size_t o_psi = o;
-    o += vec_bytes;
-    const size_t o_ckpt = o;
-    o += C * vec_bytes;
-    const size_t o_part = o;
-    o += align256b(std::max<uint64_t>(part_total, 1) * sizeof(double));
-    const size_t o_sums = o;
-    o += align256b(std::max<size_t>(R, 1) * sizeof(double));
-    const size_t o_red = o;
-    o += align256b(REDUCTION_BLOCKS * sizeof(double2));
-    const size_t o_drift = o;
-    o += align256b(std::max<size_t>(C, 1) * sizeof(double));
-    const size_t o_up = o;   // everything from here is one host-to-device copy
+    // ---- workspace of the reverse sweep: [partials][row sums][uploaded: row tables, restore counter,
+    //      programs] -----------------------------------------------------------------------------------
+    size_t ob = 0;
+    const size_t o_part = ob;
+    ob += align256b(std::max<uint64_t>(part_total, 1) * sizeof(double));
+    const size_t o_sums = ob;
+    ob += align256b(std::max<size_t>(R, 1) * sizeof(double));
+    const size_t o_up = ob;
     const size_t u_off = 0;
     const size_t u_cnt = u_off + align256b(std::max<size_t>(R, 1) * sizeof(uint64_t));
     const size_t u_count = u_cnt + align256b(std::max<size_t>(R, 1) * sizeof(uint32_t));
     const size_t u_blob = u_count + 256;
     const size_t up_bytes = u_blob + std::max<size_t>(blob_bytes, 256);
-    o += up_bytes;
-    auto* ws = static_cast<unsigned char*>(grow(s->adj, s->adj_bytes, o));
+    ob += up_bytes;
+    auto* wb = static_cast<unsigned char*>(grow(s->aux, s->aux_bytes, ob));
     std::vector<unsigned char> up(up_bytes, 0);   // restore counter starts at 0
     std::memcpy(up.data() + u_off, row_off.data(), R * sizeof(uint64_t));
     std::memcpy(up.data() + u_cnt, row_cnt.data(), R * sizeof(uint32_t));
@@ -344,31 +380,13 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
             }
         }
     }
-    void* lam = ws + o_lam;
-    void* psi = ws + o_psi;
-    const auto ckpt = [&](size_t i) { return static_cast<void*>(ws + o_ckpt + i * vec_bytes); };
-    auto* d_part = reinterpret_cast<double*>(ws + o_part);
-    auto* d_sums = reinterpret_cast<double*>(ws + o_sums);
-    auto* d_red = reinterpret_cast<double2*>(ws + o_red);
-    auto* d_drift = reinterpret_cast<double*>(ws + o_drift);
-    unsigned char* d_up = ws + o_up;
+    auto* d_part = reinterpret_cast<double*>(wb + o_part);
+    auto* d_sums = reinterpret_cast<double*>(wb + o_sums);
+    unsigned char* d_up = wb + o_up;
     auto* d_count = reinterpret_cast<uint32_t*>(d_up + u_count);
     KQ_HIP_CHECK(hipMemcpyAsync(d_up, up.data(), up_bytes, hipMemcpyHostToDevice, s->stream));
 
-    // ---- forward sweep with checkpoint copies, lambda, energy -------------------------------------------
-    std::vector<std::pair<size_t, std::function<void()>>> after;
-    for (size_t i = 0; i < C; ++i) {
-        after.emplace_back(bounds[i], [&, i] { launch_copy(s, ckpt(i), s->amps, bytes); });
-    }
-    execute_passes_with(s, fwd, after);
-    last_stats().n_gates += G;
-    launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
-    double e[2];
-    launch_dot(s, s->amps, lam, e);
-    *energy = e[0];
-
-    // ---- reverse sweep on a copy of psi (the state keeps psi(theta)) ------------------------------------
-    launch_copy(s, psi, s->amps, bytes);
+    // ---- reverse sweep --------------------------------------------------------------------------------
     const double ab = static_cast<double>(amp_bytes(s->precision));
     for (size_t j = C + 1; j-- > 0;) {
         for (const ReversePass& rp : rev[j]) {
@@ -395,14 +413,17 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     }
     std::vector<double> sums(R);
     uint32_t count = 0;
+    double2 e {0.0, 0.0};
     if (R) {
         launch_sum_rows(s->stream, d_part, reinterpret_cast<const uint64_t*>(d_up + u_off),
                         reinterpret_cast<const uint32_t*>(d_up + u_cnt), static_cast<uint32_t>(R), d_sums);
         KQ_HIP_CHECK(hipMemcpyAsync(sums.data(), d_sums, R * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     }
     KQ_HIP_CHECK(hipMemcpyAsync(&count, d_count, sizeof count, hipMemcpyDeviceToHost, s->stream));
+    KQ_HIP_CHECK(hipMemcpyAsync(&e, d_energy, sizeof e, hipMemcpyDeviceToHost, s->stream));
     KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
     profiler_collect();
+    *energy = e.x;
     std::fill(grad, grad + n_params, 0.0);
     for (size_t r = 0; r < R; ++r) {
         grad[ops[row_gate[r]].aux] += 2.0 * sums[r];

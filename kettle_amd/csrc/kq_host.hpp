@@ -66,10 +66,16 @@ void normalise_layout(const kq_state* s);
 // SWAP triples dropped as relabels: sigma receives the qubit permutation they leave
 std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uint32_t nvec, std::vector<uint8_t>& sigma);
 PermuteTables permute_tables(const std::vector<uint8_t>& bit_of);
+// The fused passes of a straight-line statevector gate list from qubit q on bit q, as a gate run plans
+// them (plan tuner, qubit moves when enabled); bit_of receives the placement the passes leave.
+std::vector<TilePass> plan_statevector_passes(kq_state* s, const std::vector<GateRec>& recs, bool allow_sweep,
+                                              std::vector<uint8_t>& bit_of);
 // planned passes in order on the state's stream, with after[j].second enqueued right after pass
-// after[j].first (after sorted by pass); returns once the stream has drained
+// after[j].first (after sorted by pass); with sync, returns once the stream has drained, else once
+// everything is enqueued (the pinned staging of the pass coefficients must not be reused before the
+// stream drains: the caller synchronises before the state runs another gate list)
 void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
-                         const std::vector<std::pair<size_t, std::function<void()>>>& after);
+                         const std::vector<std::pair<size_t, std::function<void()>>>& after, bool sync = true);
 
 // ---- measurement, interpreter, sampling (kq_measure.cpp) ---------------------------------------------
 std::mt19937 get_prng(bool has_seed, int32_t seed);

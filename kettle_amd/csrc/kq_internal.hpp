@@ -453,6 +453,8 @@ constexpr int REDUCTION_BLOCKS = 2048;
 void launch_drift_restore(kq_state* s, void* psi, const void* ckpt, double tol, double2* d_part, double* d_drift,
                           uint32_t* d_count);
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2]);
+// <a|b> in FP64 into *d_out on the device (block partials in d_part, REDUCTION_BLOCKS entries), no sync
+void launch_dot_to_device(kq_state* s, const void* a, const void* b, double2* d_part, double2* d_out);
 
 // ---- tile passes compiled at run time (kq_jit.cpp) ---------------------------------------------------
 struct JitStats

@@ -1758,6 +1758,17 @@ double launch_max_abs_diff(kq_state* s, const void* a, const void* b)
     return mx;
 }
 
+void launch_dot_to_device(kq_state* s, const void* a, const void* b, double2* d_part, double2* d_out)
+{
+    const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
+    KQ_DISPATCH(s,
+                (k_dot<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(a), static_cast<const double2*>(b), s->n_amps, d_part)),
+                (k_dot<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(a), static_cast<const float2*>(b), s->n_amps, d_part)));
+    KQ_HIP_CHECK(hipGetLastError());
+    k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(d_part, static_cast<int>(blocks), d_out);
+    KQ_HIP_CHECK(hipGetLastError());
+}
+
 void launch_dot(kq_state* s, const void* a, const void* b, double out[2])
 {
     const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);

@@ -322,7 +322,7 @@ public:
         }
     }
 
-    void finish()
+    void finish(bool sync = true)
     {
         if (!held_.empty()) {
             std::vector<std::string> srcs;
@@ -342,8 +342,10 @@ public:
             }
             held_.clear();
         }
-        KQ_HIP_CHECK(hipStreamSynchronize(s_->stream));
-        profiler_collect();
+        if (sync) {
+            KQ_HIP_CHECK(hipStreamSynchronize(s_->stream));
+            profiler_collect();
+        }
     }
 
 private:
@@ -441,8 +443,23 @@ void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
     stream.finish();
 }
 
+std::vector<TilePass> plan_statevector_passes(kq_state* s, const std::vector<GateRec>& recs, bool allow_sweep,
+                                              std::vector<uint8_t>& bit_of)
+{
+    const int max_k = fusion_max_k(s->precision, s->nvec);
+    bit_of.resize(s->nvec);
+    for (uint32_t q = 0; q < s->nvec; ++q) {
+        bit_of[q] = static_cast<uint8_t>(q);
+    }
+    const bool relabel = fusion_relabel() && !s->is_dm && max_k < static_cast<int>(s->nvec);
+    const FusionChoice fc = relabel && tuned(s, max_k) ? choose_fusion(s, recs, max_k, bit_of)
+                                                       : FusionChoice {fusion_low_qubits(), fusion_max_stages()};
+    return plan_passes(recs, s->nvec, max_k, fc.low_c, fusion_max_gates(), register_qubits(s->precision), fc.max_stages,
+                       allow_sweep, relabel ? &bit_of : nullptr);
+}
+
 void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
-                         const std::vector<std::pair<size_t, std::function<void()>>>& after)
+                         const std::vector<std::pair<size_t, std::function<void()>>>& after, bool sync)
 {
     PassStream stream(s);
     size_t a = 0;
@@ -452,7 +469,7 @@ void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
             stream.push_action(after[a].second);
         }
     }
-    stream.finish();
+    stream.finish(sync);
 }
 
 // Plans a gate list and runs its passes as they are planned.
