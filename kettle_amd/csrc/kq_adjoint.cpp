@@ -44,34 +44,31 @@ GateRec inverse_of(const GateRec& g)
     return r;
 }
 
-// d/dtheta of kettle's angle-gate matrices (as written in gate_matrix of kq_plan.cpp)
+// The generator Q of an angle gate's derivative, dG/dtheta = Q G, for kettle's angle-gate matrices (as
+// written in gate_matrix of kq_plan.cpp): RX -> (-i/2) X, RY -> (-i/2) Y, RZ -> (-i/2) Z, P -> i |1><1|, and
+// the same Q on the target of the controlled forms (their derivative vanishes where the control is 0).
+// Q does not depend on the angle.  The adjoint sweep uses dE/dtheta = 2 Re <lambda| Q |psi> with psi and
+// lambda as they are before the gate is undone.
 bool derivative_of(const kq_op& op, const GateRec& g, GateRec& d)
 {
-    const double th = op.angle;
     d = g;
     d.kind = GK_DENSE;
     std::fill(std::begin(d.m), std::end(d.m), 0.0);
     switch (op.gate) {
-        case KQ_RX: case KQ_CRX: {
-            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-            d.m[0] = -s / 2; d.m[3] = -c / 2; d.m[5] = -c / 2; d.m[6] = -s / 2;
+        case KQ_RX: case KQ_CRX:
+            d.m[3] = -0.5; d.m[5] = -0.5;
             return true;
-        }
-        case KQ_RY: case KQ_CRY: {
-            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-            d.m[0] = -s / 2; d.m[2] = -c / 2; d.m[4] = c / 2; d.m[6] = -s / 2;
+        case KQ_RY: case KQ_CRY:
+            d.m[2] = -0.5; d.m[4] = 0.5;
             return true;
-        }
-        case KQ_RZ: case KQ_CRZ: {
-            const double c = std::cos(th / 2.0), s = std::sin(th / 2.0);
-            d.m[0] = -s / 2; d.m[1] = -c / 2; d.m[6] = -s / 2; d.m[7] = c / 2;
+        case KQ_RZ: case KQ_CRZ:
+            d.kind = GK_DIAG;
+            d.m[1] = -0.5; d.m[7] = 0.5;
             return true;
-        }
-        case KQ_P: case KQ_CP: {
-            const double c = std::cos(th), s = std::sin(th);
-            d.m[6] = -s; d.m[7] = c;
+        case KQ_P: case KQ_CP:
+            d.kind = GK_DIAG;
+            d.m[7] = 1.0;
             return true;
-        }
         default: break;
     }
     return false;

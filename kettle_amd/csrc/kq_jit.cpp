@@ -1108,8 +1108,8 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
 
 // Generate the kernel source of one adjoint reverse pass (two vectors: psi `a`, lambda `b`) and the
 // layout of its coefficient buffer.  The same records as the interpreted adjoint tile kernel, as
-// straight-line code: for every record, psi <- G psi; if it has a derivative record, Re <lambda| dG |psi>
-// over the tile, reduced per wavefront into partials[slot * n_part + column]; then lambda <- G lambda.
+// straight-line code: for every record, if it has a derivative record Q, Re <lambda| Q |psi> over the
+// tile, reduced per wavefront into partials[slot * n_part + column]; then psi <- G psi, lambda <- G lambda.
 // Remaps move psi, then lambda, through one LDS tile of whole amplitudes.
 std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
 {
@@ -1264,7 +1264,6 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                 const size_t off = push8(g.m);
                 o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n";
             }
-            o << "    if (ok) { " << apply_to(g, "v") << " }\n";
             if (g.pad != 0) {
                 const TileGate& dg = tp.derivs[g.pad - 1];
                 const size_t doff = push8(dg.m);
@@ -1276,11 +1275,33 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                       << ">(v, w, dsel ? dm.m[6] : dm.m[0], dsel ? dm.m[7] : dm.m[1]);\n";
                 }
                 else {
-                    o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
+                    // the angle gates' fixed generators get their own code (no coefficients)
+                    const auto dv = record_values(dg, c64);
+                    const auto is = [&](std::array<double, 8> want) { return dv == want; };
+                    int gen = -1;
+                    if (is({0, 0, 0, -0.5, 0, -0.5, 0, 0})) {
+                        gen = 0;
+                    }
+                    else if (is({0, 0, -0.5, 0, 0.5, 0, 0, 0})) {
+                        gen = 1;
+                    }
+                    else if (is({0, -0.5, 0, 0, 0, 0, 0, 0.5})) {
+                        gen = 2;
+                    }
+                    else if (is({0, 0, 0, 0, 0, 0, 0, 1.0})) {
+                        gen = 3;
+                    }
+                    if (gen >= 0) {
+                        o << "        acc = ddot_gen<R, " << gen << ", " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w);\n";
+                    }
+                    else {
+                        o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
+                    }
                 }
                 o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
                   << "u} * n_part + col] = acc;\n      }\n    }\n";
             }
+            o << "    if (ok) { " << apply_to(g, "v") << " }\n";
             o << "    if (ok) { " << apply_to(g, "w") << " }\n  }\n";
         }
     }

@@ -774,6 +774,18 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_gate(Amp<R>* psi, Amp<R>* lam
         const uint64_t i0 = (ctrl ? insert_zero2(p, lo, hi) : insert_zero(p, t)) | cbit;
         const uint64_t i1 = i0 | tbit;
         Cx<R> x = ld<R>(psi, i0), y = ld<R>(psi, i1);
+        Cx<R> u = ld<R>(lam, i0), v = ld<R>(lam, i1);
+        if (has_d) {
+            // Re <lambda| Q |psi> before the gate is undone (dG = Q G), in FP64
+            const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
+            const double m0r = xr * d.m[0] - xi * d.m[1] + yr * d.m[2] - yi * d.m[3];
+            const double m0i = xr * d.m[1] + xi * d.m[0] + yr * d.m[3] + yi * d.m[2];
+            const double m1r = xr * d.m[4] - xi * d.m[5] + yr * d.m[6] - yi * d.m[7];
+            const double m1i = xr * d.m[5] + xi * d.m[4] + yr * d.m[7] + yi * d.m[6];
+            const double ur = u.re, ui = u.im, vr = v.re, vi = v.im;
+            accr += ur * m0r + ui * m0i + vr * m1r + vi * m1i;
+            acci += ur * m0i - ui * m0r + vr * m1i - vi * m1r;
+        }
         if (inv.kind == GK_SWAP) {
             const Cx<R> tmp = x;
             x = y;
@@ -788,18 +800,6 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint_gate(Amp<R>* psi, Amp<R>* lam
         }
         st<R>(psi, i0, x);
         st<R>(psi, i1, y);
-        Cx<R> u = ld<R>(lam, i0), v = ld<R>(lam, i1);
-        if (has_d) {
-            // mu = d (x, y), in FP64
-            const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
-            const double m0r = xr * d.m[0] - xi * d.m[1] + yr * d.m[2] - yi * d.m[3];
-            const double m0i = xr * d.m[1] + xi * d.m[0] + yr * d.m[3] + yi * d.m[2];
-            const double m1r = xr * d.m[4] - xi * d.m[5] + yr * d.m[6] - yi * d.m[7];
-            const double m1i = xr * d.m[5] + xi * d.m[4] + yr * d.m[7] + yi * d.m[6];
-            const double ur = u.re, ui = u.im, vr = v.re, vi = v.im;
-            accr += ur * m0r + ui * m0i + vr * m1r + vi * m1i;
-            acci += ur * m0i - ui * m0r + vr * m1i - vi * m1r;
-        }
         if (inv.kind == GK_SWAP) {
             const Cx<R> tmp = u;
             u = v;

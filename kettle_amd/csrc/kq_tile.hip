@@ -277,9 +277,9 @@ __device__ __forceinline__ double wave_sum(double x, int width)
 
 // ADJ = false: one state, gates applied in order.
 // ADJ = true: the reverse sweep of the adjoint gradient over two states, psi (a) and lambda (b);
-// the gate records are the inverse gates G^dag in reverse circuit order.  For each gate: psi <- G^dag
-// psi, then, if the record has a derivative record (pad = index + 1), Re <lambda| dG |psi> over the
-// tile is reduced per wavefront into partials[slot * n_part + block * waves + wave]; then
+// the gate records are the inverse gates G^dag in reverse circuit order.  For each gate: if the record
+// has a derivative record Q (pad = index + 1; dG = Q G), Re <lambda| Q |psi> over the tile is reduced
+// per wavefront into partials[slot * n_part + block * waves + wave]; then psi <- G^dag psi and
 // lambda <- G^dag lambda.
 template <typename R, bool ADJ>
 __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __restrict__ b, const TileArgs& d,
@@ -411,17 +411,6 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                 else if (code >= TV_UNIFORM_BASE) {
                     uniform_factor(fr, fi);
                 }
-                if (ok) {
-                    if (code >= TV_MERGE_BASE) {
-                        apply_merged<R>(v, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
-                    }
-                    else if (code >= TV_UNIFORM_BASE) {
-                        apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
-                    }
-                    else {
-                        apply_gate<R>(v, code, m);
-                    }
-                }
                 if (g.pad != 0) {
                     const TileGate dg = derivs[g.pad - 1];
                     const Coef<R> dm = coef_of<R>(dg);
@@ -439,6 +428,17 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                     acc = wave_sum(acc, width);
                     if ((tid & 63u) == 0) {
                         partials[static_cast<size_t>(dg.pad) * n_part + part_col] = acc;
+                    }
+                }
+                if (ok) {
+                    if (code >= TV_MERGE_BASE) {
+                        apply_merged<R>(v, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
+                    }
+                    else if (code >= TV_UNIFORM_BASE) {
+                        apply_uniform<R>(v, code - TV_UNIFORM_BASE, fr, fi);
+                    }
+                    else {
+                        apply_gate<R>(v, code, m);
                     }
                 }
                 if (ok) {

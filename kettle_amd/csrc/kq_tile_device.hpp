@@ -247,6 +247,36 @@ __device__ __forceinline__ double ddot(const Cx<R> (&v)[16], const Cx<R> (&w)[16
     return acc;
 }
 
+// Re <w| Q |v> over the pairs of register bit P (register control C) for the fixed generators of the
+// angle gates' derivatives (dG/dtheta = Q G): GEN 0: (-i/2) X, 1: (-i/2) Y, 2: (-i/2) Z, 3: i |1><1|.
+// Products per pair in R, summed in FP64.
+template <typename R, int GEN, int P, int C>
+__device__ __forceinline__ double ddot_gen(const Cx<R> (&v)[16], const Cx<R> (&w)[16])
+{
+    double acc = 0.0;
+    Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+        if constexpr (((r >> P) & 1) == 0 && (C == 0 || ((r >> (C - 1)) & 1) == 1) && C != P + 1) {
+            constexpr int r1 = r | (1 << P);
+            const Cx<R> a = v[r], b = v[r1], w0 = w[r], w1 = w[r1];
+            R t;
+            if constexpr (GEN == 0) {
+                t = (w0.re * b.im - w0.im * b.re) + (w1.re * a.im - w1.im * a.re);
+            }
+            else if constexpr (GEN == 1) {
+                t = (w1.re * a.re + w1.im * a.im) - (w0.re * b.re + w0.im * b.im);
+            }
+            else if constexpr (GEN == 2) {
+                t = (w0.re * a.im - w0.im * a.re) - (w1.re * b.im - w1.im * b.re);
+            }
+            else {
+                t = w1.im * b.re - w1.re * b.im;
+            }
+            acc += static_cast<double>(t);
+        }
+    });
+    return GEN == 3 ? acc : 0.5 * acc;
+}
+
 // uniform diagonal derivative: sum over the slots (with register control C) of Re(conj(w) f v)
 template <typename R, int C>
 __device__ __forceinline__ double ddot_uniform(const Cx<R> (&v)[16], const Cx<R> (&w)[16], R fr, R fi)
