@@ -443,6 +443,32 @@ void execute_passes(kq_state* s, const std::vector<TilePass>& passes)
     stream.finish();
 }
 
+namespace
+{
+
+// Plans of plan_statevector_passes by gate structure (kinds, bits; not the coefficients) and the
+// settings they were planned with: a repeated structure (an optimiser's next angles) takes the cached
+// passes with its own coefficients.
+struct PlanCache
+{
+    struct Entry
+    {
+        std::vector<uint32_t> key;
+        std::vector<TilePass> passes;
+        std::vector<uint8_t> bit_of;
+    };
+    std::mutex mu;
+    std::unordered_map<uint64_t, Entry> entries;
+};
+
+PlanCache& plan_cache()
+{
+    static PlanCache c;
+    return c;
+}
+
+}  // namespace
+
 std::vector<TilePass> plan_statevector_passes(kq_state* s, const std::vector<GateRec>& recs, bool allow_sweep,
                                               std::vector<uint8_t>& bit_of)
 {
@@ -452,10 +478,48 @@ std::vector<TilePass> plan_statevector_passes(kq_state* s, const std::vector<Gat
         bit_of[q] = static_cast<uint8_t>(q);
     }
     const bool relabel = fusion_relabel() && !s->is_dm && max_k < static_cast<int>(s->nvec);
-    const FusionChoice fc = relabel && tuned(s, max_k) ? choose_fusion(s, recs, max_k, bit_of)
-                                                       : FusionChoice {fusion_low_qubits(), fusion_max_stages()};
-    return plan_passes(recs, s->nvec, max_k, fc.low_c, fusion_max_gates(), register_qubits(s->precision), fc.max_stages,
-                       allow_sweep, relabel ? &bit_of : nullptr);
+    const bool tuner = relabel && tuned(s, max_k);
+    std::vector<uint32_t> key {s->nvec, static_cast<uint32_t>(s->precision), static_cast<uint32_t>(max_k),
+                               static_cast<uint32_t>(fusion_low_qubits()), static_cast<uint32_t>(fusion_max_stages()),
+                               static_cast<uint32_t>(fusion_max_gates()), relabel ? 1u : 0u, tuner ? 1u : 0u,
+                               allow_sweep ? 1u : 0u};
+    key.reserve(key.size() + 3 * recs.size());
+    for (const GateRec& g : recs) {
+        key.push_back(g.kind);
+        key.push_back(g.target);
+        key.push_back(static_cast<uint32_t>(g.control));
+    }
+    uint64_t h = 1469598103934665603ull;
+    for (const uint32_t v : key) {
+        h = (h ^ v) * 1099511628211ull;
+    }
+    PlanCache& c = plan_cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        const auto it = c.entries.find(h);
+        if (it != c.entries.end() && it->second.key == key) {
+            std::vector<TilePass> passes = it->second.passes;
+            for (TilePass& p : passes) {
+                for (size_t i = 0; i < p.gates.size(); ++i) {
+                    std::memcpy(p.gates[i].m, recs[p.src[i]].m, sizeof p.gates[i].m);
+                }
+                if (p.sweep) {
+                    std::memcpy(p.single.m, recs[p.src[0]].m, sizeof p.single.m);
+                }
+            }
+            bit_of = it->second.bit_of;
+            return passes;
+        }
+    }
+    const FusionChoice fc = tuner ? choose_fusion(s, recs, max_k, bit_of) : FusionChoice {fusion_low_qubits(), fusion_max_stages()};
+    std::vector<TilePass> passes = plan_passes(recs, s->nvec, max_k, fc.low_c, fusion_max_gates(), register_qubits(s->precision),
+                                               fc.max_stages, allow_sweep, relabel ? &bit_of : nullptr);
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.entries.size() >= 64) {
+        c.entries.clear();
+    }
+    c.entries[h] = PlanCache::Entry {std::move(key), passes, bit_of};
+    return passes;
 }
 
 void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
