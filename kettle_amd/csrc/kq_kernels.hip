@@ -480,50 +480,97 @@ __device__ __forceinline__ void pauli_coef(uint64_t i, const PauliGroup& g, cons
     }
 }
 
-// (H a)_i = sum over groups (in order) of c_g(i) a[i ^ x_g], U groups' gathers issued before their
-// coefficients are formed (measured on the 24-qubit, 248-string operator: U = 8 is slower than U = 1).
+// (H a)_i = sum over groups (in order) of c_g(i) a[i ^ x_g], for one index (batched kernels).
 template <typename R>
 __device__ __forceinline__ void h_apply_row(const Amp<R>* a, uint64_t i, const PauliGroup* groups, uint32_t ng,
                                             const PauliZ* zs, double& hr, double& hi)
 {
-    constexpr uint32_t U = 1;
     hr = 0.0;
     hi = 0.0;
-    for (uint32_t g0 = 0; g0 < ng; g0 += U) {
-        Amp<R> u[U];
+    for (uint32_t gi = 0; gi < ng; ++gi) {
+        const PauliGroup g = groups[gi];
+        double cr, ci;
+        pauli_coef<R>(i, g, zs, cr, ci);
+        const Amp<R> u = a[i ^ g.x];
+        const double br = u.x, bi = u.y;
+        hr += cr * br - ci * bi;
+        hi += cr * bi + ci * br;
+    }
+}
+
+// (H a)_i for the J indices i0 + j * T of one thread (T: threads of the grid), in the same order per index
+// as h_apply_row: per group the J gathers go out together and each string's sign mask is read once for
+// all J indices.  NARROW: every index and mask fits 32 bits (32-bit popcounts).
+constexpr int PAULI_ROWS = 4;
+template <typename R, bool NARROW>
+__device__ __forceinline__ void h_apply_rows(const Amp<R>* a, const uint64_t (&idx)[PAULI_ROWS], const bool (&in)[PAULI_ROWS],
+                                             const PauliGroup* groups, uint32_t ng, const PauliZ* zs,
+                                             double (&hr)[PAULI_ROWS], double (&hi)[PAULI_ROWS])
+{
 #pragma unroll
-        for (uint32_t j = 0; j < U; ++j) {
-            if (g0 + j < ng) {
-                u[j] = a[i ^ groups[g0 + j].x];
+    for (int j = 0; j < PAULI_ROWS; ++j) {
+        hr[j] = 0.0;
+        hi[j] = 0.0;
+    }
+    for (uint32_t gi = 0; gi < ng; ++gi) {
+        const PauliGroup g = groups[gi];
+        Amp<R> u[PAULI_ROWS];
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            u[j] = in[j] ? a[idx[j] ^ g.x] : Amp<R> {};
+        }
+        double cr[PAULI_ROWS], ci[PAULI_ROWS];
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            cr[j] = 0.0;
+            ci[j] = 0.0;
+        }
+        for (uint32_t s = 0; s < g.count; ++s) {
+            const PauliZ z = zs[g.first + s];
+#pragma unroll
+            for (int j = 0; j < PAULI_ROWS; ++j) {
+                const uint32_t par = NARROW ? (__popc(static_cast<uint32_t>(idx[j]) & static_cast<uint32_t>(z.z)) & 1u)
+                                            : (static_cast<uint32_t>(__popcll(idx[j] & z.z)) & 1u);
+                const double sg = par ? -1.0 : 1.0;
+                cr[j] += sg * z.w[0];
+                ci[j] += sg * z.w[1];
             }
         }
 #pragma unroll
-        for (uint32_t j = 0; j < U; ++j) {
-            if (g0 + j < ng) {
-                double cr, ci;
-                pauli_coef<R>(i, groups[g0 + j], zs, cr, ci);
-                const double br = u[j].x, bi = u[j].y;
-                hr += cr * br - ci * bi;
-                hi += cr * bi + ci * br;
-            }
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            const double br = u[j].x, bi = u[j].y;
+            hr[j] += cr[j] * br - ci[j] * bi;
+            hi[j] += cr[j] * bi + ci[j] * br;
         }
     }
 }
 
-template <typename R>
+template <typename R, bool NARROW>
 __global__ __launch_bounds__(BLOCK) void k_pauli_expect(const Amp<R>* a, uint64_t n, const PauliGroup* groups, int ng,
                                                         const PauliZ* zs, double2* part)
 {
     double er = 0.0, ei = 0.0;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
-    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
-        const Amp<R> v = a[i];
-        const double ar = v.x, ai = v.y;
-        double hr, hi;
-        h_apply_row<R>(a, i, groups, static_cast<uint32_t>(ng), zs, hr, hi);
-        // conj(a) * (H a)_i
-        er += ar * hr + ai * hi;
-        ei += ar * hi - ai * hr;
+    const uint64_t T = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i0 = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i0 < n; i0 += PAULI_ROWS * T) {
+        uint64_t idx[PAULI_ROWS];
+        bool in[PAULI_ROWS];
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            idx[j] = i0 + j * T;
+            in[j] = idx[j] < n;
+        }
+        double hr[PAULI_ROWS], hi[PAULI_ROWS];
+        h_apply_rows<R, NARROW>(a, idx, in, groups, static_cast<uint32_t>(ng), zs, hr, hi);
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            if (in[j]) {
+                // conj(a) * (H a)_i
+                const Amp<R> v = a[idx[j]];
+                const double ar = v.x, ai = v.y;
+                er += ar * hr[j] + ai * hi[j];
+                ei += ar * hi[j] - ai * hr[j];
+            }
+        }
     }
     const double2 r = block_sum2(er, ei);
     if (threadIdx.x == 0) {
@@ -531,20 +578,34 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_expect(const Amp<R>* a, uint64_
     }
 }
 
-template <typename R>
+template <typename R, bool NARROW>
 __global__ __launch_bounds__(BLOCK) void k_pauli_apply_sum(const Amp<R>* a, Amp<R>* b, uint64_t n, const PauliGroup* groups,
                                                            int ng, const PauliZ* zs)
 {
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * BLOCK;
-    for (uint64_t i = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i < n; i += stride) {
-        double hr, hi;
-        h_apply_row<R>(a, i, groups, static_cast<uint32_t>(ng), zs, hr, hi);
-        Amp<R> w;
-        w.x = static_cast<R>(hr);
-        w.y = static_cast<R>(hi);
-        b[i] = w;
+    const uint64_t T = static_cast<uint64_t>(gridDim.x) * BLOCK;
+    for (uint64_t i0 = blockIdx.x * uint64_t {BLOCK} + threadIdx.x; i0 < n; i0 += PAULI_ROWS * T) {
+        uint64_t idx[PAULI_ROWS];
+        bool in[PAULI_ROWS];
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            idx[j] = i0 + j * T;
+            in[j] = idx[j] < n;
+        }
+        double hr[PAULI_ROWS], hi[PAULI_ROWS];
+        h_apply_rows<R, NARROW>(a, idx, in, groups, static_cast<uint32_t>(ng), zs, hr, hi);
+#pragma unroll
+        for (int j = 0; j < PAULI_ROWS; ++j) {
+            if (in[j]) {
+                Amp<R> w;
+                w.x = static_cast<R>(hr[j]);
+                w.y = static_cast<R>(hi[j]);
+                b[idx[j]] = w;
+            }
+        }
     }
 }
+
+
 
 // in place a_i <- w (-1)^{|i & z|} a_{i ^ x}; for x != 0 each pair (i, i ^ x) is handled by the
 // index whose highest bit of x is 0
@@ -1281,9 +1342,16 @@ void launch_pauli_expectation(kq_state* s, const std::vector<PauliGroup>& groups
     auto* part = static_cast<double2*>(s->scratch);
     const unsigned blocks = grid_for(s->n_amps, 8, RED_BLOCKS);
     const int ng = static_cast<int>(groups.size());
-    KQ_DISPATCH(s,
-                (k_pauli_expect<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), s->n_amps, dg, ng, dz, part)),
-                (k_pauli_expect<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), s->n_amps, dg, ng, dz, part)));
+    if (s->nvec <= 32) {
+        KQ_DISPATCH(s,
+                    (k_pauli_expect<double, true><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), s->n_amps, dg, ng, dz, part)),
+                    (k_pauli_expect<float, true><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), s->n_amps, dg, ng, dz, part)));
+    }
+    else {
+        KQ_DISPATCH(s,
+                    (k_pauli_expect<double, false><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(s->amps), s->n_amps, dg, ng, dz, part)),
+                    (k_pauli_expect<float, false><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(s->amps), s->n_amps, dg, ng, dz, part)));
+    }
     k_reduce_partials<<<1, BLOCK, 0, s->stream>>>(part, static_cast<int>(blocks), part + RED_BLOCKS);
     KQ_HIP_CHECK(hipGetLastError());
     double2 h;
@@ -1299,11 +1367,18 @@ void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const st
     PauliGroup* dg;
     PauliZ* dz;
     upload_pauli(s, 0, groups, strings, &dg, &dz);
-    const unsigned blocks = grid_for(s->n_amps, 4);
+    const unsigned blocks = grid_for(s->n_amps, PAULI_ROWS);
     const int ng = static_cast<int>(groups.size());
-    KQ_DISPATCH(s,
-                (k_pauli_apply_sum<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), s->n_amps, dg, ng, dz)),
-                (k_pauli_apply_sum<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(d_in), static_cast<float2*>(d_out), s->n_amps, dg, ng, dz)));
+    if (s->nvec <= 32) {
+        KQ_DISPATCH(s,
+                    (k_pauli_apply_sum<double, true><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), s->n_amps, dg, ng, dz)),
+                    (k_pauli_apply_sum<float, true><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(d_in), static_cast<float2*>(d_out), s->n_amps, dg, ng, dz)));
+    }
+    else {
+        KQ_DISPATCH(s,
+                    (k_pauli_apply_sum<double, false><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), s->n_amps, dg, ng, dz)),
+                    (k_pauli_apply_sum<float, false><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const float2*>(d_in), static_cast<float2*>(d_out), s->n_amps, dg, ng, dz)));
+    }
 }
 
 void launch_pauli_apply_one(kq_state* s, uint64_t x, uint64_t z, const double w[2])
