@@ -123,7 +123,7 @@ def plain_sv_section(n: int, reps: int) -> dict:
         sv.set_basis(0)
         ka.apply_gates(sv, ops)
 
-    stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
+    stats = np.zeros(len(_lib.PASS_CLASSES), dtype=_lib.PASS_STATS_DTYPE)
     sim()
     _lib.check(lib.kq_profile_begin())
     t_sim = timed(sim)
@@ -153,7 +153,7 @@ def density_matrix_section(n: int, reps: int) -> dict:
     ops, _, _ = c.flatten()
     n_gates = int(np.sum(ops["gate"] < int(ka.Gate.M)))
     dm = ka.DensityMatrix(n)
-    stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
+    stats = np.zeros(len(_lib.PASS_CLASSES), dtype=_lib.PASS_STATS_DTYPE)
 
     def sim():
         dm.set_basis(0)
@@ -200,6 +200,31 @@ def vqe_section(n: int, layers: int, n_random: int, energies_sample: int) -> dic
         st.set_basis(0)
         r = ka.energy_and_gradient(w.circuit, w.hamiltonian, w.params, state=st)
     t_adj = (time.perf_counter() - t0) / reps
+    # one more evaluation with every pass and the H psi launch bracketed by HIP events
+    from kettle_amd import _lib
+    lib = _lib.load()
+    stats = np.zeros(len(_lib.PASS_CLASSES), dtype=_lib.PASS_STATS_DTYPE)
+    st.set_basis(0)
+    _lib.check(lib.kq_profile_begin())
+    ka.energy_and_gradient(w.circuit, w.hamiltonian, w.params, state=st)
+    _lib.check(lib.kq_profile_end(_lib.ptr(stats)))
+    parts = {}
+    for i, cls in enumerate(_lib.PASS_CLASSES):
+        s = stats[i]
+        if s["launches"]:
+            parts[cls] = {"launches": int(s["launches"]), "ms": round(float(s["ms"]), 3),
+                          "GB_per_s": round(float(s["bytes"]) / (float(s["ms"]) * 1e-3) / 1e9, 1)}
+    fw = stats[0].copy()
+    for f in ("ms", "bytes"):
+        fw[f] = stats[0][f] + stats[1][f]
+    rv = stats[3]
+    dom_rev = float(rv["ms"]) >= float(fw["ms"])
+    dom = rv if dom_rev else fw
+    dom_gbs = float(dom["bytes"]) / (float(dom["ms"]) * 1e-3) / 1e9 if dom["ms"] else 0.0
+    roofline = {"bound": "hbm", "kernel": "kq_adjoint_pass (reverse sweep)" if dom_rev else "kq_pass (forward sweep)",
+                "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None, "bytes_per_launch": (4 if dom_rev else 2) * 8 * (1 << n),
+                "classes": parts, "profiled_eval_ms": round(float(stats["ms"].sum()), 3)}
     # the FP64 reference energy on the device (complex128 forward + expectation)
     sv = ka.Statevector(n, ka.KQ_C128)
     ka.simulate(w.circuit, sv)
@@ -213,8 +238,12 @@ def vqe_section(n: int, layers: int, n_random: int, energies_sample: int) -> dic
         sv.set_basis(0)
         ka.apply_gates(sv, shifted)
         ka.expectation_value(w.hamiltonian, sv)
-    t_energy = (time.perf_counter() - t0) / energies_sample
+    t_energy = (time.perf_counter() - t0) / max(energies_sample, 1)
     t_plain = (2 * P + 1) * t_energy
+    if energies_sample == 0:
+        return {"config": f"{n}q n_local(RY,RZ; CX linear) x{layers}, {P} params, {w.hamiltonian.size()} Pauli strings",
+                "adjoint_c64_evals_per_s": round(1.0 / t_adj, 4), "energy_err_vs_fp64": abs(r.energy - e64),
+                "roofline": roofline}
     return {
         "config": f"{n}q n_local(RY,RZ; CX linear) x{layers}, {P} params, {w.hamiltonian.size()} Pauli strings",
         "adjoint_c64_evals_per_s": round(1.0 / t_adj, 4),
@@ -223,6 +252,7 @@ def vqe_section(n: int, layers: int, n_random: int, energies_sample: int) -> dic
         "energy_err_vs_fp64": abs(r.energy - e64),
         "plain_timing": f"{energies_sample} complex128 energies timed ({t_energy * 1e3:.2f} ms each), x (2P+1)",
         "checkpoint_restores": r.restores,
+        "roofline": roofline,
     }
 
 
@@ -251,11 +281,13 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
             r = b.run(use_graph=True)
         t = (time.perf_counter() - t0) / reps
         info = b.info()
+        info.update(b.plan_info())
         b.close()
         return r, t, info
 
     r64, t64, _ = timed(ka.KQ_C64)
     r, t_run, info = timed(ka.KQ_C128)
+    achieved = info["algorithmic_bytes"] / t_run / 1e9
     # plain per-instance FP64 path on a sample, scaled to the whole batch
     idx = np.unique(np.linspace(0, n_instances - 1, plain_sample).astype(int))
     t_plain_sample = 0.0
@@ -298,6 +330,13 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
         "all_c64_energy_err_vs_fp64_sample": worst64,
         "all_c64_note": "complex64 forward sweep too: informational, its energy error is that of FP32 amplitudes "
                         "and is not claimed against the 1e-6 bar (the reported mode runs the forward sweep in complex128)",
+        "roofline": {"bound": "hbm", "kernel": "whole batch run (hipGraph: tile-batch passes, Pauli, sums)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_run": round(info["algorithmic_bytes"]),
+                     "forward_passes": info["forward_passes"], "reverse_passes": info["reverse_passes"],
+                     "note": "algorithmic bytes of every launch over the whole replay's wall time (host "
+                             "read-back included): a lower bound on the passes' own rate"},
         "max_inverse_drift": float(np.max(r.drift)),
         "launches_per_batch": info["launches"],
         "device_bytes": info["device_bytes"],
@@ -306,6 +345,39 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
         "plain_warmup_s": round(t_warm, 2),
         "plain_warmup_compile_s": round(jit1["compile_seconds"] - jit0["compile_seconds"], 2),
     }
+
+
+def dense_section(n: int, reps: int) -> dict:
+    """A/B of a dense 4-qubit block (16 x 16 complex U on qubits 0..3 of an n-qubit state, in place):
+    the VALU form against the MFMA form (kq_debug_dense_block), complex128 and complex64, each launch
+    timed with HIP events after one warm-up launch.  128 flops per amplitude; bytes = one read + one
+    write of the state."""
+    import kettle_amd as ka
+    from kettle_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    q, r = np.linalg.qr(rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16)))
+    u = np.ascontiguousarray(q * (np.diag(r) / np.abs(np.diag(r)))).view(np.float64).reshape(-1)
+    out = {"workload": f"{n}q state, dense 16x16 complex block on qubits 0..3, in place", "flops_per_amp": 128}
+    for prec, name, ab in ((ka.KQ_C128, "c128", 16), (ka.KQ_C64, "c64", 8)):
+        st = ka.Statevector(n, prec)
+        res = {}
+        for form, mfma in (("valu", 0), ("mfma", 1)):
+            ms = ctypes.c_double()
+            _lib.check(lib.kq_debug_dense_block(st.handle, _lib.ptr(u), mfma, ctypes.byref(ms)))
+            t = []
+            for _ in range(reps):
+                _lib.check(lib.kq_debug_dense_block(st.handle, _lib.ptr(u), mfma, ctypes.byref(ms)))
+                t.append(ms.value)
+            avg = float(np.mean(t))
+            res[form] = {"avg_ms": round(avg, 4), "min_ms": round(min(t), 4),
+                         "GB_per_s": round(2 * ab * (1 << n) / (avg * 1e-3) / 1e9, 1),
+                         "TFLOP_per_s": round(128 * (1 << n) / (avg * 1e-3) / 1e12, 2)}
+        res["mfma_speedup"] = round(res["valu"]["avg_ms"] / res["mfma"]["avg_ms"], 3)
+        out[name] = res
+        del st
+    return out
 
 
 def main() -> int:
@@ -347,6 +419,10 @@ def main() -> int:
     ap.add_argument("--jit-priority", type=int, default=1,
                     help="wave priority around a tile's HBM traffic in the run-time compiled passes: 0 off, "
                          "1 loads, 2 loads and stores")
+    ap.add_argument("--only", choices=["c64", "plain", "dm", "vqe", "batch", "dense"], default=None,
+                    help="run one side section alone and print its JSON (profiling runs)")
+    ap.add_argument("--jit-nontemporal", type=int, default=0,
+                    help="nontemporal hint on the run-time compiled passes' HBM traffic: 0 off, 1 loads, 2 loads and stores")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()
@@ -368,6 +444,18 @@ def main() -> int:
     log(f"device {name.value.decode()} CUs={ncu.value} mem={mem.value / 2**30:.0f} GiB")
 
     n = args.qubits
+    if args.only:
+        ka.set_jit_priority(args.jit_priority)
+        ka.set_jit_nontemporal(args.jit_nontemporal)
+        only = {"c64": lambda: c64_section(args.c64_qubits, args.layers, args.c64_steps),
+                "plain": lambda: plain_sv_section(20, 5),
+                "dm": lambda: density_matrix_section(args.dm_qubits, 2),
+                "vqe": lambda: vqe_section(args.vqe_qubits, 20, 200, args.vqe_energies),
+                "batch": lambda: batch_section(args.batch_instances, args.batch_reps, 32),
+                "dense": lambda: dense_section(args.qubits, 5)}[args.only]
+        log(f"{args.only} section only")
+        print(json.dumps({args.only: only()}))
+        return 0
     if args.calib_copy:
         # a plain copy kernel of known size: reads and writes 16 * 2^n bytes per launch
         src = ka.Statevector(n, ka.KQ_C128)
@@ -397,6 +485,7 @@ def main() -> int:
     waves, per_cu, normalise = (int(x) for x in args.jit_tuning.split(","))
     ka.set_jit_tuning(waves, per_cu, bool(normalise))
     ka.set_jit_priority(args.jit_priority)
+    ka.set_jit_nontemporal(args.jit_nontemporal)
     for i in range(args.warmup):
         step()
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -404,7 +493,7 @@ def main() -> int:
     tuner = ka.fusion_autotune_info()
     _, n_passes, n_tiles = ka.plan_stats()
     _lib.check(lib.kq_synchronize())
-    stats = np.zeros(3, dtype=_lib.PASS_STATS_DTYPE)
+    stats = np.zeros(len(_lib.PASS_CLASSES), dtype=_lib.PASS_STATS_DTYPE)
     _lib.check(lib.kq_profile_begin())
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -441,6 +530,7 @@ def main() -> int:
     for f in ("ms", "bytes"):
         tile[f] = stats[0][f] + stats[1][f]
     tile["launches"] = stats[0]["launches"] + stats[1]["launches"]
+    fastest = min(float(stats[c]["min_ms"]) for c in (0, 1) if stats[c]["launches"]) if tile["launches"] else 0.0
     kernel_ms = float(stats["ms"].sum()) / args.steps
     achieved = float(tile["bytes"]) / (float(tile["ms"]) * 1e-3) / 1e9 if tile["ms"] else 0.0
     jit_used = jit_end["cache_hits"] > jit_warm["cache_hits"]
@@ -491,13 +581,20 @@ def main() -> int:
     del sv
     if not args.no_stream_probe:
         log("hbm stream probe")
-        gbps = (ctypes.c_double * 4)()
-        _lib.check(lib.kq_stream_probe(n, gbps))
-        probe = {"read": round(gbps[0], 1), "write": round(gbps[1], 1), "in_place": round(gbps[2], 1),
-                 "copy": round(gbps[3], 1), "unit": "GB/s", "elements": f"2^{n} x 16 B"}
+        det = (ctypes.c_double * 24)()
+        _lib.check(lib.kq_stream_probe_shapes(n, det))
+        modes = ("read", "write", "in_place", "copy")
+        shapes = ("persistent_2_per_cu", "persistent_8_per_cu", "block_per_4096", "block_per_4096_nontemporal",
+                  "block_per_4096_prio", "block_per_4096_nontemporal_prio")
+        best = [max(det[4 * sh + m] for sh in range(6)) for m in range(4)]
+        probe = {m: round(best[i], 1) for i, m in enumerate(modes)}
+        probe.update({"unit": "GB/s", "elements": f"2^{n} x 16 B", "best_of": "6 launch shapes",
+                      "shapes": {sh: [round(det[4 * j + m], 1) for m in range(4)] for j, sh in enumerate(shapes)}})
         out["hbm_stream_probe"] = probe
         # the tile pass reads and writes every amplitude once, like the in-place stream
-        roofline["frac_of_in_place_stream"] = round(achieved / max(gbps[2], 1e-9), 4)
+        roofline["frac_of_in_place_stream"] = round(achieved / max(best[2], 1e-9), 4)
+        # fastest single pass launch of the timed steps against the probe
+        roofline["fastest_launch_GB_per_s"] = round(2 * 16 * (1 << n) / (fastest * 1e-3) / 1e9, 1) if fastest > 0 else None
     if args.c64_qubits > 0:
         log(f"complex64 section: {args.c64_qubits} qubits")
         out["complex64_large"] = c64_section(args.c64_qubits, args.layers, args.c64_steps)

@@ -172,6 +172,9 @@ int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise);
  * issues its tile's HBM loads, 2 also while it issues its stores, so that waves starting or finishing
  * a tile go ahead of the waves computing one.  No counterpart in the reference. */
 int kq_set_jit_priority(int mode);
+/* Nontemporal hint on the run-time pass kernels' HBM traffic: 0 (default) off, 1 on the tile loads,
+ * 2 on the loads and the stores.  No counterpart in the reference. */
+int kq_set_jit_nontemporal(int mode);
 /* Counters of the run-time compiled kernels; uncapped counts kernels compiled again without the
  * register cap because they spilled under it. */
 int kq_jit_info(int* mode, size_t* compiled, size_t* cache_hits, size_t* failures, double* compile_seconds,
@@ -284,27 +287,51 @@ int kq_batch_set_parameters(kq_batch* b, size_t index, const double* values);
 int kq_batch_run(kq_batch* b, int use_graph, double* energies, double* grads, double* drift);
 int kq_batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params_total, size_t* device_bytes,
                   size_t* n_launches);
+/* Plan totals over all instances (after the first run): forward and reverse tile passes, and the
+ * algorithmic bytes of one run (one read and/or write of every amplitude of every vector that each
+ * launch updates: reset, forward passes, lambda = H psi, reverse passes on psi and lambda, drift). */
+int kq_batch_plan_info(const kq_batch* b, size_t* fwd_passes, size_t* rev_passes, double* algorithmic_bytes);
+/* Drift-triggered recompute: after every later kq_batch_run, each instance whose drift exceeds
+ * drift_tol (or is NaN) is evaluated again by kq_energy_gradient on a complex128 work state of its
+ * qubit count (held by the batch, with a restore at every checkpoint), and that energy and
+ * gradient replace the batched ones in the outputs; drift[] still reports the batched sweep's.
+ * drift_tol < 0 (the default) turns it off; 0 recomputes every instance with any drift.
+ * kq_batch_recomputed reports how many instances the last run recomputed, and in total. */
+int kq_batch_set_drift_recompute(kq_batch* b, double drift_tol);
+int kq_batch_recomputed(const kq_batch* b, size_t* last_run, size_t* total);
 int kq_batch_destroy(kq_batch* b);
 
 /* ---- profiling ------------------------------------------------------------------------------
- * While enabled, every gate pass launched by kq_simulate / kq_apply_gates is bracketed by HIP
- * events on the state's own stream; durations and algorithmic bytes (read + write of every
- * amplitude a pass touches) accumulate per class:
+ * While enabled, every gate pass launched by kq_simulate / kq_apply_gates / kq_energy_gradient
+ * and every Pauli-operator launch of kq_energy_gradient is bracketed by HIP events on the state's
+ * own stream; durations and algorithmic bytes accumulate per class:
  *   0 = tile pass whose tile is the low contiguous qubits (LDS-fused low-index gates)
  *   1 = tile pass that includes higher qubits (strided high-index gates)
- *   2 = single-gate sweep (lone diagonal gates) */
+ *   2 = single-gate sweep (lone diagonal gates)
+ *   3 = adjoint reverse tile pass (two vectors and derivative partials)
+ *   4 = lambda = H psi (read psi + write lambda counted; the gathers per X-mask group re-read psi)
+ * Bytes: one read + one write of every amplitude of every vector a launch updates. */
+#define KQ_PASS_CLASSES 5
 typedef struct kq_pass_stats {
     double   ms;          /* summed kernel time */
     uint64_t launches;
     double   bytes;       /* summed algorithmic bytes */
     uint64_t gates;       /* gates applied by these launches */
+    double   min_ms;      /* shortest and longest launch */
+    double   max_ms;
 } kq_pass_stats;
 int kq_profile_begin(void);
 /* Device bandwidth reference: GB/s of plain streams of 16-byte elements over two 2^n_qubits
  * buffers (16 <= n_qubits <= 32, 2 x 16 * 2^n bytes of device memory for the call): read only,
- * write only, read-modify-write in place, copy.  No counterpart in the reference. */
+ * write only, read-modify-write in place, copy, each the best of six launch shapes.
+ * kq_stream_probe_shapes gives every shape: gbps[4 * shape + mode], shapes 0 / 1 = a persistent
+ * grid of 2 / 8 blocks of 256 threads per CU, 2 = one block per 4096-element chunk (the tile
+ * passes' shape), 3 = shape 2 with nontemporal loads and stores, 4 / 5 = shapes 2 / 3 with the
+ * wave priority raised while the loads issue (as the tile passes do).  No counterpart in the
+ * reference. */
 int kq_stream_probe(uint32_t n_qubits, double gbps[4]);
-int kq_profile_end(kq_pass_stats out[3]);
+int kq_stream_probe_shapes(uint32_t n_qubits, double gbps[24]);
+int kq_profile_end(kq_pass_stats out[KQ_PASS_CLASSES]);
 
 /* ---- debug / parity probes ------------------------------------------------------------------
  * Device-computed pair maps in kettle's generator order (gate_pair_generator.hpp:24-160):
@@ -351,6 +378,15 @@ int kq_debug_jit_source(uint32_t n_qubits, int precision, const kq_op* ops, size
 int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op* ops, size_t n_ops,
                   size_t lds_bytes, uint32_t* order, uint32_t* pass_of, int32_t* pass_kind,
                   double* pass_cost, size_t* n_passes);
+/* A/B probe of the vector ALUs against the matrix cores: psi[r, :] <- U psi[r, :] for every row r of
+ * 16 amplitudes (U on qubits 0..3, 16 x 16 complex, row-major, interleaved FP64; rounded to FP32 for a
+ * complex64 state), by one launch of the VALU form (use_mfma = 0) or of the MFMA form (16x16x4
+ * matrix instructions); *ms (nullable) is that launch's duration from HIP events.  Statevectors of
+ * at least 8 qubits.  No counterpart in the reference (kettle has no multi-qubit dense gate). */
+int kq_debug_dense_block(kq_state* state, const double* u, int use_mfma, double* ms);
+/* Lane layout of the 16x16x4 matrix instruction (FP64 for KQ_C128, FP32 for KQ_C64): one wave runs
+ * D = A x B once with lane l holding operands a[l], b[l]; d[4 l + e] is element e of lane l's D. */
+int kq_debug_mfma16_layout(int precision, const double* a, const double* b, double* d);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@ from .circuit import (ControlFlowBooleanKind, ControlFlowPredicate, Gate, Matrix
                       extend_circuit)
 from .api import (DensityMatrix, EnergyGradient, FusionPlan, PackedPauliOperator, PauliOperator, PauliPhase,
                   PauliTerm, QuantumNoise, debug_jit_compile, debug_jit_source, debug_plan, jit_info, set_jit,
-                  set_jit_tuning, set_jit_priority,
+                  set_jit_tuning, set_jit_priority, set_jit_nontemporal,
                   SimulationResult, SparsePauliString, Statevector, apply_gates, calculate_probabilities_raw,
                   energy_and_gradient, expectation_value, partial_trace, perform_measurements_as_counts,
                   perform_measurements_as_counts_marginal, perform_measurements_as_counts_raw,

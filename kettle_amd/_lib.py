@@ -31,9 +31,10 @@ OP_DTYPE = np.dtype(
 )
 assert OP_DTYPE.itemsize == 88
 
-# kq_pass_stats[3]
-PASS_STATS_DTYPE = np.dtype([("ms", "<f8"), ("launches", "<u8"), ("bytes", "<f8"), ("gates", "<u8")])
-PASS_CLASSES = ("tile_low", "tile_high", "sweep")
+# kq_pass_stats[KQ_PASS_CLASSES]
+PASS_STATS_DTYPE = np.dtype([("ms", "<f8"), ("launches", "<u8"), ("bytes", "<f8"), ("gates", "<u8"),
+                             ("min_ms", "<f8"), ("max_ms", "<f8")])
+PASS_CLASSES = ("tile_low", "tile_high", "sweep", "adjoint_reverse", "pauli_apply")
 
 LOGGER_FN = ctypes.CFUNCTYPE(None, c_void_p, c_int32, POINTER(c_double), POINTER(c_int8), c_size_t)
 
@@ -92,6 +93,9 @@ _SIGNATURES = {
     "kq_debug_pair_map_2q": (c_int, [c_uint32, c_uint32, c_uint32, c_int, c_void_p]),
     "kq_set_jit": (c_int, [c_int]),
     "kq_stream_probe": (c_int, [c_uint32, c_void_p]),
+    "kq_stream_probe_shapes": (c_int, [c_uint32, c_void_p]),
+    "kq_debug_dense_block": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_double)]),
+    "kq_debug_mfma16_layout": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "kq_sample_circuit_shots": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p,
                                         c_uint64, c_int, c_int32, c_uint64, c_void_p, POINTER(c_uint64)]),
     "kq_set_fusion_low_qubits": (c_int, [c_int]),
@@ -103,6 +107,7 @@ _SIGNATURES = {
     "kq_debug_check_relabel": (c_int, [c_uint32, c_void_p, c_size_t, c_void_p, POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_set_jit_tuning": (c_int, [c_int, c_int, c_int]),
     "kq_set_jit_priority": (c_int, [c_int]),
+    "kq_set_jit_nontemporal": (c_int, [c_int]),
     "kq_jit_info": (c_int, [POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double),
                             POINTER(c_size_t)]),
     "kq_debug_jit_compile": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, POINTER(c_size_t), POINTER(c_size_t)]),
@@ -116,6 +121,9 @@ _SIGNATURES = {
     "kq_batch_set_parameters": (c_int, [c_void_p, c_size_t, c_void_p]),
     "kq_batch_run": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "kq_batch_info": (c_int, [c_void_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t)]),
+    "kq_batch_set_drift_recompute": (c_int, [c_void_p, c_double]),
+    "kq_batch_plan_info": (c_int, [c_void_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(c_double)]),
+    "kq_batch_recomputed": (c_int, [c_void_p, POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_batch_destroy": (c_int, [c_void_p]),
     "kq_debug_plan": (c_int, [c_uint32, c_int, c_int, c_void_p, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
                               c_void_p, c_void_p]),

@@ -269,6 +269,11 @@ def set_jit_priority(mode: int) -> None:
     check(load().kq_set_jit_priority(int(mode)))
 
 
+def set_jit_nontemporal(mode: int) -> None:
+    """Nontemporal hint on the run-time pass kernels' HBM traffic: 0 off, 1 loads, 2 loads and stores."""
+    check(load().kq_set_jit_nontemporal(int(mode)))
+
+
 def jit_info() -> dict:
     c = _lib.ctypes
     mode, comp, hits, fails, secs, unc = c.c_int(), c.c_size_t(), c.c_size_t(), c.c_size_t(), c.c_double(), c.c_size_t()

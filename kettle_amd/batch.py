@@ -92,6 +92,23 @@ class EnergyGradientBatch:
             o += p
         return BatchResult(energies, out, drift)
 
+    def plan_info(self) -> dict:
+        """Forward / reverse tile passes over all instances and one run's algorithmic bytes."""
+        a, b, c = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_double()
+        check(load().kq_batch_plan_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"forward_passes": a.value, "reverse_passes": b.value, "algorithmic_bytes": c.value}
+
+    def set_drift_recompute(self, drift_tol: float | None) -> None:
+        """After each later run, instances whose drift exceeds drift_tol get their energy and gradient
+        from the single-state complex128 adjoint instead (None turns it off)."""
+        check(load().kq_batch_set_drift_recompute(self._h, -1.0 if drift_tol is None else float(drift_tol)))
+
+    def recomputed(self) -> tuple[int, int]:
+        """(instances recomputed by the last run, in total)."""
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        check(load().kq_batch_recomputed(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def info(self) -> dict:
         a, b, c, d = (ctypes.c_size_t() for _ in range(4))
         check(load().kq_batch_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), ctypes.byref(d)))

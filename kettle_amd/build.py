@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 KQ_SOURCES = ["kq_kernels.hip", "kq_tile.hip", "kq_plan.cpp", "kq_api.cpp", "kq_run.cpp", "kq_measure.cpp", "kq_pauli.cpp",
-              "kq_adjoint.cpp", "kq_debug.cpp", "kq_batch.cpp", "kq_jit.cpp"]
+              "kq_adjoint.cpp", "kq_debug.cpp", "kq_batch.cpp", "kq_jit.cpp", "kq_dense.hip"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -292,7 +292,8 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         }
         s->bit_of = identity ? std::vector<uint8_t> {} : bit_of;
     }
-    launch_pauli_apply_sum(s, s->amps, lam, groups, strings);
+    profiled(s, 4, 2.0 * static_cast<double>(amp_bytes(s->precision) * s->n_amps), strings.size(),
+             [&] { launch_pauli_apply_sum(s, s->amps, lam, groups, strings); });
     launch_dot_to_device(s, s->amps, lam, d_red, d_energy);
     launch_copy(s, psi, s->amps, bytes);   // the reverse sweep runs on a copy: the state keeps psi(theta)
 
@@ -389,7 +390,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         for (const ReversePass& rp : rev[j]) {
             const TileProgram& tp = rp.tp;
             if (rp.fn) {
-                profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(), [&] {
+                profiled(s, 3, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(), [&] {
                     launch_jit_adjoint_pass(s, rp.fn, tp, psi, lam, reinterpret_cast<const double*>(d_up + u_blob + rp.blob_off),
                                             d_part + rp.part_off);
                 });
@@ -400,7 +401,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
             const unsigned char* ly = sp + align256b(tp.steps.size() * sizeof(TileStep));
             const unsigned char* dv = ly + align256b(tp.layouts.size() * sizeof(TileLayout));
             const unsigned char* tm = dv + align256b(std::max<size_t>(tp.derivs.size(), 1) * sizeof(TileGate));
-            profiled(s, 1, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
+            profiled(s, 3, 4.0 * ab * static_cast<double>(s->n_amps), tp.gates.size(),
                      [&] { launch_adjoint_tile_pass(s, psi, lam, tp, g, sp, ly, dv, tm, d_part + rp.part_off); });
         }
         // psi is now the forward state at the segment's start: the checkpoint taken there

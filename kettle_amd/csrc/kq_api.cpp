@@ -143,6 +143,9 @@ void profiler_collect()
         float ms = 0.0f;
         KQ_HIP_CHECK(hipEventSynchronize(e.b));
         KQ_HIP_CHECK(hipEventElapsedTime(&ms, e.a, e.b));
+        kq_pass_stats& c = p.cls[e.cls];
+        c.min_ms = c.launches == 0 ? ms : std::min(c.min_ms, static_cast<double>(ms));
+        c.max_ms = std::max(c.max_ms, static_cast<double>(ms));
         p.cls[e.cls].ms += ms;
         p.cls[e.cls].launches += 1;
         p.cls[e.cls].bytes += e.bytes;
@@ -353,13 +356,13 @@ int kq_profile_begin(void)
     });
 }
 
-int kq_profile_end(kq_pass_stats out[3])
+int kq_profile_end(kq_pass_stats out[KQ_PASS_CLASSES])
 {
     return guarded([&] {
         PassProfiler& p = profiler();
         profiler_collect();
         if (out) {
-            for (int i = 0; i < 3; ++i) {
+            for (int i = 0; i < KQ_PASS_CLASSES; ++i) {
                 out[i] = p.cls[i];
             }
         }
@@ -372,6 +375,24 @@ int kq_stream_probe(uint32_t n_qubits, double gbps[4])
     return guarded([&] {
         if (n_qubits < 16 || n_qubits > 32 || !gbps) {
             throw InvalidArg {"stream probe: 16 <= n_qubits <= 32 and a 4-value output"};
+        }
+        ensure_init();
+        double all[24];
+        stream_probe(n_qubits, all);
+        for (int m = 0; m < 4; ++m) {
+            gbps[m] = 0.0;
+            for (int sh = 0; sh < 6; ++sh) {
+                gbps[m] = std::max(gbps[m], all[4 * sh + m]);
+            }
+        }
+    });
+}
+
+int kq_stream_probe_shapes(uint32_t n_qubits, double gbps[24])
+{
+    return guarded([&] {
+        if (n_qubits < 16 || n_qubits > 32 || !gbps) {
+            throw InvalidArg {"stream probe: 16 <= n_qubits <= 32 and a 24-value output"};
         }
         ensure_init();
         stream_probe(n_qubits, gbps);
@@ -399,6 +420,15 @@ int kq_set_jit_priority(int mode)
     return guarded([&] {
         JitTuning t = jit_tuning();
         t.priority = mode;
+        set_jit_tuning(t);
+    });
+}
+
+int kq_set_jit_nontemporal(int mode)
+{
+    return guarded([&] {
+        JitTuning t = jit_tuning();
+        t.nontemporal = mode;
         set_jit_tuning(t);
     });
 }
@@ -489,6 +519,36 @@ int kq_batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params_total
             throw InvalidArg {"batch: null handle"};
         }
         batch_info(b, n_instances, n_params_total, device_bytes, n_launches);
+    });
+}
+
+int kq_batch_plan_info(const kq_batch* b, size_t* fwd_passes, size_t* rev_passes, double* algorithmic_bytes)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_plan_info(b, fwd_passes, rev_passes, algorithmic_bytes);
+    });
+}
+
+int kq_batch_set_drift_recompute(kq_batch* b, double drift_tol)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_set_drift_recompute(b, drift_tol);
+    });
+}
+
+int kq_batch_recomputed(const kq_batch* b, size_t* last_run, size_t* total)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_recomputed(b, last_run, total);
     });
 }
 

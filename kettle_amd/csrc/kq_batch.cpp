@@ -13,12 +13,16 @@
 //   distance of every uncomputed state from |0...0> (inverse-gate drift), in one launch
 // The sequence contains no allocation, no host synchronisation and no memset, so after a first
 // eager run it is captured in a hipGraph and replayed by later calls.
+// With a drift tolerance set, every instance whose drift exceeds it after the run is evaluated again
+// by the single-state checkpointed adjoint in complex128 (on a work state per qubit count, kept by
+// the batch), and that energy and gradient replace the batched ones.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "kq_host.hpp"
 #include "kq_internal.hpp"
 
 namespace kq
@@ -45,6 +49,10 @@ struct BatchInstance
     uint64_t n_amps = 0;
     size_t grad_off = 0;               // first gradient entry of the instance in the output
     size_t pauli_off = 0;              // byte offset of the groups (strings follow) in the Pauli arena
+    // the operator as given, for a single-state recompute
+    std::vector<uint32_t> h_off, h_q;
+    std::vector<uint8_t> h_k, h_ph;
+    std::vector<double> h_c;
 };
 
 struct BatchRound
@@ -83,6 +91,8 @@ struct kq_batch
     size_t prog_bytes = 0;
     size_t table_bytes = 0;
     size_t pauli_bytes = 0;
+    size_t fwd_passes = 0, rev_passes = 0;
+    double algorithmic_bytes = 0.0;     // per run: one read and/or write of every amplitude each launch updates
     // device memory
     void* psi_fwd = nullptr;            // forward states when the forward precision differs
     void* psi = nullptr;
@@ -100,6 +110,11 @@ struct kq_batch
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     size_t n_launches = 0;
+    // drift-triggered recompute (disabled while negative)
+    double recompute_tol = -1.0;
+    size_t last_recomputed = 0;
+    size_t total_recomputed = 0;
+    std::vector<std::pair<uint32_t, kq_state*>> work;   // complex128 work state per qubit count
 };
 
 namespace kq
@@ -281,6 +296,21 @@ void build(kq_batch* b)
         grad += in.n_params;
     }
     b->total_amps = amps;
+    {
+        const double af = static_cast<double>(amp_bytes(b->fwd_precision));
+        const double ar = static_cast<double>(ab);
+        b->fwd_passes = b->rev_passes = 0;
+        b->algorithmic_bytes = 0.0;
+        for (const BatchInstance& in : b->inst) {
+            const double A = static_cast<double>(in.n_amps);
+            b->fwd_passes += in.fwd.size();
+            b->rev_passes += in.bwd.size();
+            // reset, forward passes, lambda = H psi, narrowing, reverse passes (two vectors), drift read
+            b->algorithmic_bytes += A * (af + 2.0 * af * static_cast<double>(in.fwd.size()) + af + ar
+                                         + (b->fwd_precision != b->precision ? af + ar : 0.0)
+                                         + 4.0 * ar * static_cast<double>(in.bwd.size()) + ar);
+        }
+    }
     b->prog_bytes = std::max<size_t>(prog, 256);
     b->pauli_bytes = std::max<size_t>(pauli, 256);
     // rounds, per tile size (which fixes the workgroup size)
@@ -551,6 +581,13 @@ void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, s
         }
     }
     build_pauli(n_qubits, n_strings, off, q, k, phases, coeffs, in.groups, in.strings);
+    if (n_strings > 0) {   // build_pauli has checked the arrays
+        in.h_off.assign(off, off + n_strings + 1);
+        in.h_q.assign(q, q + off[n_strings]);
+        in.h_k.assign(k, k + off[n_strings]);
+        in.h_ph.assign(phases, phases + n_strings);
+        in.h_c.assign(coeffs, coeffs + 2 * n_strings);
+    }
     b->inst.push_back(std::move(in));
     b->planned = false;
     if (index) {
@@ -569,6 +606,42 @@ void batch_set_parameters(kq_batch* b, size_t index, const double* values)
     }
     in.params.assign(values, values + in.n_params);
     b->coeffs_dirty = true;
+}
+
+// energy and gradient of instance i by the single-state checkpointed adjoint in complex128, with a
+// restore at every checkpoint (drift tolerance 0)
+void recompute_instance(kq_batch* b, size_t i, double* energy, double* grad)
+{
+    const BatchInstance& in = b->inst[i];
+    kq_state* st = nullptr;
+    for (auto& w : b->work) {
+        if (w.first == in.n) {
+            st = w.second;
+        }
+    }
+    if (st == nullptr) {
+        st = create_state(in.n, KQ_C128, false);
+        b->work.emplace_back(in.n, st);
+    }
+    st->bit_of.clear();   // the adjoint runs from |0...0> with qubit q on bit q
+    launch_set_basis(st, 0);
+    std::vector<kq_op> ops = in.ops;
+    for (kq_op& op : ops) {
+        if (op.aux >= 0) {
+            op.angle = in.params[static_cast<size_t>(op.aux)];
+        }
+    }
+    const size_t n_strings = in.h_ph.size();
+    double e = 0.0;
+    std::vector<double> g(std::max<size_t>(in.n_params, 1), 0.0);
+    energy_gradient(st, ops.data(), ops.size(), n_strings, in.h_off.data(), in.h_q.data(), in.h_k.data(), in.h_ph.data(),
+                    in.h_c.data(), in.n_params, &e, g.data(), 0, 0.0, nullptr);
+    if (energy) {
+        *energy = e;
+    }
+    if (grad) {
+        std::copy(g.begin(), g.begin() + static_cast<std::ptrdiff_t>(in.n_params), grad);
+    }
 }
 
 void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, double* drift)
@@ -632,6 +705,50 @@ void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, dou
             grads[in.grad_off + static_cast<size_t>(op.aux)] += 2.0 * sv[r];
         }
     }
+    b->last_recomputed = 0;
+    if (b->recompute_tol >= 0.0) {
+        for (size_t i = 0; i < n; ++i) {
+            if (!(dr[i] <= b->recompute_tol)) {   // NaN drift is recomputed too
+                recompute_instance(b, i, energies ? energies + i : nullptr, grads ? grads + b->inst[i].grad_off : nullptr);
+                ++b->last_recomputed;
+            }
+        }
+        b->total_recomputed += b->last_recomputed;
+    }
+}
+
+void batch_plan_info(const kq_batch* b, size_t* fwd_passes, size_t* rev_passes, double* algorithmic_bytes)
+{
+    if (!b->planned) {
+        throw InvalidArg {"batch: not planned yet (run it once)"};
+    }
+    if (fwd_passes) {
+        *fwd_passes = b->fwd_passes;
+    }
+    if (rev_passes) {
+        *rev_passes = b->rev_passes;
+    }
+    if (algorithmic_bytes) {
+        *algorithmic_bytes = b->algorithmic_bytes;
+    }
+}
+
+void batch_set_drift_recompute(kq_batch* b, double drift_tol)
+{
+    if (std::isnan(drift_tol)) {
+        throw InvalidArg {"batch: drift tolerance is NaN"};
+    }
+    b->recompute_tol = drift_tol;
+}
+
+void batch_recomputed(const kq_batch* b, size_t* last_run, size_t* total)
+{
+    if (last_run) {
+        *last_run = b->last_recomputed;
+    }
+    if (total) {
+        *total = b->total_recomputed;
+    }
 }
 
 kq_batch* batch_create(int precision, int forward_precision, hipStream_t stream)
@@ -652,6 +769,10 @@ void batch_destroy(kq_batch* b)
         (void) hipStreamSynchronize(b->stream);
     }
     release(b);
+    for (auto& w : b->work) {
+        (void) kq_state_destroy(w.second);
+    }
+    b->work.clear();
     if (b->stream) {
         (void) hipStreamDestroy(b->stream);
     }

@@ -470,3 +470,97 @@ int kq_debug_plan(uint32_t n_qubits, int precision, int is_density, const kq_op*
 }
 
 }  // extern "C"
+
+int kq_debug_dense_block(kq_state* s, const double* u, int use_mfma, double* ms)
+{
+    return guarded([&] {
+        check_state(s);
+        if (!u) {
+            throw InvalidArg {"dense block: null matrix"};
+        }
+        if (s->is_dm) {
+            throw InvalidArg {"dense block: statevectors only"};
+        }
+        ensure_init();
+        normalise_layout(s);
+        // U in the amplitude precision, in the state's scratch (the reductions' scratch is not in use here)
+        const size_t bytes = 512 * amp_bytes(s->precision) / 2;
+        void* d_u = scratch(s, bytes);
+        if (s->precision == KQ_C128) {
+            KQ_HIP_CHECK(hipMemcpyAsync(d_u, u, bytes, hipMemcpyHostToDevice, s->stream));
+        }
+        else {
+            std::vector<float> f(u, u + 512);
+            KQ_HIP_CHECK(hipMemcpyAsync(d_u, f.data(), bytes, hipMemcpyHostToDevice, s->stream));
+            KQ_HIP_CHECK(hipStreamSynchronize(s->stream));   // f leaves scope
+        }
+        hipEvent_t a, b;
+        KQ_HIP_CHECK(hipEventCreate(&a));
+        KQ_HIP_CHECK(hipEventCreate(&b));
+        try {
+            KQ_HIP_CHECK(hipEventRecord(a, s->stream));
+            launch_dense4(s, d_u, use_mfma != 0);
+            KQ_HIP_CHECK(hipEventRecord(b, s->stream));
+            KQ_HIP_CHECK(hipEventSynchronize(b));
+            float t = 0.0f;
+            KQ_HIP_CHECK(hipEventElapsedTime(&t, a, b));
+            if (ms) {
+                *ms = t;
+            }
+        }
+        catch (...) {
+            (void) hipEventDestroy(a);
+            (void) hipEventDestroy(b);
+            throw;
+        }
+        (void) hipEventDestroy(a);
+        (void) hipEventDestroy(b);
+    });
+}
+
+int kq_debug_mfma16_layout(int precision, const double* a, const double* b, double* d)
+{
+    return guarded([&] {
+        if (!a || !b || !d || (precision != KQ_C128 && precision != KQ_C64)) {
+            throw InvalidArg {"mfma layout: bad arguments"};
+        }
+        ensure_init();
+        const size_t es = precision == KQ_C128 ? 8 : 4;
+        void* buf = nullptr;
+        KQ_HIP_CHECK(hipMalloc(&buf, 384 * es));
+        try {
+            std::vector<unsigned char> h(384 * es);
+            for (int i = 0; i < 64; ++i) {
+                if (es == 8) {
+                    std::memcpy(h.data() + i * 8, a + i, 8);
+                    std::memcpy(h.data() + (64 + i) * 8, b + i, 8);
+                }
+                else {
+                    const float fa = static_cast<float>(a[i]), fb = static_cast<float>(b[i]);
+                    std::memcpy(h.data() + i * 4, &fa, 4);
+                    std::memcpy(h.data() + (64 + i) * 4, &fb, 4);
+                }
+            }
+            KQ_HIP_CHECK(hipMemcpy(buf, h.data(), 128 * es, hipMemcpyHostToDevice));
+            auto* base = static_cast<unsigned char*>(buf);
+            launch_mfma16_layout(nullptr, precision, base, base + 64 * es, base + 128 * es);
+            KQ_HIP_CHECK(hipDeviceSynchronize());
+            KQ_HIP_CHECK(hipMemcpy(h.data(), base + 128 * es, 256 * es, hipMemcpyDeviceToHost));
+            for (int i = 0; i < 256; ++i) {
+                if (es == 8) {
+                    std::memcpy(d + i, h.data() + i * 8, 8);
+                }
+                else {
+                    float f;
+                    std::memcpy(&f, h.data() + i * 4, 4);
+                    d[i] = f;
+                }
+            }
+        }
+        catch (...) {
+            (void) hipFree(buf);
+            throw;
+        }
+        (void) hipFree(buf);
+    });
+}

@@ -246,7 +246,7 @@ void* scratch(kq_state* s, size_t bytes);
 struct PassProfiler
 {
     bool enabled = false;
-    kq_pass_stats cls[3] {};
+    kq_pass_stats cls[KQ_PASS_CLASSES] {};
     // events recorded but not yet read: (class, start, stop, bytes, gates)
     struct Pending { int cls; hipEvent_t a; hipEvent_t b; double bytes; uint64_t gates; };
     std::vector<Pending> pending;
@@ -426,7 +426,11 @@ void launch_adjoint_gate(kq_state* s, void* psi, void* lam, const GateRec& inv, 
 void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes);
 void launch_copy_kernel(kq_state* s, void* dst, const void* src, uint64_t n_amps);
 // GB/s of plain 16-byte streams over 2^n_qubits elements: read, write, in place, copy
-void stream_probe(uint32_t n_qubits, double gbps[4]);
+void stream_probe(uint32_t n_qubits, double gbps[24]);   // [launch shape][read, write, in place, copy]
+// psi[r, :] <- U psi[r, :] over index bits 0..3 (kq_dense.hip); d_u: 16 x 16 complex in the state's precision
+void launch_dense4(kq_state* s, const void* d_u, bool use_mfma);
+// one wave, one 16x16x4 matrix instruction (FP64 for KQ_C128, FP32 otherwise): d[4 l + e] = element e of lane l
+void launch_mfma16_layout(hipStream_t st, int precision, const void* a, const void* b, void* d);   // [launch shape][read, write, in place, copy]
 
 // Tables of the one-pass index-bit permutation (k_permute_bits): tile local bit j is source index bit
 // src_of[j] (a single-bit mask) and destination bit dst_of[j]; block bit i is source bit free_src[i]
@@ -477,6 +481,7 @@ struct JitTuning
     int blocks_per_cu = 0;
     bool normalise = true;   // uncontrolled one-qubit gates with their pivot divided out
     int priority = 1;        // wave priority around HBM traffic: 0 off, 1 loads, 2 loads and stores
+    int nontemporal = 0;     // nontemporal hint on HBM traffic: 0 off, 1 loads, 2 loads and stores
 };
 void set_jit_tuning(const JitTuning& t);
 JitTuning jit_tuning();
@@ -511,5 +516,8 @@ void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, s
 void batch_set_parameters(kq_batch* b, size_t index, const double* values);
 void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, double* drift);
 void batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params, size_t* device_bytes, size_t* n_launches);
+void batch_set_drift_recompute(kq_batch* b, double drift_tol);
+void batch_plan_info(const kq_batch* b, size_t* fwd_passes, size_t* rev_passes, double* algorithmic_bytes);
+void batch_recomputed(const kq_batch* b, size_t* last_run, size_t* total);
 
 }  // namespace kq

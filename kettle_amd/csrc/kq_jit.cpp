@@ -715,9 +715,9 @@ int jit_mode() { return cache().mode; }
 void set_jit_tuning(const JitTuning& t)
 {
     if (t.min_waves_per_simd < 0 || t.min_waves_per_simd > 8 || t.blocks_per_cu < 0 || t.blocks_per_cu > 64
-        || t.priority < 0 || t.priority > 2) {
-        throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8], blocks per CU in [0, 64] and priority "
-                          "in [0, 2]"};
+        || t.priority < 0 || t.priority > 2 || t.nontemporal < 0 || t.nontemporal > 2) {
+        throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8], blocks per CU in [0, 64], priority "
+                          "and nontemporal mode in [0, 2]"};
     }
     std::lock_guard<std::mutex> lk(cache().mu);
     cache().tuning = t;
@@ -826,13 +826,16 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
             pending.clear();
         }
         o << ";\n";
+        const char* nt_ld = tune.nontemporal > 0 ? "true" : "false";
+        const char* nt_st = tune.nontemporal > 1 ? "true" : "false";
         for (int r = 0; r < 16; ++r) {
             if (load) {
-                o << "    { const Amp<R> x = a[off ^ " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n";
+                o << "    { const Amp<R> x = amp_load<" << nt_ld << ">(a + (off ^ " << hex64(L.hbm_slot[r]) << ")); v[" << r
+                  << "] = {x.x, x.y}; }\n";
             }
             else {
-                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off ^ " << hex64(L.hbm_slot[r])
-                  << "] = x; }\n";
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; amp_store<" << nt_st << ">(a + (off ^ "
+                  << hex64(L.hbm_slot[r]) << "), x); }\n";
             }
         }
         o << "  }\n";
@@ -1113,6 +1116,7 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
 // Remaps move psi, then lambda, through one LDS tile of whole amplitudes.
 std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
 {
+    const JitTuning tune = jit_tuning();
     coefs.clear();
     const bool c64 = precision == KQ_C64;
     const int nt = 1 << tp.n_thr_bits;
@@ -1146,14 +1150,17 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
       << "  const uint64_t col = uint64_t {blockIdx.x} * " << waves << "u + (tid >> 6);\n";
     const auto io = [&](const TileLayout& L, bool load) {
         o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
+        const char* nt_ld = tune.nontemporal > 0 ? "true" : "false";
+        const char* nt_st = tune.nontemporal > 1 ? "true" : "false";
         for (int r = 0; r < 16; ++r) {
+            const std::string at = "(off ^ " + hex64(L.hbm_slot[r]) + ")";
             if (load) {
-                o << "    { const Amp<R> x = a[off ^ " << hex64(L.hbm_slot[r]) << "]; v[" << r << "] = {x.x, x.y}; }\n"
-                  << "    { const Amp<R> y = b[off ^ " << hex64(L.hbm_slot[r]) << "]; w[" << r << "] = {y.x, y.y}; }\n";
+                o << "    { const Amp<R> x = amp_load<" << nt_ld << ">(a + " << at << "); v[" << r << "] = {x.x, x.y}; }\n"
+                  << "    { const Amp<R> y = amp_load<" << nt_ld << ">(b + " << at << "); w[" << r << "] = {y.x, y.y}; }\n";
             }
             else {
-                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; a[off ^ " << hex64(L.hbm_slot[r]) << "] = x; }\n"
-                  << "    { Amp<R> y; y.x = w[" << r << "].re; y.y = w[" << r << "].im; b[off ^ " << hex64(L.hbm_slot[r]) << "] = y; }\n";
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; amp_store<" << nt_st << ">(a + " << at << ", x); }\n"
+                  << "    { Amp<R> y; y.x = w[" << r << "].re; y.y = w[" << r << "].im; amp_store<" << nt_st << ">(b + " << at << ", y); }\n";
             }
         }
         o << "  }\n";

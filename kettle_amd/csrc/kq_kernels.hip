@@ -1579,10 +1579,13 @@ void launch_copy(kq_state* s, void* dst, const void* src, size_t bytes)
 }
 
 // ---- HBM stream probe ---------------------------------------------------------------------------------
-// What this device's HBM delivers to plain streams of 16-byte elements (16 per thread in flight,
-// a few blocks per CU): read only, write only, read-modify-write in place, copy.  The reference
-// point for the tile passes, which read and write every amplitude once.
-template <int MODE>
+// What this device's HBM delivers to plain streams of 16-byte elements, 16 per thread in flight:
+// read only, write only, read-modify-write in place, copy; each in six launch shapes (a persistent
+// grid of 2 or 8 blocks per CU; one block per 4096-element chunk like a tile pass, with plain or
+// nontemporal loads and stores, and each of those with the wave priority raised while the loads
+// issue, as the tile passes do).  The reference point for the tile passes, which read and write
+// every amplitude once.
+template <int MODE, bool NT, bool PRIO>
 __global__ __launch_bounds__(BLOCK) void k_stream_probe(const double2* __restrict__ src, double2* __restrict__ dst,
                                                         uint64_t n, double* __restrict__ sink)
 {
@@ -1591,14 +1594,25 @@ __global__ __launch_bounds__(BLOCK) void k_stream_probe(const double2* __restric
     double acc = 0.0;
     for (uint64_t c = blockIdx.x * chunk; c + chunk <= n; c += uint64_t {gridDim.x} * chunk) {
         double2 v[U];
+        if constexpr (PRIO) {
+            __builtin_amdgcn_s_setprio(3);
+        }
 #pragma unroll
         for (int j = 0; j < U; ++j) {
+            const uint64_t i = c + uint64_t {BLOCK} * j + threadIdx.x;
             if constexpr (MODE == 1) {
                 v[j] = make_double2(static_cast<double>(j), 1.0);
             }
-            else {
-                v[j] = src[c + uint64_t {BLOCK} * j + threadIdx.x];
+            else if constexpr (NT) {
+                v[j].x = __builtin_nontemporal_load(&src[i].x);
+                v[j].y = __builtin_nontemporal_load(&src[i].y);
             }
+            else {
+                v[j] = src[i];
+            }
+        }
+        if constexpr (PRIO) {
+            __builtin_amdgcn_s_setprio(0);
         }
         if constexpr (MODE == 0) {
 #pragma unroll
@@ -1609,7 +1623,14 @@ __global__ __launch_bounds__(BLOCK) void k_stream_probe(const double2* __restric
         else {
 #pragma unroll
             for (int j = 0; j < U; ++j) {
-                dst[c + uint64_t {BLOCK} * j + threadIdx.x] = v[j];
+                const uint64_t i = c + uint64_t {BLOCK} * j + threadIdx.x;
+                if constexpr (NT) {
+                    __builtin_nontemporal_store(v[j].x, &dst[i].x);
+                    __builtin_nontemporal_store(v[j].y, &dst[i].y);
+                }
+                else {
+                    dst[i] = v[j];
+                }
             }
         }
     }
@@ -1618,7 +1639,7 @@ __global__ __launch_bounds__(BLOCK) void k_stream_probe(const double2* __restric
     }
 }
 
-void stream_probe(uint32_t n_qubits, double gbps[4])
+void stream_probe(uint32_t n_qubits, double gbps[24])
 {
     const uint64_t n = uint64_t {1} << n_qubits;
     double2* a = nullptr;
@@ -1639,7 +1660,10 @@ void stream_probe(uint32_t n_qubits, double gbps[4])
         KQ_HIP_CHECK(hipGetDevice(&dev));
         hipDeviceProp_t prop;
         KQ_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-        const unsigned grid = static_cast<unsigned>(prop.multiProcessorCount) * 2;
+        const auto cus = static_cast<unsigned>(prop.multiProcessorCount);
+        // n >= 2^16, so every shape has at least 16 blocks and the chunk loop covers all n elements
+        const auto chunks = static_cast<unsigned>(n / (uint64_t {BLOCK} * 16));
+        const unsigned grids[6] = {2 * cus, 8 * cus, chunks, chunks, chunks, chunks};
         hipEvent_t e0, e1;
         KQ_HIP_CHECK(hipEventCreate(&e0));
         KQ_HIP_CHECK(hipEventCreate(&e1));
@@ -1657,11 +1681,19 @@ void stream_probe(uint32_t n_qubits, double gbps[4])
             KQ_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return bytes / (static_cast<double>(ms) / reps * 1.0e-3) / 1.0e9;
         };
-        gbps[0] = time([&] { k_stream_probe<0><<<grid, BLOCK>>>(a, b, n, sink); }, one);
-        gbps[1] = time([&] { k_stream_probe<1><<<grid, BLOCK>>>(a, b, n, sink); }, one);
-        gbps[2] = time([&] { k_stream_probe<2><<<grid, BLOCK>>>(a, a, n, sink); }, 2 * one);
-        gbps[3] = time([&] { k_stream_probe<3><<<grid, BLOCK>>>(a, b, n, sink); }, 2 * one);
-        KQ_HIP_CHECK(hipGetLastError());
+        const auto shape = [&]<bool NT, bool PRIO>(unsigned grid, double* g) {
+            g[0] = time([&] { k_stream_probe<0, NT, PRIO><<<grid, BLOCK>>>(a, b, n, sink); }, one);
+            g[1] = time([&] { k_stream_probe<1, NT, PRIO><<<grid, BLOCK>>>(a, b, n, sink); }, one);
+            g[2] = time([&] { k_stream_probe<2, NT, PRIO><<<grid, BLOCK>>>(a, a, n, sink); }, 2 * one);
+            g[3] = time([&] { k_stream_probe<3, NT, PRIO><<<grid, BLOCK>>>(a, b, n, sink); }, 2 * one);
+            KQ_HIP_CHECK(hipGetLastError());
+        };
+        shape.template operator()<false, false>(grids[0], gbps + 0);
+        shape.template operator()<false, false>(grids[1], gbps + 4);
+        shape.template operator()<false, false>(grids[2], gbps + 8);
+        shape.template operator()<true, false>(grids[3], gbps + 12);
+        shape.template operator()<false, true>(grids[4], gbps + 16);
+        shape.template operator()<true, true>(grids[5], gbps + 20);
         (void) hipEventDestroy(e0);
         (void) hipEventDestroy(e1);
     }

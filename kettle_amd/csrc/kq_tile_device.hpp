@@ -53,6 +53,33 @@ __device__ __forceinline__ void wave_priority()
     __builtin_amdgcn_s_setprio(PRIO);
 }
 
+// HBM access of one amplitude, with the nontemporal hint (streamed once per pass, not worth keeping in
+// the caches) when NT is set; the two halves merge into one 8- or 16-byte access
+template <bool NT, typename A>
+__device__ __forceinline__ A amp_load(const A* p)
+{
+    if constexpr (NT) {
+        A x;
+        x.x = __builtin_nontemporal_load(&p->x);
+        x.y = __builtin_nontemporal_load(&p->y);
+        return x;
+    }
+    else {
+        return *p;
+    }
+}
+template <bool NT, typename A>
+__device__ __forceinline__ void amp_store(A* p, const A& x)
+{
+    if constexpr (NT) {
+        __builtin_nontemporal_store(x.x, &p->x);
+        __builtin_nontemporal_store(x.y, &p->y);
+    }
+    else {
+        *p = x;
+    }
+}
+
 // the eight coefficients of a gate record, already in registers
 template <typename R>
 struct Coef

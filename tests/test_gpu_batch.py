@@ -119,6 +119,39 @@ def test_graph_replay_and_parameter_updates():
     b.close()
 
 
+def test_drift_triggered_recompute():
+    """drift_tol = 0 forces the complex128 single-state recompute of every complex64 instance that
+    drifted at all: its energy and gradient then match the complex128 path to 1e-12 / 1e-10; a loose
+    tolerance recomputes nothing and leaves the batched results as they were."""
+    insts = small_instances(16, 21)
+    b = make_batch(insts, ka.KQ_C64)
+    plain = b.run(use_graph=False)
+    assert b.recomputed() == (0, 0)
+    b.set_drift_recompute(0.0)
+    for use_graph in (False, True):
+        r = b.run(use_graph=use_graph)
+        drifted = np.nonzero(r.drift > 0.0)[0]
+        assert len(drifted) >= len(insts) // 2
+        assert b.recomputed()[0] == len(drifted)
+        for i in drifted:
+            one = single(insts[i], ka.KQ_C128)
+            assert abs(r.energies[i] - one.energy) <= 1e-12, i
+            assert np.max(np.abs(r.gradients[i] - one.gradient), initial=0.0) <= 1e-10, i
+        for i in np.nonzero(r.drift == 0.0)[0]:
+            assert r.energies[i] == plain.energies[i], i
+    assert b.recomputed()[1] == 2 * len(drifted)
+    b.set_drift_recompute(1.0)
+    r = b.run()
+    assert b.recomputed()[0] == 0
+    assert np.array_equal(r.energies, plain.energies)
+    b.set_drift_recompute(None)
+    b.run()
+    assert b.recomputed()[0] == 0
+    with pytest.raises(ka.KqError):
+        b.set_drift_recompute(float("nan"))
+    b.close()
+
+
 def test_batch_errors():
     b = EnergyGradientBatch(ka.KQ_C64)
     with pytest.raises(ka.KqError, match="no instances"):
