@@ -421,8 +421,9 @@ def main() -> int:
                          "1 loads, 2 loads and stores")
     ap.add_argument("--only", choices=["c64", "plain", "dm", "vqe", "batch", "dense"], default=None,
                     help="run one side section alone and print its JSON (profiling runs)")
-    ap.add_argument("--jit-nontemporal", type=int, default=0,
-                    help="nontemporal hint on the run-time compiled passes' HBM traffic: 0 off, 1 loads, 2 loads and stores")
+    ap.add_argument("--jit-nontemporal", type=int, default=3,
+                    help="nontemporal hint on the run-time compiled passes' HBM traffic: 0 off, 1 loads, 2 loads and "
+                         "stores, 3 loads and stores of vectors above 256 MiB")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="only run N plain copies of a --qubits complex128 state (HBM counter calibration)")
     args = ap.parse_args()

@@ -172,8 +172,9 @@ int kq_set_jit_tuning(int min_waves_per_simd, int blocks_per_cu, int normalise);
  * issues its tile's HBM loads, 2 also while it issues its stores, so that waves starting or finishing
  * a tile go ahead of the waves computing one.  No counterpart in the reference. */
 int kq_set_jit_priority(int mode);
-/* Nontemporal hint on the run-time pass kernels' HBM traffic: 0 (default) off, 1 on the tile loads,
- * 2 on the loads and the stores.  No counterpart in the reference. */
+/* Nontemporal hint on the run-time pass kernels' HBM traffic: 0 off, 1 on the tile loads, 2 on the
+ * loads and the stores, 3 (default) as 2 for vectors larger than 256 MiB (the Infinity Cache) and as
+ * 0 for smaller ones.  No counterpart in the reference. */
 int kq_set_jit_nontemporal(int mode);
 /* Counters of the run-time compiled kernels; uncapped counts kernels compiled again without the
  * register cap because they spilled under it. */
@@ -359,10 +360,12 @@ int kq_debug_check_relabel(uint32_t n_qubits, const kq_op* ops, size_t n_ops, do
  * gate list; fails with the compiler log if any does not compile.  No GPU needed. */
 int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
                          size_t* n_kernels, size_t* failures);
-/* Host-only: pass pass_index of a straight-line gate list as a statevector runs it (qubit moves and
- * restore passes included).  info[0] = 1 for a sweep of one gate (coefs: its 8 matrix values,
- * info[3] = target bit, info[4] = control bit + 1 or 0), else 0 for a tile pass with its generated
- * kernel source, info[1] = workgroups, info[2] = threads per workgroup, coefs: its coefficient buffer. */
+/* Host-only: pass pass_index of a straight-line gate list as a statevector runs it (qubit moves, the
+ * last pass's return of every qubit to its own bit and any restore passes included).  info[0] = 1 for
+ * a sweep of one gate (coefs: its 8 matrix values, info[3] = target bit, info[4] = control bit + 1 or
+ * 0), else 0 for a tile pass with its generated kernel source, or 2 for one that stores out of place
+ * (its kernel takes a fourth argument, the destination buffer, which then holds the state);
+ * info[1] = workgroups, info[2] = threads per workgroup, coefs: its coefficient buffer. */
 int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes, size_t pass_index,
                   char* src, size_t src_cap, size_t* src_len, double* coefs, size_t coef_cap, size_t* n_coefs,
                   uint64_t info[5], size_t* n_passes);

@@ -270,7 +270,8 @@ def set_jit_priority(mode: int) -> None:
 
 
 def set_jit_nontemporal(mode: int) -> None:
-    """Nontemporal hint on the run-time pass kernels' HBM traffic: 0 off, 1 loads, 2 loads and stores."""
+    """Nontemporal hint on the run-time pass kernels' HBM traffic: 0 off, 1 loads, 2 loads and stores,
+    3 (default) loads and stores of vectors larger than the 256 MiB Infinity Cache."""
     check(load().kq_set_jit_nontemporal(int(mode)))
 
 

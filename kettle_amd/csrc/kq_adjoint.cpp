@@ -112,6 +112,46 @@ struct ReversePass
     std::vector<double> coefs;
 };
 
+// Compiled reverse-pass kernels by pass structure (adjoint_structure_key): an optimiser's next call on the
+// same circuit structure reuses them without generating their sources again.
+class ReverseKernels
+{
+public:
+    const hipFunction_t* find(const std::vector<uint64_t>& key)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        const auto it = map_.find(hash(key));
+        return (it != map_.end() && it->second.first == key) ? &it->second.second : nullptr;
+    }
+    void add(std::vector<uint64_t>&& key, hipFunction_t fn)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (map_.size() >= 4096) {
+            map_.clear();
+        }
+        const uint64_t h = hash(key);
+        map_[h] = {std::move(key), fn};
+    }
+
+private:
+    static uint64_t hash(const std::vector<uint64_t>& key)
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (const uint64_t v : key) {
+            h = (h ^ v) * 1099511628211ull;
+        }
+        return h;
+    }
+    std::mutex mu_;
+    std::unordered_map<uint64_t, std::pair<std::vector<uint64_t>, hipFunction_t>> map_;
+};
+
+ReverseKernels& reverse_kernels()
+{
+    static ReverseKernels c;
+    return c;
+}
+
 // Gate-by-gate adjoint for states too small for a tile (fewer qubits than one thread's register block).
 void energy_gradient_unfused(kq_state* s, const kq_op* ops, const std::vector<GateRec>& recs,
                              const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings, size_t n_params,
@@ -328,19 +368,32 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     }
     const size_t R = row_gate.size();
     // reverse passes as run-time compiled kernels where the state is large enough for them; the
-    // programs of the others (and of any that fail to compile) go to the interpreted tile kernel
+    // programs of the others (and of any that fail to compile) go to the interpreted tile kernel.  A pass
+    // whose structure was seen before takes its kernel from the cache and only its coefficients anew.
     if (jit_wanted(s) && jit_available()) {
         std::vector<std::string> srcs;
         std::vector<ReversePass*> who;
+        std::vector<std::vector<uint64_t>> keys;
+        ReverseKernels& cache = reverse_kernels();
         for (auto& seg : rev) {
             for (ReversePass& rp : seg) {
+                std::vector<uint64_t> key = adjoint_structure_key(rp.tp, s->precision);
+                if (const hipFunction_t* fn = cache.find(key)) {
+                    rp.fn = *fn;
+                    adjoint_coefs(rp.tp, s->precision, rp.coefs);
+                    continue;
+                }
                 srcs.push_back(jit_adjoint_source(rp.tp, s->precision, rp.coefs));
                 who.push_back(&rp);
+                keys.push_back(std::move(key));
             }
         }
         const auto fns = jit_kernels(srcs);
         for (size_t i = 0; i < who.size(); ++i) {
             who[i]->fn = fns[i];
+            if (fns[i]) {
+                cache.add(std::move(keys[i]), fns[i]);
+            }
         }
     }
     for (auto& seg : rev) {

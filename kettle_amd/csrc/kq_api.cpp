@@ -235,6 +235,9 @@ int kq_state_destroy(kq_state* s)
         if (s->amps) {
             (void) hipFree(s->amps);
         }
+        if (s->spare) {
+            (void) hipFree(s->spare);
+        }
         if (s->scratch) {
             (void) hipFree(s->scratch);
         }

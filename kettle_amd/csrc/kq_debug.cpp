@@ -31,6 +31,7 @@ namespace
 struct DebugPass
 {
     bool sweep = false;
+    bool out_of_place = false;   // stores into the second buffer, which then becomes the state's
     std::string src;
     std::vector<double> coefs;
     uint32_t blocks = 0;
@@ -56,11 +57,19 @@ std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_o
     }
     std::vector<uint8_t> bit_of;
     std::vector<uint8_t> sigma;
+    // as a gate run: the last pass puts every qubit back on its own bit when it can (in place, or out of
+    // place into the second buffer)
+    std::vector<uint8_t> final_bits;
     if (fusion_relabel()) {
         recs = exchanges_as_relabels(recs, n_qubits, sigma);
+        final_bits.resize(n_qubits);
+        for (uint32_t q = 0; q < n_qubits; ++q) {
+            final_bits[sigma[q]] = static_cast<uint8_t>(q);
+        }
     }
     auto passes = plan_passes(recs, n_qubits, max_k, fusion_low_qubits(), fusion_max_gates(), register_qubits(precision),
-                              fusion_max_stages(), true, fusion_relabel() ? &bit_of : nullptr);
+                              fusion_max_stages(), true, fusion_relabel() ? &bit_of : nullptr, nullptr,
+                              fusion_relabel() ? &final_bits : nullptr, true);
     if (!sigma.empty() && bit_of.size() == n_qubits) {
         const std::vector<uint8_t> planned = bit_of;
         for (uint32_t q = 0; q < n_qubits; ++q) {
@@ -87,6 +96,7 @@ std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_o
             TileProgram tp;
             lower_tile_pass(p, precision, n_qubits, tp);
             d.src = jit_source(tp, precision, d.coefs);
+            d.out_of_place = !tp.out_freepos.empty();
             d.blocks = tile_pass_blocks(tp);
             d.threads = 1u << tp.n_thr_bits;
         }
@@ -407,7 +417,7 @@ int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_o
             const double* from = d.sweep ? d.gate.m : d.coefs.data();
             std::memcpy(coefs, from, std::min(coef_cap, *n_coefs) * sizeof(double));
         }
-        info[0] = d.sweep ? 1 : 0;
+        info[0] = d.sweep ? 1 : (d.out_of_place ? 2 : 0);
         info[1] = d.blocks;
         info[2] = d.threads;
         info[3] = d.sweep ? d.gate.target : 0;

@@ -78,8 +78,11 @@ struct TilePass
     int c = 0;                   // low contiguous qubits 0..c-1 in the tile
     int rq = 0;                  // register bits per thread
     std::vector<int> qubits;     // qubits[j] = index bit of local bit j (ascending)
-    std::vector<int> out_qubits; // index bit local bit j is stored to (empty: qubits[j]); a permutation
-                                 // of `qubits` that moves qubits onto the low bits for the next pass
+    std::vector<int> out_qubits; // index bit local bit j is stored to (empty: qubits[j]); in place a
+                                 // permutation of `qubits` that moves qubits onto the low bits for the
+                                 // next pass, out of place any bits (with out_free, a permutation of all)
+    std::vector<int> out_free;   // out of place (non-empty): the pass stores into a second buffer, and
+                                 // the i-th index bit outside the tile (ascending) goes to out_free[i]
     std::vector<GateRec> gates;  // tile-encoded records
     std::vector<TileStage> stages;
     bool load_direct = false;    // load HBM straight into stage 0's layout
@@ -175,6 +178,7 @@ struct TileProgram
     std::vector<TileGate> derivs;     // adjoint passes: derivative records (pad = partial slot)
     std::vector<TileTerm> terms;      // terms of merged diagonal runs
     std::vector<uint32_t> deriv_src;  // input record index of each derivative slot
+    std::vector<uint8_t> out_freepos; // out of place: destination bit of each freepos entry (empty: in place)
 };
 
 // Lower a planned pass into the tile kernel's program (coefficients rounded to the precision).
@@ -233,6 +237,9 @@ struct kq_state
     // qubit q of the state is stored on index bit bit_of[q] (empty: bit q); only gate passes of a
     // statevector leave it permuted, every other operation restores bit q first
     std::vector<uint8_t> bit_of;
+    // a second buffer of the state's size: the destination of passes that store out of place (the
+    // buffers then trade places); allocated by the first such pass and kept until the state is destroyed
+    void* spare = nullptr;
 };
 
 namespace kq
@@ -280,10 +287,15 @@ bool is_diag(const GateRec& g);
 // Plan fused tile passes over a straight-line list of records.
 // allow_sweep: a pass holding a single diagonal gate may run as a plain sweep (off for adjoint plans)
 // With `emit`, each pass is handed over as soon as it is planned and the returned list is empty.
+// final_bits (with bit_of): the placement the last pass should leave (qubit -> index bit).  The last
+// pass then takes in the bits it needs to store every qubit there, in place when the displaced qubits'
+// bits and destinations fit its tile, else (out_of_place) into a second buffer when the bits that hold
+// the destinations' low qubits fit; otherwise the placement stays as planned.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates, uint32_t nvec, int max_k,
                                   int low_c, int max_gates, int rq, int max_stages, bool allow_sweep = true,
                                   std::vector<uint8_t>* bit_of = nullptr,
-                                  const std::function<void(TilePass&&)>* emit = nullptr);
+                                  const std::function<void(TilePass&&)>* emit = nullptr,
+                                  const std::vector<uint8_t>* final_bits = nullptr, bool out_of_place = false);
 // gate-free tile passes that store every qubit q back on index bit q
 std::vector<TilePass> plan_restore(std::vector<uint8_t> bit_of, uint32_t nvec, int max_k, int low_c, int rq);
 // whether statevector gate passes may move qubits onto other index bits (kq_set_fusion_relabel)
@@ -481,8 +493,14 @@ struct JitTuning
     int blocks_per_cu = 0;
     bool normalise = true;   // uncontrolled one-qubit gates with their pivot divided out
     int priority = 1;        // wave priority around HBM traffic: 0 off, 1 loads, 2 loads and stores
-    int nontemporal = 0;     // nontemporal hint on HBM traffic: 0 off, 1 loads, 2 loads and stores
+    int nontemporal = 3;     // nontemporal hint on HBM traffic: 0 off, 1 loads, 2 loads and stores,
+                             // 3 loads and stores of vectors larger than the 256 MiB Infinity Cache
 };
+// the nontemporal mode (0..2) a pass over vectors of 2^nvec amplitudes of `amp_bytes` bytes runs with
+inline int nontemporal_mode(int mode, uint32_t nvec, size_t amp_bytes)
+{
+    return mode < 3 ? mode : ((amp_bytes << nvec) > (size_t {256} << 20) ? 2 : 0);
+}
 void set_jit_tuning(const JitTuning& t);
 JitTuning jit_tuning();
 uint32_t device_cu_count();
@@ -497,10 +515,15 @@ std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
 hipFunction_t jit_lookup(const std::string& source);
 // compile without loading (no device needed): number of failures, last log in *log
 size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
+// out-of-place programs (tp.out_freepos) store into s->spare, which must be allocated
 void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);
 // kernel source of one adjoint reverse pass over (psi, lambda) with derivative reductions (the records of
 // an adjoint-lowered program); coefs receives its coefficient buffer
 std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
+// the coefficient buffer of an adjoint reverse pass alone (what jit_adjoint_source also returns)
+void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& coefs, std::vector<size_t>* offs = nullptr);
+// everything of an adjoint reverse pass its generated source depends on besides the coefficients
+std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision);
 // partials: [slot][blocks * waves] doubles, as the interpreted adjoint pass writes them
 void launch_jit_adjoint_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, void* psi, void* lam,
                              const double* d_coefs, double* d_partials);

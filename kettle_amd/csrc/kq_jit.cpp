@@ -715,9 +715,9 @@ int jit_mode() { return cache().mode; }
 void set_jit_tuning(const JitTuning& t)
 {
     if (t.min_waves_per_simd < 0 || t.min_waves_per_simd > 8 || t.blocks_per_cu < 0 || t.blocks_per_cu > 64
-        || t.priority < 0 || t.priority > 2 || t.nontemporal < 0 || t.nontemporal > 2) {
+        || t.priority < 0 || t.priority > 2 || t.nontemporal < 0 || t.nontemporal > 3) {
         throw InvalidArg {"jit tuning: min waves per SIMD must be in [0, 8], blocks per CU in [0, 64], priority "
-                          "and nontemporal mode in [0, 2]"};
+                          "in [0, 2] and nontemporal mode in [0, 3]"};
     }
     std::lock_guard<std::mutex> lk(cache().mu);
     cache().tuning = t;
@@ -745,7 +745,11 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     if (tune.min_waves_per_simd > 0) {
         o << " __attribute__((amdgpu_waves_per_eu(" << tune.min_waves_per_simd << ")))";
     }
-    o << " void kq_pass(kq::dev::Amp<R>* __restrict__ a, const double* __restrict__ cf, uint32_t n_blocks)\n{\n"
+    // an out-of-place pass (the state's last pass putting every qubit on its final bit) loads from `a`
+    // and stores into `dst`, at the destination bits of every tile and outside bit
+    const bool oop = !tp.out_freepos.empty();
+    o << " void kq_pass(kq::dev::Amp<R>* __restrict__ a, const double* __restrict__ cf, uint32_t n_blocks"
+      << (oop ? ", kq::dev::Amp<R>* __restrict__ dst" : "") << ")\n{\n"
       << "  using namespace kq::dev;\n";
     bool remaps = tp.out_layout != 0;
     for (const TileStep& st : tp.steps) {
@@ -820,22 +824,25 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         return e;
     };
     const auto io = [&](const TileLayout& L, bool load) {
-        o << "  {\n    const uint64_t off = (base | (" << hbm_thread_expr(L, tp.n_thr_bits) << "))";
+        const bool to_dst = oop && !load;
+        o << "  {\n    const uint64_t off = (" << (to_dst ? "(" + deposit_expr(tp.out_freepos) + ")" : std::string {"base"})
+          << " | (" << hbm_thread_expr(L, tp.n_thr_bits) << "))";
         if (!load) {
             o << " ^ (" << pending_xor(L.hbm_slot, "0ull") << ")";
             pending.clear();
         }
         o << ";\n";
-        const char* nt_ld = tune.nontemporal > 0 ? "true" : "false";
-        const char* nt_st = tune.nontemporal > 1 ? "true" : "false";
+        const int ntm = nontemporal_mode(tune.nontemporal, static_cast<uint32_t>(tp.k + tp.freepos.size()), c64 ? 8 : 16);
+        const char* nt_ld = ntm > 0 ? "true" : "false";
+        const char* nt_st = ntm > 1 ? "true" : "false";
         for (int r = 0; r < 16; ++r) {
             if (load) {
                 o << "    { const Amp<R> x = amp_load<" << nt_ld << ">(a + (off ^ " << hex64(L.hbm_slot[r]) << ")); v[" << r
                   << "] = {x.x, x.y}; }\n";
             }
             else {
-                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; amp_store<" << nt_st << ">(a + (off ^ "
-                  << hex64(L.hbm_slot[r]) << "), x); }\n";
+                o << "    { Amp<R> x; x.x = v[" << r << "].re; x.y = v[" << r << "].im; amp_store<" << nt_st << ">("
+                  << (to_dst ? "dst" : "a") << " + (off ^ " << hex64(L.hbm_slot[r]) << "), x); }\n";
             }
         }
         o << "  }\n";
@@ -1093,7 +1100,7 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         if (!same_lds) {
             remap(A, B);
         }
-        else {
+        else if (!oop) {
             // Same registers, other HBM offsets: the store moves qubits to other bits of the tile, so
             // a thread writes amplitudes other threads of the workgroup read.  Every load of the
             // workgroup must have completed first: a full barrier (it waits for outstanding loads).
@@ -1107,6 +1114,110 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     io(tp.layouts[cur], false);
     o << "  }\n}\n";
     return head + pre.str() + o.str();
+}
+
+// The coefficient buffer of an adjoint reverse pass in the order its generated kernel reads it: per
+// record (steps in order, each step's records in order) the terms of a merged diagonal run, or the 8
+// values of a uniform diagonal or of a gate, then the 8 values of its derivative record if it has one;
+// complex64 values as packed floats.  offs (optional) receives each of those entries' offsets in that
+// order.  It depends on the coefficients only, so a pass of known structure needs no new source.
+void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& coefs, std::vector<size_t>* offs)
+{
+    const bool c64 = precision == KQ_C64;
+    coefs.clear();
+    if (offs) {
+        offs->clear();
+    }
+    const auto push8 = [&](const double* m) {
+        if (offs) {
+            offs->push_back(coefs.size());
+        }
+        if (c64) {
+            for (int i = 0; i < 4; ++i) {
+                coefs.push_back(m[i]);   // 8 packed floats
+            }
+        }
+        else {
+            coefs.insert(coefs.end(), m, m + 8);
+        }
+    };
+    for (const TileStep& st : tp.steps) {
+        for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
+            const TileGate& g = tp.gates[gi];
+            if (g.code >= TV_MERGE_BASE) {
+                for (uint64_t j = 0; j < g.glb_sel; ++j) {
+                    const TileTerm& t = tp.terms[g.lane_sel + j];
+                    if (offs) {
+                        offs->push_back(coefs.size());
+                    }
+                    if (c64) {
+                        coefs.push_back(t.d[0]);
+                        coefs.push_back(t.d[1]);
+                    }
+                    else {
+                        coefs.insert(coefs.end(), t.d, t.d + 4);
+                    }
+                }
+            }
+            else if (g.code >= TV_UNIFORM_BASE) {
+                double m[8];
+                const auto mv = record_values(g, c64);
+                std::copy(mv.begin(), mv.end(), m);
+                push8(c64 ? reinterpret_cast<const double*>(g.m) : m);
+            }
+            else {
+                push8(g.m);
+            }
+            if (g.pad != 0) {
+                push8(tp.derivs[g.pad - 1].m);
+            }
+        }
+    }
+}
+
+// Structure of an adjoint reverse pass: everything its generated source depends on besides the
+// coefficients (records' variants, conditions and selections, derivative generators, steps, layouts,
+// outside bits), with the precision and the kernel tuning; passes with equal keys have equal sources.
+std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision)
+{
+    const JitTuning t = jit_tuning();
+    std::vector<uint64_t> key {static_cast<uint64_t>(precision), static_cast<uint64_t>(tp.k), static_cast<uint64_t>(tp.n_thr_bits),
+                               tp.out_layout, static_cast<uint64_t>(t.nontemporal), static_cast<uint64_t>(t.priority),
+                               static_cast<uint64_t>(t.min_waves_per_simd), tp.gates.size(), tp.steps.size(), tp.layouts.size()};
+    const auto rec = [&](const TileGate& g) {
+        key.push_back((uint64_t {g.code} << 32) | g.lane_ctrl);
+        key.push_back((uint64_t {g.lane_sel} << 32) | g.pad);
+        key.push_back(g.glb_ctrl);
+        key.push_back(g.glb_sel);
+    };
+    for (const TileGate& g : tp.gates) {
+        rec(g);
+    }
+    for (const TileGate& d : tp.derivs) {
+        rec(d);
+        for (const double v : record_values(d, precision == KQ_C64)) {
+            key.push_back(std::bit_cast<uint64_t>(v));   // the generator's form is chosen from these
+        }
+    }
+    for (const TileTerm& m : tp.terms) {
+        key.push_back((uint64_t {m.lane_ctrl} << 32) | m.lane_sel);
+        key.push_back(m.glb_ctrl);
+        key.push_back(m.glb_sel);
+    }
+    for (const TileStep& st : tp.steps) {
+        key.push_back((uint64_t {st.g_begin} << 32) | st.g_count);
+        key.push_back(st.layout);
+    }
+    for (const TileLayout& L : tp.layouts) {
+        const auto* w = reinterpret_cast<const uint32_t*>(&L);
+        for (size_t i = 0; i < sizeof(TileLayout) / sizeof(uint32_t); ++i) {
+            key.push_back(w[i]);
+        }
+    }
+    for (const uint8_t b : tp.freepos) {
+        key.push_back(b);
+    }
+    return key;
 }
 
 // Generate the kernel source of one adjoint reverse pass (two vectors: psi `a`, lambda `b`) and the
@@ -1150,8 +1261,9 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
       << "  const uint64_t col = uint64_t {blockIdx.x} * " << waves << "u + (tid >> 6);\n";
     const auto io = [&](const TileLayout& L, bool load) {
         o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
-        const char* nt_ld = tune.nontemporal > 0 ? "true" : "false";
-        const char* nt_st = tune.nontemporal > 1 ? "true" : "false";
+        const int ntm = nontemporal_mode(tune.nontemporal, static_cast<uint32_t>(tp.k + tp.freepos.size()), c64 ? 8 : 16);
+        const char* nt_ld = ntm > 0 ? "true" : "false";
+        const char* nt_st = ntm > 1 ? "true" : "false";
         for (int r = 0; r < 16; ++r) {
             const std::string at = "(off ^ " + hex64(L.hbm_slot[r]) + ")";
             if (load) {
@@ -1182,29 +1294,11 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
         }
         o << "  }\n";
     };
-    const auto push8 = [&](const double* m) {
-        const size_t off = coefs.size();
-        if (c64) {
-            for (int i = 0; i < 4; ++i) {
-                coefs.push_back(m[i]);   // 8 packed floats
-            }
-        }
-        else {
-            coefs.insert(coefs.end(), m, m + 8);
-        }
-        return off;
-    };
-    const auto push_term = [&](const TileTerm& t) {
-        const size_t off = coefs.size();
-        if (c64) {
-            coefs.push_back(t.d[0]);
-            coefs.push_back(t.d[1]);
-        }
-        else {
-            coefs.insert(coefs.end(), t.d, t.d + 4);
-        }
-        return off;
-    };
+    // offsets of the coefficient buffer, consumed in the order adjoint_coefs laid it out
+    std::vector<size_t> offs;
+    adjoint_coefs(tp, precision, coefs, &offs);
+    size_t next_off = 0;
+    const auto take = [&]() { return offs.at(next_off++); };
     // the statement applying record g to vector u (coefficients / factors already in scope)
     const auto apply_to = [&](const TileGate& g, const char* u) {
         std::ostringstream a;
@@ -1244,7 +1338,7 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                 o << "    R f0r = 1, f0i = 0, f1r = 1, f1i = 0;\n";
                 for (uint64_t j = 0; j < g.glb_sel; ++j) {
                     const TileTerm& t = tp.terms[g.lane_sel + j];
-                    const size_t off = push_term(t);
+                    const size_t off = take();
                     o << "    { R d0r, d0i, d1r, d1i; term_at<R>(cf + " << off << ", d0r, d0i, d1r, d1i); const bool tk = "
                       << cond_expr(t.glb_ctrl, t.lane_ctrl) << ";";
                     if (uniform) {
@@ -1260,20 +1354,17 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                 }
             }
             else if (g.code >= TV_UNIFORM_BASE) {
-                double m[8];
-                const auto mv = record_values(g, c64);
-                std::copy(mv.begin(), mv.end(), m);
-                const size_t off = push8(c64 ? reinterpret_cast<const double*>(g.m) : m);
+                const size_t off = take();
                 o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n    const bool sel = " << sel_expr(g.lane_sel, g.glb_sel)
                   << ";\n    const R fr = sel ? m.m[6] : m.m[0], fi = sel ? m.m[7] : m.m[1];\n";
             }
             else {
-                const size_t off = push8(g.m);
+                const size_t off = take();
                 o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n";
             }
             if (g.pad != 0) {
                 const TileGate& dg = tp.derivs[g.pad - 1];
-                const size_t doff = push8(dg.m);
+                const size_t doff = take();
                 o << "    {\n      const Coef<R> dm = coef_at<R>(cf + " << doff << ");\n      double acc = 0.0;\n"
                   << "      if (" << cond_expr(dg.glb_ctrl, dg.lane_ctrl) << ") {\n";
                 if (dg.code >= TV_UNIFORM_BASE) {
@@ -1524,10 +1615,17 @@ void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const
     if (tp.freepos.size() > 31 || tp.freepos.size() + tp.k != s->nvec) {
         throw InvalidArg {"jit pass: bad program shape"};
     }
+    const bool oop = !tp.out_freepos.empty();
+    if (oop && (s->spare == nullptr || tp.out_freepos.size() != tp.freepos.size())) {
+        throw InvalidArg {"jit pass: out-of-place pass without a destination buffer"};
+    }
     void* amps = s->amps;
+    void* dst = s->spare;
     const double* cf = d_coefs;
     uint32_t n_blocks = tile_pass_blocks(tp);
-    void* args[] = {&amps, &cf, &n_blocks};
+    void* args3[] = {&amps, &cf, &n_blocks};
+    void* args4[] = {&amps, &cf, &n_blocks, &dst};
+    void** args = oop ? args4 : args3;
     const JitTuning tune = jit_tuning();
     unsigned grid = n_blocks;
     if (tune.blocks_per_cu > 0) {

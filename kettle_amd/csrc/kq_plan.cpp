@@ -572,7 +572,8 @@ std::vector<uint8_t> choose_low_qubits(const std::vector<GateRec>& gates, const 
 // it qubit q is index bit q throughout.
 std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t nvec, int max_k, int low_c, int max_gates,
                                   int rq_max, int max_stages, bool allow_sweep, std::vector<uint8_t>* bit_of_io,
-                                  const std::function<void(TilePass&&)>* emit)
+                                  const std::function<void(TilePass&&)>* emit, const std::vector<uint8_t>* final_bits,
+                                  bool out_of_place)
 {
     std::vector<TilePass> passes;
     const auto hand_over = [&](TilePass&& p) {
@@ -640,6 +641,44 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
             hand_over(std::move(p));
             remaining = std::move(deferred);
             continue;
+        }
+        // The last pass stores every qubit on its final bit when the bits that takes fit the tile.  Index
+        // bit b holds a qubit whose final bit is dest[b] (a permutation).  In place, the tile must hold
+        // every bit that moves (the moved bits are closed under dest); out of place it must hold the bits
+        // whose contents land on the destination's low bits 0..cs-1, so that each wavefront still stores
+        // runs of 2^cs amplitudes: cs = c when those bits fit, else fewer, down to 3 (runs of 64 or 128
+        // bytes).
+        std::vector<int> dest;
+        bool fold_oop = false;
+        if (relabel && deferred.empty() && final_bits && final_bits->size() == nvec && *final_bits != bit_of) {
+            std::vector<uint32_t> qubit_on(nvec);
+            for (uint32_t q = 0; q < nvec; ++q) {
+                qubit_on[bit_of[q]] = q;
+            }
+            std::vector<int> d(nvec);
+            uint64_t moved = 0;
+            for (uint32_t b = 0; b < nvec; ++b) {
+                d[b] = (*final_bits)[qubit_on[b]];
+                moved |= static_cast<uint64_t>(d[b] != static_cast<int>(b)) << b;
+            }
+            if (std::popcount(q_mask | moved) <= k) {
+                q_mask |= moved;
+                dest = std::move(d);
+            }
+            else if (out_of_place) {
+                for (int cs = c; cs >= 3 && dest.empty(); --cs) {
+                    uint64_t low_dest = 0;
+                    for (uint32_t b = 0; b < nvec; ++b) {
+                        low_dest |= static_cast<uint64_t>(d[b] < cs) << b;
+                    }
+                    if (std::popcount(q_mask | low_dest) <= k) {
+                        q_mask |= low_dest;
+                        dest = d;
+                        fold_oop = true;
+                    }
+                }
+            }
+            q_count = std::popcount(q_mask);
         }
         // fill Q with the lowest free qubits (longer contiguous runs coalesce better)
         for (uint32_t q = 0; q < nvec && q_count < k; ++q) {
@@ -722,7 +761,21 @@ std::vector<TilePass> plan_passes(const std::vector<GateRec>& gates_in, uint32_t
             }
             complete_stage(p.io_layout, top, 0, k, rq);
         }
-        if (relabel && !deferred.empty()) {
+        if (!dest.empty()) {
+            p.out_qubits.resize(p.qubits.size());
+            for (size_t j = 0; j < p.qubits.size(); ++j) {
+                p.out_qubits[j] = dest[static_cast<size_t>(p.qubits[j])];
+            }
+            if (fold_oop) {
+                for (uint32_t b = 0; b < nvec; ++b) {
+                    if (!((q_mask >> b) & 1ull)) {
+                        p.out_free.push_back(dest[b]);
+                    }
+                }
+            }
+            bit_of = *final_bits;
+        }
+        else if (relabel && !deferred.empty()) {
             const auto nb = choose_low_qubits(gates_in, deferred, bit_of, q_mask, c, k, rq, max_gates, max_stages);
             if (nb != bit_of) {
                 // local bit j holds the qubit on index bit p.qubits[j]; after the store it is on nb[qubit]
@@ -1228,11 +1281,33 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
         out.steps.push_back(step);
     }
     const bool store_cur = p.store_direct || same_regs(p.io_layout, *cur, p.rq);
+    const bool oop = !p.out_free.empty();
+    uint64_t dest_free = 0;   // out of place: the destination bits of the outside bits
+    if (oop) {
+        // every index bit goes to one destination bit: the tile's bits to out_qubits, the others to out_free
+        if (p.out_free.size() != out.freepos.size() || p.out_qubits.size() != p.qubits.size()) {
+            throw InvalidArg {"tile lowering: out-of-place store bits do not match the tile"};
+        }
+        uint64_t seen = 0;
+        for (const std::vector<int>* v : {&p.out_qubits, &p.out_free}) {
+            for (const int b : *v) {
+                if (b < 0 || b >= static_cast<int>(nvec) || ((seen >> b) & 1ull)) {
+                    throw InvalidArg {"tile lowering: out-of-place store bits are not a permutation"};
+                }
+                seen |= uint64_t {1} << b;
+            }
+        }
+        for (const int b : p.out_free) {
+            dest_free |= uint64_t {1} << b;
+        }
+        out.out_freepos.assign(p.out_free.begin(), p.out_free.end());
+    }
     if (!p.out_qubits.empty() && p.out_qubits != p.qubits) {
-        // the store moves qubits to other index bits of the tile: the store layout's HBM offsets
-        // follow out_qubits (its LDS offsets are the current layout's when no remap is needed)
-        if (p.out_qubits.size() != p.qubits.size()
-            || !std::is_permutation(p.out_qubits.begin(), p.out_qubits.end(), p.qubits.begin())) {
+        // the store moves qubits to other index bits (of the tile, or out of place of the state): the
+        // store layout's HBM offsets follow out_qubits (its LDS offsets are the current layout's when no
+        // remap is needed)
+        if (!oop && (p.out_qubits.size() != p.qubits.size()
+                     || !std::is_permutation(p.out_qubits.begin(), p.out_qubits.end(), p.qubits.begin()))) {
             throw InvalidArg {"tile lowering: store bits are not a permutation of the tile bits"};
         }
         out.layouts.push_back(moved_store_layout(p, ab));
@@ -1248,9 +1323,12 @@ void lower_tile_pass(const TilePass& p, int precision, uint32_t nvec, TileProgra
     // every layout must address each amplitude of the state exactly once: the slot bits, the
     // thread bits and the free qubits are disjoint and together cover all nvec index bits
     const uint64_t all = nvec >= 64 ? ~0ull : ((uint64_t {1} << nvec) - 1);
-    const uint64_t free_bits = all & ~qmask;
+    const uint64_t free_bits_src = all & ~qmask;
     const uint32_t tile_bytes = static_cast<uint32_t>((size_t {1} << p.k) * ab);
-    for (const TileLayout& L : out.layouts) {
+    for (size_t li = 0; li < out.layouts.size(); ++li) {
+        const TileLayout& L = out.layouts[li];
+        // an out-of-place store layout addresses the destination, whose outside bits are out_free's
+        const uint64_t free_bits = (oop && li == out.out_layout) ? dest_free : free_bits_src;
         uint64_t slot_bits = 0, thr_bits = 0;
         for (int q = 0; q < p.rq; ++q) {
             slot_bits |= L.hbm_slot[1u << q];

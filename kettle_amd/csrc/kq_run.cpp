@@ -36,9 +36,12 @@ uint64_t sweep_amps(const GateRec& g, uint64_t n_amps)
 }
 
 void execute_passes(kq_state* s, const std::vector<TilePass>& passes);
-void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of);
+// final_bits: the placement the last pass should store (see plan_passes); out_of_place: it may store
+// into the state's spare buffer
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of,
+                  const std::vector<uint8_t>* final_bits = nullptr, bool out_of_place = false);
 void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of, int low_c,
-                  int max_stages);
+                  int max_stages, const std::vector<uint8_t>* final_bits = nullptr, bool out_of_place = false);
 // tile settings of one gate run, chosen by the plan tuner
 struct FusionChoice
 {
@@ -119,15 +122,36 @@ PermuteTables permute_tables(const std::vector<uint8_t>& bit_of)
     return t;
 }
 
+// The state's spare buffer (the destination of out-of-place passes), allocated on first use; false if
+// it cannot be allocated.
+bool ensure_spare(kq_state* s)
+{
+    if (s->spare) {
+        return true;
+    }
+    try {
+        s->spare = device_alloc(s->n_amps * amp_bytes(s->precision));
+    }
+    catch (const AllocError&) {
+        return false;
+    }
+    return true;
+}
+
 // Moves every qubit q of the state back on index bit q in one out-of-place pass (k_permute_bits) into
-// a new buffer that replaces the state's.  False (nothing done) if the state has fewer than 12 index
-// bits or the second buffer cannot be allocated.
+// the spare buffer, or a new buffer, which replaces the state's.  False (nothing done) if the state has
+// fewer than 12 index bits or no second buffer can be had.
 bool permute_out_of_place(kq_state* s)
 {
     if (s->nvec < 12 || s->nvec > 63) {
         return false;
     }
     const PermuteTables t = permute_tables(s->bit_of);
+    if (s->spare) {
+        launch_permute_bits(s, s->amps, s->spare, t);
+        std::swap(s->amps, s->spare);
+        return true;
+    }
     void* dst = nullptr;
     try {
         dst = device_alloc(s->n_amps * amp_bytes(s->precision));
@@ -200,6 +224,13 @@ std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uin
     return out;
 }
 
+// Whether the last pass of a statevector gate run may store out of place: only run-time compiled passes
+// do, into the state's spare buffer (allocated here when it can be).
+bool fold_out_of_place(kq_state* s, int max_k)
+{
+    return jit_wanted(s) && jit_available() && max_k < static_cast<int>(s->nvec) && ensure_spare(s);
+}
+
 void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relabel)
 {
     PlanStats& st = last_stats();
@@ -227,6 +258,11 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
         normalise_layout(s);
     }
     std::vector<uint8_t> bit_of = s->bit_of;
+    // a gate run leaves every qubit on its own bit when its last pass can store them there
+    std::vector<uint8_t> identity_bits(s->nvec);
+    for (uint32_t q = 0; q < s->nvec; ++q) {
+        identity_bits[q] = static_cast<uint8_t>(q);
+    }
     if (relabel) {
         std::vector<uint8_t> sigma;
         const std::vector<GateRec> rest = exchanges_as_relabels(recs, s->nvec, sigma);
@@ -241,7 +277,13 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
             if (!rest.empty()) {
                 const FusionChoice fc = tuned(s, max_k) ? choose_fusion(s, rest, max_k, bit_of)
                                                         : FusionChoice {fusion_low_qubits(), fusion_max_stages()};
-                plan_and_run(s, rest, max_k, &bit_of, fc.low_c, fc.max_stages);
+                // qubit q ends on the bit of the relabelled qubit sigma[q]: for q on bit q, that one's
+                // final bit is q
+                std::vector<uint8_t> final_bits(s->nvec);
+                for (uint32_t q = 0; q < s->nvec; ++q) {
+                    final_bits[sigma[q]] = static_cast<uint8_t>(q);
+                }
+                plan_and_run(s, rest, max_k, &bit_of, fc.low_c, fc.max_stages, &final_bits, fold_out_of_place(s, max_k));
             }
             std::vector<uint8_t> out(s->nvec);
             bool identity = true;
@@ -261,10 +303,13 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
             }
         }
         const FusionChoice fc = choose_fusion(s, recs, max_k, bit_of);
-        plan_and_run(s, recs, max_k, &bit_of, fc.low_c, fc.max_stages);
+        plan_and_run(s, recs, max_k, &bit_of, fc.low_c, fc.max_stages, &identity_bits, fold_out_of_place(s, max_k));
+    }
+    else if (relabel) {
+        plan_and_run(s, recs, max_k, &bit_of, &identity_bits, fold_out_of_place(s, max_k));
     }
     else {
-        plan_and_run(s, recs, max_k, relabel ? &bit_of : nullptr);
+        plan_and_run(s, recs, max_k, nullptr);
     }
     if (relabel) {
         bool identity = true;
@@ -413,6 +458,33 @@ private:
             KQ_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_->stream));
             profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
                      [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d)); });
+            if (!tp.out_freepos.empty()) {
+                std::swap(s_->amps, s_->spare);   // the pass stored the state into the spare buffer
+            }
+            return;
+        }
+        if (!tp.out_freepos.empty()) {
+            // no compiled kernel for an out-of-place pass: the interpreted kernel runs it in place without
+            // moving qubits, then one permutation pass stores every index bit b at its destination
+            TilePass in_place = p;
+            std::vector<uint8_t> src_of(s_->nvec);   // destination bit -> source bit
+            for (size_t j = 0; j < p.qubits.size(); ++j) {
+                src_of[static_cast<size_t>(p.out_qubits[j])] = static_cast<uint8_t>(p.qubits[j]);
+            }
+            for (size_t i = 0; i < tp.freepos.size(); ++i) {
+                src_of[tp.out_freepos[i]] = tp.freepos[i];
+            }
+            in_place.out_qubits.clear();
+            in_place.out_free.clear();
+            Item plain;
+            plain.pass = std::move(in_place);
+            lower_tile_pass(plain.pass, s_->precision, s_->nvec, plain.prog);
+            --st.n_passes;
+            --st.n_tile_passes;
+            launch(plain);
+            ++st.n_passes;
+            launch_permute_bits(s_, s_->amps, s_->spare, permute_tables(src_of));
+            std::swap(s_->amps, s_->spare);
             return;
         }
         const size_t g_b = align256(tp.gates.size() * sizeof(TileGate)), s_b = align256(tp.steps.size() * sizeof(TileStep)),
@@ -538,17 +610,19 @@ void execute_passes_with(kq_state* s, const std::vector<TilePass>& passes,
 
 // Plans a gate list and runs its passes as they are planned.
 void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of, int low_c,
-                  int max_stages)
+                  int max_stages, const std::vector<uint8_t>* final_bits, bool out_of_place)
 {
     PassStream stream(s);
     const std::function<void(TilePass&&)> emit = [&](TilePass&& p) { stream.push(std::move(p)); };
-    plan_passes(recs, s->nvec, max_k, low_c, fusion_max_gates(), register_qubits(s->precision), max_stages, true, bit_of, &emit);
+    plan_passes(recs, s->nvec, max_k, low_c, fusion_max_gates(), register_qubits(s->precision), max_stages, true, bit_of, &emit,
+                final_bits, out_of_place);
     stream.finish();
 }
 
-void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of)
+void plan_and_run(kq_state* s, const std::vector<GateRec>& recs, int max_k, std::vector<uint8_t>* bit_of,
+                  const std::vector<uint8_t>* final_bits, bool out_of_place)
 {
-    plan_and_run(s, recs, max_k, bit_of, fusion_low_qubits(), fusion_max_stages());
+    plan_and_run(s, recs, max_k, bit_of, fusion_low_qubits(), fusion_max_stages(), final_bits, out_of_place);
 }
 
 // ---- plan tuner ------------------------------------------------------------------------------

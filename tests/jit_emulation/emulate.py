@@ -49,8 +49,8 @@ thread_local EmuDim threadIdx;
 EmuDim blockIdx;
 EmuDim gridDim;
 std::barrier<>* emu_barrier = nullptr;
-extern "C" void emu_launch(void (*fn)(void*, const double*, uint32_t), void* a, const double* cf, uint32_t blocks,
-                           uint32_t threads)
+template <typename F>
+static void emu_run(F call, uint32_t blocks, uint32_t threads)
 {
     gridDim.x = blocks;
     for (uint32_t b = 0; b < blocks; ++b) {
@@ -59,12 +59,23 @@ extern "C" void emu_launch(void (*fn)(void*, const double*, uint32_t), void* a, 
         emu_barrier = &bar;
         std::vector<std::thread> pool;
         for (uint32_t t = 0; t < threads; ++t) {
-            pool.emplace_back([=] { threadIdx.x = t; fn(a, cf, blocks); });
+            pool.emplace_back([=] { threadIdx.x = t; call(); });
         }
         for (auto& th : pool) {
             th.join();
         }
     }
+}
+extern "C" void emu_launch(void (*fn)(void*, const double*, uint32_t), void* a, const double* cf, uint32_t blocks,
+                           uint32_t threads)
+{
+    emu_run([=] { fn(a, cf, blocks); }, blocks, threads);
+}
+// out-of-place passes: a fourth argument, the destination buffer
+extern "C" void emu_launch_to(void (*fn)(void*, const double*, uint32_t, void*), void* a, const double* cf, uint32_t blocks,
+                              uint32_t threads, void* dst)
+{
+    emu_run([=] { fn(a, cf, blocks, dst); }, blocks, threads);
 }
 """
 
@@ -97,7 +108,7 @@ def _passes(ops: np.ndarray, n: int, precision: int):
         coefs = np.zeros(max(nc.value, 1), dtype=np.float64)
         _lib.check(lib.kq_debug_pass(*args, buf, ln.value + 1, c.byref(ln), _lib.ptr(coefs), coefs.size, c.byref(nc),
                                      _lib.ptr(info), c.byref(npas)))
-        out.append({"sweep": bool(info[0]), "src": buf.value.decode(), "coefs": coefs[:nc.value].copy(),
+        out.append({"sweep": int(info[0]) == 1, "to_other": int(info[0]) == 2, "src": buf.value.decode(), "coefs": coefs[:nc.value].copy(),
                     "blocks": int(info[1]), "threads": int(info[2]), "target": int(info[3]), "control": int(info[4]) - 1})
         i += 1
         if i >= npas.value:
@@ -110,7 +121,7 @@ _BUILD = Path(tempfile.gettempdir()) / "kq_jit_emulation"
 def _compile(sources: list[tuple[str, str]]) -> ctypes.CDLL:
     """sources: (function name, host source); one shared library (cached by content)."""
     _BUILD.mkdir(exist_ok=True)
-    key = hashlib.sha1("".join(s for _, s in sources).encode()).hexdigest()[:16]
+    key = hashlib.sha1((RUNTIME + "".join(s for _, s in sources)).encode()).hexdigest()[:16]
     lib_path = _BUILD / f"emu_{key}.so"
     if not lib_path.exists():
         files = []
@@ -160,6 +171,9 @@ def run(ops: np.ndarray, n: int, initial: np.ndarray, precision: int = ka.KQ_C12
     fn_t = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32)
     launch = lib.emu_launch
     launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]
+    launch_to = lib.emu_launch_to
+    launch_to.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                          ctypes.c_void_p]
     c64 = precision == ka.KQ_C64
     psi = np.ascontiguousarray(initial, dtype=np.complex64 if c64 else np.complex128).copy()
     for i, p in enumerate(passes):
@@ -175,5 +189,12 @@ def run(ops: np.ndarray, n: int, initial: np.ndarray, precision: int = ka.KQ_C12
             continue
         fn = getattr(lib, f"kq_pass_{i}")
         cf = np.ascontiguousarray(p["coefs"] if p["coefs"].size else np.zeros(1))
-        launch(ctypes.cast(fn, ctypes.c_void_p), psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"])
+        if p["to_other"]:
+            # stores into a second buffer, which then holds the state (every entry is written once)
+            dst = np.full_like(psi, np.nan)
+            launch_to(ctypes.cast(fn, ctypes.c_void_p), psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"],
+                      dst.ctypes.data)
+            psi = dst
+        else:
+            launch(ctypes.cast(fn, ctypes.c_void_p), psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"])
     return psi.astype(np.complex128)

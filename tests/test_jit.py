@@ -195,6 +195,30 @@ def test_jit_tunings_match_the_oracle(tuning):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", [1, 2], ids=["loads", "loads-stores"])
+def test_jit_nontemporal_modes_match_the_oracle(mode):
+    """Nontemporal loads (and stores) forced on a 21-qubit state (32 MiB, where the automatic mode leaves
+    them off), against the oracle."""
+    n = 21
+    c = random_gate_circuit(n, 120, np.random.default_rng(8100 + mode))
+    ka.set_jit(2)
+    ka.set_jit_nontemporal(mode)
+    try:
+        out = run_sv(c)
+    finally:
+        ka.set_jit_nontemporal(3)
+    ref, _ = orc.sv_simulate(basis("0" * n), n, ops_of(c))
+    assert np.max(np.abs(out - ref)) <= TOL128
+
+
+def test_set_jit_nontemporal_rejects_bad_mode():
+    from kettle_amd import _lib
+    assert _lib.load().kq_set_jit_nontemporal(4) == _lib.KQ_ERR_INVALID
+    assert _lib.load().kq_set_jit_nontemporal(-1) == _lib.KQ_ERR_INVALID
+    ka.set_jit_nontemporal(3)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
 def test_jit_flipped_forms_20q(precision):
     """Six low qubits per tile and up to 16 stages on the layered circuit: one-qubit gates (dense, real,

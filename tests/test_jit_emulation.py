@@ -55,3 +55,23 @@ def test_random_gates_complex128(n):
 def test_random_gates_complex64_low_four():
     ka.set_fusion_low_qubits(4)
     check(random_gate_circuit(14, 200, np.random.default_rng(9200)), 14, ka.KQ_C64, 2e-5)
+
+
+@pytest.mark.parametrize("low,stages,layers", [(6, 16, 8), (4, 12, 12)], ids=["runs-of-64", "runs-of-8"])
+def test_last_pass_stores_out_of_place_in_qubit_order(low, stages, layers):
+    """The last pass of a gate run with qubit moves stores every qubit back on its own bit, out of place
+    into a second buffer: with the tile holding the bits bound for index bits 0..5 (stores in runs of 64
+    amplitudes), and with room for only bits 0..2 of them (runs of 8).  The emulated state, read in qubit
+    order with no restore pass after it, equals the oracle's."""
+    ka.set_fusion_low_qubits(low)
+    ka.set_fusion(-1, 0, stages, 0.0)
+    try:
+        ops, _, _ = workloads.qft_plus_layers(n=16, layers=layers).flatten()
+        passes = emulate._passes(np.ascontiguousarray(ops), 16, ka.KQ_C128)
+        assert passes[-1]["to_other"] and not any(p["to_other"] for p in passes[:-1])
+        out = emulate.run(ops, 16, basis("0" * 16), ka.KQ_C128)
+    finally:
+        ka.set_fusion(-1, 0, 12, 0.0)
+        ka.set_fusion_autotune(True)
+    ref, _ = orc.sv_simulate(basis("0" * 16), 16, ops)
+    assert np.max(np.abs(out - ref)) <= 1e-12
