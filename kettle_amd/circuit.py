@@ -429,13 +429,15 @@ class QuantumCircuit:
         preds: list[int] = []
         self._flatten_into(ops, preds, values, slot)
         arr = np.zeros(len(ops), dtype=OP_DTYPE)
-        for i, (g, q0, q1, aux, angle, u) in enumerate(ops):
-            arr[i]["gate"] = g
-            arr[i]["q0"] = q0
-            arr[i]["q1"] = q1
-            arr[i]["aux"] = aux
-            arr[i]["angle"] = angle
-            arr[i]["u"] = u
+        if ops:
+            # one column at a time (a record-by-record fill costs more than the simulation of small states)
+            g, q0, q1, aux, angle, u = zip(*ops)
+            arr["gate"] = g
+            arr["q0"] = q0
+            arr["q1"] = q1
+            arr["aux"] = aux
+            arr["angle"] = angle
+            arr["u"] = u
         return arr, np.asarray(preds, dtype=np.int32), order
 
     def _flatten_into(self, ops, preds, values, slot) -> None:
