@@ -390,6 +390,13 @@ int kq_debug_dense_block(kq_state* state, const double* u, int use_mfma, double*
 /* Lane layout of the 16x16x4 matrix instruction (FP64 for KQ_C128, FP32 for KQ_C64): one wave runs
  * D = A x B once with lane l holding operands a[l], b[l]; d[4 l + e] is element e of lane l's D. */
 int kq_debug_mfma16_layout(int precision, const double* a, const double* b, double* d);
+/* Fault injection for tests of the callers' error paths (no counterpart in the reference): any single
+ * device allocation of more than alloc_cap_bytes fails with KQ_ERR_ALLOC (0: no cap), and, when
+ * fail_status is non-zero, the entry-point call after the next fail_after_calls ones returns
+ * fail_status without doing anything, with the message "injected fault" (once).  Process-wide; this
+ * call and kq_last_error are never failed.  kettle's C++ layer turns the statuses into its exception
+ * types like any other failure (std::out_of_range for KQ_ERR_OUT_OF_RANGE, else std::runtime_error). */
+int kq_debug_set_faults(size_t alloc_cap_bytes, int fail_status, int fail_after_calls);
 
 #ifdef __cplusplus
 }

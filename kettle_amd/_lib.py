@@ -96,6 +96,7 @@ _SIGNATURES = {
     "kq_stream_probe_shapes": (c_int, [c_uint32, c_void_p]),
     "kq_debug_dense_block": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_double)]),
     "kq_debug_mfma16_layout": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
+    "kq_debug_set_faults": (c_int, [c_size_t, c_int, c_int]),
     "kq_sample_circuit_shots": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_void_p, c_size_t, c_size_t, c_void_p,
                                         c_uint64, c_int, c_int32, c_uint64, c_void_p, POINTER(c_uint64)]),
     "kq_set_fusion_low_qubits": (c_int, [c_int]),

@@ -2,6 +2,7 @@
 // initialisation, state creation and copies, profiling switches and the batched-gradient handles.
 // The gate runs, measurement and sampling, Pauli operators, the adjoint gradient and the debug entry
 // points live in kq_run.cpp, kq_measure.cpp, kq_pauli.cpp, kq_adjoint.cpp and kq_debug.cpp.
+#include <atomic>
 #include <algorithm>
 #include <unordered_map>
 #include <functional>
@@ -63,8 +64,33 @@ void check_state(const kq_state* s)
     }
 }
 
+namespace
+{
+// kq_debug_set_faults: a cap on single device allocations, and a status the call after the next
+// g_fail_after ones returns
+std::atomic<size_t> g_alloc_cap {0};
+std::atomic<int> g_fail_status {0};
+std::atomic<int> g_fail_after {0};
+}  // namespace
+
+int take_injected_fault()
+{
+    if (g_fail_status.load(std::memory_order_relaxed) == 0) {
+        return 0;
+    }
+    if (g_fail_after.fetch_sub(1) > 0) {
+        return 0;
+    }
+    return g_fail_status.exchange(0);
+}
+
 void* device_alloc(size_t bytes)
 {
+    const size_t cap = g_alloc_cap.load(std::memory_order_relaxed);
+    if (cap != 0 && bytes > cap) {
+        throw AllocError {"device allocation of " + std::to_string(bytes) + " bytes exceeds the cap of " + std::to_string(cap)
+                          + " bytes"};
+    }
     void* p = nullptr;
     const hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess || p == nullptr) {
@@ -165,6 +191,18 @@ using namespace kq;
 extern "C" {
 
 const char* kq_last_error(void) { return g_last_error.c_str(); }
+
+int kq_debug_set_faults(size_t alloc_cap_bytes, int fail_status, int fail_after_calls)
+{
+    if (fail_after_calls < 0 || fail_status < 0) {
+        set_error("fault injection: negative call count or status");
+        return KQ_ERR_INVALID;
+    }
+    g_alloc_cap = alloc_cap_bytes;
+    g_fail_after = fail_after_calls;
+    g_fail_status = fail_status;
+    return KQ_OK;
+}
 
 int kq_init(int device)
 {

@@ -220,16 +220,7 @@ void lower_instance(BatchInstance& in, int precision, int fwd_precision)
     }
 }
 
-void* balloc(size_t bytes)
-{
-    void* p = nullptr;
-    const hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
-    if (e != hipSuccess || p == nullptr) {
-        (void) hipGetLastError();
-        throw AllocError {"batch: hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};
-    }
-    return p;
-}
+void* balloc(size_t bytes) { return device_alloc(std::max<size_t>(bytes, 256)); }
 
 void release(kq_batch* b)
 {

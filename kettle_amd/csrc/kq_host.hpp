@@ -16,9 +16,16 @@ namespace kq
 
 // Runs the body of a C entry point: maps the library's exception types to kq.h status codes and
 // keeps the message for kq_last_error().
+// the status an injected fault makes this call return (0: none); see kq_debug_set_faults
+int take_injected_fault();
+
 template <typename F>
 int guarded(F&& f)
 {
+    if (const int injected = take_injected_fault(); injected != 0) {
+        set_error("injected fault");
+        return injected;
+    }
     try {
         f();
         return KQ_OK;

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include <kettle/kettle.hpp>
+#include <kq.h>
 
 namespace
 {
@@ -426,6 +427,28 @@ int main()
             REQUIRE(bits == "0x" || bits == "1x");
         }
         REQUIRE(std::abs(state[0] - std::complex<double> {1.0, 0.0}) < 1e-15);
+    });
+
+    // ---- failures crossing the C-ABI (kq_debug_set_faults) ---------------------------------------------
+    run("injected faults reach kettle's exception types", [] {
+        // a cap on single device allocations below one 20-qubit state: simulate fails with std::runtime_error
+        REQUIRE(kq_debug_set_faults(size_t {1} << 20, 0, 0) == KQ_OK);
+        auto big = ket::QuantumCircuit {20};
+        big.add_h_gate(0);
+        auto s = ket::Statevector {20};
+        require_throws<std::runtime_error>([&] { ket::simulate(big, s); }, "exceeds the cap");
+        REQUIRE(kq_debug_set_faults(0, 0, 0) == KQ_OK);
+        // a forced status on the next libkq call: out-of-range stays std::out_of_range, others runtime_error
+        auto c = ket::QuantumCircuit {2};
+        c.add_x_gate(0);
+        auto s2 = ket::Statevector {2};
+        REQUIRE(kq_debug_set_faults(0, KQ_ERR_OUT_OF_RANGE, 0) == KQ_OK);
+        require_throws<std::out_of_range>([&] { ket::simulate(c, s2); }, "injected fault");
+        REQUIRE(kq_debug_set_faults(0, KQ_ERR_HIP, 0) == KQ_OK);
+        require_throws<std::runtime_error>([&] { ket::simulate(c, s2); }, "injected fault");
+        // the fault fires once: the same call then succeeds
+        ket::simulate(c, s2);
+        REQUIRE(ket::almost_eq(s2, ket::Statevector {"10"}));
     });
 
     std::printf("%d passed, %d failed\n", g_passed, g_failed);

@@ -147,3 +147,20 @@ def test_invalid_arguments_are_rejected_before_device_work():
     assert lib.kq_state_upload(None, None) == _lib.KQ_ERR_INVALID
     assert lib.kq_simulate(None, 1, None, 0, None, 0, 0, 0, None, 0, _lib.LOGGER_FN(), None) == _lib.KQ_ERR_INVALID
     assert lib.kq_sample_probabilities(None, 0, 1, 0, 0, None) == _lib.KQ_ERR_INVALID
+
+
+def test_injected_fault_fires_once_after_the_given_calls():
+    """kq_debug_set_faults: the call after the next fail_after_calls ones returns the forced status with
+    its message, once; the setter itself never fails (host-only calls, no device needed)."""
+    lib = _lib.load()
+    try:
+        assert lib.kq_debug_set_faults(0, _lib.KQ_ERR_RUNTIME, 2) == _lib.KQ_OK
+        assert lib.kq_set_fusion_low_qubits(5) == _lib.KQ_OK
+        assert lib.kq_set_fusion_low_qubits(5) == _lib.KQ_OK
+        assert lib.kq_set_fusion_low_qubits(5) == _lib.KQ_ERR_RUNTIME
+        assert lib.kq_last_error() == b"injected fault"
+        assert lib.kq_set_fusion_low_qubits(5) == _lib.KQ_OK
+        assert lib.kq_debug_set_faults(0, 0, -1) == _lib.KQ_ERR_INVALID
+    finally:
+        lib.kq_debug_set_faults(0, 0, 0)
+        lib.kq_set_fusion_autotune(1)
