@@ -109,7 +109,7 @@ struct ReversePass
     size_t blob_off = 0;      // its program (interpreted) or coefficients (compiled) in the blob
     uint64_t part_off = 0;    // its first derivative partial
     hipFunction_t fn = nullptr;   // the compiled pass kernel, or null: the interpreted tile kernel runs it
-    std::vector<double> coefs;
+    AdjointCoefs coefs;           // its coefficient buffer and derivative factors (compiled kernels)
 };
 
 // Compiled reverse-pass kernels by pass structure (adjoint_structure_key): an optimiser's next call on the
@@ -377,10 +377,10 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         ReverseKernels& cache = reverse_kernels();
         for (auto& seg : rev) {
             for (ReversePass& rp : seg) {
-                std::vector<uint64_t> key = adjoint_structure_key(rp.tp, s->precision);
+                adjoint_coefs(rp.tp, s->precision, rp.coefs);
+                std::vector<uint64_t> key = adjoint_structure_key(rp.tp, s->precision, rp.coefs);
                 if (const hipFunction_t* fn = cache.find(key)) {
                     rp.fn = *fn;
-                    adjoint_coefs(rp.tp, s->precision, rp.coefs);
                     continue;
                 }
                 srcs.push_back(jit_adjoint_source(rp.tp, s->precision, rp.coefs));
@@ -396,10 +396,16 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
             }
         }
     }
+    // each derivative row's factor, in row order (compiled passes divide the vectors by their pivots)
+    std::vector<double> row_scale;
+    row_scale.reserve(R);
     for (auto& seg : rev) {
         for (ReversePass& rp : seg) {
             rp.blob_off = blob_bytes;
-            blob_bytes += rp.fn ? align256b(std::max<size_t>(rp.coefs.size(), 1) * sizeof(double)) : program_bytes(rp.tp);
+            blob_bytes += rp.fn ? align256b(std::max<size_t>(rp.coefs.coefs.size(), 1) * sizeof(double)) : program_bytes(rp.tp);
+            for (size_t d = 0; d < rp.tp.derivs.size(); ++d) {
+                row_scale.push_back(rp.fn ? rp.coefs.dscale[d] : 1.0);
+            }
         }
     }
 
@@ -424,7 +430,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     for (const auto& seg : rev) {
         for (const ReversePass& rp : seg) {
             if (rp.fn) {
-                std::memcpy(up.data() + u_blob + rp.blob_off, rp.coefs.data(), rp.coefs.size() * sizeof(double));
+                std::memcpy(up.data() + u_blob + rp.blob_off, rp.coefs.coefs.data(), rp.coefs.coefs.size() * sizeof(double));
             }
             else {
                 write_program(up.data() + u_blob + rp.blob_off, rp.tp);
@@ -477,7 +483,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     *energy = e.x;
     std::fill(grad, grad + n_params, 0.0);
     for (size_t r = 0; r < R; ++r) {
-        grad[ops[row_gate[r]].aux] += 2.0 * sums[r];
+        grad[ops[row_gate[r]].aux] += 2.0 * sums[r] * row_scale[r];
     }
     if (n_restores) {
         *n_restores = count;

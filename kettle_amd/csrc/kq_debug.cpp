@@ -357,7 +357,7 @@ int kq_debug_adjoint_jit_compile(uint32_t n_qubits, int precision, const kq_op* 
         for (const auto& p : passes) {
             TileProgram tp;
             lower_tile_pass(p, precision, n_qubits, tp, &dptr);
-            std::vector<double> coefs;
+            AdjointCoefs coefs;
             srcs.push_back(jit_adjoint_source(tp, precision, coefs));
         }
         std::string log;

@@ -519,11 +519,22 @@ size_t jit_compile_only(const std::vector<std::string>& sources, std::string* lo
 void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);
 // kernel source of one adjoint reverse pass over (psi, lambda) with derivative reductions (the records of
 // an adjoint-lowered program); coefs receives its coefficient buffer
-std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
-// the coefficient buffer of an adjoint reverse pass alone (what jit_adjoint_source also returns)
-void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& coefs, std::vector<size_t>* offs = nullptr);
+// What the generated kernel of an adjoint reverse pass reads besides its structure: the coefficient
+// buffer; per derivative record, the factor its reduced partials need (uncontrolled one-qubit gates run
+// with their pivot divided out, so inside the pass psi and lambda are the true vectors divided by the
+// product F of the pivots so far, and Re <lambda|Q|psi> comes out divided by |F|^2); and the choice of
+// form of every such gate (which depends on the coefficients, and on which the source depends).
+struct AdjointCoefs
+{
+    std::vector<double> coefs;
+    std::vector<double> dscale;     // per derivative record: multiply its reduced partials by this
+    std::vector<std::string> forms; // per record: its normalised statements (empty: the general form)
+};
+std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoefs& out);
+// the coefficients, derivative factors and forms alone (what jit_adjoint_source also returns)
+void adjoint_coefs(const TileProgram& tp, int precision, AdjointCoefs& out, std::vector<size_t>* offs = nullptr);
 // everything of an adjoint reverse pass its generated source depends on besides the coefficients
-std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision);
+std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision, const AdjointCoefs& c);
 // partials: [slot][blocks * waves] doubles, as the interpreted adjoint pass writes them
 void launch_jit_adjoint_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, void* psi, void* lam,
                              const double* d_coefs, double* d_partials);

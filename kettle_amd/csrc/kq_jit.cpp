@@ -309,7 +309,8 @@ std::array<double, 8> record_values(const TileGate& g, bool c64)
 // X gates pending.  Only forms closed under that conjugation take it: an axis rotation is its own
 // conjugate, a rotation with the diagonal pivot [[1, t], [-t, 1]] becomes [[1, -t], [t, 1]].
 bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::vector<double>& coefs,
-                     std::complex<double>& factor, bool& factor_complex, const std::string& flip = {})
+                     std::complex<double>& factor, bool& factor_complex, const std::string& flip = {},
+                     const std::vector<std::string>& vecs = {"v"}, double floor_abs = 0.0)
 {
     if (g.code >= TV_UNIFORM_BASE || g.lane_ctrl != 0 || g.glb_ctrl != 0 || g.code % 5 != 0) {
         return false;
@@ -326,7 +327,13 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
     // form, and the kernel's structure stays the same for almost every angle.
     const auto diag_pivot_of = [](double diag, double off) { return std::abs(diag) >= 1.0e-6 * std::abs(off); };
     // amplitudes grow by 1 / |factor| inside the pass: keep that far from overflow
-    const double floor = c64 ? 1.0e-20 : 1.0e-200;
+    const double floor = floor_abs > 0.0 ? floor_abs : (c64 ? 1.0e-20 : 1.0e-200);
+    // the statement for every vector the pass carries (the adjoint's psi and lambda take the same gate)
+    const auto each = [&](const std::string& head, const std::string& args) {
+        for (const std::string& vec : vecs) {
+            o << "  " << head << "(" << vec << ", " << args << ");\n";
+        }
+    };
     const auto fits = [&](std::complex<double> pivot) { return std::abs(factor * pivot) >= floor; };
     if (kind == TK_REAL) {
         const bool rotation = m[6] == m[0] && m[4] == -m[2];
@@ -344,8 +351,8 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
             return false;
         }
         const std::string t = val((diag_pivot ? m[2] : m[0]) / pivot);
-        o << "  orth<R, " << P << ", " << form << ">(v, " << (flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")")
-          << ");\n";
+        each("orth<R, " + std::to_string(P) + ", " + std::to_string(form) + ">",
+             flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")");
         factor *= pivot;
         return true;
     }
@@ -358,8 +365,7 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         if (pivot == 0.0 || !fits(pivot)) {
             return false;
         }
-        o << "  axis<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << val((diag_pivot ? m[3] : m[0]) / pivot)
-          << ");\n";
+        each("axis<R, " + std::to_string(P) + ", " + std::to_string(diag_pivot ? 0 : 1) + ">", val((diag_pivot ? m[3] : m[0]) / pivot));
         factor *= pivot;
         return true;
     }
@@ -374,8 +380,11 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         const std::complex<double> d = std::complex<double> {m[6], m[7]} / d0;
         const auto dr = val(d.real());
         const auto di = val(d.imag());
-        o << "  { const Coef<R> m {{0, 0, 0, 0, 0, 0, " << dr << ", " << di << "}}; apply<R, K_DIAG1, " << P
-          << ", 0>(v, m); }\n";
+        o << "  { const Coef<R> m {{0, 0, 0, 0, 0, 0, " << dr << ", " << di << "}};";
+        for (const std::string& vec : vecs) {
+            o << " apply<R, K_DIAG1, " << P << ", 0>(" << vec << ", m);";
+        }
+        o << " }\n";
         factor *= d0;
         factor_complex = true;
         return true;
@@ -395,7 +404,7 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         const auto near = [](double a, double b) { return std::abs(a - b) <= ULPS * std::max(1.0, std::abs(b)); };
         if (diag_pivot && near(z.imag(), 0.0)) {
             if (near(z.real(), 1.0) && near(x.real(), 0.0) && near(y.real(), 0.0) && near(y.imag(), x.imag())) {
-                o << "  axis<R, " << P << ", 0>(v, " << val(x.imag()) << ");\n";
+                each("axis<R, " + std::to_string(P) + ", 0>", val(x.imag()));
                 factor *= pivot;
                 factor_complex = true;
                 return true;
@@ -405,8 +414,8 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
                 const bool reflection = near(z.real(), -1.0) && near(y.real(), x.real());
                 if ((rotation || reflection) && (flip.empty() || rotation)) {
                     const std::string t = val(x.real());
-                    o << "  orth<R, " << P << ", " << (rotation ? 0 : 1) << ">(v, "
-                      << (flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")") << ");\n";
+                    each("orth<R, " + std::to_string(P) + ", " + std::to_string(rotation ? 0 : 1) + ">",
+                         flip.empty() ? t : "((" + flip + ") ? -" + t + " : " + t + ")");
                     factor *= pivot;
                     factor_complex = true;
                     return true;
@@ -418,8 +427,8 @@ bool emit_normalised(std::ostringstream& o, const TileGate& g, bool c64, std::ve
         }
         const auto xr = val(x.real()), xi = val(x.imag()), yr = val(y.real()), yi = val(y.imag());
         const auto zr = val(z.real()), zi = val(z.imag());
-        o << "  dense1<R, " << P << ", " << (diag_pivot ? 0 : 1) << ">(v, " << xr << ", " << xi << ", " << yr << ", "
-          << yi << ", " << zr << ", " << zi << ");\n";
+        each("dense1<R, " + std::to_string(P) + ", " + std::to_string(diag_pivot ? 0 : 1) + ">",
+             xr + ", " + xi + ", " + yr + ", " + yi + ", " + zr + ", " + zi);
         factor *= pivot;
         factor_complex = true;
         return true;
@@ -1118,13 +1127,20 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
 
 // The coefficient buffer of an adjoint reverse pass in the order its generated kernel reads it: per
 // record (steps in order, each step's records in order) the terms of a merged diagonal run, or the 8
-// values of a uniform diagonal or of a gate, then the 8 values of its derivative record if it has one;
-// complex64 values as packed floats.  offs (optional) receives each of those entries' offsets in that
-// order.  It depends on the coefficients only, so a pass of known structure needs no new source.
-void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& coefs, std::vector<size_t>* offs)
+// values of a uniform diagonal, or the normalised values of an uncontrolled one-qubit gate (its pivot
+// divided out, as in the forward passes, the pivot going into the pass factor F), or the 8 values of any
+// other gate, then the 8 values of its derivative record if it has one; finally F itself, which
+// multiplies both vectors before the store.  complex64 values as packed floats.  offs (optional)
+// receives the offsets of the terms, uniform and general gate values and derivative values in that
+// order (normalised gates carry their offsets in their statements).  Depends on the coefficients only,
+// so a pass of known structure and forms needs no new source.
+void adjoint_coefs(const TileProgram& tp, int precision, AdjointCoefs& out, std::vector<size_t>* offs)
 {
     const bool c64 = precision == KQ_C64;
+    std::vector<double>& coefs = out.coefs;
     coefs.clear();
+    out.dscale.assign(tp.derivs.size(), 1.0);
+    out.forms.assign(tp.gates.size(), std::string {});
     if (offs) {
         offs->clear();
     }
@@ -1141,9 +1157,18 @@ void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& co
             coefs.insert(coefs.end(), m, m + 8);
         }
     };
+    // Products of psi and lambda grow by 1 / |F|^2 inside the pass: complex64 keeps |F| >= 1e-12, so they
+    // stay far below the FP32 range.
+    const double floor = c64 ? 1.0e-12 : 1.0e-200;
+    std::complex<double> factor {1.0, 0.0};
+    bool factor_complex = false;
+    bool normalised = false;
     for (const TileStep& st : tp.steps) {
         for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
             const TileGate& g = tp.gates[gi];
+            if (g.pad != 0) {
+                out.dscale[g.pad - 1] = std::norm(factor);   // |F|^2 where the derivative is taken
+            }
             if (g.code >= TV_MERGE_BASE) {
                 for (uint64_t j = 0; j < g.glb_sel; ++j) {
                     const TileTerm& t = tp.terms[g.lane_sel + j];
@@ -1166,19 +1191,30 @@ void adjoint_coefs(const TileProgram& tp, int precision, std::vector<double>& co
                 push8(c64 ? reinterpret_cast<const double*>(g.m) : m);
             }
             else {
-                push8(g.m);
+                std::ostringstream form;
+                if (emit_normalised(form, g, c64, coefs, factor, factor_complex, {}, {"v", "w"}, floor)) {
+                    out.forms[gi] = form.str();
+                    normalised = true;
+                }
+                else {
+                    push8(g.m);
+                }
             }
             if (g.pad != 0) {
                 push8(tp.derivs[g.pad - 1].m);
             }
         }
     }
+    if (normalised) {
+        coefs.push_back(factor.real());
+        coefs.push_back(factor.imag());
+    }
 }
 
 // Structure of an adjoint reverse pass: everything its generated source depends on besides the
 // coefficients (records' variants, conditions and selections, derivative generators, steps, layouts,
 // outside bits), with the precision and the kernel tuning; passes with equal keys have equal sources.
-std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision)
+std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision, const AdjointCoefs& c)
 {
     const JitTuning t = jit_tuning();
     std::vector<uint64_t> key {static_cast<uint64_t>(precision), static_cast<uint64_t>(tp.k), static_cast<uint64_t>(tp.n_thr_bits),
@@ -1217,6 +1253,15 @@ std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision
     for (const uint8_t b : tp.freepos) {
         key.push_back(b);
     }
+    // the normalised gates' statements (their form, and where their values sit in the buffer)
+    for (const std::string& f : c.forms) {
+        key.push_back(f.size());
+        for (size_t i = 0; i < f.size(); i += 8) {
+            uint64_t w = 0;
+            std::memcpy(&w, f.data() + i, std::min<size_t>(8, f.size() - i));
+            key.push_back(w);
+        }
+    }
     return key;
 }
 
@@ -1224,11 +1269,12 @@ std::vector<uint64_t> adjoint_structure_key(const TileProgram& tp, int precision
 // layout of its coefficient buffer.  The same records as the interpreted adjoint tile kernel, as
 // straight-line code: for every record, if it has a derivative record Q, Re <lambda| Q |psi> over the
 // tile, reduced per wavefront into partials[slot * n_part + column]; then psi <- G psi, lambda <- G lambda.
-// Remaps move psi, then lambda, through one LDS tile of whole amplitudes.
-std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
+// Remaps move psi, then lambda, through one LDS tile of whole amplitudes.  Uncontrolled one-qubit gates
+// run with their pivot divided out (adjoint_coefs), the product of the pivots multiplying both vectors
+// before the store.
+std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoefs& ac)
 {
     const JitTuning tune = jit_tuning();
-    coefs.clear();
     const bool c64 = precision == KQ_C64;
     const int nt = 1 << tp.n_thr_bits;
     const int waves = (nt + 63) / 64;
@@ -1296,7 +1342,7 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
     };
     // offsets of the coefficient buffer, consumed in the order adjoint_coefs laid it out
     std::vector<size_t> offs;
-    adjoint_coefs(tp, precision, coefs, &offs);
+    adjoint_coefs(tp, precision, ac, &offs);
     size_t next_off = 0;
     const auto take = [&]() { return offs.at(next_off++); };
     // the statement applying record g to vector u (coefficients / factors already in scope)
@@ -1358,7 +1404,7 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                 o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n    const bool sel = " << sel_expr(g.lane_sel, g.glb_sel)
                   << ";\n    const R fr = sel ? m.m[6] : m.m[0], fi = sel ? m.m[7] : m.m[1];\n";
             }
-            else {
+            else if (ac.forms[gi].empty()) {
                 const size_t off = take();
                 o << "    const Coef<R> m = coef_at<R>(cf + " << off << ");\n";
             }
@@ -1399,9 +1445,20 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, std::vector
                 o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
                   << "u} * n_part + col] = acc;\n      }\n    }\n";
             }
+            if (!ac.forms[gi].empty()) {
+                // uncontrolled one-qubit gate, pivot divided out, on both vectors
+                o << "  " << ac.forms[gi] << "  }\n";
+                continue;
+            }
             o << "    if (ok) { " << apply_to(g, "v") << " }\n";
             o << "    if (ok) { " << apply_to(g, "w") << " }\n  }\n";
         }
+    }
+    if (std::any_of(ac.forms.begin(), ac.forms.end(), [](const std::string& f) { return !f.empty(); })) {
+        // the product of the pivots taken out, last in the buffer, back into both vectors
+        const size_t nf = ac.coefs.size();
+        o << "  scale<R, 0>(v, static_cast<R>(cf[" << nf - 2 << "]), static_cast<R>(cf[" << nf - 1 << "]));\n"
+          << "  scale<R, 0>(w, static_cast<R>(cf[" << nf - 2 << "]), static_cast<R>(cf[" << nf - 1 << "]));\n";
     }
     if (tp.out_layout != cur) {
         remap(tp.layouts[cur], tp.layouts[tp.out_layout]);
