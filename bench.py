@@ -38,10 +38,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per tile-pass launch over algorithmic bytes (one read + one write of the state), from
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on the 28-qubit version of this workload (kq_pass),
-# FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (the factors a plain copy kernel of known size
-# reads): measured HBM bytes per dispatch / (2 * 16 * 2^28 B), the excess being coefficient tables
-TRAFFIC_PER_ALGORITHMIC_BYTE = 1.0004
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this 30-qubit workload (kq_pass, the 14 passes of one
+# call), FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0 (the factors a plain copy kernel of known
+# size reads): measured HBM bytes per dispatch / (2 * 16 * 2^30 B), the excess being coefficient tables
+TRAFFIC_PER_ALGORITHMIC_BYTE = 1.0003
 
 
 def log(msg: str) -> None:
