@@ -107,6 +107,13 @@ constexpr uint32_t TV_UNIFORM_BASE = TK_COUNT * 4 * 5;
 // outside bit, i.e. every slot) and the register control C, then applies them in one step.
 constexpr uint32_t TV_MERGE_BASE = TV_UNIFORM_BASE + 5;
 constexpr uint32_t TV_END = TV_MERGE_BASE + 25;
+// Derivative records only: Re <w| Q |v> for the fixed generators of the angle gates (dG/dtheta = Q G):
+// variant TV_DGEN_BASE + gen * 20 + P * 5 + C with gen 0: (-i/2) X (RX), 1: (-i/2) Y (RY), 2: (-i/2) Z
+// (RZ), 3: i |1><1| (P), on register position P with register control C; no coefficients needed.
+constexpr uint32_t TV_DGEN_BASE = TV_END;
+constexpr uint32_t TV_DGEN_END = TV_DGEN_BASE + 4 * 20;
+// the generator a derivative matrix is (0..3, as above), or -1
+int derivative_generator(const double m[8]);
 
 // one diagonal gate inside a merged run: its conditions, selection and the two diagonal entries
 struct TileTerm

@@ -1413,34 +1413,19 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoef
                 const size_t doff = take();
                 o << "    {\n      const Coef<R> dm = coef_at<R>(cf + " << doff << ");\n      double acc = 0.0;\n"
                   << "      if (" << cond_expr(dg.glb_ctrl, dg.lane_ctrl) << ") {\n";
-                if (dg.code >= TV_UNIFORM_BASE) {
+                if (dg.code >= TV_DGEN_BASE && dg.code < TV_DGEN_END) {
+                    // an angle gate's fixed generator: its own code, no coefficients
+                    const uint32_t pc = (dg.code - TV_DGEN_BASE) % 20;
+                    o << "        acc = ddot_gen<R, " << (dg.code - TV_DGEN_BASE) / 20 << ", " << pc / 5 << ", " << pc % 5
+                      << ">(v, w);\n";
+                }
+                else if (dg.code >= TV_UNIFORM_BASE) {
                     o << "        const bool dsel = " << sel_expr(dg.lane_sel, dg.glb_sel) << ";\n"
                       << "        acc = ddot_uniform<R, " << (dg.code - TV_UNIFORM_BASE)
                       << ">(v, w, dsel ? dm.m[6] : dm.m[0], dsel ? dm.m[7] : dm.m[1]);\n";
                 }
                 else {
-                    // the angle gates' fixed generators get their own code (no coefficients)
-                    const auto dv = record_values(dg, c64);
-                    const auto is = [&](std::array<double, 8> want) { return dv == want; };
-                    int gen = -1;
-                    if (is({0, 0, 0, -0.5, 0, -0.5, 0, 0})) {
-                        gen = 0;
-                    }
-                    else if (is({0, 0, -0.5, 0, 0.5, 0, 0, 0})) {
-                        gen = 1;
-                    }
-                    else if (is({0, -0.5, 0, 0, 0, 0, 0, 0.5})) {
-                        gen = 2;
-                    }
-                    else if (is({0, 0, 0, 0, 0, 0, 0, 1.0})) {
-                        gen = 3;
-                    }
-                    if (gen >= 0) {
-                        o << "        acc = ddot_gen<R, " << gen << ", " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w);\n";
-                    }
-                    else {
-                        o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
-                    }
+                    o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
                 }
                 o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
                   << "u} * n_part + col] = acc;\n      }\n    }\n";

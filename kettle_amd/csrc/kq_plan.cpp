@@ -1050,8 +1050,27 @@ uint32_t tile_kind(uint32_t kind)
     throw InvalidArg {"tile lowering: unknown gate kind"};
 }
 
+int derivative_generator(const double m[8])
+{
+    const auto is = [&](std::initializer_list<double> want) { return std::equal(want.begin(), want.end(), m); };
+    if (is({0, 0, 0, -0.5, 0, -0.5, 0, 0})) {
+        return 0;
+    }
+    if (is({0, 0, -0.5, 0, 0.5, 0, 0, 0})) {
+        return 1;
+    }
+    if (is({0, -0.5, 0, 0, 0, 0, 0, 0.5})) {
+        return 2;
+    }
+    if (is({0, 0, 0, 0, 0, 0, 0, 1.0})) {
+        return 3;
+    }
+    return -1;
+}
+
 // One encoded gate record -> kernel variant.  derivative: the record is a derivative dG (never
-// given the d0 = 1 treatment, its diagonal may have d0 = 0).
+// given the d0 = 1 treatment, its diagonal may have d0 = 0); on a register target, an angle gate's
+// fixed generator gets its own variant (TV_DGEN_BASE), which needs no coefficients.
 TileGate lower_gate(const GateRec& g, int precision, bool derivative)
 {
     TileGate t {};
@@ -1077,6 +1096,10 @@ TileGate lower_gate(const GateRec& g, int precision, bool derivative)
             throw InvalidArg {"tile lowering: control equals target"};
         }
         t.code = tile_code(tile_kind(g.kind), tidx, creg);
+        const int gen = derivative ? derivative_generator(g.m) : -1;
+        if (gen >= 0) {
+            t.code = TV_DGEN_BASE + static_cast<uint32_t>(gen) * 20 + tidx * 5 + creg;
+        }
     }
     else {
         if (!diag) {

@@ -135,6 +135,25 @@ __device__ __forceinline__ double ddot_gate(const Cx<R> (&v)[16], const Cx<R> (&
     return 0.0;
 }
 
+#define KQ_DGEN_CASE(G, P, C) \
+    case (G) * 20 + (P) * 5 + (C): return ddot_gen<R, G, P, C>(v, w);
+#define KQ_DGEN_ALL(G) \
+    KQ_DGEN_CASE(G, 0, 0) KQ_DGEN_CASE(G, 0, 2) KQ_DGEN_CASE(G, 0, 3) KQ_DGEN_CASE(G, 0, 4) \
+    KQ_DGEN_CASE(G, 1, 0) KQ_DGEN_CASE(G, 1, 1) KQ_DGEN_CASE(G, 1, 3) KQ_DGEN_CASE(G, 1, 4) \
+    KQ_DGEN_CASE(G, 2, 0) KQ_DGEN_CASE(G, 2, 1) KQ_DGEN_CASE(G, 2, 2) KQ_DGEN_CASE(G, 2, 4) \
+    KQ_DGEN_CASE(G, 3, 0) KQ_DGEN_CASE(G, 3, 1) KQ_DGEN_CASE(G, 3, 2) KQ_DGEN_CASE(G, 3, 3)
+
+// an angle gate's fixed generator (TV_DGEN_BASE variants): gc = gen * 20 + P * 5 + C
+template <typename R>
+__device__ __forceinline__ double ddot_gen_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t gc)
+{
+    switch (gc) {
+        KQ_DGEN_ALL(0) KQ_DGEN_ALL(1) KQ_DGEN_ALL(2) KQ_DGEN_ALL(3)
+        default: break;
+    }
+    return 0.0;
+}
+
 template <typename R>
 __device__ __forceinline__ double ddot_uniform_gate(const Cx<R> (&v)[16], const Cx<R> (&w)[16], uint32_t c, R fr, R fi)
 {
@@ -416,7 +435,10 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                     const Coef<R> dm = coef_of<R>(dg);
                     double acc = 0.0;
                     if ((base & dg.glb_ctrl) == dg.glb_ctrl && (tid & dg.lane_ctrl) == dg.lane_ctrl) {
-                        if (dg.code >= TV_UNIFORM_BASE) {
+                        if (dg.code >= TV_DGEN_BASE) {
+                            acc = ddot_gen_gate<R>(v, w, dg.code - TV_DGEN_BASE);
+                        }
+                        else if (dg.code >= TV_UNIFORM_BASE) {
                             const bool sel = dg.lane_sel ? ((tid & dg.lane_sel) != 0) : ((base & dg.glb_sel) != 0);
                             acc = ddot_uniform_gate<R>(v, w, dg.code - TV_UNIFORM_BASE, sel ? dm.m[6] : dm.m[0],
                                                        sel ? dm.m[7] : dm.m[1]);
