@@ -148,6 +148,12 @@ class QuantumCircuit:
         self._elements: list = []
         self._params: dict[str, _ParamData] = {}
         self._param_count = 0
+        self._version = 0          # bumped by every change of the element list
+        self._flat_cache = None    # flatten()'s arrays of the last straight-line structure
+
+    def _append(self, el) -> None:
+        self._elements.append(el)
+        self._version += 1
 
     # -- sizes -------------------------------------------------------------------------------
     def n_qubits(self) -> int:
@@ -207,7 +213,7 @@ class QuantumCircuit:
                 self._one(gate, t)
             return
         self._check_qubit(int(target), "qubit", gate.name)
-        self._elements.append(_GateEl(gate, int(target)))
+        self._append(_GateEl(gate, int(target)))
 
     def _one_angle(self, gate: Gate, target, angle=None, parameterized=None):
         if isinstance(target, Iterable) and not isinstance(target, str):
@@ -217,13 +223,13 @@ class QuantumCircuit:
         self._check_qubit(int(target), "qubit", gate.name)
         if isinstance(angle, str):
             pid = self._use_param(angle)
-            self._elements.append(_GateEl(gate, int(target), param=pid))
+            self._append(_GateEl(gate, int(target), param=pid))
             return None
         if parameterized:
             pid = self._new_param(float(angle))
-            self._elements.append(_GateEl(gate, int(target), angle=float(angle), param=pid))
+            self._append(_GateEl(gate, int(target), angle=float(angle), param=pid))
             return pid
-        self._elements.append(_GateEl(gate, int(target), angle=float(angle)))
+        self._append(_GateEl(gate, int(target), angle=float(angle)))
         return None
 
     def _ctrl(self, gate: Gate, control, target=None) -> None:
@@ -233,7 +239,7 @@ class QuantumCircuit:
             return
         self._check_qubit(int(control), "control qubit", gate.name)
         self._check_qubit(int(target), "target qubit", gate.name)
-        self._elements.append(_GateEl(gate, int(control), int(target)))
+        self._append(_GateEl(gate, int(control), int(target)))
 
     def _ctrl_angle(self, gate: Gate, control, target=None, angle=None, parameterized=None):
         if target is None:
@@ -244,13 +250,13 @@ class QuantumCircuit:
         self._check_qubit(int(target), "target qubit", gate.name)
         if isinstance(angle, str):
             pid = self._use_param(angle)
-            self._elements.append(_GateEl(gate, int(control), int(target), param=pid))
+            self._append(_GateEl(gate, int(control), int(target), param=pid))
             return None
         if parameterized:
             pid = self._new_param(float(angle))
-            self._elements.append(_GateEl(gate, int(control), int(target), angle=float(angle), param=pid))
+            self._append(_GateEl(gate, int(control), int(target), angle=float(angle), param=pid))
             return pid
-        self._elements.append(_GateEl(gate, int(control), int(target), angle=float(angle)))
+        self._append(_GateEl(gate, int(control), int(target), angle=float(angle)))
         return None
 
     # -- kettle's builder names -------------------------------------------------------------------
@@ -292,7 +298,7 @@ class QuantumCircuit:
                 self.add_u_gate(matrix, t)
             return
         self._check_qubit(int(target), "qubit", "U")
-        self._elements.append(_GateEl(Gate.U, int(target), matrix=matrix))
+        self._append(_GateEl(Gate.U, int(target), matrix=matrix))
 
     def add_cu_gate(self, matrix: Matrix2X2, control, target=None) -> None:
         if target is None:
@@ -301,7 +307,7 @@ class QuantumCircuit:
             return
         self._check_qubit(int(control), "control qubit", "CU")
         self._check_qubit(int(target), "target qubit", "CU")
-        self._elements.append(_GateEl(Gate.CU, int(control), int(target), matrix=matrix))
+        self._append(_GateEl(Gate.CU, int(control), int(target), matrix=matrix))
 
     def add_m_gate(self, target, bit=None) -> None:
         """``add_m_gate(q)`` stores the outcome in bit q; ``add_m_gate(q, bit)`` in ``bit``."""
@@ -315,7 +321,7 @@ class QuantumCircuit:
         bit = int(target) if bit is None else int(bit)
         self._check_qubit(int(target), "qubit", "M")
         self._check_bit(bit)
-        self._elements.append(_GateEl(Gate.M, int(target), bit))
+        self._append(_GateEl(Gate.M, int(target), bit))
 
     # compound helpers that expand to primitive gates before simulation (circuit.cpp:801-903)
     def add_swap_gate(self, t0, t1=None) -> None:
@@ -381,7 +387,7 @@ class QuantumCircuit:
         for b in predicate.bit_indices_to_check:
             self._check_bit(b)
         self._merge_params(subcircuit)
-        self._elements.append(_IfEl(predicate, subcircuit))
+        self._append(_IfEl(predicate, subcircuit))
 
     def add_if_not_statement(self, bit: int, subcircuit: "QuantumCircuit") -> None:
         self.add_if_statement(ControlFlowPredicate([int(bit)], [0], ControlFlowBooleanKind.IF), subcircuit)
@@ -391,21 +397,22 @@ class QuantumCircuit:
             predicate = ControlFlowPredicate([int(predicate)], [1], ControlFlowBooleanKind.IF)
         for b in predicate.bit_indices_to_check:
             self._check_bit(b)
-        self._elements.append(_IfEl(predicate, if_sub, else_sub))
+        self._append(_IfEl(predicate, if_sub, else_sub))
 
     def add_if_not_else_statement(self, bit: int, if_sub: "QuantumCircuit", else_sub: "QuantumCircuit") -> None:
         self.add_if_else_statement(ControlFlowPredicate([int(bit)], [0], ControlFlowBooleanKind.IF), if_sub, else_sub)
 
     def add_classical_register_circuit_logger(self) -> None:
-        self._elements.append(_LogEl(OP_LOG_CREG))
+        self._append(_LogEl(OP_LOG_CREG))
 
     def add_statevector_circuit_logger(self) -> None:
-        self._elements.append(_LogEl(OP_LOG_STATE))
+        self._append(_LogEl(OP_LOG_STATE))
 
     def pop_back(self) -> None:
         if not self._elements:
             raise RuntimeError("ERROR: called `pop_back()` on a `QuantumCircuit` instance with no circuit elements.\n")
         self._elements.pop()
+        self._version += 1
 
     # -- flattening ---------------------------------------------------------------------------------
     def parameter_order(self) -> list[str]:
@@ -425,6 +432,16 @@ class QuantumCircuit:
             if data.value is None:
                 raise RuntimeError("ERROR: cannot perform simulation with an uninitialized parameter value.\n")
             values[pid] = data.value
+        # a straight-line circuit flattened before with the same elements and parameter order: only the
+        # parameterised angles change (an optimiser's next call)
+        key = (self._version, len(self._elements), tuple(order))
+        cache = self._flat_cache
+        if cache is not None and cache[0] == key:
+            _, base, where, pids, preds_c = cache
+            arr = base.copy()
+            if len(where):
+                arr["angle"][where] = [values[pid] for pid in pids]
+            return arr, preds_c.copy(), order
         ops: list[tuple] = []
         preds: list[int] = []
         self._flatten_into(ops, preds, values, slot)
@@ -438,7 +455,12 @@ class QuantumCircuit:
             arr["aux"] = aux
             arr["angle"] = angle
             arr["u"] = u
-        return arr, np.asarray(preds, dtype=np.int32), order
+        preds_a = np.asarray(preds, dtype=np.int32)
+        if all(isinstance(el, _GateEl) for el in self._elements):
+            where = np.array([i for i, el in enumerate(self._elements) if el.param is not None], dtype=np.int64)
+            pids = [self._elements[i].param for i in where]
+            self._flat_cache = (key, arr.copy(), where, pids, preds_a.copy())
+        return arr, preds_a, order
 
     def _flatten_into(self, ops, preds, values, slot) -> None:
         zero_u = (0.0,) * 8
@@ -481,3 +503,4 @@ def extend_circuit(left: QuantumCircuit, right: QuantumCircuit) -> None:
         raise RuntimeError("ERROR: cannot extend circuits with different numbers of qubits.\n")
     left._merge_params(right)
     left._elements.extend(right._elements)
+    left._version += 1

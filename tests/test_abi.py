@@ -164,3 +164,35 @@ def test_injected_fault_fires_once_after_the_given_calls():
     finally:
         lib.kq_debug_set_faults(0, 0, 0)
         lib.kq_set_fusion_autotune(1)
+
+
+def test_flatten_cache_follows_parameters_and_elements():
+    """flatten() of a straight-line circuit reuses its arrays for the same elements and parameter order
+    and refills only the parameterised angles; new values, appended and popped elements and another
+    parameter order all give what a fresh flatten gives."""
+    import numpy as np
+    import kettle_amd as ka
+    from kettle_amd import workloads
+
+    w = workloads.vqe(n=6, layers=3, n_random=4, seed=3)
+    c = w.circuit
+
+    def fresh(order=None):
+        c._flat_cache = None
+        return c.flatten(order)[0]
+
+    first = c.flatten(w.params)[0]
+    assert np.array_equal(first, fresh(w.params))
+    rng = np.random.default_rng(5)
+    for pid in w.params:
+        c.set_parameter_value(pid, float(rng.uniform(-3, 3)))
+    again = c.flatten(w.params)[0]
+    assert not np.array_equal(again["angle"], first["angle"]) and np.array_equal(again, fresh(w.params))
+    c.add_rx_gate(2, 0.25)
+    grown = c.flatten(w.params)[0]
+    assert len(grown) == len(first) + 1 and np.array_equal(grown, fresh(w.params))
+    c.pop_back()
+    assert np.array_equal(c.flatten(w.params)[0], fresh(w.params))
+    rev = list(reversed(w.params))
+    assert np.array_equal(c.flatten(rev)[0], fresh(rev))
+    assert not np.array_equal(c.flatten(rev)[0]["aux"], c.flatten(w.params)[0]["aux"])
