@@ -275,9 +275,10 @@ def batch_section(n_instances: int, reps: int, plain_sample: int) -> dict:
         b.run(use_graph=True)                 # captures the graph
         log(f"batch: {n_instances} instances, forward precision {forward_precision}: setup + first runs "
             f"{time.perf_counter() - t0:.1f} s")
+        thetas = [inst.theta for inst in insts]
         t0 = time.perf_counter()
         for i in range(reps):
-            b.set_parameters(i % n_instances, insts[i % n_instances].theta)   # an optimiser step touches instances
+            b.set_all_parameters(thetas)   # an optimiser step: every instance's programs are lowered again
             r = b.run(use_graph=True)
         t = (time.perf_counter() - t0) / reps
         info = b.info()

@@ -285,6 +285,9 @@ int kq_batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops,
                  size_t n_strings, const uint32_t* term_offsets, const uint32_t* term_qubits,
                  const uint8_t* term_kinds, const uint8_t* phases, const double* coeffs, size_t* index);
 int kq_batch_set_parameters(kq_batch* b, size_t index, const double* values);
+/* Every instance's parameters at once (an optimiser step): values holds them in add order, n_params
+ * per instance back to back.  The next run lowers the changed instances again on host threads. */
+int kq_batch_set_all_parameters(kq_batch* b, const double* values);
 int kq_batch_run(kq_batch* b, int use_graph, double* energies, double* grads, double* drift);
 int kq_batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params_total, size_t* device_bytes,
                   size_t* n_launches);

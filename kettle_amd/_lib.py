@@ -120,6 +120,7 @@ _SIGNATURES = {
     "kq_batch_add": (c_int, [c_void_p, c_uint32, c_void_p, c_size_t, c_size_t, c_size_t, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p, POINTER(c_size_t)]),
     "kq_batch_set_parameters": (c_int, [c_void_p, c_size_t, c_void_p]),
+    "kq_batch_set_all_parameters": (c_int, [c_void_p, c_void_p]),
     "kq_batch_run": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "kq_batch_info": (c_int, [c_void_p, POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_batch_set_drift_recompute": (c_int, [c_void_p, c_double]),

@@ -79,6 +79,17 @@ class EnergyGradientBatch:
             raise ValueError(f"instance {index} has {self._n_params[index]} parameters, got {v.size}")
         check(load().kq_batch_set_parameters(self._h, int(index), ptr(v)))
 
+    def set_all_parameters(self, values) -> None:
+        """Every instance's parameters (an optimiser step): one array per instance, in add order."""
+        parts = [np.asarray(v, dtype=np.float64).ravel() for v in values]
+        if len(parts) != len(self._n_params):
+            raise ValueError(f"{len(self._n_params)} instances, got {len(parts)} parameter arrays")
+        for i, (v, p) in enumerate(zip(parts, self._n_params)):
+            if v.size != p:
+                raise ValueError(f"instance {i} has {p} parameters, got {v.size}")
+        flat = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros(0))
+        check(load().kq_batch_set_all_parameters(self._h, ptr(flat)))
+
     def run(self, use_graph: bool = True) -> BatchResult:
         n = len(self._n_params)
         energies = np.zeros(n)

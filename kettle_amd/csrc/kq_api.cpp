@@ -542,6 +542,16 @@ int kq_batch_set_parameters(kq_batch* b, size_t index, const double* values)
     });
 }
 
+int kq_batch_set_all_parameters(kq_batch* b, const double* values)
+{
+    return guarded([&] {
+        if (!b) {
+            throw InvalidArg {"batch: null handle"};
+        }
+        batch_set_all_parameters(b, values);
+    });
+}
+
 int kq_batch_run(kq_batch* b, int use_graph, double* energies, double* grads, double* drift)
 {
     return guarded([&] {

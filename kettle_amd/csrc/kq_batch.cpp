@@ -17,9 +17,13 @@
 // by the single-state checkpointed adjoint in complex128 (on a work state per qubit count, kept by
 // the batch), and that energy and gradient replace the batched ones.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kq_host.hpp"
@@ -35,6 +39,7 @@ struct BatchInstance
     std::vector<kq_op> ops;
     size_t n_params = 0;
     std::vector<double> params;
+    bool dirty = true;                 // parameters changed since its programs were last written
     std::vector<PauliGroup> groups;
     std::vector<PauliZ> strings;
     // plan
@@ -45,6 +50,9 @@ struct BatchInstance
     // lowered programs and their byte offsets in the program arena
     std::vector<TileProgram> fwd_prog, bwd_prog;
     std::vector<size_t> fwd_off, bwd_off;
+    size_t fwd_end = 0, bwd_end = 0;   // end of the last forward / reverse program
+    // per reverse pass, per derivative slot: |F|^2, F the pivots divided out of the gates before it
+    std::vector<std::vector<double>> bwd_dscale;
     uint64_t amp_off = 0;              // offset of psi / lambda in the arenas (amplitudes)
     uint64_t n_amps = 0;
     size_t grad_off = 0;               // first gradient entry of the instance in the output
@@ -69,6 +77,15 @@ struct BatchRound
     size_t row_base = 0;               // first global row of the round
 };
 
+// one derivative slot of a reverse pass: its instance, gate, pass and slot
+struct BatchRow
+{
+    uint32_t inst;
+    uint32_t gate;
+    uint32_t pass;
+    uint32_t slot;
+};
+
 }  // namespace kq
 
 struct kq_batch
@@ -81,7 +98,7 @@ struct kq_batch
     bool coeffs_dirty = true;
     // plan-wide tables
     std::vector<kq::BatchRound> rounds;
-    std::vector<std::pair<uint32_t, uint32_t>> rows;   // global derivative row -> (instance, gate index)
+    std::vector<kq::BatchRow> rows;     // global derivative rows
     uint64_t total_amps = 0;
     uint64_t part_doubles = 0;          // largest adjoint round's partials
     uint32_t pauli_blocks = 0;
@@ -104,8 +121,8 @@ struct kq_batch
     double* slot_values = nullptr;
     double* energies = nullptr;
     double* drift = nullptr;
-    // host staging of the program arena
-    std::vector<unsigned char> prog_host;
+    // host staging of the program arena (pinned)
+    unsigned char* prog_host = nullptr;
     // captured launch sequence
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -212,11 +229,14 @@ void lower_instance(BatchInstance& in, int precision, int fwd_precision)
     for (size_t p = 0; p < in.fwd.size(); ++p) {
         patch(in.fwd[p], recs);
         lower_tile_pass(in.fwd[p], fwd_precision, in.nv, in.fwd_prog[p]);
+        normalise_program(in.fwd_prog[p], fwd_precision, nullptr);
     }
     in.bwd_prog.resize(in.bwd.size());
+    in.bwd_dscale.resize(in.bwd.size());
     for (size_t p = 0; p < in.bwd.size(); ++p) {
         patch(in.bwd[p], inv);
         lower_tile_pass(in.bwd[p], precision, in.nv, in.bwd_prog[p], &dptr);
+        normalise_program(in.bwd_prog[p], precision, &in.bwd_dscale[p]);
     }
 }
 
@@ -237,6 +257,10 @@ void release(kq_batch* b)
             (void) hipFree(*p);
             *p = nullptr;
         }
+    }
+    if (b->prog_host) {
+        (void) hipHostFree(b->prog_host);
+        b->prog_host = nullptr;
     }
     for (double** p : {&b->part, &b->slot_values, &b->energies, &b->drift}) {
         if (*p) {
@@ -277,10 +301,12 @@ void build(kq_batch* b)
             in.fwd_off.push_back(prog);
             prog += program_bytes(tp);
         }
+        in.fwd_end = prog;
         for (const TileProgram& tp : in.bwd_prog) {
             in.bwd_off.push_back(prog);
             prog += program_bytes(tp);
         }
+        in.bwd_end = prog;
         in.pauli_off = pauli;
         pauli += align256(in.groups.size() * sizeof(PauliGroup)) + align256(in.strings.size() * sizeof(PauliZ));
         in.grad_off = grad;
@@ -367,7 +393,7 @@ void build(kq_batch* b)
                     for (size_t sl = 0; sl < tp.derivs.size(); ++sl) {
                         roff.push_back(part + sl * it.n_part);
                         rcnt.push_back(it.n_part);
-                        b->rows.emplace_back(ii, in.bwd_gate[tp.deriv_src[sl]]);
+                        b->rows.push_back({ii, in.bwd_gate[tp.deriv_src[sl]], static_cast<uint32_t>(r), static_cast<uint32_t>(sl)});
                     }
                     part += tp.derivs.size() * it.n_part;
                     blocks += nb;
@@ -465,25 +491,89 @@ void build(kq_batch* b)
     }
     KQ_HIP_CHECK(hipMemcpyAsync(b->pauli, ph.data(), ph.size(), hipMemcpyHostToDevice, b->stream));
     KQ_HIP_CHECK(hipStreamSynchronize(b->stream));
-    b->prog_host.assign(b->prog_bytes, 0);
+    KQ_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&b->prog_host), std::max<size_t>(b->prog_bytes, 256), hipHostMallocDefault));
+    std::memset(b->prog_host, 0, b->prog_bytes);
     b->planned = true;
     b->coeffs_dirty = true;
 }
 
+// Writes the pass programs of the instances whose parameters changed (every instance after a build)
+// into the pinned staging arena and copies their byte ranges to the device.  Re-lowering runs on up to
+// 16 host threads, one instance at a time each.
 void upload_programs(kq_batch* b, bool relower)
 {
-    for (BatchInstance& in : b->inst) {
+    std::vector<uint32_t> todo;
+    for (uint32_t i = 0; i < b->inst.size(); ++i) {
+        if (!relower || b->inst[i].dirty) {
+            todo.push_back(i);
+        }
+    }
+    const auto one = [&](BatchInstance& in) {
         if (relower) {
             lower_instance(in, b->precision, b->fwd_precision);
         }
+        const auto put = [&](const TileProgram& tp, size_t off, size_t end) {
+            // new angles change a program's values and variants, never its size
+            if (program_bytes(tp) != end - off) {
+                throw InvalidArg {"batch: a re-lowered pass program changed size"};
+            }
+            check_tile_records(tp);
+            write_program(tp, b->prog_host + off);
+        };
         for (size_t p = 0; p < in.fwd_prog.size(); ++p) {
-            write_program(in.fwd_prog[p], b->prog_host.data() + in.fwd_off[p]);
+            put(in.fwd_prog[p], in.fwd_off[p], p + 1 < in.fwd_off.size() ? in.fwd_off[p + 1] : in.fwd_end);
         }
         for (size_t p = 0; p < in.bwd_prog.size(); ++p) {
-            write_program(in.bwd_prog[p], b->prog_host.data() + in.bwd_off[p]);
+            put(in.bwd_prog[p], in.bwd_off[p], p + 1 < in.bwd_off.size() ? in.bwd_off[p + 1] : in.bwd_end);
+        }
+    };
+    std::atomic<size_t> next {0};
+    std::exception_ptr err;
+    std::mutex err_mu;
+    const auto work = [&] {
+        for (size_t j = next++; j < todo.size(); j = next++) {
+            try {
+                one(b->inst[todo[j]]);
+            }
+            catch (...) {
+                std::lock_guard<std::mutex> lk(err_mu);
+                if (!err) {
+                    err = std::current_exception();
+                }
+                next = todo.size();
+            }
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t workers = std::min<size_t>({todo.size() / 8 + 1, hw, 16});
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < workers; ++w) {
+        pool.emplace_back(work);
+    }
+    work();
+    for (auto& t : pool) {
+        t.join();
+    }
+    if (err) {
+        std::rethrow_exception(err);
+    }
+    // an instance's programs are contiguous: one copy per changed instance, or one for the arena
+    if (todo.size() == b->inst.size()) {
+        KQ_HIP_CHECK(hipMemcpyAsync(b->prog, b->prog_host, b->prog_bytes, hipMemcpyHostToDevice, b->stream));
+    }
+    else {
+        for (const uint32_t i : todo) {
+            const BatchInstance& in = b->inst[i];
+            const size_t begin = !in.fwd_off.empty() ? in.fwd_off[0] : (!in.bwd_off.empty() ? in.bwd_off[0] : in.bwd_end);
+            if (in.bwd_end > begin) {
+                KQ_HIP_CHECK(hipMemcpyAsync(static_cast<unsigned char*>(b->prog) + begin, b->prog_host + begin,
+                                            in.bwd_end - begin, hipMemcpyHostToDevice, b->stream));
+            }
         }
     }
-    KQ_HIP_CHECK(hipMemcpyAsync(b->prog, b->prog_host.data(), b->prog_bytes, hipMemcpyHostToDevice, b->stream));
+    for (const uint32_t i : todo) {
+        b->inst[i].dirty = false;
+    }
     b->coeffs_dirty = false;
 }
 
@@ -596,6 +686,25 @@ void batch_set_parameters(kq_batch* b, size_t index, const double* values)
         throw InvalidArg {"batch: null parameter values"};
     }
     in.params.assign(values, values + in.n_params);
+    in.dirty = true;
+    b->coeffs_dirty = true;
+}
+
+void batch_set_all_parameters(kq_batch* b, const double* values)
+{
+    size_t total = 0;
+    for (const BatchInstance& in : b->inst) {
+        total += in.n_params;
+    }
+    if (total > 0 && values == nullptr) {
+        throw InvalidArg {"batch: null parameter values"};
+    }
+    size_t o = 0;
+    for (BatchInstance& in : b->inst) {
+        in.params.assign(values + o, values + o + in.n_params);
+        in.dirty = true;
+        o += in.n_params;
+    }
     b->coeffs_dirty = true;
 }
 
@@ -691,9 +800,11 @@ void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, dou
         std::fill(grads, grads + total, 0.0);
         // rows are in a fixed order, so the sums over gates sharing a parameter are deterministic
         for (size_t r = 0; r < b->rows.size(); ++r) {
-            const BatchInstance& in = b->inst[b->rows[r].first];
-            const kq_op& op = in.ops[b->rows[r].second];
-            grads[in.grad_off + static_cast<size_t>(op.aux)] += 2.0 * sv[r];
+            const BatchRow& row = b->rows[r];
+            const BatchInstance& in = b->inst[row.inst];
+            const kq_op& op = in.ops[row.gate];
+            // the pass carries psi / F and lambda / F: the slot holds Re <lambda| Q |psi> / |F|^2
+            grads[in.grad_off + static_cast<size_t>(op.aux)] += 2.0 * sv[r] * in.bwd_dscale[row.pass][row.slot];
         }
     }
     b->last_recomputed = 0;

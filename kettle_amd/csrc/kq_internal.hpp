@@ -114,6 +114,13 @@ constexpr uint32_t TV_DGEN_BASE = TV_END;
 constexpr uint32_t TV_DGEN_END = TV_DGEN_BASE + 4 * 20;
 // the generator a derivative matrix is (0..3, as above), or -1
 int derivative_generator(const double m[8]);
+// Uncontrolled one-qubit gates with their pivot entry divided out, for the interpreted tile kernel:
+// variant TV_NORM_BASE + form * 4 + P on register position P; forms 0..3 real rotation / reflection
+// (t in the first coefficient: [[1, t], [-t, 1]], [[1, t], [t, -1]], [[t, 1], [-1, t]], [[t, 1], [1, -t]]),
+// 4..5 axis rotation ([[1, i t], [i t, 1]], [[t, i], [i, t]]), 6..7 complex ([[1, x], [y, z]],
+// [[x, 1], [y, z]], x, y, z in the first six).
+constexpr uint32_t TV_NORM_BASE = TV_DGEN_END;
+constexpr uint32_t TV_NORM_END = TV_NORM_BASE + 8 * 4;
 
 // one diagonal gate inside a merged run: its conditions, selection and the two diagonal entries
 struct TileTerm
@@ -188,6 +195,14 @@ struct TileProgram
     std::vector<uint8_t> out_freepos; // out of place: destination bit of each freepos entry (empty: in place)
 };
 
+// Rewrites the uncontrolled one-qubit records of a lowered pass (for the interpreted tile kernel) in
+// the TV_NORM forms, a diagonal as d0 = 1 with d1 / d0, and appends to its last step one uniform record that
+// multiplies every slot by F, the product of the pivots taken out; inside the pass the vectors are the
+// true ones divided by F, so a derivative record reduces Re <w|Q|v> / |F|^2: dscale (optional) receives
+// |F|^2 for each derivative slot.  Always appends exactly one record (F = 1 when nothing is rewritten),
+// so a program's size does not depend on its angles.
+void normalise_program(TileProgram& tp, int precision, std::vector<double>* dscale);
+
 // Lower a planned pass into the tile kernel's program (coefficients rounded to the precision).
 // deriv_of_src (adjoint passes): for input record i, its derivative dG (or null); a gate with one
 // gets TileGate::pad = derivative index + 1 and its derivative record the next partial slot.
@@ -204,6 +219,8 @@ void merge_diagonal_runs(const TileGate* gates, uint32_t count, int precision, s
 // Default LDS placement of a tile: entry of local index l (its own inverse; linear over GF(2)).
 inline uint32_t lds_swizzle(uint32_t l) { return l ^ (((l >> 4) ^ (l >> 8)) & 15u); }
 
+// throws unless every step and gate record of the program is one the tile kernel runs
+void check_tile_records(const TileProgram& prog);
 uint32_t tile_pass_blocks(const TileProgram& prog);
 uint32_t tile_pass_waves(const TileProgram& prog);
 
@@ -555,6 +572,7 @@ void batch_add(kq_batch* b, uint32_t n_qubits, const kq_op* ops, size_t n_ops, s
                const uint32_t* off, const uint32_t* q, const uint8_t* k, const uint8_t* phases, const double* coeffs,
                size_t* index);
 void batch_set_parameters(kq_batch* b, size_t index, const double* values);
+void batch_set_all_parameters(kq_batch* b, const double* values);
 void batch_run(kq_batch* b, bool use_graph, double* energies, double* grads, double* drift);
 void batch_info(const kq_batch* b, size_t* n_instances, size_t* n_params, size_t* device_bytes, size_t* n_launches);
 void batch_set_drift_recompute(kq_batch* b, double drift_tol);

@@ -939,17 +939,34 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_batch(const Amp<R>* __restrict_
     using GZ = const __attribute__((address_space(1))) PauliZ;
     const auto* groups = (const PauliGroup*) (GGroup*) it.groups;
     const auto* zs = (const PauliZ*) (GZ*) it.zs;
-    const uint64_t end = min(it.n_amps, static_cast<uint64_t>(local + 1) * BATCH_AMPS);
+    // the block's BATCH_AMPS = PAULI_ROWS * BLOCK rows: thread t takes rows t + j * BLOCK, their gathers
+    // issued together per group
+    static_assert(BATCH_AMPS == PAULI_ROWS * BLOCK, "one row set per thread and block");
+    uint64_t idx[PAULI_ROWS];
+    bool in[PAULI_ROWS];
+#pragma unroll
+    for (int j = 0; j < PAULI_ROWS; ++j) {
+        idx[j] = static_cast<uint64_t>(local) * BATCH_AMPS + threadIdx.x + static_cast<uint64_t>(j) * BLOCK;
+        in[j] = idx[j] < it.n_amps;
+    }
+    double hr[PAULI_ROWS], hi_[PAULI_ROWS];
+    if (it.n_amps <= (uint64_t {1} << 32)) {
+        h_apply_rows<R, true>(a, idx, in, groups, it.n_groups, zs, hr, hi_);
+    }
+    else {
+        h_apply_rows<R, false>(a, idx, in, groups, it.n_groups, zs, hr, hi_);
+    }
     double e = 0.0;
-    for (uint64_t i = static_cast<uint64_t>(local) * BATCH_AMPS + threadIdx.x; i < end; i += BLOCK) {
-        double hr, hi_;
-        h_apply_row<R>(a, i, groups, it.n_groups, zs, hr, hi_);
-        const Amp<R> v = a[i];
-        e += static_cast<double>(v.x) * hr + static_cast<double>(v.y) * hi_;
-        Amp<RO> w;
-        w.x = static_cast<RO>(hr);
-        w.y = static_cast<RO>(hi_);
-        b[i] = w;
+#pragma unroll
+    for (int j = 0; j < PAULI_ROWS; ++j) {
+        if (in[j]) {
+            const Amp<R> v = a[idx[j]];
+            e += static_cast<double>(v.x) * hr[j] + static_cast<double>(v.y) * hi_[j];
+            Amp<RO> w;
+            w.x = static_cast<RO>(hr[j]);
+            w.y = static_cast<RO>(hi_[j]);
+            b[idx[j]] = w;
+        }
     }
     const double2 r = block_sum2(e, 0.0);
     if (threadIdx.x == 0) {

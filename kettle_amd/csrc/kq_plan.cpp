@@ -5,6 +5,8 @@
 // theta / 2.0 for RX/RY/RZ and of theta for P (operations.cpp:111-173), M_SQRT1_2 and 0.5
 // literals for H, T, SX (operations.cpp:13-109).  The device never evaluates a transcendental.
 #include <algorithm>
+#include <array>
+#include <complex>
 #include <bit>
 #include <cmath>
 #include <cstring>
@@ -1166,7 +1168,116 @@ void set_term_coefs(TileTerm& term, int precision, const double d[4])
     }
 }
 
+// a record's coefficients from doubles (complex64 records hold 8 packed floats)
+void store_values(TileGate& g, int precision, const std::array<double, 8>& v)
+{
+    std::memset(g.m, 0, sizeof g.m);
+    if (precision == KQ_C64) {
+        float f[8];
+        for (int i = 0; i < 8; ++i) {
+            f[i] = static_cast<float>(v[static_cast<size_t>(i)]);
+        }
+        std::memcpy(g.m, f, sizeof f);
+    }
+    else {
+        std::memcpy(g.m, v.data(), sizeof g.m);
+    }
+}
 }  // namespace
+
+void normalise_program(TileProgram& tp, int precision, std::vector<double>* dscale)
+{
+    if (dscale) {
+        dscale->assign(tp.derivs.size(), 1.0);
+    }
+    if (tp.steps.empty()) {
+        throw InvalidArg {"normalise: program without steps"};
+    }
+    // the vectors grow by 1 / |F| inside the pass: keep F far from underflow (products of two
+    // complex64 vectors by 1 / |F|^2 stay inside the FP32 range)
+    const double floor = precision == KQ_C64 ? 1.0e-12 : 1.0e-200;
+    const auto diag_pivot_of = [](double diag, double off) { return std::abs(diag) >= 1.0e-6 * std::abs(off); };
+    std::complex<double> factor {1.0, 0.0};
+    const auto fits = [&](std::complex<double> pivot) { return std::abs(pivot) > 0.0 && std::abs(factor * pivot) >= floor; };
+    for (const TileStep& st : tp.steps) {
+        for (uint32_t gi = st.g_begin; gi < st.g_begin + st.g_count; ++gi) {
+            TileGate& g = tp.gates[gi];
+            if (g.pad != 0 && dscale) {
+                (*dscale)[g.pad - 1] = std::norm(factor);   // |F|^2 where the derivative is taken
+            }
+            if (g.code >= TV_UNIFORM_BASE || g.lane_ctrl != 0 || g.glb_ctrl != 0 || g.code % 5 != 0) {
+                continue;
+            }
+            const uint32_t kind = g.code / 20, P = (g.code / 5) % 4;
+            std::array<double, 8> m;
+            coef_values(g, precision, m.data());
+            std::array<double, 8> out {};
+            uint32_t code = 0;
+            std::complex<double> pivot;
+            if (kind == TK_REAL) {
+                const bool rotation = m[6] == m[0] && m[4] == -m[2];
+                const bool reflection = m[6] == -m[0] && m[4] == m[2];
+                if (!rotation && !reflection) {
+                    continue;
+                }
+                const bool dp = diag_pivot_of(m[0], m[2]);
+                pivot = dp ? m[0] : m[2];
+                out[0] = (dp ? m[2] : m[0]) / pivot.real();
+                code = TV_NORM_BASE + ((dp ? 0u : 2u) + (rotation ? 0u : 1u)) * 4 + P;
+            }
+            else if (kind == TK_AXIS) {
+                if (m[6] != m[0] || m[5] != m[3]) {
+                    continue;
+                }
+                const bool dp = diag_pivot_of(m[0], m[3]);
+                pivot = dp ? m[0] : m[3];
+                out[0] = (dp ? m[3] : m[0]) / pivot.real();
+                code = TV_NORM_BASE + (dp ? 4u : 5u) * 4 + P;
+            }
+            else if (kind == TK_DIAG) {
+                pivot = {m[0], m[1]};
+                if (std::abs(pivot) < 0.5) {
+                    continue;
+                }
+                const std::complex<double> d = std::complex<double> {m[6], m[7]} / pivot;
+                out[0] = 1.0;
+                out[6] = d.real();
+                out[7] = d.imag();
+                code = tile_code(TK_DIAG1, P, 0);
+            }
+            else if (kind == TK_DENSE) {
+                const std::complex<double> m00 {m[0], m[1]}, m01 {m[2], m[3]}, m10 {m[4], m[5]}, m11 {m[6], m[7]};
+                const bool dp = diag_pivot_of(std::abs(m00), std::abs(m01));
+                pivot = dp ? m00 : m01;
+                if (std::abs(pivot) == 0.0) {
+                    continue;
+                }
+                const std::complex<double> x = (dp ? m01 : m00) / pivot, y = m10 / pivot, z = m11 / pivot;
+                out = {x.real(), x.imag(), y.real(), y.imag(), z.real(), z.imag(), 0.0, 0.0};
+                code = TV_NORM_BASE + (dp ? 6u : 7u) * 4 + P;
+            }
+            else {
+                continue;   // swaps stay renames, d0 = 1 diagonals are normalised already
+            }
+            if (!fits(pivot)) {
+                continue;
+            }
+            factor *= pivot;
+            g.code = code;
+            store_values(g, precision, out);
+        }
+    }
+    // the pivots' product back into every slot: a uniform diagonal record at the end of the last step
+    TileStep& last = tp.steps.back();
+    if (last.g_begin + last.g_count != tp.gates.size()) {
+        throw InvalidArg {"normalise: the last step does not end the record list"};
+    }
+    TileGate f {};
+    f.code = TV_UNIFORM_BASE;   // every slot, selection bits 0: the factor in the first entry
+    store_values(f, precision, {factor.real(), factor.imag(), 0.0, 0.0, 0.0, 0.0, factor.real(), factor.imag()});
+    tp.gates.push_back(f);
+    ++last.g_count;
+}
 
 // Merge diagonal gates of one step.  Diagonal gates commute with each other, and with a non-diagonal
 // gate unless it targets their target or register-control position, so every diagonal gate without

@@ -106,6 +106,26 @@ __device__ __forceinline__ void apply_gate(Cx<R> (&v)[16], uint32_t code, const 
     }
 }
 
+#define KQ_NORM_CASES(P) \
+    case 0 * 4 + (P): orth<R, P, 0>(v, m.m[0]); break; \
+    case 1 * 4 + (P): orth<R, P, 1>(v, m.m[0]); break; \
+    case 2 * 4 + (P): orth<R, P, 2>(v, m.m[0]); break; \
+    case 3 * 4 + (P): orth<R, P, 3>(v, m.m[0]); break; \
+    case 4 * 4 + (P): axis<R, P, 0>(v, m.m[0]); break; \
+    case 5 * 4 + (P): axis<R, P, 1>(v, m.m[0]); break; \
+    case 6 * 4 + (P): dense1<R, P, 0>(v, m.m[0], m.m[1], m.m[2], m.m[3], m.m[4], m.m[5]); break; \
+    case 7 * 4 + (P): dense1<R, P, 1>(v, m.m[0], m.m[1], m.m[2], m.m[3], m.m[4], m.m[5]); break;
+
+// an uncontrolled one-qubit gate with its pivot divided out (TV_NORM_BASE variants): fp = form * 4 + P
+template <typename R>
+__device__ __forceinline__ void apply_norm(Cx<R> (&v)[16], uint32_t fp, const Coef<R>& m)
+{
+    switch (fp) {
+        KQ_NORM_CASES(0) KQ_NORM_CASES(1) KQ_NORM_CASES(2) KQ_NORM_CASES(3)
+        default: __builtin_unreachable();
+    }
+}
+
 template <typename R>
 __device__ __forceinline__ void apply_uniform(Cx<R> (&v)[16], uint32_t c, R fr, R fi)
 {
@@ -404,7 +424,10 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                 if (!lane_ok) {
                     continue;   // a control on a thread bit is 0 for this lane
                 }
-                if (code >= TV_MERGE_BASE) {
+                if (code >= TV_NORM_BASE) {
+                    apply_norm<R>(v, code - TV_NORM_BASE, m);
+                }
+                else if (code >= TV_MERGE_BASE) {
                     const uint32_t key = code - TV_MERGE_BASE;
                     R f0r, f0i, f1r, f1i;
                     merged_factors<R>(terms, g.lane_sel, static_cast<uint32_t>(g.glb_sel), key >= 20, base, tid, f0r, f0i,
@@ -422,12 +445,13 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
             }
             else {
                 const bool ok = glb_ok && lane_ok;
+                const bool norm = code >= TV_NORM_BASE;
                 R fr = 0, fi = 0, f0r = 0, f0i = 0;
-                if (code >= TV_MERGE_BASE) {
+                if (!norm && code >= TV_MERGE_BASE) {
                     merged_factors<R>(terms, g.lane_sel, static_cast<uint32_t>(g.glb_sel), code - TV_MERGE_BASE >= 20, base,
                                       tid, f0r, f0i, fr, fi);
                 }
-                else if (code >= TV_UNIFORM_BASE) {
+                else if (!norm && code >= TV_UNIFORM_BASE) {
                     uniform_factor(fr, fi);
                 }
                 if (g.pad != 0) {
@@ -453,7 +477,10 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                     }
                 }
                 if (ok) {
-                    if (code >= TV_MERGE_BASE) {
+                    if (norm) {
+                        apply_norm<R>(v, code - TV_NORM_BASE, m);
+                    }
+                    else if (code >= TV_MERGE_BASE) {
                         apply_merged<R>(v, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
                     }
                     else if (code >= TV_UNIFORM_BASE) {
@@ -464,7 +491,10 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                     }
                 }
                 if (ok) {
-                    if (code >= TV_MERGE_BASE) {
+                    if (norm) {
+                        apply_norm<R>(w, code - TV_NORM_BASE, m);
+                    }
+                    else if (code >= TV_MERGE_BASE) {
                         apply_merged<R>(w, code - TV_MERGE_BASE, f0r, f0i, fr, fi);
                     }
                     else if (code >= TV_UNIFORM_BASE) {
@@ -565,22 +595,7 @@ TileArgs check_program(const kq_state* s, const TileProgram& prog)
     if (prog.freepos.size() > static_cast<size_t>(MAX_FREE_Q) || prog.freepos.size() + prog.k != s->nvec) {
         throw InvalidArg {"tile pass: free qubits do not complete the state"};
     }
-    for (const TileStep& st : prog.steps) {
-        if (st.layout >= prog.layouts.size() || st.g_begin + st.g_count > prog.gates.size()) {
-            throw InvalidArg {"tile pass: step out of range"};
-        }
-    }
-    for (const TileGate& g : prog.gates) {
-        if (g.pad > prog.derivs.size()) {
-            throw InvalidArg {"tile pass: derivative index out of range"};
-        }
-        if (g.code >= TV_END || (g.code < TV_UNIFORM_BASE && g.code % 5 == (g.code / 5) % 4 + 1)) {
-            throw InvalidArg {"tile pass: unknown gate variant"};
-        }
-        if (g.code >= TV_MERGE_BASE && (static_cast<uint64_t>(g.lane_sel) + g.glb_sel > prog.terms.size())) {
-            throw InvalidArg {"tile pass: merged run out of range"};
-        }
-    }
+    check_tile_records(prog);
     TileArgs d {};
     d.n_thr_bits = prog.n_thr_bits;
     d.n_steps = static_cast<int>(prog.steps.size());
@@ -601,6 +616,27 @@ TileArgs check_program(const kq_state* s, const TileProgram& prog)
 }
 
 }  // namespace
+
+void check_tile_records(const TileProgram& prog)
+{
+    for (const TileStep& st : prog.steps) {
+        if (st.layout >= prog.layouts.size() || st.g_begin + st.g_count > prog.gates.size()) {
+            throw InvalidArg {"tile pass: step out of range"};
+        }
+    }
+    for (const TileGate& g : prog.gates) {
+        if (g.pad > prog.derivs.size()) {
+            throw InvalidArg {"tile pass: derivative index out of range"};
+        }
+        const bool norm = g.code >= TV_NORM_BASE && g.code < TV_NORM_END;
+        if ((g.code >= TV_END && !norm) || (g.code < TV_UNIFORM_BASE && g.code % 5 == (g.code / 5) % 4 + 1)) {
+            throw InvalidArg {"tile pass: unknown gate variant"};
+        }
+        if (!norm && g.code >= TV_MERGE_BASE && (static_cast<uint64_t>(g.lane_sel) + g.glb_sel > prog.terms.size())) {
+            throw InvalidArg {"tile pass: merged run out of range"};
+        }
+    }
+}
 
 uint32_t tile_pass_blocks(const TileProgram& prog) { return 1u << prog.freepos.size(); }
 uint32_t tile_pass_waves(const TileProgram& prog) { return ((1u << prog.n_thr_bits) + 63) / 64; }

@@ -115,6 +115,18 @@ def test_graph_replay_and_parameter_updates():
         one = single(insts[i], ka.KQ_C128)
         assert abs(rep.energies[i] - one.energy) <= 1e-12
         assert np.max(np.abs(rep.gradients[i] - one.gradient)) <= 1e-10
+    # every instance at once (an optimiser step): only the changed programs are written again
+    values = [rng.uniform(-math.pi, math.pi, size=len(inst.theta)) for inst in insts]
+    b.set_all_parameters(values)
+    for inst, v in zip(insts, values):
+        inst.theta = v
+    rep = b.run(use_graph=True)
+    for i in (1, 9, 15):
+        one = single(insts[i], ka.KQ_C128)
+        assert abs(rep.energies[i] - one.energy) <= 1e-12
+        assert np.max(np.abs(rep.gradients[i] - one.gradient)) <= 1e-10
+    with pytest.raises(ValueError):
+        b.set_all_parameters(values[:-1])
     assert b.info()["launches"] > 0
     b.close()
 
