@@ -57,11 +57,17 @@ void ensure_init()
     }
 }
 
-void check_state(const kq_state* s)
+void check_state_lazy(const kq_state* s)
 {
     if (s == nullptr || s->amps == nullptr) {
         throw InvalidArg {"null or destroyed state handle"};
     }
+}
+
+void check_state(const kq_state* s)
+{
+    check_state_lazy(s);
+    materialise_basis(const_cast<kq_state*>(s));
 }
 
 namespace
@@ -115,7 +121,7 @@ void* grow(void*& buf, size_t& have, size_t need)
     return buf;
 }
 
-kq_state* create_state(uint32_t n, int precision, bool dm)
+kq_state* create_state(uint32_t n, int precision, bool dm, bool pending_zero)
 {
     if (precision != KQ_C128 && precision != KQ_C64) {
         throw InvalidArg {"precision must be KQ_C128 or KQ_C64"};
@@ -138,8 +144,13 @@ kq_state* create_state(uint32_t n, int precision, bool dm)
     try {
         s->amps = device_alloc(s->n_amps * amp_bytes(precision));
         KQ_HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-        launch_set_basis(s, 0);
-        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        if (pending_zero) {
+            s->lazy_basis = 0;   // |0...0>, written by the first operation
+        }
+        else {
+            launch_set_basis(s, 0);
+            KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        }
     }
     catch (...) {
         if (s->amps) {
@@ -245,7 +256,7 @@ int kq_sv_create(kq_state** out, uint32_t n_qubits, int precision)
         if (!out) {
             throw InvalidArg {"null output pointer"};
         }
-        *out = create_state(n_qubits, precision, false);
+        *out = create_state(n_qubits, precision, false, true);
     });
 }
 
@@ -255,7 +266,7 @@ int kq_dm_create(kq_state** out, uint32_t n_qubits, int precision)
         if (!out) {
             throw InvalidArg {"null output pointer"};
         }
-        *out = create_state(n_qubits, precision, true);
+        *out = create_state(n_qubits, precision, true, true);
     });
 }
 
@@ -298,7 +309,7 @@ int kq_state_destroy(kq_state* s)
 int kq_state_info(const kq_state* s, uint32_t* n_qubits, int* precision, int* is_density)
 {
     return guarded([&] {
-        check_state(s);
+        check_state_lazy(s);
         if (n_qubits) { *n_qubits = s->n; }
         if (precision) { *precision = s->precision; }
         if (is_density) { *is_density = s->is_dm; }
@@ -319,12 +330,13 @@ int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes)
 int kq_state_upload(kq_state* s, const double* host)
 {
     return guarded([&] {
-        check_state(s);
+        check_state_lazy(s);
         if (!host) {
             throw InvalidArg {"null host buffer"};
         }
         ensure_init();
         s->bit_of.clear();   // overwritten: qubit q on bit q
+        s->lazy_basis = -1;
         if (s->precision == KQ_C128) {
             KQ_HIP_CHECK(hipMemcpy(s->amps, host, s->n_amps * 16, hipMemcpyHostToDevice));
         }
@@ -357,22 +369,22 @@ int kq_state_download(const kq_state* s, double* host)
 int kq_state_set_basis(kq_state* s, uint64_t index)
 {
     return guarded([&] {
-        check_state(s);
+        check_state_lazy(s);
         const uint64_t N = uint64_t {1} << s->n;
         if (index >= N) {
             throw InvalidArg {"basis index out of range"};
         }
         ensure_init();
-        s->bit_of.clear();   // overwritten: qubit q on bit q
-        launch_set_basis(s, s->is_dm ? index + index * N : index);
-        KQ_HIP_CHECK(hipStreamSynchronize(s->stream));
+        // written by the next operation (a gate run's first pass builds it without reading the buffer)
+        s->bit_of.clear();
+        s->lazy_basis = static_cast<int64_t>(s->is_dm ? index + index * N : index);
     });
 }
 
 int kq_state_copy(kq_state* dst, const kq_state* src)
 {
     return guarded([&] {
-        check_state(dst);
+        check_state_lazy(dst);
         check_state(src);
         if (dst->n_amps != src->n_amps || dst->precision != src->precision || dst->is_dm != src->is_dm) {
             throw InvalidArg {"state copy: shapes differ"};
@@ -380,6 +392,7 @@ int kq_state_copy(kq_state* dst, const kq_state* src)
         ensure_init();
         normalise_layout(src);
         dst->bit_of.clear();
+        dst->lazy_basis = -1;
         KQ_HIP_CHECK(hipStreamSynchronize(src->stream));
         launch_copy_kernel(dst, dst->amps, src->amps, dst->n_amps);
         KQ_HIP_CHECK(hipStreamSynchronize(dst->stream));

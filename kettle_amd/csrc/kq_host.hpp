@@ -58,10 +58,15 @@ int guarded(F&& f)
 
 // ---- library and states (kq_api.cpp) ----------------------------------------------------------------
 void ensure_init();                 // selects the library's device (initialises it once)
+// a usable state; a pending basis state (kq_state::lazy_basis) is written first
 void check_state(const kq_state* s);
+// a usable state, a pending basis state left pending (callers that overwrite the state or build it)
+void check_state_lazy(const kq_state* s);
 void* device_alloc(size_t bytes);   // hipMalloc or AllocError
 void* grow(void*& buf, size_t& have, size_t need);   // device buffer of at least `need` bytes
-kq_state* create_state(uint32_t n, int precision, bool dm);
+// pending_zero: |0...0> is left pending (kq_state::lazy_basis) instead of written now; callers that
+// write the buffer directly create states without it
+kq_state* create_state(uint32_t n, int precision, bool dm, bool pending_zero = false);
 
 // ---- gate runs (kq_run.cpp) ---------------------------------------------------------------------------
 // device records of a straight-line list through the fused passes (qubit moves where allowed)
@@ -70,6 +75,8 @@ void run_ops(kq_state* s, const kq_op* ops, size_t n);
 // puts qubit q back on index bit q (after gate passes that moved qubits)
 void normalise_layout(kq_state* s);
 void normalise_layout(const kq_state* s);
+// writes a pending basis state (kq_state::lazy_basis) into the state's buffer
+void materialise_basis(kq_state* s);
 // SWAP triples dropped as relabels: sigma receives the qubit permutation they leave
 std::vector<GateRec> exchanges_as_relabels(const std::vector<GateRec>& recs, uint32_t nvec, std::vector<uint8_t>& sigma);
 PermuteTables permute_tables(const std::vector<uint8_t>& bit_of);

@@ -264,6 +264,10 @@ struct kq_state
     // a second buffer of the state's size: the destination of passes that store out of place (the
     // buffers then trade places); allocated by the first such pass and kept until the state is destroyed
     void* spare = nullptr;
+    // >= 0: the state is the basis vector of this stored index, not yet written to `amps` (a reset
+    // waits for the first operation: a gate run builds its first pass's tiles from the index, anything
+    // else writes the vector first)
+    int64_t lazy_basis = -1;
 };
 
 namespace kq
@@ -339,6 +343,7 @@ int fusion_max_k_for(int precision, uint32_t nvec, size_t lds_bytes);
 // LDS bytes one workgroup may use on the current device (queried at initialisation)
 size_t& device_lds_per_block();
 constexpr size_t TILE_STATIC_LDS = 0;      // static LDS of the tile kernel besides the tile
+constexpr size_t TILE_LDS_ENTRY = 8;       // LDS bytes per amplitude of the tile kernel's remaps (complex128: halves)
 int fusion_max_gates();
 void set_fusion(int max_k, int max_gates, int max_stages, double max_cost);
 double fusion_max_cost();
@@ -531,16 +536,19 @@ uint32_t device_cu_count();
 uint32_t& device_cus();
 bool jit_wanted(const kq_state* s);
 bool jit_available();
-// kernel source of one forward pass; coefs receives its coefficient buffer
-std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs);
+// kernel source of one forward pass; coefs receives its coefficient buffer.  from_basis: the pass
+// builds its input tiles from a basis index (a kernel argument) instead of loading the state
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, bool from_basis = false);
 // compiled kernels for the sources (null where compilation failed); compiles missing ones in parallel
 std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
 // the compiled kernel of a source if it is cached (a cache hit), else null; never compiles
 hipFunction_t jit_lookup(const std::string& source);
 // compile without loading (no device needed): number of failures, last log in *log
 size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
-// out-of-place programs (tp.out_freepos) store into s->spare, which must be allocated
-void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs);
+// out-of-place programs (tp.out_freepos) store into s->spare, which must be allocated; basis: the
+// stored index of the basis state a from_basis kernel builds (null for the others)
+void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs,
+                     const uint64_t* basis = nullptr);
 // kernel source of one adjoint reverse pass over (psi, lambda) with derivative reductions (the records of
 // an adjoint-lowered program); coefs receives its coefficient buffer
 // What the generated kernel of an adjoint reverse pass reads besides its structure: the coefficient

@@ -741,7 +741,7 @@ bool jit_wanted(const kq_state* s)
 }
 
 // Generate the kernel source of one forward tile pass and the layout of its coefficient buffer.
-std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs)
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, bool from_basis)
 {
     coefs.clear();
     const bool c64 = precision == KQ_C64;
@@ -757,8 +757,10 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     // an out-of-place pass (the state's last pass putting every qubit on its final bit) loads from `a`
     // and stores into `dst`, at the destination bits of every tile and outside bit
     const bool oop = !tp.out_freepos.empty();
+    // from_basis: the state is the basis vector |basis> not yet written; the pass builds its tiles from
+    // that index instead of loading them
     o << " void kq_pass(kq::dev::Amp<R>* __restrict__ a, const double* __restrict__ cf, uint32_t n_blocks"
-      << (oop ? ", kq::dev::Amp<R>* __restrict__ dst" : "") << ")\n{\n"
+      << (oop ? ", kq::dev::Amp<R>* __restrict__ dst" : "") << (from_basis ? ", uint64_t basis" : "") << ")\n{\n"
       << "  using namespace kq::dev;\n";
     bool remaps = tp.out_layout != 0;
     for (const TileStep& st : tp.steps) {
@@ -771,6 +773,9 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     // Blocks walk the tiles with a grid stride.  With more tiles than blocks, the next tile's amplitudes
     // are loaded into registers before the current tile's gates run, so HBM reads overlap the arithmetic.
     const bool persistent = tune.blocks_per_cu > 0;
+    if (persistent && from_basis) {
+        throw InvalidArg {"jit: a pass from a basis state has no persistent form"};
+    }
     o << "  const uint32_t tid = threadIdx.x;\n  Cx<R> v[16];\n";
     if (persistent) {
         o << "  const uint64_t thr_in = " << hbm_thread_expr(tp.layouts[0], tp.n_thr_bits) << ";\n"
@@ -922,7 +927,34 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
         return off;
     };
-    if (!persistent) {
+    if (from_basis) {
+        // Every tile but the one holding the basis amplitude is 0, and the pass's gates (linear, acting
+        // within each tile) leave it 0: those workgroups store zeros where the pass stores and end.
+        // The one that holds it builds its tile, amplitude (off ^ slot) 1 at the basis index and 0
+        // elsewhere, and runs the pass.  Nothing is read.
+        uint64_t free_mask = 0;
+        for (const uint8_t q : tp.freepos) {
+            free_mask |= uint64_t {1} << q;
+        }
+        const TileLayout& S = tp.layouts[tp.out_layout];
+        const int ntm = nontemporal_mode(tune.nontemporal, static_cast<uint32_t>(tp.k + tp.freepos.size()), c64 ? 8 : 16);
+        o << "  if ((basis & " << hex64(free_mask) << ") != base) {\n"
+          << "    const uint64_t zoff = " << (oop ? "(" + deposit_expr(tp.out_freepos) + ")" : std::string {"base"}) << " | ("
+          << hbm_thread_expr(S, tp.n_thr_bits) << ");\n"
+          << "    Amp<R> zero;\n    zero.x = R(0);\n    zero.y = R(0);\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    amp_store<" << (ntm > 1 ? "true" : "false") << ">(" << (oop ? "dst" : "a") << " + (zoff ^ "
+              << hex64(S.hbm_slot[r]) << "), zero);\n";
+        }
+        o << "    return;\n  }\n";
+        const TileLayout& L = tp.layouts[0];
+        o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
+        for (int r = 0; r < 16; ++r) {
+            o << "    v[" << r << "] = {((off ^ " << hex64(L.hbm_slot[r]) << ") == basis) ? R(1) : R(0), R(0)};\n";
+        }
+        o << "  }\n";
+    }
+    else if (!persistent) {
         if (tune.priority > 0) {
             o << "  wave_priority<3>();\n";
         }
@@ -1652,7 +1684,7 @@ size_t jit_compile_only(const std::vector<std::string>& sources, std::string* lo
     return failures;
 }
 
-void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs)
+void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs, const uint64_t* basis)
 {
     if (tp.freepos.size() > 31 || tp.freepos.size() + tp.k != s->nvec) {
         throw InvalidArg {"jit pass: bad program shape"};
@@ -1665,10 +1697,14 @@ void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const
     void* dst = s->spare;
     const double* cf = d_coefs;
     uint32_t n_blocks = tile_pass_blocks(tp);
-    void* args3[] = {&amps, &cf, &n_blocks};
-    void* args4[] = {&amps, &cf, &n_blocks, &dst};
-    void** args = oop ? args4 : args3;
+    uint64_t b = basis ? *basis : 0;
+    void* args3[] = {&amps, &cf, &n_blocks, &b};
+    void* args4[] = {&amps, &cf, &n_blocks, &dst, &b};
+    void** args = oop ? args4 : args3;   // the basis index is read only by kernels built from a basis
     const JitTuning tune = jit_tuning();
+    if (basis && tune.blocks_per_cu > 0) {
+        throw InvalidArg {"jit pass: a pass from a basis state has no persistent form"};
+    }
     unsigned grid = n_blocks;
     if (tune.blocks_per_cu > 0) {
         grid = static_cast<unsigned>(std::min<uint64_t>(n_blocks, uint64_t {device_cu_count()} * tune.blocks_per_cu));

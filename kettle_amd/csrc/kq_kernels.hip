@@ -1196,6 +1196,7 @@ void launch_gate_sweep(kq_state* s, const GateRec& g)
 
 void launch_set_basis(kq_state* s, uint64_t index)
 {
+    s->lazy_basis = -1;   // written now: no pending basis state may overwrite it later
     const unsigned blocks = grid_for(s->n_amps, 4);
     KQ_DISPATCH(s, (k_set_basis<double><<<blocks, BLOCK, 0, s->stream>>>(static_cast<double2*>(s->amps), s->n_amps, index)),
                 (k_set_basis<float><<<blocks, BLOCK, 0, s->stream>>>(static_cast<float2*>(s->amps), s->n_amps, index)));

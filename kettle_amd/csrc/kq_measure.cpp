@@ -503,7 +503,7 @@ int kq_simulate(kq_state* s, uint32_t circuit_n_qubits, const kq_op* ops, size_t
                 void* logger_user)
 {
     return guarded([&] {
-        check_state(s);
+        check_state_lazy(s);
         if (n_ops > 0 && ops == nullptr) {
             throw InvalidArg {"null op array"};
         }

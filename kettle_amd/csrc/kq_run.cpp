@@ -168,8 +168,17 @@ bool permute_out_of_place(kq_state* s)
 
 // Stores every qubit q of the state back on index bit q, if gate passes left them permuted: one
 // out-of-place pass when a second buffer fits, else gate-free in-place tile passes.
+void materialise_basis(kq_state* s)
+{
+    if (s->lazy_basis >= 0) {
+        ensure_init();
+        launch_set_basis(s, static_cast<uint64_t>(s->lazy_basis));   // clears lazy_basis
+    }
+}
+
 void normalise_layout(kq_state* s)
 {
+    materialise_basis(s);
     if (s->bit_of.empty()) {
         return;
     }
@@ -254,8 +263,8 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
     // statevector passes may leave qubits on other index bits (the state's bit_of); everything
     // else, and density matrices, run with qubit q on bit q
     const bool relabel = allow_relabel && fusion_relabel() && !s->is_dm;
-    if (!relabel) {
-        normalise_layout(s);
+    if (!relabel && !s->bit_of.empty()) {
+        normalise_layout(s);   // (a pending basis state keeps qubit q on bit q and stays pending)
     }
     std::vector<uint8_t> bit_of = s->bit_of;
     // a gate run leaves every qubit on its own bit when its last pass can store them there
@@ -338,10 +347,14 @@ public:
     {
         Item it;
         it.pass = std::move(p);
+        const bool first = first_;
+        first_ = false;
         if (!it.pass.sweep) {
             lower_tile_pass(it.pass, s_->precision, s_->nvec, it.prog);
             if (use_jit_) {
-                it.src = jit_source(it.prog, s_->precision, it.coefs);
+                // the run's first pass on a pending basis state builds its tiles from the basis index
+                it.from_basis = first && s_->lazy_basis >= 0 && jit_tuning().blocks_per_cu == 0;
+                it.src = jit_source(it.prog, s_->precision, it.coefs, it.from_basis);
                 it.fn = jit_lookup(it.src);
             }
         }
@@ -357,6 +370,7 @@ public:
     // passes pushed later, whether or not those passes wait for their kernels to compile
     void push_action(std::function<void()> action)
     {
+        first_ = false;
         Item it;
         it.action = std::move(action);
         if (held_.empty()) {
@@ -401,6 +415,7 @@ private:
         std::string src;
         std::vector<double> coefs;
         hipFunction_t fn = nullptr;
+        bool from_basis = false;        // compiled to build the state's pending basis vector
         std::function<void()> action;   // set: not a pass, run this instead
     };
 
@@ -435,6 +450,9 @@ private:
 
     void launch(Item& it)
     {
+        if (s_->lazy_basis >= 0 && !(it.from_basis && it.fn)) {
+            materialise_basis(s_);   // anything but a pass built from the basis index reads the buffer
+        }
         if (it.action) {
             it.action();
             return;
@@ -456,8 +474,17 @@ private:
             auto [d, h] = reserve(bytes);
             std::memcpy(h, it.coefs.data(), it.coefs.size() * sizeof(double));
             KQ_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_->stream));
-            profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
-                     [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d)); });
+            if (it.from_basis) {
+                // no read: only the store moves bytes
+                const uint64_t basis = static_cast<uint64_t>(s_->lazy_basis);
+                s_->lazy_basis = -1;
+                profiled(s_, cls, ab * static_cast<double>(s_->n_amps), p.gates.size(),
+                         [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d), &basis); });
+            }
+            else {
+                profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
+                         [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d)); });
+            }
             if (!tp.out_freepos.empty()) {
                 std::swap(s_->amps, s_->spare);   // the pass stored the state into the spare buffer
             }
@@ -502,6 +529,7 @@ private:
 
     kq_state* s_;
     bool use_jit_;
+    bool first_ = true;
     std::vector<Item> held_;
     size_t used_ = 0;
 };
@@ -744,7 +772,7 @@ extern "C" {
 int kq_apply_gates(kq_state* s, const kq_op* ops, size_t n_ops)
 {
     return guarded([&] {
-        check_state(s);
+        check_state_lazy(s);
         if (n_ops > 0 && ops == nullptr) {
             throw InvalidArg {"null op array"};
         }

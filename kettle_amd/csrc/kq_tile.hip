@@ -361,18 +361,43 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
 
     uint32_t cur = 0;
     const auto remap_one = [&](Cx<R> (&u)[16], const TileLayout& A, const TileLayout& B, uint32_t xa, uint32_t xb) {
-        __syncthreads();
-        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            Amp<R> x;
-            x.x = u[r].re;
-            x.y = u[r].im;
-            *reinterpret_cast<Amp<R>*>(smem_raw + (xa ^ A.lds_slot[r])) = x;
-        });
-        __syncthreads();
-        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
-            const Amp<R> x = *reinterpret_cast<const Amp<R>*>(smem_raw + (xb ^ B.lds_slot[r]));
-            u[r] = {x.x, x.y};
-        });
+        if constexpr (sizeof(R) == 8) {
+            // complex128: the real parts, then the imaginary parts, through 8-byte entries (the layouts'
+            // 16-byte offsets halved: the same swizzle in half the LDS, so more workgroups per CU)
+            const uint32_t ha = xa >> 1, hb = xb >> 1;
+            const auto half = [&]<bool IM>() __attribute__((always_inline)) {
+                __syncthreads();
+                Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+                    *reinterpret_cast<R*>(smem_raw + (ha ^ (A.lds_slot[r] >> 1))) = IM ? u[r].im : u[r].re;
+                });
+                __syncthreads();
+                Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+                    const R x = *reinterpret_cast<const R*>(smem_raw + (hb ^ (B.lds_slot[r] >> 1)));
+                    if constexpr (IM) {
+                        u[r].im = x;
+                    }
+                    else {
+                        u[r].re = x;
+                    }
+                });
+            };
+            half.template operator()<false>();
+            half.template operator()<true>();
+        }
+        else {
+            __syncthreads();
+            Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+                Amp<R> x;
+                x.x = u[r].re;
+                x.y = u[r].im;
+                *reinterpret_cast<Amp<R>*>(smem_raw + (xa ^ A.lds_slot[r])) = x;
+            });
+            __syncthreads();
+            Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+                const Amp<R> x = *reinterpret_cast<const Amp<R>*>(smem_raw + (xb ^ B.lds_slot[r]));
+                u[r] = {x.x, x.y};
+            });
+        }
     };
     const auto remap = [&](uint32_t to) {
         const TileLayout& A = layouts[cur];
@@ -609,7 +634,7 @@ TileArgs check_program(const kq_state* s, const TileProgram& prog)
     if (prog.freepos.size() > 31) {
         throw InvalidArg {"tile pass: grid too large"};
     }
-    if ((size_t {1} << prog.k) * amp_bytes(s->precision) + TILE_STATIC_LDS > device_lds_per_block()) {
+    if ((size_t {1} << prog.k) * TILE_LDS_ENTRY + TILE_STATIC_LDS > device_lds_per_block()) {
         throw InvalidArg {"tile pass: tile does not fit the workgroup's LDS"};
     }
     return d;
@@ -650,7 +675,7 @@ void launch_tile_pass(kq_state* s, const TileProgram& prog, const void* d_gates,
     }
     const unsigned blocks = tile_pass_blocks(prog);
     const unsigned threads = 1u << prog.n_thr_bits;
-    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
+    const size_t lds = (size_t {1} << prog.k) * TILE_LDS_ENTRY;
     const auto* g = static_cast<const TileGate*>(d_gates);
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
@@ -675,7 +700,7 @@ void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* ar
         throw InvalidArg {"tile batch: bad launch shape"};
     }
     const unsigned threads = 1u << (k - 4);
-    const size_t lds = (size_t {1} << k) * amp_bytes(precision);
+    const size_t lds = (size_t {1} << k) * TILE_LDS_ENTRY;
     if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
         throw InvalidArg {"tile batch: tile does not fit the workgroup's LDS"};
     }
@@ -713,7 +738,7 @@ void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgr
     const unsigned blocks = tile_pass_blocks(prog);
     const unsigned threads = 1u << prog.n_thr_bits;
     const uint32_t n_part = blocks * tile_pass_waves(prog);
-    const size_t lds = (size_t {1} << prog.k) * amp_bytes(s->precision);
+    const size_t lds = (size_t {1} << prog.k) * TILE_LDS_ENTRY;
     const auto* g = static_cast<const TileGate*>(d_gates);
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
