@@ -67,7 +67,9 @@ void check_state_lazy(const kq_state* s)
 void check_state(const kq_state* s)
 {
     check_state_lazy(s);
-    materialise_basis(const_cast<kq_state*>(s));
+    auto* w = const_cast<kq_state*>(s);
+    materialise_basis(w);
+    w->zero_mask = 0;   // the call may change the state: no known zeros are kept
 }
 
 namespace
@@ -337,6 +339,7 @@ int kq_state_upload(kq_state* s, const double* host)
         ensure_init();
         s->bit_of.clear();   // overwritten: qubit q on bit q
         s->lazy_basis = -1;
+        s->zero_mask = 0;
         if (s->precision == KQ_C128) {
             KQ_HIP_CHECK(hipMemcpy(s->amps, host, s->n_amps * 16, hipMemcpyHostToDevice));
         }
@@ -393,6 +396,7 @@ int kq_state_copy(kq_state* dst, const kq_state* src)
         normalise_layout(src);
         dst->bit_of.clear();
         dst->lazy_basis = -1;
+        dst->zero_mask = 0;
         KQ_HIP_CHECK(hipStreamSynchronize(src->stream));
         launch_copy_kernel(dst, dst->amps, src->amps, dst->n_amps);
         KQ_HIP_CHECK(hipStreamSynchronize(dst->stream));

@@ -268,6 +268,11 @@ struct kq_state
     // waits for the first operation: a gate run builds its first pass's tiles from the index, anything
     // else writes the vector first)
     int64_t lazy_basis = -1;
+    // Known zeros: every amplitude whose stored index has (index & zero_mask) != zero_val is 0 (a basis
+    // state: every bit known; each tile pass clears its tile's bits).  Gate runs skip the zero tiles;
+    // any other operation forgets it (zero_mask = 0).
+    uint64_t zero_mask = 0;
+    uint64_t zero_val = 0;
 };
 
 namespace kq
@@ -536,19 +541,23 @@ uint32_t device_cu_count();
 uint32_t& device_cus();
 bool jit_wanted(const kq_state* s);
 bool jit_available();
-// kernel source of one forward pass; coefs receives its coefficient buffer.  from_basis: the pass
-// builds its input tiles from a basis index (a kernel argument) instead of loading the state
-std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, bool from_basis = false);
+// What a forward pass kernel starts from: the state (load); a basis state not yet written, built from
+// its index (basis: the kernel's last argument); or the state with known-zero tiles, those whose outside
+// bits under a mask differ from given values (skip_zero: the last two arguments, zmask and zval), which
+// are skipped (in place: the buffer holds the zeros) or stored as zeros (out of place).
+enum class PassInput : int { load = 0, basis = 1, skip_zero = 2 };
+// kernel source of one forward pass; coefs receives its coefficient buffer
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, PassInput input = PassInput::load);
 // compiled kernels for the sources (null where compilation failed); compiles missing ones in parallel
 std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
 // the compiled kernel of a source if it is cached (a cache hit), else null; never compiles
 hipFunction_t jit_lookup(const std::string& source);
 // compile without loading (no device needed): number of failures, last log in *log
 size_t jit_compile_only(const std::vector<std::string>& sources, std::string* log);
-// out-of-place programs (tp.out_freepos) store into s->spare, which must be allocated; basis: the
-// stored index of the basis state a from_basis kernel builds (null for the others)
+// out-of-place programs (tp.out_freepos) store into s->spare, which must be allocated; extra: the values
+// of the kernel's input mode (PassInput::basis: the stored basis index; skip_zero: zmask, zval)
 void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs,
-                     const uint64_t* basis = nullptr);
+                     const uint64_t* extra = nullptr, size_t n_extra = 0);
 // kernel source of one adjoint reverse pass over (psi, lambda) with derivative reductions (the records of
 // an adjoint-lowered program); coefs receives its coefficient buffer
 // What the generated kernel of an adjoint reverse pass reads besides its structure: the coefficient

@@ -741,8 +741,10 @@ bool jit_wanted(const kq_state* s)
 }
 
 // Generate the kernel source of one forward tile pass and the layout of its coefficient buffer.
-std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, bool from_basis)
+std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, PassInput input)
 {
+    const bool from_basis = input == PassInput::basis;
+    const bool skip_zero = input == PassInput::skip_zero;
     coefs.clear();
     const bool c64 = precision == KQ_C64;
     const int nt = 1 << tp.n_thr_bits;
@@ -759,8 +761,10 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     const bool oop = !tp.out_freepos.empty();
     // from_basis: the state is the basis vector |basis> not yet written; the pass builds its tiles from
     // that index instead of loading them
+    // skip_zero: the workgroups whose outside bits under zmask differ from zval hold a zero tile
     o << " void kq_pass(kq::dev::Amp<R>* __restrict__ a, const double* __restrict__ cf, uint32_t n_blocks"
-      << (oop ? ", kq::dev::Amp<R>* __restrict__ dst" : "") << (from_basis ? ", uint64_t basis" : "") << ")\n{\n"
+      << (oop ? ", kq::dev::Amp<R>* __restrict__ dst" : "") << (from_basis ? ", uint64_t basis" : "")
+      << (skip_zero ? ", uint64_t zmask, uint64_t zval" : "") << ")\n{\n"
       << "  using namespace kq::dev;\n";
     bool remaps = tp.out_layout != 0;
     for (const TileStep& st : tp.steps) {
@@ -773,8 +777,8 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
     // Blocks walk the tiles with a grid stride.  With more tiles than blocks, the next tile's amplitudes
     // are loaded into registers before the current tile's gates run, so HBM reads overlap the arithmetic.
     const bool persistent = tune.blocks_per_cu > 0;
-    if (persistent && from_basis) {
-        throw InvalidArg {"jit: a pass from a basis state has no persistent form"};
+    if (persistent && input != PassInput::load) {
+        throw InvalidArg {"jit: a pass from a basis state or over zero tiles has no persistent form"};
     }
     o << "  const uint32_t tid = threadIdx.x;\n  Cx<R> v[16];\n";
     if (persistent) {
@@ -927,18 +931,12 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
         }
         return off;
     };
-    if (from_basis) {
-        // Every tile but the one holding the basis amplitude is 0, and the pass's gates (linear, acting
-        // within each tile) leave it 0: those workgroups store zeros where the pass stores and end.
-        // The one that holds it builds its tile, amplitude (off ^ slot) 1 at the basis index and 0
-        // elsewhere, and runs the pass.  Nothing is read.
-        uint64_t free_mask = 0;
-        for (const uint8_t q : tp.freepos) {
-            free_mask |= uint64_t {1} << q;
-        }
+    // zeros stored where the pass stores, by a workgroup whose tile is 0 (and stays 0 under the pass's
+    // gates, which are linear and act within each tile), which then ends
+    const auto zero_store = [&](const std::string& cond) {
         const TileLayout& S = tp.layouts[tp.out_layout];
         const int ntm = nontemporal_mode(tune.nontemporal, static_cast<uint32_t>(tp.k + tp.freepos.size()), c64 ? 8 : 16);
-        o << "  if ((basis & " << hex64(free_mask) << ") != base) {\n"
+        o << "  if (" << cond << ") {\n"
           << "    const uint64_t zoff = " << (oop ? "(" + deposit_expr(tp.out_freepos) + ")" : std::string {"base"}) << " | ("
           << hbm_thread_expr(S, tp.n_thr_bits) << ");\n"
           << "    Amp<R> zero;\n    zero.x = R(0);\n    zero.y = R(0);\n";
@@ -947,6 +945,25 @@ std::string jit_source(const TileProgram& tp, int precision, std::vector<double>
               << hex64(S.hbm_slot[r]) << "), zero);\n";
         }
         o << "    return;\n  }\n";
+    };
+    if (skip_zero) {
+        if (oop) {
+            zero_store("(base & zmask) != zval");   // the destination buffer holds no zeros yet
+        }
+        else {
+            // in place: the buffer already holds the zeros
+            o << "  if ((base & zmask) != zval) {\n    return;\n  }\n";
+        }
+    }
+    if (from_basis) {
+        // Every tile but the one holding the basis amplitude is 0: those workgroups store zeros.  The one
+        // that holds it builds its tile, amplitude (off ^ slot) 1 at the basis index and 0 elsewhere, and
+        // runs the pass.  Nothing is read.
+        uint64_t free_mask = 0;
+        for (const uint8_t q : tp.freepos) {
+            free_mask |= uint64_t {1} << q;
+        }
+        zero_store("(basis & " + hex64(free_mask) + ") != base");
         const TileLayout& L = tp.layouts[0];
         o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
         for (int r = 0; r < 16; ++r) {
@@ -1684,7 +1701,8 @@ size_t jit_compile_only(const std::vector<std::string>& sources, std::string* lo
     return failures;
 }
 
-void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs, const uint64_t* basis)
+void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const double* d_coefs, const uint64_t* extra,
+                     size_t n_extra)
 {
     if (tp.freepos.size() > 31 || tp.freepos.size() + tp.k != s->nvec) {
         throw InvalidArg {"jit pass: bad program shape"};
@@ -1693,17 +1711,26 @@ void launch_jit_pass(kq_state* s, hipFunction_t fn, const TileProgram& tp, const
     if (oop && (s->spare == nullptr || tp.out_freepos.size() != tp.freepos.size())) {
         throw InvalidArg {"jit pass: out-of-place pass without a destination buffer"};
     }
+    if (n_extra > 2) {
+        throw InvalidArg {"jit pass: too many arguments"};
+    }
     void* amps = s->amps;
     void* dst = s->spare;
     const double* cf = d_coefs;
     uint32_t n_blocks = tile_pass_blocks(tp);
-    uint64_t b = basis ? *basis : 0;
-    void* args3[] = {&amps, &cf, &n_blocks, &b};
-    void* args4[] = {&amps, &cf, &n_blocks, &dst, &b};
-    void** args = oop ? args4 : args3;   // the basis index is read only by kernels built from a basis
+    uint64_t x[2] = {n_extra > 0 ? extra[0] : 0, n_extra > 1 ? extra[1] : 0};
+    // the kernel's parameters: a, cf, n_blocks, [dst], then its input mode's values (basis, or zmask and zval)
+    void* args[6] = {&amps, &cf, &n_blocks};
+    size_t na = 3;
+    if (oop) {
+        args[na++] = &dst;
+    }
+    for (size_t i = 0; i < n_extra; ++i) {
+        args[na++] = &x[i];
+    }
     const JitTuning tune = jit_tuning();
-    if (basis && tune.blocks_per_cu > 0) {
-        throw InvalidArg {"jit pass: a pass from a basis state has no persistent form"};
+    if (n_extra > 0 && tune.blocks_per_cu > 0) {
+        throw InvalidArg {"jit pass: a pass from a basis state or over zero tiles has no persistent form"};
     }
     unsigned grid = n_blocks;
     if (tune.blocks_per_cu > 0) {

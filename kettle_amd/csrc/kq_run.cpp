@@ -171,8 +171,11 @@ bool permute_out_of_place(kq_state* s)
 void materialise_basis(kq_state* s)
 {
     if (s->lazy_basis >= 0) {
+        const uint64_t index = static_cast<uint64_t>(s->lazy_basis);
         ensure_init();
-        launch_set_basis(s, static_cast<uint64_t>(s->lazy_basis));   // clears lazy_basis
+        launch_set_basis(s, index);   // clears lazy_basis
+        s->zero_mask = s->n_amps - 1;   // e_index: every other amplitude is 0
+        s->zero_val = index;
     }
 }
 
@@ -182,6 +185,7 @@ void normalise_layout(kq_state* s)
     if (s->bit_of.empty()) {
         return;
     }
+    s->zero_mask = 0;
     if (permute_out_of_place(s)) {
         s->bit_of.clear();
         PlanStats& st = last_stats();
@@ -252,6 +256,7 @@ void run_records(kq_state* s, const std::vector<GateRec>& recs, bool allow_relab
     // tiles smaller than one thread's register block (states of a few qubits) run gate by gate
     if (max_k < register_qubits(s->precision)) {
         normalise_layout(s);
+        s->zero_mask = 0;   // (known zeros are followed through tile passes only)
         for (const auto& g : recs) {
             profiled(s, 2, 2.0 * ab * static_cast<double>(sweep_amps(g, s->n_amps)), 1, [&] { launch_gate_sweep(s, g); });
             ++st.n_passes;
@@ -341,7 +346,18 @@ size_t align256(size_t x) { return (x + 255) & ~size_t {255}; }
 class PassStream
 {
 public:
-    explicit PassStream(kq_state* s) : s_(s), use_jit_(jit_wanted(s) && jit_available()) {}
+    explicit PassStream(kq_state* s) : s_(s), use_jit_(jit_wanted(s) && jit_available())
+    {
+        // the known zeros the passes start from, followed as they are pushed (launch order)
+        if (s->lazy_basis >= 0) {
+            mask_ = s->n_amps - 1;
+            val_ = static_cast<uint64_t>(s->lazy_basis);
+        }
+        else {
+            mask_ = s->zero_mask;
+            val_ = s->zero_val & s->zero_mask;
+        }
+    }
 
     void push(TilePass&& p)
     {
@@ -351,13 +367,49 @@ public:
         first_ = false;
         if (!it.pass.sweep) {
             lower_tile_pass(it.pass, s_->precision, s_->nvec, it.prog);
+            uint64_t free_mask = 0;
+            for (const uint8_t q : it.prog.freepos) {
+                free_mask |= uint64_t {1} << q;
+            }
+            if (use_jit_ && jit_tuning().blocks_per_cu == 0) {
+                // the run's first pass on a pending basis state builds its tiles from the basis index; a
+                // pass whose outside bits include known ones skips (or zero-fills) the zero tiles
+                if (first && s_->lazy_basis >= 0) {
+                    it.input = PassInput::basis;
+                }
+                else if ((mask_ & free_mask) != 0) {
+                    it.input = PassInput::skip_zero;
+                    it.zmask = mask_ & free_mask;
+                    it.zval = val_ & it.zmask;
+                }
+            }
             if (use_jit_) {
-                // the run's first pass on a pending basis state builds its tiles from the basis index
-                it.from_basis = first && s_->lazy_basis >= 0 && jit_tuning().blocks_per_cu == 0;
-                it.src = jit_source(it.prog, s_->precision, it.coefs, it.from_basis);
+                it.src = jit_source(it.prog, s_->precision, it.coefs, it.input);
                 it.fn = jit_lookup(it.src);
             }
+            // after the pass: its tile's bits are unknown; out of place, the outside bits move
+            if (it.prog.out_freepos.empty()) {
+                mask_ &= free_mask;
+            }
+            else {
+                uint64_t m = 0, v = 0;
+                for (size_t i = 0; i < it.prog.freepos.size(); ++i) {
+                    const uint64_t from = uint64_t {1} << it.prog.freepos[i], to = uint64_t {1} << it.prog.out_freepos[i];
+                    if (mask_ & from) {
+                        m |= to;
+                        v |= (val_ & from) ? to : 0;
+                    }
+                }
+                mask_ = m;
+                val_ = v;
+            }
+            val_ &= mask_;
         }
+        else {
+            mask_ = 0;   // (sweeps: not followed)
+        }
+        it.mask_after = mask_;
+        it.val_after = val_;
         if (held_.empty() && (it.pass.sweep || !use_jit_ || it.fn)) {
             launch(it);
         }
@@ -371,6 +423,7 @@ public:
     void push_action(std::function<void()> action)
     {
         first_ = false;
+        mask_ = 0;   // an action may change the state in any way
         Item it;
         it.action = std::move(action);
         if (held_.empty()) {
@@ -415,8 +468,10 @@ private:
         std::string src;
         std::vector<double> coefs;
         hipFunction_t fn = nullptr;
-        bool from_basis = false;        // compiled to build the state's pending basis vector
-        std::function<void()> action;   // set: not a pass, run this instead
+        PassInput input = PassInput::load;   // what its compiled kernel starts from
+        uint64_t zmask = 0, zval = 0;        // skip_zero: the known outside bits
+        uint64_t mask_after = 0, val_after = 0;   // the known zeros once it has run
+        std::function<void()> action;        // set: not a pass, run this instead
     };
 
     // device and pinned staging regions of `bytes` for one program
@@ -450,9 +505,16 @@ private:
 
     void launch(Item& it)
     {
-        if (s_->lazy_basis >= 0 && !(it.from_basis && it.fn)) {
+        if (s_->lazy_basis >= 0 && !(it.input == PassInput::basis && it.fn)) {
             materialise_basis(s_);   // anything but a pass built from the basis index reads the buffer
         }
+        launch_item(it);
+        s_->zero_mask = it.mask_after;
+        s_->zero_val = it.val_after;
+    }
+
+    void launch_item(Item& it)
+    {
         if (it.action) {
             it.action();
             return;
@@ -474,12 +536,23 @@ private:
             auto [d, h] = reserve(bytes);
             std::memcpy(h, it.coefs.data(), it.coefs.size() * sizeof(double));
             KQ_HIP_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s_->stream));
-            if (it.from_basis) {
+            if (it.input == PassInput::basis) {
                 // no read: only the store moves bytes
                 const uint64_t basis = static_cast<uint64_t>(s_->lazy_basis);
                 s_->lazy_basis = -1;
                 profiled(s_, cls, ab * static_cast<double>(s_->n_amps), p.gates.size(),
-                         [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d), &basis); });
+                         [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d), &basis, 1); });
+            }
+            else if (it.input == PassInput::skip_zero) {
+                // the tiles that may be non-zero are read and written; out of place, the others are written
+                const uint64_t tiles = tile_pass_blocks(tp);
+                const uint64_t live = tiles >> std::popcount(it.zmask);
+                const double tile_bytes = ab * static_cast<double>(uint64_t {1} << tp.k);
+                const double moved = tile_bytes * (2.0 * static_cast<double>(live)
+                                                   + (tp.out_freepos.empty() ? 0.0 : static_cast<double>(tiles - live)));
+                const uint64_t zx[2] = {it.zmask, it.zval};
+                profiled(s_, cls, moved, p.gates.size(),
+                         [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d), zx, 2); });
             }
             else {
                 profiled(s_, cls, 2.0 * ab * static_cast<double>(s_->n_amps), p.gates.size(),
@@ -530,6 +603,7 @@ private:
     kq_state* s_;
     bool use_jit_;
     bool first_ = true;
+    uint64_t mask_ = 0, val_ = 0;   // known zeros after the passes pushed so far
     std::vector<Item> held_;
     size_t used_ = 0;
 };
