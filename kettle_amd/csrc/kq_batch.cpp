@@ -18,6 +18,7 @@
 // the batch), and that energy and gradient replace the batched ones.
 #include <algorithm>
 #include <atomic>
+#include <bit>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -53,6 +54,9 @@ struct BatchInstance
     size_t fwd_end = 0, bwd_end = 0;   // end of the last forward / reverse program
     // per reverse pass, per derivative slot: |F|^2, F the pivots divided out of the gates before it
     std::vector<std::vector<double>> bwd_dscale;
+    // per forward pass: the index bits known to be 0 wherever the state is non-zero when it starts (the
+    // sweep starts from |0...0>; a pass's tile bits are unknown after it)
+    std::vector<uint64_t> fwd_zero_mask;
     uint64_t amp_off = 0;              // offset of psi / lambda in the arenas (amplitudes)
     uint64_t n_amps = 0;
     size_t grad_off = 0;               // first gradient entry of the instance in the output
@@ -105,6 +109,7 @@ struct kq_batch
     size_t pauli_items_off = 0;
     size_t e_row_off_off = 0, e_row_cnt_off = 0;
     size_t ones_off = 0;
+    bool reset = true;                  // some instance has no forward pass: the arena is reset first
     size_t prog_bytes = 0;
     size_t table_bytes = 0;
     size_t pauli_bytes = 0;
@@ -318,14 +323,37 @@ void build(kq_batch* b)
         const double ar = static_cast<double>(ab);
         b->fwd_passes = b->rev_passes = 0;
         b->algorithmic_bytes = 0.0;
-        for (const BatchInstance& in : b->inst) {
+        b->reset = false;
+        for (BatchInstance& in : b->inst) {
+            b->reset = b->reset || in.fwd_prog.empty();
+        }
+        for (BatchInstance& in : b->inst) {
             const double A = static_cast<double>(in.n_amps);
             b->fwd_passes += in.fwd.size();
             b->rev_passes += in.bwd.size();
-            // reset, forward passes, lambda = H psi, narrowing, reverse passes (two vectors), drift read
-            b->algorithmic_bytes += A * (af + 2.0 * af * static_cast<double>(in.fwd.size()) + af + ar
-                                         + (b->fwd_precision != b->precision ? af + ar : 0.0)
-                                         + 4.0 * ar * static_cast<double>(in.bwd.size()) + ar);
+            // Forward passes from |0...0>: the first writes the state without reading it (its tile 0 built,
+            // the other tiles stored as zeros); a later one reads and writes the tiles whose outside bits
+            // under the known-zero mask are all 0 and skips the others.
+            in.fwd_zero_mask.assign(in.fwd_prog.size(), 0);
+            uint64_t known = in.n_amps - 1;
+            double fwd_bytes = 0.0;
+            for (size_t p = 0; p < in.fwd_prog.size(); ++p) {
+                uint64_t free_mask = 0;
+                for (const uint8_t q : in.fwd_prog[p].freepos) {
+                    free_mask |= uint64_t {1} << q;
+                }
+                in.fwd_zero_mask[p] = known & free_mask;
+                const double live = A / static_cast<double>(uint64_t {1} << std::popcount(in.fwd_zero_mask[p]));
+                fwd_bytes += p == 0 ? af * A : 2.0 * af * live;
+                known &= free_mask;
+            }
+            // (reset), forward passes, lambda = H psi, narrowing, reverse passes (two vectors), drift read
+            b->algorithmic_bytes += A * (af + ar + (b->fwd_precision != b->precision ? af + ar : 0.0)
+                                         + 4.0 * ar * static_cast<double>(in.bwd.size()) + ar)
+                                  + fwd_bytes;
+        }
+        if (b->reset) {
+            b->algorithmic_bytes += af * static_cast<double>(b->total_amps);
         }
     }
     b->prog_bytes = std::max<size_t>(prog, 256);
@@ -387,6 +415,10 @@ void build(kq_batch* b)
                     it.n_steps = static_cast<int32_t>(tp.steps.size());
                     it.out_layout = tp.out_layout;
                     it.block_begin = blocks;
+                    if (!adjoint && !b->reset) {
+                        it.input = r == 0 ? 1u : (in.fwd_zero_mask[r] != 0 ? 2u : 0u);
+                        it.zmask = in.fwd_zero_mask[r];
+                    }
                     const uint32_t nb = tile_pass_blocks(tp);
                     it.n_part = nb * tile_pass_waves(tp);
                     it.partials = part;   // offset in doubles, made an address later
@@ -585,8 +617,11 @@ void enqueue(kq_batch* b)
     size_t launches = 0;
     const bool mixed = b->fwd_precision != b->precision;
     void* fwd = mixed ? b->psi_fwd : b->psi;
-    launch_batch_reset(b->stream, b->fwd_precision, fwd, b->total_amps, reinterpret_cast<const uint64_t*>(tab + b->ones_off), n);
-    launches += 2;
+    if (b->reset) {
+        // (otherwise every state's first forward pass writes its |0...0> itself)
+        launch_batch_reset(b->stream, b->fwd_precision, fwd, b->total_amps, reinterpret_cast<const uint64_t*>(tab + b->ones_off), n);
+        launches += 2;
+    }
     for (const BatchRound& rd : b->rounds) {
         if (rd.adjoint) {
             continue;

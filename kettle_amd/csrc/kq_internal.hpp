@@ -177,7 +177,10 @@ struct BatchTileItem
     uint32_t out_layout;
     uint32_t n_part;
     uint32_t block_begin;
-    uint32_t pad;
+    uint32_t input;          // forward items: 0 load the state; 1 the state is |0...0>, not written yet (the
+                             // workgroup of tile 0 builds it, the others store zeros); 2 skip the tiles whose
+                             // outside bits under zmask are not all 0 (they hold zeros already)
+    uint64_t zmask;
 };
 
 struct TileProgram

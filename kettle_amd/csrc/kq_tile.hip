@@ -58,7 +58,8 @@ struct TileArgs
     int n_thr_bits;
     int n_steps;
     uint32_t out_layout;
-    uint32_t pad;
+    uint32_t input;          // as BatchTileItem::input (0 for single-state passes)
+    uint64_t zmask;
 };
 
 // a gate record's coefficients: doubles (complex128) or the floats packed in its first 32 bytes
@@ -346,7 +347,30 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
 
     Cx<R> v[16];
     Cx<R> w[ADJ ? 16 : 1];
-    {
+    if (!ADJ && d.input == 1) {
+        // |0...0> not written yet: every tile but tile 0 is zero and stays zero under the pass's gates
+        // (linear, within the tile), so its workgroup stores zeros where the pass stores and ends
+        if (base != 0) {
+            const TileLayout& S = layouts[d.out_layout];
+            const uint64_t off = base | hbm_thread_part(S, tid, tb);
+            Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+                Amp<R> z;
+                z.x = 0;
+                z.y = 0;
+                a[off + S.hbm_slot[r]] = z;
+            });
+            return;
+        }
+        const TileLayout& L = layouts[0];
+        const uint64_t off = hbm_thread_part(L, tid, tb);
+        Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
+            v[r] = {(off + L.hbm_slot[r]) == 0 ? R(1) : R(0), R(0)};
+        });
+    }
+    else {
+        if (!ADJ && d.input == 2 && (base & d.zmask) != 0) {
+            return;   // a zero tile, whose zeros the buffer holds
+        }
         const TileLayout& L = layouts[0];
         const uint64_t off = base | hbm_thread_part(L, tid, tb);
         Unroll<0, 16>::run([&]<int r>() __attribute__((always_inline)) {
@@ -582,7 +606,7 @@ __global__ __launch_bounds__(TILE_BLOCK) void k_tile_batch(Amp<R>* __restrict__ 
         }
     }
     const BatchTileItem& it = items[lo];
-    const TileArgs d {it.free_mask, it.n_thr_bits, it.n_steps, it.out_layout, 0};
+    const TileArgs d {it.free_mask, it.n_thr_bits, it.n_steps, it.out_layout, ADJ ? 0u : it.input, it.zmask};
     // the tables live in global memory: address-space-1 pointers keep their reads scalar
     using GStep = const __attribute__((address_space(1))) TileStep;
     using GLayout = const __attribute__((address_space(1))) TileLayout;
