@@ -112,7 +112,10 @@ int kq_state_device_ptr(kq_state* s, void** ptr, size_t* bytes);
 /* Host <-> device copies of the whole state, interleaved complex doubles (2*N or 2*N*N). */
 int kq_state_upload(kq_state* s, const double* host);
 int kq_state_download(const kq_state* s, double* host);
-/* Computational basis state |index> (or |index><index| for a density matrix). */
+/* Computational basis state |index> (or |index><index| for a density matrix).  Nothing is written
+ * yet (nor by kq_sv_create / kq_dm_create for their |0...0>): the next gate run builds the state in its
+ * first pass, and any other call on the state writes it first.  A device pointer fetched earlier does
+ * not see it until it is written. */
 int kq_state_set_basis(kq_state* s, uint64_t index);
 int kq_state_copy(kq_state* dst, const kq_state* src);
 
