@@ -375,6 +375,15 @@ int kq_debug_jit_compile(uint32_t n_qubits, int precision, const kq_op* ops, siz
 int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes, size_t pass_index,
                   char* src, size_t src_cap, size_t* src_len, double* coefs, size_t coef_cap, size_t* n_coefs,
                   uint64_t info[5], size_t* n_passes);
+/* Host-only: as kq_debug_pass, for a run that starts from the basis state |basis> not yet written
+ * (kq_state_set_basis): info[5] = the pass kernel's input (0: it loads the state; 1: it builds its
+ * tiles from the basis index, its fourth argument after the destination buffer of an out-of-place
+ * pass, and reads nothing; 2: it skips the tiles whose outside index bits under info[6] differ from
+ * info[7], taking those two as its last arguments, and stores zeros there when it stores out of
+ * place), info[6] the known-bit mask, info[7] the known values (the basis index for input 1). */
+int kq_debug_pass_from_basis(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                             size_t pass_index, uint64_t basis, char* src, size_t src_cap, size_t* src_len, double* coefs,
+                             size_t coef_cap, size_t* n_coefs, uint64_t info[8], size_t* n_passes);
 /* Host-only: generate and compile (without loading) the run-time adjoint reverse-pass kernels of a
  * straight-line parameterised gate list (ops with aux >= 0 carry derivatives), the whole list as one
  * reverse segment; fails with the compiler log if any does not compile.  No GPU needed. */

@@ -105,6 +105,9 @@ _SIGNATURES = {
     "kq_fusion_autotune_info": (c_int, [POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_debug_pass": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, c_size_t, c_char_p, c_size_t,
                               POINTER(c_size_t), c_void_p, c_size_t, POINTER(c_size_t), c_void_p, POINTER(c_size_t)]),
+    "kq_debug_pass_from_basis": (c_int, [c_uint32, c_int, c_void_p, c_size_t, c_size_t, c_size_t, c_uint64, c_char_p,
+                                          c_size_t, POINTER(c_size_t), c_void_p, c_size_t, POINTER(c_size_t), c_void_p,
+                                          POINTER(c_size_t)]),
     "kq_debug_check_relabel": (c_int, [c_uint32, c_void_p, c_size_t, c_void_p, POINTER(c_size_t), POINTER(c_size_t)]),
     "kq_set_jit_tuning": (c_int, [c_int, c_int, c_int]),
     "kq_set_jit_priority": (c_int, [c_int]),

@@ -37,9 +37,14 @@ struct DebugPass
     uint32_t blocks = 0;
     uint32_t threads = 0;
     GateRec gate {};
+    PassInput input = PassInput::load;   // what its kernel starts from (runs from a basis state)
+    uint64_t zmask = 0, zval = 0;        // skip_zero: the known outside bits; basis: zval = the index
 };
 
-std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes)
+// basis >= 0: the run starts from that basis state, not written yet (the first pass builds it, later
+// passes skip the tiles known to be zero, the restore passes after the run start from a written state)
+std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                                    int64_t basis = -1)
 {
     if (n_qubits < 4 || n_qubits > 40 || (precision != KQ_C128 && precision != KQ_C64) || (!ops && n_ops)) {
         throw InvalidArg {"debug jit: bad arguments"};
@@ -76,6 +81,7 @@ std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_o
             bit_of[q] = planned[sigma[q]];
         }
     }
+    const size_t run_passes = passes.size();
     bool moved = false;
     for (size_t q = 0; q < bit_of.size(); ++q) {
         moved = moved || bit_of[q] != q;
@@ -85,17 +91,33 @@ std::vector<DebugPass> debug_passes(uint32_t n_qubits, int precision, const kq_o
             passes.push_back(std::move(p));
         }
     }
+    if (basis >= 0 && static_cast<uint64_t>(basis) >= (uint64_t {1} << n_qubits)) {
+        throw InvalidArg {"debug jit: basis index out of range"};
+    }
+    KnownZeros kz;
+    if (basis >= 0) {
+        kz.mask = (uint64_t {1} << n_qubits) - 1;
+        kz.val = static_cast<uint64_t>(basis);
+        kz.basis = true;
+    }
     std::vector<DebugPass> out;
-    for (const auto& p : passes) {
+    for (size_t i = 0; i < passes.size(); ++i) {
+        const TilePass& p = passes[i];
         DebugPass d;
+        if (i == run_passes) {
+            kz.forget();   // a read restores the order, and writes a pending basis state first
+        }
         if (p.sweep) {
             d.sweep = true;
             d.gate = p.single;
+            kz.forget();
         }
         else {
             TileProgram tp;
             lower_tile_pass(p, precision, n_qubits, tp);
-            d.src = jit_source(tp, precision, d.coefs);
+            d.input = kz.input_for(tp, i == 0, true, d.zmask, d.zval);
+            kz.after(tp);
+            d.src = jit_source(tp, precision, d.coefs, d.input);
             d.out_of_place = !tp.out_freepos.empty();
             d.blocks = tile_pass_blocks(tp);
             d.threads = 1u << tp.n_thr_bits;
@@ -422,6 +444,45 @@ int kq_debug_pass(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_o
         info[2] = d.threads;
         info[3] = d.sweep ? d.gate.target : 0;
         info[4] = d.sweep ? static_cast<uint64_t>(d.gate.control + 1) : 0;
+    });
+}
+
+int kq_debug_pass_from_basis(uint32_t n_qubits, int precision, const kq_op* ops, size_t n_ops, size_t lds_bytes,
+                             size_t pass_index, uint64_t basis, char* src, size_t src_cap, size_t* src_len, double* coefs,
+                             size_t coef_cap, size_t* n_coefs, uint64_t info[8], size_t* n_passes)
+{
+    return guarded([&] {
+        if (!src_len || !n_coefs || !info || !n_passes) {
+            throw InvalidArg {"debug pass: null output"};
+        }
+        if (basis >= (uint64_t {1} << 62)) {
+            throw InvalidArg {"debug pass: basis index out of range"};
+        }
+        const auto passes = debug_passes(n_qubits, precision, ops, n_ops, lds_bytes, static_cast<int64_t>(basis));
+        *n_passes = passes.size();
+        if (pass_index >= passes.size()) {
+            throw InvalidArg {"debug pass: pass index out of range"};
+        }
+        const DebugPass& d = passes[pass_index];
+        *src_len = d.src.size();
+        *n_coefs = d.sweep ? 8 : d.coefs.size();
+        if (src && src_cap) {
+            const size_t n = std::min(src_cap - 1, d.src.size());
+            std::memcpy(src, d.src.data(), n);
+            src[n] = '\0';
+        }
+        if (coefs) {
+            const double* from = d.sweep ? d.gate.m : d.coefs.data();
+            std::memcpy(coefs, from, std::min(coef_cap, *n_coefs) * sizeof(double));
+        }
+        info[0] = d.sweep ? 1 : (d.out_of_place ? 2 : 0);
+        info[1] = d.blocks;
+        info[2] = d.threads;
+        info[3] = d.sweep ? d.gate.target : 0;
+        info[4] = d.sweep ? static_cast<uint64_t>(d.gate.control + 1) : 0;
+        info[5] = static_cast<uint64_t>(d.input);
+        info[6] = d.zmask;
+        info[7] = d.zval;
     });
 }
 

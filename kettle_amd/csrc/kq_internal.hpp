@@ -551,6 +551,26 @@ bool jit_available();
 enum class PassInput : int { load = 0, basis = 1, skip_zero = 2 };
 // kernel source of one forward pass; coefs receives its coefficient buffer
 std::string jit_source(const TileProgram& tp, int precision, std::vector<double>& coefs, PassInput input = PassInput::load);
+// The known zeros of a state followed through a run of passes in launch order: every amplitude whose
+// stored index has (index & mask) != val is 0; basis: the state is that basis vector, not written yet.
+struct KnownZeros
+{
+    uint64_t mask = 0;
+    uint64_t val = 0;
+    bool basis = false;
+    // the input of a tile pass with this program (first: the run's first pass; compiled: it runs as a
+    // compiled kernel without the persistent form); zmask / zval: the known outside bits for skip_zero
+    PassInput input_for(const TileProgram& tp, bool first, bool compiled, uint64_t& zmask, uint64_t& zval) const;
+    // after a tile pass: its tile bits are unknown, an out-of-place pass moves the outside bits
+    void after(const TileProgram& tp);
+    // after anything else
+    void forget()
+    {
+        mask = 0;
+        val = 0;
+        basis = false;
+    }
+};
 // compiled kernels for the sources (null where compilation failed); compiles missing ones in parallel
 std::vector<hipFunction_t> jit_kernels(const std::vector<std::string>& sources);
 // the compiled kernel of a source if it is cached (a cache hit), else null; never compiles

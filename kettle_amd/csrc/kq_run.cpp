@@ -200,6 +200,54 @@ void normalise_layout(kq_state* s)
 
 void normalise_layout(const kq_state* s) { normalise_layout(const_cast<kq_state*>(s)); }
 
+PassInput KnownZeros::input_for(const TileProgram& tp, bool first, bool compiled, uint64_t& zmask, uint64_t& zval) const
+{
+    zmask = 0;
+    zval = 0;
+    if (!compiled) {
+        return PassInput::load;
+    }
+    if (first && basis) {
+        zval = val;   // the basis index
+        return PassInput::basis;
+    }
+    uint64_t free_mask = 0;
+    for (const uint8_t q : tp.freepos) {
+        free_mask |= uint64_t {1} << q;
+    }
+    if ((mask & free_mask) == 0) {
+        return PassInput::load;
+    }
+    zmask = mask & free_mask;
+    zval = val & zmask;
+    return PassInput::skip_zero;
+}
+
+void KnownZeros::after(const TileProgram& tp)
+{
+    uint64_t free_mask = 0;
+    for (const uint8_t q : tp.freepos) {
+        free_mask |= uint64_t {1} << q;
+    }
+    if (tp.out_freepos.empty()) {
+        mask &= free_mask;
+    }
+    else {
+        uint64_t m = 0, v = 0;
+        for (size_t i = 0; i < tp.freepos.size(); ++i) {
+            const uint64_t from = uint64_t {1} << tp.freepos[i], to = uint64_t {1} << tp.out_freepos[i];
+            if (mask & from) {
+                m |= to;
+                v |= (val & from) ? to : 0;
+            }
+        }
+        mask = m;
+        val = v;
+    }
+    val &= mask;
+    basis = false;
+}
+
 // kettle's SWAP gate is three controlled X gates, CX(a, b) CX(b, a) CX(a, b) (circuit.cpp:801-810).
 // Where qubits may sit on any index bit, the exchange moves no amplitude: the triple is dropped and
 // every later gate on a acts on b and vice versa.  Returns the other gates, relabelled; sigma[q] is
@@ -350,12 +398,13 @@ public:
     {
         // the known zeros the passes start from, followed as they are pushed (launch order)
         if (s->lazy_basis >= 0) {
-            mask_ = s->n_amps - 1;
-            val_ = static_cast<uint64_t>(s->lazy_basis);
+            kz_.mask = s->n_amps - 1;
+            kz_.val = static_cast<uint64_t>(s->lazy_basis);
+            kz_.basis = true;
         }
         else {
-            mask_ = s->zero_mask;
-            val_ = s->zero_val & s->zero_mask;
+            kz_.mask = s->zero_mask;
+            kz_.val = s->zero_val & s->zero_mask;
         }
     }
 
@@ -367,49 +416,20 @@ public:
         first_ = false;
         if (!it.pass.sweep) {
             lower_tile_pass(it.pass, s_->precision, s_->nvec, it.prog);
-            uint64_t free_mask = 0;
-            for (const uint8_t q : it.prog.freepos) {
-                free_mask |= uint64_t {1} << q;
-            }
-            if (use_jit_ && jit_tuning().blocks_per_cu == 0) {
-                // the run's first pass on a pending basis state builds its tiles from the basis index; a
-                // pass whose outside bits include known ones skips (or zero-fills) the zero tiles
-                if (first && s_->lazy_basis >= 0) {
-                    it.input = PassInput::basis;
-                }
-                else if ((mask_ & free_mask) != 0) {
-                    it.input = PassInput::skip_zero;
-                    it.zmask = mask_ & free_mask;
-                    it.zval = val_ & it.zmask;
-                }
-            }
+            // the run's first pass on a pending basis state builds its tiles from the basis index; a pass
+            // whose outside bits include known ones skips (or zero-fills) the zero tiles
+            it.input = kz_.input_for(it.prog, first, use_jit_ && jit_tuning().blocks_per_cu == 0, it.zmask, it.zval);
             if (use_jit_) {
                 it.src = jit_source(it.prog, s_->precision, it.coefs, it.input);
                 it.fn = jit_lookup(it.src);
             }
-            // after the pass: its tile's bits are unknown; out of place, the outside bits move
-            if (it.prog.out_freepos.empty()) {
-                mask_ &= free_mask;
-            }
-            else {
-                uint64_t m = 0, v = 0;
-                for (size_t i = 0; i < it.prog.freepos.size(); ++i) {
-                    const uint64_t from = uint64_t {1} << it.prog.freepos[i], to = uint64_t {1} << it.prog.out_freepos[i];
-                    if (mask_ & from) {
-                        m |= to;
-                        v |= (val_ & from) ? to : 0;
-                    }
-                }
-                mask_ = m;
-                val_ = v;
-            }
-            val_ &= mask_;
+            kz_.after(it.prog);
         }
         else {
-            mask_ = 0;   // (sweeps: not followed)
+            kz_.forget();   // (sweeps: not followed)
         }
-        it.mask_after = mask_;
-        it.val_after = val_;
+        it.mask_after = kz_.mask;
+        it.val_after = kz_.val;
         if (held_.empty() && (it.pass.sweep || !use_jit_ || it.fn)) {
             launch(it);
         }
@@ -423,7 +443,7 @@ public:
     void push_action(std::function<void()> action)
     {
         first_ = false;
-        mask_ = 0;   // an action may change the state in any way
+        kz_.forget();   // an action may change the state in any way
         Item it;
         it.action = std::move(action);
         if (held_.empty()) {
@@ -603,7 +623,7 @@ private:
     kq_state* s_;
     bool use_jit_;
     bool first_ = true;
-    uint64_t mask_ = 0, val_ = 0;   // known zeros after the passes pushed so far
+    KnownZeros kz_;   // known zeros after the passes pushed so far
     std::vector<Item> held_;
     size_t used_ = 0;
 };

@@ -77,6 +77,27 @@ extern "C" void emu_launch_to(void (*fn)(void*, const double*, uint32_t, void*),
 {
     emu_run([=] { fn(a, cf, blocks, dst); }, blocks, threads);
 }
+// passes of a run from a basis state: one (the basis index) or two (known-bit mask and values) more arguments
+extern "C" void emu_launch_x1(void (*fn)(void*, const double*, uint32_t, uint64_t), void* a, const double* cf,
+                              uint32_t blocks, uint32_t threads, uint64_t x)
+{
+    emu_run([=] { fn(a, cf, blocks, x); }, blocks, threads);
+}
+extern "C" void emu_launch_x2(void (*fn)(void*, const double*, uint32_t, uint64_t, uint64_t), void* a, const double* cf,
+                              uint32_t blocks, uint32_t threads, uint64_t x, uint64_t y)
+{
+    emu_run([=] { fn(a, cf, blocks, x, y); }, blocks, threads);
+}
+extern "C" void emu_launch_to_x1(void (*fn)(void*, const double*, uint32_t, void*, uint64_t), void* a, const double* cf,
+                                 uint32_t blocks, uint32_t threads, void* dst, uint64_t x)
+{
+    emu_run([=] { fn(a, cf, blocks, dst, x); }, blocks, threads);
+}
+extern "C" void emu_launch_to_x2(void (*fn)(void*, const double*, uint32_t, void*, uint64_t, uint64_t), void* a,
+                                 const double* cf, uint32_t blocks, uint32_t threads, void* dst, uint64_t x, uint64_t y)
+{
+    emu_run([=] { fn(a, cf, blocks, dst, x, y); }, blocks, threads);
+}
 """
 
 
@@ -198,3 +219,62 @@ def run(ops: np.ndarray, n: int, initial: np.ndarray, precision: int = ka.KQ_C12
         else:
             launch(ctypes.cast(fn, ctypes.c_void_p), psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"])
     return psi.astype(np.complex128)
+
+
+def _passes_from_basis(ops: np.ndarray, n: int, precision: int, basis: int):
+    c = ctypes
+    lib = _lib.load()
+    out = []
+    i = 0
+    while True:
+        ln, nc, npas = c.c_size_t(), c.c_size_t(), c.c_size_t()
+        info = np.zeros(8, dtype=np.uint64)
+        args = (n, precision, _lib.ptr(ops), len(ops), 160 * 1024, i, basis)
+        _lib.check(lib.kq_debug_pass_from_basis(*args, None, 0, c.byref(ln), None, 0, c.byref(nc), _lib.ptr(info),
+                                                c.byref(npas)))
+        buf = c.create_string_buffer(ln.value + 1)
+        coefs = np.zeros(max(nc.value, 1), dtype=np.float64)
+        _lib.check(lib.kq_debug_pass_from_basis(*args, buf, ln.value + 1, c.byref(ln), _lib.ptr(coefs), coefs.size,
+                                                c.byref(nc), _lib.ptr(info), c.byref(npas)))
+        out.append({"sweep": int(info[0]) == 1, "to_other": int(info[0]) == 2, "src": buf.value.decode(),
+                    "coefs": coefs[:nc.value].copy(), "blocks": int(info[1]), "threads": int(info[2]),
+                    "target": int(info[3]), "control": int(info[4]) - 1, "input": int(info[5]),
+                    "zmask": int(info[6]), "zval": int(info[7])})
+        i += 1
+        if i >= npas.value:
+            return out
+
+
+def run_from_basis(ops: np.ndarray, n: int, basis: int, precision: int = ka.KQ_C128):
+    """The state after the gate list from the basis state |basis>, as the generated kernels of a run from a
+    pending reset compute it: the buffer starts as NaN (a pass that read it would spread NaN), the first
+    pass builds the state, later passes skip known-zero tiles.  Returns (state, input mode per tile pass)."""
+    passes = _passes_from_basis(np.ascontiguousarray(ops), n, precision, basis)
+    tiles = [(f"kq_pass_{i}", host_source(p["src"], f"kq_pass_{i}")) for i, p in enumerate(passes) if not p["sweep"]]
+    lib = _compile(tiles)
+    vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+    shims = {
+        (False, 0): (lib.emu_launch, []), (True, 0): (lib.emu_launch_to, [vp]),
+        (False, 1): (lib.emu_launch_x1, [u64]), (True, 1): (lib.emu_launch_to_x1, [vp, u64]),
+        (False, 2): (lib.emu_launch_x2, [u64, u64]), (True, 2): (lib.emu_launch_to_x2, [vp, u64, u64]),
+    }
+    for f, extra in shims.values():
+        f.argtypes = [vp, vp, vp, u32, u32] + extra
+    c64 = precision == ka.KQ_C64
+    psi = np.full(1 << n, np.nan + 1j * np.nan, dtype=np.complex64 if c64 else np.complex128)
+    modes = []
+    for i, p in enumerate(passes):
+        if p["sweep"]:
+            raise RuntimeError("run_from_basis: sweep passes are not emulated here")
+        fn = ctypes.cast(getattr(lib, f"kq_pass_{i}"), vp)
+        cf = np.ascontiguousarray(p["coefs"] if p["coefs"].size else np.zeros(1))
+        extra = [] if p["input"] == 0 else ([p["zval"]] if p["input"] == 1 else [p["zmask"], p["zval"]])
+        launch, _ = shims[(p["to_other"], len(extra))]
+        modes.append(p["input"])
+        if p["to_other"]:
+            dst = np.full_like(psi, np.nan)
+            launch(fn, psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"], dst.ctypes.data, *extra)
+            psi = dst
+        else:
+            launch(fn, psi.ctypes.data, cf.ctypes.data, p["blocks"], p["threads"], *extra)
+    return psi.astype(np.complex128), modes
