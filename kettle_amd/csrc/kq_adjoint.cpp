@@ -231,6 +231,8 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     const int max_k = fusion_max_k(s->precision, s->nvec);
     const int rq = register_qubits(s->precision);
     if (max_k < rq) {
+        materialise_basis(s);   // the unfused sweep reads the state directly
+        s->zero_mask = 0;
         std::vector<PauliGroup> groups;
         std::vector<PauliZ> strings;
         build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
@@ -502,15 +504,19 @@ int kq_energy_gradient(kq_state* state, const kq_op* ops, size_t n_ops, size_t n
                        size_t* n_restores)
 {
     return guarded([&] {
-        check_state(state);
+        // a pending basis state is built by the forward sweep's first pass (which reads nothing)
+        check_state_lazy(state);
         if (!energy || (n_params > 0 && !grad)) {
             throw InvalidArg {"null output"};
         }
         ensure_init();
-        normalise_layout(state);
+        if (!state->bit_of.empty()) {
+            normalise_layout(state);
+        }
         last_stats() = PlanStats {};
         energy_gradient(state, ops, n_ops, n_strings, term_offsets, term_qubits, term_kinds, phases, coeffs, n_params,
                         energy, grad, checkpoint_every, drift_tol, n_restores);
+        state->zero_mask = 0;
     });
 }
 
