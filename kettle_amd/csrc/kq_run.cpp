@@ -446,8 +446,10 @@ public:
         kz_.forget();   // an action may change the state in any way
         Item it;
         it.action = std::move(action);
+        it.mask_after = 0;
+        it.val_after = 0;
         if (held_.empty()) {
-            it.action();
+            launch(it);   // (writes a pending basis state first)
         }
         else {
             held_.push_back(std::move(it));
@@ -528,6 +530,7 @@ private:
         if (s_->lazy_basis >= 0 && !(it.input == PassInput::basis && it.fn)) {
             materialise_basis(s_);   // anything but a pass built from the basis index reads the buffer
         }
+        s_->zero_mask = 0;   // (nothing is known if the launch fails)
         launch_item(it);
         s_->zero_mask = it.mask_after;
         s_->zero_val = it.val_after;
@@ -559,9 +562,9 @@ private:
             if (it.input == PassInput::basis) {
                 // no read: only the store moves bytes
                 const uint64_t basis = static_cast<uint64_t>(s_->lazy_basis);
-                s_->lazy_basis = -1;
                 profiled(s_, cls, ab * static_cast<double>(s_->n_amps), p.gates.size(),
                          [&] { launch_jit_pass(s_, it.fn, tp, reinterpret_cast<const double*>(d), &basis, 1); });
+                s_->lazy_basis = -1;   // written by the pass just launched
             }
             else if (it.input == PassInput::skip_zero) {
                 // the tiles that may be non-zero are read and written; out of place, the others are written
