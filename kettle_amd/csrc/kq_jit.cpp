@@ -1354,6 +1354,37 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoef
     }
     o << ";\n  const uint64_t base = " << deposit_expr(tp.freepos) << ";\n"
       << "  const uint64_t col = uint64_t {blockIdx.x} * " << waves << "u + (tid >> 6);\n";
+    // Derivative partials, complex64 passes of whole wavefronts: each lane writes its value into a
+    // wave-private LDS area (16 derivatives x 64 lanes); every 16 derivatives and at the end the wave
+    // sums them together, 4 lanes per derivative over 16 values each, then 2 shuffle steps, instead of a
+    // 6-step shuffle tree per derivative.  The fences order the area's writes and reads within the wave.
+    const bool staged = c64 && nt >= 64;
+    constexpr int STAGE = 16;
+    std::vector<uint32_t> pend;
+    if (staged) {
+        o << "  __shared__ double dpart[" << waves * STAGE * 64 << "];\n"
+          << "  double* const dp = dpart + (tid >> 6) * " << STAGE * 64 << "u;\n  const uint32_t lane = tid & 63u;\n";
+    }
+    const auto flush = [&] {
+        if (pend.empty()) {
+            return;
+        }
+        o << "  {\n    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n"
+          << "    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, \"workgroup\");\n"
+          << "    const uint32_t j = lane >> 2;\n    double s = 0.0;\n"
+          << "    if (j < " << pend.size() << "u) {\n      const double* q = dp + j * 64u + (lane & 3u) * 16u;\n";
+        for (int i = 0; i < 16; ++i) {
+            o << "      s += q[" << i << "];\n";
+        }
+        o << "    }\n    s += __shfl_xor(s, 1, 64);\n    s += __shfl_xor(s, 2, 64);\n"
+          << "    if ((lane & 3u) == 0u && j < " << pend.size() << "u) {\n      uint32_t slot = " << pend[0] << "u;\n";
+        for (size_t i = 1; i < pend.size(); ++i) {
+            o << "      slot = j == " << i << "u ? " << pend[i] << "u : slot;\n";
+        }
+        o << "      partials[uint64_t {slot} * n_part + col] = s;\n    }\n"
+          << "    __builtin_amdgcn_fence(__ATOMIC_RELEASE, \"workgroup\");\n  }\n";
+        pend.clear();
+    };
     const auto io = [&](const TileLayout& L, bool load) {
         o << "  {\n    const uint64_t off = base | (" << hbm_thread_expr(L, tp.n_thr_bits) << ");\n";
         const int ntm = nontemporal_mode(tune.nontemporal, static_cast<uint32_t>(tp.k + tp.freepos.size()), c64 ? 8 : 16);
@@ -1476,8 +1507,17 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoef
                 else {
                     o << "        acc = ddot<R, " << (dg.code / 5) % 4 << ", " << dg.code % 5 << ">(v, w, dm);\n";
                 }
-                o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
-                  << "u} * n_part + col] = acc;\n      }\n    }\n";
+                if (staged) {
+                    o << "      }\n      dp[" << pend.size() * 64 << "u + lane] = acc;\n    }\n";
+                    pend.push_back(dg.pad);
+                    if (pend.size() == STAGE) {
+                        flush();
+                    }
+                }
+                else {
+                    o << "      }\n      acc = wsum(acc);\n      if ((tid & 63u) == 0u) {\n        partials[uint64_t {" << dg.pad
+                      << "u} * n_part + col] = acc;\n      }\n    }\n";
+                }
             }
             if (!ac.forms[gi].empty()) {
                 // uncontrolled one-qubit gate, pivot divided out, on both vectors
@@ -1488,6 +1528,7 @@ std::string jit_adjoint_source(const TileProgram& tp, int precision, AdjointCoef
             o << "    if (ok) { " << apply_to(g, "w") << " }\n  }\n";
         }
     }
+    flush();
     if (std::any_of(ac.forms.begin(), ac.forms.end(), [](const std::string& f) { return !f.empty(); })) {
         // the product of the pivots taken out, last in the buffer, back into both vectors
         const size_t nf = ac.coefs.size();
