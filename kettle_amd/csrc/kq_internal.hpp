@@ -352,6 +352,11 @@ int fusion_max_k_for(int precision, uint32_t nvec, size_t lds_bytes);
 size_t& device_lds_per_block();
 constexpr size_t TILE_STATIC_LDS = 0;      // static LDS of the tile kernel besides the tile
 constexpr size_t TILE_LDS_ENTRY = 8;       // LDS bytes per amplitude of the tile kernel's remaps (complex128: halves)
+// derivative staging of the adjoint tile kernel: TILE_STAGE derivatives x 64 lanes of doubles and their
+// slots, per whole wavefront, after the tile in LDS
+constexpr uint32_t TILE_STAGE = 8;
+constexpr uint32_t TILE_STAGE_BYTES = TILE_STAGE * 64 * 8 + 256;
+size_t adjoint_tile_lds(int k);
 int fusion_max_gates();
 void set_fusion(int max_k, int max_gates, int max_stages, double max_cost);
 double fusion_max_cost();

@@ -438,6 +438,37 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
     const int width = (1 << tb) < 64 ? (1 << tb) : 64;
     const uint32_t waves = (blockDim.x + 63) / 64;
     const uint32_t part_col = block * waves + tid / 64;
+    // Derivative partials of whole wavefronts are staged: each lane writes its value into a wave-private
+    // LDS area after the tile (TILE_STAGE derivatives x 64 lanes, and their slots); when it is full and at
+    // the end the wave sums them together, 8 lanes per derivative over 8 values each, then 3 shuffle
+    // steps, instead of a 6-step shuffle tree per derivative.  The fences order the area's writes and
+    // reads within the wave.
+    const bool staged = ADJ && width == 64;
+    double* const stage = reinterpret_cast<double*>(smem_raw + (128u << tb) + (tid >> 6) * TILE_STAGE_BYTES);
+    uint32_t* const stage_slot = reinterpret_cast<uint32_t*>(stage + TILE_STAGE * 64);
+    const uint32_t lane = tid & 63u;
+    uint32_t n_staged = 0;
+    const auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const uint32_t j = lane >> 3;
+        double x = 0.0;
+        if (j < n_staged) {
+            const double* q = stage + j * 64u + (lane & 7u) * 8u;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                x += q[i];
+            }
+        }
+        x += __shfl_xor(x, 1, 64);
+        x += __shfl_xor(x, 2, 64);
+        x += __shfl_xor(x, 4, 64);
+        if ((lane & 7u) == 0u && j < n_staged) {
+            partials[static_cast<size_t>(stage_slot[j]) * n_part + part_col] = x;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        n_staged = 0;
+    };
 
     for (int si = 0; si < d.n_steps; ++si) {
         const TileStep st = steps[si];
@@ -520,9 +551,20 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
                             acc = ddot_gate<R>(v, w, dg.code % 20, dm);
                         }
                     }
-                    acc = wave_sum(acc, width);
-                    if ((tid & 63u) == 0) {
-                        partials[static_cast<size_t>(dg.pad) * n_part + part_col] = acc;
+                    if (staged) {
+                        stage[n_staged * 64u + lane] = acc;
+                        if (lane == 0) {
+                            stage_slot[n_staged] = dg.pad;
+                        }
+                        if (++n_staged == TILE_STAGE) {
+                            flush();
+                        }
+                    }
+                    else {
+                        acc = wave_sum(acc, width);
+                        if ((tid & 63u) == 0) {
+                            partials[static_cast<size_t>(dg.pad) * n_part + part_col] = acc;
+                        }
                     }
                 }
                 if (ok) {
@@ -557,6 +599,11 @@ __device__ __forceinline__ void tile_body(Amp<R>* __restrict__ a, Amp<R>* __rest
         }
     }
 
+    if constexpr (ADJ) {
+        if (staged && n_staged > 0) {
+            flush();
+        }
+    }
     if (d.out_layout != cur) {
         remap(d.out_layout);
     }
@@ -687,6 +734,13 @@ void check_tile_records(const TileProgram& prog)
     }
 }
 
+// dynamic LDS of an adjoint tile pass: the tile, then the derivative staging area of each whole wavefront
+size_t adjoint_tile_lds(int k)
+{
+    const size_t threads = size_t {1} << (k - 4);
+    return (size_t {1} << k) * TILE_LDS_ENTRY + (threads >= 64 ? threads / 64 * TILE_STAGE_BYTES : 0);
+}
+
 uint32_t tile_pass_blocks(const TileProgram& prog) { return 1u << prog.freepos.size(); }
 uint32_t tile_pass_waves(const TileProgram& prog) { return ((1u << prog.n_thr_bits) + 63) / 64; }
 
@@ -724,7 +778,7 @@ void launch_tile_batch(hipStream_t stream, int precision, bool adjoint, void* ar
         throw InvalidArg {"tile batch: bad launch shape"};
     }
     const unsigned threads = 1u << (k - 4);
-    const size_t lds = (size_t {1} << k) * TILE_LDS_ENTRY;
+    const size_t lds = adjoint ? adjoint_tile_lds(k) : (size_t {1} << k) * TILE_LDS_ENTRY;
     if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
         throw InvalidArg {"tile batch: tile does not fit the workgroup's LDS"};
     }
@@ -762,7 +816,10 @@ void launch_adjoint_tile_pass(kq_state* s, void* psi, void* lam, const TileProgr
     const unsigned blocks = tile_pass_blocks(prog);
     const unsigned threads = 1u << prog.n_thr_bits;
     const uint32_t n_part = blocks * tile_pass_waves(prog);
-    const size_t lds = (size_t {1} << prog.k) * TILE_LDS_ENTRY;
+    const size_t lds = adjoint_tile_lds(prog.k);
+    if (lds + TILE_STATIC_LDS > device_lds_per_block()) {
+        throw InvalidArg {"tile pass: tile and derivative staging do not fit the workgroup's LDS"};
+    }
     const auto* g = static_cast<const TileGate*>(d_gates);
     const auto* st = static_cast<const TileStep*>(d_steps);
     const auto* ly = static_cast<const TileLayout*>(d_layouts);
