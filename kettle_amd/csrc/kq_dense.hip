@@ -6,8 +6,9 @@
 //   MFMA: one 16-row tile (256 amplitudes) per wave, new = Psi_tile x U^T as four real 16x16x16
 //         products (re.re, -im.im, re.im, im.re), each four 16x16x4 matrix instructions
 //         (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32); U^T's 4 x 16 slices stay in registers.
-// Operand layouts of the 16x16x4 instructions (lane l of 64): A[i][k] from lane i + 16 k, B[k][j]
-// from lane j + 16 k, D[4 g + e][j] in element e of lane j + 16 g.
+// Operand layouts of the 16x16x4 instructions (lane l of 64, measured with k_mfma16_layout on gfx950):
+// A[i][k] from lane i + 16 k and B[k][j] from lane j + 16 k for both; the result row held in element e
+// of lane j + 16 g differs: D[4 g + e][j] for f32, D[4 e + g][j] for f64 (rows interleaved).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,11 +34,14 @@ template <> struct Mfma16<double>
 {
     using V = v4d;
     static __device__ __forceinline__ V mac(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // result row of element e in lane group g
+    static __device__ __forceinline__ uint32_t row(uint32_t g, int e) { return 4 * e + g; }
 };
 template <> struct Mfma16<float>
 {
     using V = v4f;
     static __device__ __forceinline__ V mac(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ uint32_t row(uint32_t g, int e) { return 4 * g + e; }
 };
 
 // u: 16 x 16 complex, row-major, interleaved (re, im), in the amplitude precision
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(DENSE_BLOCK) void k_dense4_mfma(typename DenseAmp<R
             A y;
             y.x = dr[e];
             y.y = di[e];
-            tile[16 * (4 * hi + e) + lo] = y;   // D[row 4 hi + e][j = lo]
+            tile[16 * M::row(hi, e) + lo] = y;   // D[row][j = lo]
         }
     }
 }

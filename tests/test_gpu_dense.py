@@ -36,6 +36,39 @@ def apply_dense(state: ka.Statevector, u: np.ndarray, use_mfma: bool) -> float:
     return ms.value
 
 
+def mfma16(precision, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """One 16x16x4 matrix instruction on lane operands a[l], b[l]; returns [lane, element] of the result."""
+    d = np.zeros(256)
+    _lib.check(_lib.load().kq_debug_mfma16_layout(precision, _lib.ptr(np.ascontiguousarray(a, dtype=np.float64)),
+                                                   _lib.ptr(np.ascontiguousarray(b, dtype=np.float64)), _lib.ptr(d)))
+    return d.reshape(64, 4)
+
+
+@pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
+def test_mfma16_lane_layout(precision):
+    """The layout the MFMA form relies on: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, and
+    D[i][j] in lane j + 16 g, element e, with (g, e) = (i % 4, i // 4) for the f64 instruction and
+    (i // 4, i % 4) for the f32 one.  Checked with a random integer product (exact in both precisions)
+    against numpy, and with one-hot operands, so that a transposed or permuted operand fails."""
+    rng = np.random.default_rng(5)
+    A = rng.integers(-8, 9, size=(16, 4)).astype(np.float64)
+    B = rng.integers(-8, 9, size=(4, 16)).astype(np.float64)
+    lanes = np.arange(64)
+    a = A[lanes % 16, lanes // 16]
+    b = B[lanes // 16, lanes % 16]
+    d = mfma16(precision, a, b)
+    i, j = np.meshgrid(np.arange(16), np.arange(16), indexing="ij")
+    g, e = (i % 4, i // 4) if precision == ka.KQ_C128 else (i // 4, i % 4)
+    np.testing.assert_array_equal(d[j + 16 * g, e], A @ B)
+    for p in (0, 1, 5, 16, 37, 63):            # one A entry: only row p % 16 holds B's row p // 16
+        onehot = np.zeros(64)
+        onehot[p] = 1.0
+        d = mfma16(precision, onehot, b)
+        want = np.zeros((16, 16))
+        want[p % 16] = B[p // 16]
+        np.testing.assert_array_equal(d[j + 16 * g, e], want)
+
+
 def random_state(n: int, seed: int) -> np.ndarray:
     rng = np.random.default_rng(seed)
     v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
@@ -43,8 +76,7 @@ def random_state(n: int, seed: int) -> np.ndarray:
 
 
 @pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
-@pytest.mark.parametrize("use_mfma", [False, pytest.param(True, marks=pytest.mark.skip(
-    reason="the MFMA form's output rows are permuted: its 16x16x4 lane layout is being checked"))], ids=["valu", "mfma"])
+@pytest.mark.parametrize("use_mfma", [False, True], ids=["valu", "mfma"])
 @pytest.mark.parametrize("n", [8, 14])
 def test_dense_block_equals_kettle_gates(precision, use_mfma, n):
     ops, _, _ = workloads.random_circuit(4, depth=8, seed=n + 3).flatten()
@@ -59,7 +91,6 @@ def test_dense_block_equals_kettle_gates(precision, use_mfma, n):
 
 
 @pytest.mark.parametrize("precision", [ka.KQ_C128, ka.KQ_C64], ids=["c128", "c64"])
-@pytest.mark.skip(reason="the MFMA form's output rows are permuted: its 16x16x4 lane layout is being checked")
 def test_dense_block_forms_agree_on_haar_unitary(precision):
     rng = np.random.default_rng(7)
     q, r = np.linalg.qr(rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16)))
