@@ -236,6 +236,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
         std::vector<PauliGroup> groups;
         std::vector<PauliZ> strings;
         build_pauli(s->n, n_strings, off, q, k, phases, coeffs, groups, strings);
+        sort_pauli_groups_by_tile(groups);
         energy_gradient_unfused(s, ops, recs, groups, strings, n_params, energy, grad);
         return;
     }
@@ -263,6 +264,7 @@ void energy_gradient(kq_state* s, const kq_op* ops, size_t n_ops, size_t n_strin
     std::vector<PauliGroup> groups;
     std::vector<PauliZ> strings;
     build_pauli(s->n, n_strings, off, q_at.data(), k, phases, coeffs, groups, strings);
+    sort_pauli_groups_by_tile(groups);
     std::vector<size_t> bounds;
     if (every == 0) {
         const size_t stride = std::max<size_t>(1, static_cast<size_t>(std::ceil(std::sqrt(static_cast<double>(P)))));

@@ -8,6 +8,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/kq.h"
@@ -458,6 +459,16 @@ void launch_max_rows(hipStream_t st, const double* rows, const uint64_t* d_offs,
 void launch_pauli_expectation(kq_state* s, const std::vector<PauliGroup>& groups, const std::vector<PauliZ>& strings,
                               double out[2]);
 // b_i = sum_g (sum_{s in g} w_s (-1)^{|i & z_s|}) a_{i ^ x_g}  (b = H a), same precision as a
+// H psi runs over tiles of 2^PAULI_TILE_BITS amplitudes once the state holds at least 16 of them
+constexpr int PAULI_TILE_BITS = 12;
+// stable order of the groups by their flip mask's bits above the tile (the partner tile), so that
+// the tiled H psi stages each distinct partner tile once; the sums' terms only change order
+inline void sort_pauli_groups_by_tile(std::vector<PauliGroup>& groups)
+{
+    std::stable_sort(groups.begin(), groups.end(), [](const PauliGroup& l, const PauliGroup& r) {
+        return (l.x >> PAULI_TILE_BITS) < (r.x >> PAULI_TILE_BITS);
+    });
+}
 void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const std::vector<PauliGroup>& groups,
                             const std::vector<PauliZ>& strings);
 // in-place (c, x, z) operator: a_i <- w (-1)^{|i & z|} a_{i ^ x}

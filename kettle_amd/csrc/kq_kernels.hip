@@ -605,6 +605,79 @@ __global__ __launch_bounds__(BLOCK) void k_pauli_apply_sum(const Amp<R>* a, Amp<
     }
 }
 
+// b = H a over tiles of 2^PAULI_TILE_BITS amplitudes, one workgroup per output tile t: a group with flip
+// mask x reads tile t ^ (x >> PAULI_TILE_BITS) at offsets i ^ (x & (TILE - 1)).  The partner tile is
+// staged in LDS and kept for the following groups while their high flip bits agree, so groups sorted
+// by x >> PAULI_TILE_BITS (sort_pauli_groups_by_tile) read each distinct partner tile once per output
+// tile instead of the state once per group.  The sign parity of a string splits into the tile's bits
+// (uniform) and the lane's offset bits.  Needs a != b and n = n_tiles * TILE.
+template <typename R>
+__global__ __launch_bounds__(BLOCK) void k_pauli_apply_tiled(const Amp<R>* __restrict__ a, Amp<R>* __restrict__ b,
+                                                             uint64_t n_tiles, const PauliGroup* groups, int ng,
+                                                             const PauliZ* zs)
+{
+    constexpr uint32_t TILE = 1u << PAULI_TILE_BITS;
+    constexpr int PER = TILE / BLOCK;
+    __shared__ Amp<R> part[TILE];
+    for (uint64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        double hr[PER], hi[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            hr[k] = 0.0;
+            hi[k] = 0.0;
+        }
+        uint64_t staged = ~uint64_t {0};
+        for (int gi = 0; gi < ng; ++gi) {
+            const PauliGroup g = groups[gi];
+            const uint64_t hx = g.x >> PAULI_TILE_BITS;
+            if (hx != staged) {   // uniform over the workgroup
+                __syncthreads();
+                const Amp<R>* src = a + ((t ^ hx) << PAULI_TILE_BITS);
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    part[threadIdx.x + k * BLOCK] = src[threadIdx.x + k * BLOCK];
+                }
+                __syncthreads();
+                staged = hx;
+            }
+            const uint32_t xl = static_cast<uint32_t>(g.x) & (TILE - 1);
+            double cr[PER], ci[PER];
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                cr[k] = 0.0;
+                ci[k] = 0.0;
+            }
+            for (uint32_t s = 0; s < g.count; ++s) {
+                const PauliZ z = zs[g.first + s];
+                const uint32_t tpar = static_cast<uint32_t>(__popcll(t & (z.z >> PAULI_TILE_BITS))) & 1u;
+                const uint32_t zl = static_cast<uint32_t>(z.z) & (TILE - 1);
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const uint32_t il = threadIdx.x + k * BLOCK;
+                    const double sg = ((__popc(il & zl) & 1u) ^ tpar) ? -1.0 : 1.0;
+                    cr[k] += sg * z.w[0];
+                    ci[k] += sg * z.w[1];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const Amp<R> u = part[(threadIdx.x + k * BLOCK) ^ xl];
+                const double br = u.x, bi = u.y;
+                hr[k] += cr[k] * br - ci[k] * bi;
+                hi[k] += cr[k] * bi + ci[k] * br;
+            }
+        }
+        Amp<R>* dst = b + (t << PAULI_TILE_BITS);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            Amp<R> w;
+            w.x = static_cast<R>(hr[k]);
+            w.y = static_cast<R>(hi[k]);
+            dst[threadIdx.x + k * BLOCK] = w;
+        }
+    }
+}
+
 
 
 // in place a_i <- w (-1)^{|i & z|} a_{i ^ x}; for x != 0 each pair (i, i ^ x) is handled by the
@@ -1385,8 +1458,16 @@ void launch_pauli_apply_sum(kq_state* s, const void* d_in, void* d_out, const st
     PauliGroup* dg;
     PauliZ* dz;
     upload_pauli(s, 0, groups, strings, &dg, &dz);
-    const unsigned blocks = grid_for(s->n_amps, PAULI_ROWS);
     const int ng = static_cast<int>(groups.size());
+    const uint64_t n_tiles = s->n_amps >> PAULI_TILE_BITS;
+    if (d_in != d_out && n_tiles >= 16) {
+        const unsigned tb = static_cast<unsigned>(std::min<uint64_t>(n_tiles, uint64_t {1} << 20));
+        KQ_DISPATCH(s,
+                    (k_pauli_apply_tiled<double><<<tb, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), n_tiles, dg, ng, dz)),
+                    (k_pauli_apply_tiled<float><<<tb, BLOCK, 0, s->stream>>>(static_cast<const float2*>(d_in), static_cast<float2*>(d_out), n_tiles, dg, ng, dz)));
+        return;
+    }
+    const unsigned blocks = grid_for(s->n_amps, PAULI_ROWS);
     if (s->nvec <= 32) {
         KQ_DISPATCH(s,
                     (k_pauli_apply_sum<double, true><<<blocks, BLOCK, 0, s->stream>>>(static_cast<const double2*>(d_in), static_cast<double2*>(d_out), s->n_amps, dg, ng, dz)),
